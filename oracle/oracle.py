@@ -1,0 +1,166 @@
+"""ctypes access to the test oracle (liboracle.so, the FP64 C restatement) and to the compiled reference (_ref/libxcref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the product
+package.  Both libraries expose the same operations:
+
+    setup(accuracy, Z, xyz_bohr, spec, spherical, ids) -> handle
+    npoints / nblocks / nfunctions
+    grid()            -> (xyzw [G,4], block_atom [B], block_count [B])
+    integrate(P)      -> (E_xc, N_e, V_packed, seconds)          (V starts at zero here)
+    block_values(b)   -> (indices, f, fx, fy, fz)
+    functional(ids, rho, sigma) -> (eps, vrho, vsigma)
+    c2s(lLow, lHigh)  -> matrix
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libxcref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class _Library:
+    """One of the two CPU libraries; prefix is 'xqo_' (oracle) or 'xcref_' (reference)."""
+
+    def __init__(self, path, prefix):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        f = self._fn
+        f("setup").restype = ctypes.c_void_p
+        f("setup").argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _ip, _ip]
+        for name in ("npoints", "nblocks", "nfunctions"):
+            f(name).restype = ctypes.c_int
+            f(name).argtypes = [ctypes.c_void_p]
+        f("grid").restype = None
+        f("grid").argtypes = [ctypes.c_void_p, _dp, _ip, _ip]
+        f("integrate").restype = ctypes.c_int
+        f("integrate").argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        f("block_values").restype = ctypes.c_int
+        f("block_values").argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
+        f("functional").restype = ctypes.c_int
+        f("functional").argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        f("c2s").restype = None
+        f("c2s").argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        f("free").restype = None
+        f("free").argtypes = [ctypes.c_void_p]
+
+    def _fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def functional(self, ids, rho, sigma):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        n = rho.size
+        e, vr, vs = np.zeros(n), np.zeros(n), np.zeros(n)
+        st = self._fn("functional")(len(ids), _i(ids), n, _d(rho), _d(sigma), _d(e), _d(vr), _d(vs))
+        if st != 0:
+            raise ValueError("functional status {:d}".format(st))
+        return e, vr, vs
+
+    def c2s(self, lLow, lHigh):
+        nc = sum((l + 1) * (l + 2) // 2 for l in range(lLow, lHigh + 1))
+        ns = sum(2 * l + 1 for l in range(lLow, lHigh + 1))
+        out = np.zeros((nc, ns))
+        self._fn("c2s")(lLow, lHigh, _d(out))
+        return out
+
+    def setup(self, accuracy, Z, xyz_bohr, spec, spherical, ids):
+        return _Handle(self, accuracy, Z, xyz_bohr, spec, spherical, ids)
+
+
+class _Handle:
+    def __init__(self, lib, accuracy, Z, xyz_bohr, spec, spherical, ids):
+        self.lib = lib
+        self._Z = np.ascontiguousarray(Z, dtype=np.int32)
+        self._xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+        self._spec = np.ascontiguousarray(spec, dtype=np.float64)
+        self._ids = np.ascontiguousarray(ids, dtype=np.int32)
+        st = np.zeros(1, dtype=np.int32)
+        self.h = lib._fn("setup")(int(accuracy), len(self._Z), _i(self._Z), _d(self._xyz), self._spec.size, _d(self._spec),
+                                  int(bool(spherical)), len(self._ids), _i(self._ids), _i(st))
+        if not self.h or st[0] != 0:
+            raise ValueError("setup status {:d}".format(int(st[0])))
+        self.npoints = lib._fn("npoints")(self.h)
+        self.nblocks = lib._fn("nblocks")(self.h)
+        self.nfunctions = lib._fn("nfunctions")(self.h)
+
+    def grid(self):
+        xyzw = np.zeros((self.npoints, 4))
+        ba = np.zeros(self.nblocks, dtype=np.int32)
+        bc = np.zeros(self.nblocks, dtype=np.int32)
+        self.lib._fn("grid")(self.h, _d(xyzw), _i(ba), _i(bc))
+        return xyzw, ba, bc
+
+    def integrate(self, P_packed, threads=1, V=None):
+        """E_xc, N_e, V (packed; incremented from V if given, else from zero) and the call's wall time."""
+        P = np.ascontiguousarray(P_packed, dtype=np.float64)
+        n = self.nfunctions
+        if P.size != n * (n + 1) // 2:
+            raise ValueError("P has {:d} entries, expected {:d}".format(P.size, n * (n + 1) // 2))
+        V = np.zeros_like(P) if V is None else np.ascontiguousarray(V, dtype=np.float64).copy()
+        e, ne, sec = np.zeros(1), np.zeros(1), np.zeros(1)
+        st = self.lib._fn("integrate")(self.h, _d(P), int(threads), _d(e), _d(ne), _d(V), _d(sec))
+        if st != 0:
+            raise ValueError("integrate status {:d}".format(st))
+        return float(e[0]), float(ne[0]), V, float(sec[0])
+
+    def block_values(self, block):
+        n = self.nfunctions
+        xyzw, ba, bc = self.grid() if not hasattr(self, "_bc") else (None, None, self._bc)
+        self._bc = bc
+        npb = int(bc[block])
+        idx = np.zeros(n, dtype=np.int32)
+        f, fx, fy, fz = (np.zeros(n * npb) for _ in range(4))
+        nb = self.lib._fn("block_values")(self.h, int(block), _i(idx), _d(f), _d(fx), _d(fy), _d(fz))
+        shape = (nb, npb)
+        return idx[:nb].copy(), f[:nb * npb].reshape(shape), fx[:nb * npb].reshape(shape), \
+            fy[:nb * npb].reshape(shape), fz[:nb * npb].reshape(shape)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib._fn("free")(self.h)
+            self.h = None
+
+
+_oracle = None
+_reference = None
+
+
+def oracle():
+    """The C restatement (built by `make -C oracle`)."""
+    global _oracle
+    if _oracle is None:
+        _oracle = _Library(ORACLE_PATH, "xqo_")
+    return _oracle
+
+
+def reference():
+    """The compiled reference path (built by `make -C oracle ref`); raises FileNotFoundError when absent."""
+    global _reference
+    if _reference is None:
+        _reference = _Library(REF_PATH, "xcref_")
+    return _reference
+
+
+def reference_available():
+    return os.path.exists(REF_PATH)
+
+
+# . The compiled reference names its integrate "inCore" flag where the oracle takes a thread count; both are ints.

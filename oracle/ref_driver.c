@@ -1,0 +1,259 @@
+/*==================================================================================================================================
+! Driver for the compiled pDynamo3 reference C path (test infrastructure only).
+!
+! This file is NOT part of the product.  It is linked, by oracle/Makefile, against the reference's own C sources compiled where
+! they lie under the reference tree (DFTGrid.c, DFTGridWeights.c, DFTIntegrator.c, the vendored libxc "cxc", the Gaussian basis
+! C files, pCore and pScientific arrays), producing oracle/_ref/libxcref.so.  It exists only to write golden vectors and to time
+! the reference's own CPU path; no product code loads it.
+!
+! Each entry point reproduces what the reference's Python/Cython layer does before calling C:
+!   - basis set construction follows GaussianBasis.pyx:__setstate__ (pyrex/pMolecule.QCModel.GaussianBases.GaussianBasis.pyx:114-137)
+!     followed by Finalize (GaussianBasisContainer.pyx:136) and the container index arrays (GaussianBasisContainer.pyx:100-112);
+!   - grid construction calls DFTGrid_Construct (DFTGrid.c:105) as DFTGrid.pyx:57-73 does;
+!   - the quadrature calls DFTIntegrator_Integrate (DFTIntegrator.c:159) as DFTIntegrator.pyx:23-73 does (restricted case).
+!=================================================================================================================================*/
+# include <math.h>
+# include <stdlib.h>
+# include <string.h>
+# include <time.h>
+
+# include "Coordinates3.h"
+# include "DFTFunctionalModel.h"
+# include "DFTGrid.h"
+# include "DFTIntegrator.h"
+# include "GaussianBasis.h"
+# include "GaussianBasisContainer.h"
+# include "GaussianBasisContainerIntegrals_f1Op1.h"
+# include "GridFunctionDataBlock.h"
+# include "IntegerArray1D.h"
+# include "SymmetricMatrix.h"
+
+typedef struct {
+    Integer                 nAtoms      ;
+    Integer                 nFunctions  ;
+    Coordinates3           *coordinates ;
+    IntegerArray1D         *atomicNumbers ;
+    DFTGrid                *grid        ;
+    GaussianBasisContainer *bases       ;
+    GaussianBasis         **unique      ;
+    Integer                 nUnique     ;
+    DFTFunctionalModel     *functional  ;
+    IntegerArray1D         *cfp, *fc    ;
+} RefHandle ;
+
+/* . Basis specification: a flat array of doubles, per element:
+!    Z, nShells, then per shell: lLow, lHigh, nPrimitives, then per primitive: exponent, coefficient(lLow..lHigh) (normalized). */
+static GaussianBasis *MakeBasis ( const double *spec, int *consumed, int spherical )
+{
+    int p = 0, s, q, c, z, nShells ;
+    GaussianBasis *b ;
+    z       = ( int ) spec[p++] ;
+    nShells = ( int ) spec[p++] ;
+    b = GaussianBasis_Allocate ( nShells ) ;
+    b->atomicNumber = z ;
+    b->isSpherical  = ( spherical ? True : False ) ;
+    b->pNormalized  = True ;
+    for ( s = 0 ; s < nShells ; s++ )
+    {
+        int lLow  = ( int ) spec[p++] ;
+        int lHigh = ( int ) spec[p++] ;
+        int nP    = ( int ) spec[p++] ;
+        GaussianBasis_AllocateShell ( b, s, lHigh, lLow, nP ) ;
+        for ( q = 0 ; q < nP ; q++ )
+        {
+            b->shells[s].primitives[q].exponent  = spec[p] ;
+            b->shells[s].primitives[q].exponent0 = spec[p] ; p++ ;
+            for ( c = 0 ; c < ( lHigh - lLow + 1 ) ; c++ )
+            {
+                b->shells[s].primitives[q].coefficients [c] = spec[p] ;
+                b->shells[s].primitives[q].coefficients0[c] = spec[p] ; p++ ;
+            }
+        }
+    }
+    GaussianBasis_UnnormalizePrimitives ( b ) ;
+    {
+        Status st = Status_OK ;
+        GaussianBasis_Finalize ( b, &st ) ;
+    }
+    *consumed = p ;
+    return b ;
+}
+
+void *xcref_setup ( int accuracy, int nAtoms, const int *Z, const double *xyzBohr,
+                    int nSpec, const double *spec, int spherical, int nIDs, const int *ids, int *status )
+{
+    RefHandle *h = ( RefHandle * ) calloc ( 1, sizeof ( RefHandle ) ) ;
+    Status     st = Status_OK ;
+    int        i, p, u ;
+    h->nAtoms        = nAtoms ;
+    h->coordinates   = Coordinates3_Allocate ( nAtoms, &st ) ;
+    h->atomicNumbers = IntegerArray1D_AllocateWithExtent ( nAtoms, &st ) ;
+    for ( i = 0 ; i < nAtoms ; i++ )
+    {
+        Coordinates3_SetRow ( h->coordinates, i, xyzBohr[3*i], xyzBohr[3*i+1], xyzBohr[3*i+2] ) ;
+        Array1D_Item ( h->atomicNumbers, i ) = Z[i] ;
+    }
+    /* . Unique bases. */
+    h->unique  = ( GaussianBasis ** ) calloc ( 128, sizeof ( GaussianBasis * ) ) ;
+    h->nUnique = 0 ;
+    for ( p = 0 ; p < nSpec ; )
+    {
+        int used = 0 ;
+        GaussianBasis *b = MakeBasis ( &spec[p], &used, spherical ) ;
+        h->unique[h->nUnique++] = b ;
+        p += used ;
+    }
+    /* . Container: shares the unique entries (not the owner). */
+    h->bases = GaussianBasisContainer_Allocate ( nAtoms, &st ) ;
+    for ( i = 0 ; i < nAtoms ; i++ )
+    {
+        GaussianBasis *b = NULL ;
+        for ( u = 0 ; u < h->nUnique ; u++ ) if ( h->unique[u]->atomicNumber == Z[i] ) b = h->unique[u] ;
+        if ( b == NULL ) { *status = 3 ; return NULL ; }
+        h->bases->entries[i] = b ;
+    }
+    h->nFunctions = GaussianBasisContainer_NumberOfFunctions ( h->bases ) ;
+    h->cfp = IntegerArray1D_AllocateWithExtent ( nAtoms + 1   , &st ) ;
+    h->fc  = IntegerArray1D_AllocateWithExtent ( h->nFunctions, &st ) ;
+    GaussianBasisContainer_MakeIndexArrays ( h->bases, h->cfp, h->fc, &st ) ;
+    /* . Functional. */
+    {
+        IntegerArray1D *idArray = IntegerArray1D_AllocateWithExtent ( nIDs, &st ) ;
+        for ( i = 0 ; i < nIDs ; i++ ) Array1D_Item ( idArray, i ) = ids[i] ;
+        h->functional = DFTFunctionalModel_MakeFromIDs ( idArray, True, &st ) ;
+        IntegerArray1D_Deallocate ( &idArray ) ;
+    }
+    /* . Grid. */
+    h->grid = DFTGrid_Construct ( ( DFTGridAccuracy ) accuracy, h->atomicNumbers, h->coordinates, &st ) ;
+    DFTGrid_MakeRecords ( h->grid, &st ) ;
+    *status = ( int ) st ;
+    return ( void * ) h ;
+}
+
+int xcref_nfunctions ( void *vh ) { return ( ( RefHandle * ) vh )->nFunctions ; }
+int xcref_npoints    ( void *vh ) { return DFTGrid_NumberOfPoints  ( ( ( RefHandle * ) vh )->grid ) ; }
+int xcref_nblocks    ( void *vh ) { return DFTGrid_NumberOfRecords ( ( ( RefHandle * ) vh )->grid ) ; }
+int xcref_estimated_points ( void *vh )
+{
+    RefHandle *h = ( RefHandle * ) vh ;
+    Status st = Status_OK ;
+    return DFTGrid_EstimatedPoints ( h->grid->accuracy, h->atomicNumbers, &st ) ;
+}
+
+/* . Grid points (x,y,z,w per point, in record order) and per-block (atom, count). */
+void xcref_grid ( void *vh, double *xyzw, int *blockAtom, int *blockCount )
+{
+    RefHandle *h = ( RefHandle * ) vh ;
+    int r, i, n = 0 ;
+    for ( r = 0 ; r < h->grid->numberOfRecords ; r++ )
+    {
+        DFTGridPointBlock *b = h->grid->records[r] ;
+        blockAtom [r] = b->atom ;
+        blockCount[r] = b->numberOfPoints ;
+        for ( i = 0 ; i < b->numberOfPoints ; i++, n++ )
+        {
+            xyzw[4*n  ] = Coordinates3_Item ( b->coordinates3, i, 0 ) ;
+            xyzw[4*n+1] = Coordinates3_Item ( b->coordinates3, i, 1 ) ;
+            xyzw[4*n+2] = Coordinates3_Item ( b->coordinates3, i, 2 ) ;
+            xyzw[4*n+3] = Array1D_Item ( b->weights, i ) ;
+        }
+    }
+}
+
+/* . One quadrature: returns E_xc and N_e; V (packed lower triangle) is incremented in place.  seconds = wall time of the call. */
+int xcref_integrate ( void *vh, const double *P, int inCore, double *eXC, double *rho, double *V, double *seconds )
+{
+    RefHandle *h = ( RefHandle * ) vh ;
+    Status st = Status_OK ;
+    int n = h->nFunctions, k, size = ( n * ( n + 1 ) ) / 2 ;
+    struct timespec t0, t1 ;
+    SymmetricMatrix *dP = SymmetricMatrix_AllocateWithExtent ( n, &st ) ;
+    SymmetricMatrix *fA = SymmetricMatrix_AllocateWithExtent ( n, &st ) ;
+    for ( k = 0 ; k < size ; k++ ) { dP->data[k] = P[k] ; fA->data[k] = V[k] ; }
+    clock_gettime ( CLOCK_MONOTONIC, &t0 ) ;
+    DFTIntegrator_Integrate ( h->functional, h->grid, h->bases, h->coordinates, dP, NULL, ( inCore ? True : False ), False,
+                              eXC, rho, fA, NULL, NULL, &st ) ;
+    clock_gettime ( CLOCK_MONOTONIC, &t1 ) ;
+    for ( k = 0 ; k < size ; k++ ) V[k] = fA->data[k] ;
+    if ( seconds != NULL ) *seconds = ( t1.tv_sec - t0.tv_sec ) + 1.0e-9 * ( t1.tv_nsec - t0.tv_nsec ) ;
+    SymmetricMatrix_Deallocate ( &dP ) ;
+    SymmetricMatrix_Deallocate ( &fA ) ;
+    return ( int ) st ;
+}
+
+/* . Basis values and first derivatives of one block after the reference's |phi| > bfTol filter.
+!    Outputs: nb, indices[nb], f/fx/fy/fz [nb x nPoints] row-major.  Returns nb. */
+int xcref_block_values ( void *vh, int record, int *indices, double *f, double *fx, double *fy, double *fz )
+{
+    RefHandle *h = ( RefHandle * ) vh ;
+    Status st = Status_OK ;
+    DFTGridPointBlock *b = h->grid->records[record] ;
+    GridFunctionDataBlock *d = GridFunctionDataBlock_Allocate ( h->nFunctions, b->numberOfPoints, 1, &st ) ;
+    int i, p, nb ;
+    GaussianBasisContainerIntegrals_f1Op1ir123 ( h->bases, h->coordinates, b->coordinates3, True, &(h->grid->bfTolerance), d, &st ) ;
+    nb = d->numberOfFunctions ;
+    for ( i = 0 ; i < nb ; i++ )
+    {
+        indices[i] = Array1D_Item ( d->indices, i ) ;
+        for ( p = 0 ; p < b->numberOfPoints ; p++ )
+        {
+            f [i*b->numberOfPoints+p] = Array2D_Item ( d->f , i, p ) ;
+            fx[i*b->numberOfPoints+p] = Array2D_Item ( d->fX, i, p ) ;
+            fy[i*b->numberOfPoints+p] = Array2D_Item ( d->fY, i, p ) ;
+            fz[i*b->numberOfPoints+p] = Array2D_Item ( d->fZ, i, p ) ;
+        }
+    }
+    GridFunctionDataBlock_Deallocate ( &d ) ;
+    return nb ;
+}
+
+/* . Functional values for an unpolarized density: eps (per particle), vrho, vsigma, summed over the functionals as
+!    DFTFunctionalModel_Evaluate does (DFTFunctionalModel.c:108-154). */
+int xcref_functional ( int nIDs, const int *ids, int n, const double *rho, const double *sigma,
+                       double *eXC, double *vRho, double *vSigma )
+{
+    int f, i ;
+    double *e = ( double * ) malloc ( n * sizeof ( double ) ), *vr = ( double * ) malloc ( n * sizeof ( double ) ),
+           *vs = ( double * ) malloc ( n * sizeof ( double ) ) ;
+    for ( i = 0 ; i < n ; i++ ) { eXC[i] = 0.0 ; vRho[i] = 0.0 ; vSigma[i] = 0.0 ; }
+    for ( f = 0 ; f < nIDs ; f++ )
+    {
+        xc_func_type func ;
+        if ( xc_func_init ( &func, ids[f], XC_UNPOLARIZED ) < 0 ) { free ( e ) ; free ( vr ) ; free ( vs ) ; return 3 ; }
+        xc_gga_exc_vxc ( &func, n, rho, sigma, e, vr, vs ) ;
+        for ( i = 0 ; i < n ; i++ ) { eXC[i] += e[i] ; vRho[i] += vr[i] ; vSigma[i] += vs[i] ; }
+        xc_func_end ( &func ) ;
+    }
+    free ( e ) ; free ( vr ) ; free ( vs ) ;
+    return 0 ;
+}
+
+/* . Cartesian to spherical transformation for lLow..lHigh (nc x ns, row-major). */
+void xcref_c2s ( int lLow, int lHigh, double *out )
+{
+    Status st = Status_OK ;
+    RealArray2D *t = GaussianBasis_TransformationCartesianToSpherical ( lLow, lHigh, &st ) ;
+    int i, j ;
+    for ( i = 0 ; i < View2D_Rows ( t ) ; i++ )
+        for ( j = 0 ; j < View2D_Columns ( t ) ; j++ ) out[i*View2D_Columns ( t )+j] = Array2D_Item ( t, i, j ) ;
+    RealArray2D_Deallocate ( &t ) ;
+}
+
+void xcref_free ( void *vh )
+{
+    RefHandle *h = ( RefHandle * ) vh ;
+    int u ;
+    Status st = Status_OK ;
+    if ( h == NULL ) return ;
+    DFTGrid_Deallocate ( &h->grid, &st ) ;
+    DFTFunctionalModel_Deallocate ( &h->functional ) ;
+    for ( u = 0 ; u < h->nUnique ; u++ ) GaussianBasis_Deallocate ( &h->unique[u] ) ;
+    free ( h->unique ) ;
+    free ( h->bases->entries ) ;
+    IntegerArray1D_Deallocate ( &h->cfp ) ;
+    IntegerArray1D_Deallocate ( &h->fc ) ;
+    free ( h->bases ) ;
+    Coordinates3_Deallocate ( &h->coordinates ) ;
+    IntegerArray1D_Deallocate ( &h->atomicNumbers ) ;
+    free ( h ) ;
+}

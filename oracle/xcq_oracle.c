@@ -1,0 +1,898 @@
+/*==================================================================================================================================
+! xcq_oracle.c -- FP64 CPU restatement of pDynamo3's DFT exchange-correlation quadrature.
+!
+! TEST INFRASTRUCTURE ONLY.  This file is the oracle that the GPU library is checked against, and the "port" CPU baseline that
+! bench.py times.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it (through oracle/oracle.py); the
+! product library (pdynamo3_amd) never links, loads or calls it.
+!
+! Every function cites the reference code it follows (paths relative to the pDynamo3 source root).  The arithmetic is written
+! from the mathematics; where the reference fixes an order or a constant (Lebedev orbit order, the Becke polynomial constants,
+! the libxc-1.2.0 thresholds) the same values are used so that results agree to rounding.
+!
+! Parity pins (see tests/test_oracle.py):
+!   - grid: point count, block structure and (x,y,z,w) checksums against the compiled reference (tests/golden/*grid*),
+!     Lebedev invariants (sum w = 1, exact monomial integration), radial closed form;
+!   - basis: per-block values/gradients and kept-function indices against the compiled reference, c2s against the reference's
+!     own fixture C2STransformations_0to6.yaml, gradients against finite differences, <phi|phi> = 1 by quadrature;
+!   - functionals: against the compiled libxc-1.2.0 on edge and bulk (rho, sigma) points, and against finite differences;
+!   - quadrature: E_xc, N_e and V_xc against the compiled reference's DFTIntegrator_Integrate.
+!=================================================================================================================================*/
+# include <math.h>
+# include <stdlib.h>
+# include <string.h>
+# include <time.h>
+# ifdef _OPENMP
+# include <omp.h>
+# endif
+
+# include "../pdynamo3_amd/csrc/lebedev_tables.h"
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Constants.
+!---------------------------------------------------------------------------------------------------------------------------------*/
+/* Bragg radii in Angstrom, index = atomic number (DFTGrid.c:29-40). */
+static const double BRAGG[120] = {
+    0.75, 0.35, 0.35, 1.45, 1.05, 0.85, 0.70, 0.65, 0.60, 0.50, 0.50, 1.80, 1.50, 1.25, 1.10, 1.00, 1.00, 1.00, 1.00, 2.20,
+    1.80, 1.60, 1.40, 1.35, 1.40, 1.40, 1.40, 1.35, 1.35, 1.35, 1.35, 1.30, 1.25, 1.15, 1.15, 1.15, 1.15, 2.35, 2.00, 1.80,
+    1.55, 1.45, 1.45, 1.35, 1.30, 1.35, 1.40, 1.60, 1.55, 1.55, 1.45, 1.45, 1.40, 1.40, 1.40, 2.60, 2.15, 1.95, 1.85, 1.85,
+    1.85, 1.85, 1.85, 1.85, 1.80, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75, 1.55, 1.45, 1.35, 1.35, 1.30, 1.35, 1.35, 1.35,
+    1.50, 1.90, 1.80, 1.60, 1.90, 1.90, 1.90, 2.60, 2.15, 1.95, 1.80, 1.80, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75,
+    1.75, 1.75, 1.75, 1.75, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55 } ;
+/* Angstrom -> bohr as the C side of the reference computes it (Units.h:35, Constants.h:15). */
+# define ANGSTROM_TO_BOHR ( 1.0e-10 / 5.291772083e-11 )
+/* Per-accuracy tolerances (DFTGrid.c:43,46). */
+static const double BF_TOL [5] = { 1.0e-8 , 1.0e-9 , 1.0e-10, 1.0e-12, 1.0e-15 } ;
+static const double RHO_TOL[5] = { 1.0e-13, 1.0e-14, 1.0e-15, 1.0e-17, 1.0e-20 } ;
+# define BLOCK_SIZE      128      /* DFTGrid.c:82 */
+# define MIN_L           9        /* DFTGrid.c:101 */
+# define R_CUT_FACTOR    0.2      /* DFTGrid.c:102 */
+# define W_TOL           1.0e-30  /* DFTGrid.c:103 */
+/* Becke polynomial cell function s(nu) = 1/2 - A * sum_t C_t nu^(2t+1), C_t = (-1)^t binom(10,t)/(2t+1) normalised by A
+!  (DFTGridWeights.c:21-35; the same 16-digit decimal constants). */
+static const double BECKE_A = 1.850069046020527e+00 ;
+static const double BECKE_C[11] = { 1.000000000000000e+00, -3.333333333333333e+00,  9.000000000000000e+00, -17.142857142857142e+00,
+                                   23.333333333333332e+00, -22.909090909090910e+00, 16.153846153846153e+00,  -8.000000000000000e+00,
+                                    2.647058823529412e+00,  -0.526315789473684e+00,  0.047619047619048e+00 } ;
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Grid parameters (DFTGrid.c:474-523, 528-535, 562-575).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+static void AtomParameters ( int accuracy, int z, int *nR, int *lMax, double *rMax )
+{
+    switch ( accuracy )
+    {
+        case 0: *nR = ( z < 11 ? 21 : z < 19 ? 42 : z < 37 ? 75 : 84 ) ; *rMax = 20.0 ; *lMax = 23 ; break ;
+        case 1: *nR = ( z < 11 ? 35 : z < 19 ? 70 : z < 37 ? 95 : 104 ) ; *rMax = 25.0 ; *lMax = 35 ; break ;
+        case 2: *nR = ( z < 11 ? 49 : z < 19 ? 88 : z < 37 ? 112 : 123 ) ; *rMax = 30.0 ; *lMax = ( z < 11 ? 35 : 41 ) ; break ;
+        case 3: *nR = ( z < 11 ? 70 : z < 19 ? 123 : z < 37 ? 130 : 155 ) ; *rMax = 35.0 ;
+                *lMax = ( z < 11 ? 41 : z < 19 ? 47 : z < 89 ? 53 : 59 ) ; break ;
+        default: *nR = ( z < 11 ? 100 : z < 19 ? 125 : z < 37 ? 160 : 205 ) ; *rMax = 35.0 ; *lMax = 65 ; break ;
+    }
+}
+
+static double BraggRadius ( int z )
+{
+    if ( z < 0 ) z = 0 ; else if ( z >= 120 ) z = 119 ;
+    return ANGSTROM_TO_BOHR * BRAGG[z] ;
+}
+
+/* Handy's Euler-Maclaurin m=2 radial rule: x_i = (i+1)/(n+1), r = R (x/(1-x))^2, w = 2 R^3 x^5 / (1-x)^7 / (n+1). */
+static void RadialPoints ( int n, double R, double *r, double *w )
+{
+    double d = 1.0 + ( double ) n ;
+    for ( int i = 0 ; i < n ; i++ )
+    {
+        double x = ( double ) ( i + 1 ) / d ;
+        r[i] = R * pow ( x / ( 1.0 - x ), 2.0 ) ;
+        w[i] = 2.0 * pow ( R, 3.0 ) * pow ( x, 5.0 ) / pow ( 1.0 - x, 7.0 ) / d ;
+    }
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Lebedev-Laikov rules (Lebedev.c:62-193 for the size <-> degree maps; orbit expansion order Lebedev.c:1565-1719).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+int xqo_lebedev_size_for_l ( int l )
+{
+    static const int small[16] = { 6, 6, 14, 26, 38, 50, 74, 86, 110, 146, 170, 194, 230, 266, 302, 350 } ;
+    static const int large[22] = { 0, 0, 0, 0, 0, 434, 590, 770, 974, 1202, 1454, 1730, 2030, 2354, 2702, 3074, 3470, 3890,
+                                   4334, 4802, 5294, 5810 } ;
+    if ( l < 3 ) return 6 ;
+    if ( l <= 31 ) { int k = ( l - 1 + 1 ) / 2 ; return small[k] ; }   /* ceil((l-1)/2) */
+    if ( l <= 131 ) { int k = ( l - 5 + 5 ) / 6 ; return large[k] ; }  /* ceil((l-5)/6) */
+    return 0 ;
+}
+
+int xqo_lebedev_l_for_size ( int n )
+{
+    static const int sizes[32] = { 6, 14, 26, 38, 50, 74, 86, 110, 146, 170, 194, 230, 266, 302, 350, 434, 590, 770, 974, 1202,
+                                   1454, 1730, 2030, 2354, 2702, 3074, 3470, 3890, 4334, 4802, 5294, 5810 } ;
+    static const int ls[32] = { 3, 5, 7, 9, 11, 13, 15, 17, 19, 21, 23, 25, 27, 29, 31, 35, 41, 47, 53, 59, 65, 71, 77, 83, 89, 95,
+                                101, 107, 113, 119, 125, 131 } ;
+    for ( int k = 0 ; k < 32 ; k++ ) if ( n <= sizes[k] ) return ls[k] ;
+    return 0 ;
+}
+
+/* Expand one octahedral orbit in the reference's point order.  Templates: each row gives, for x, y, z, which generator value
+!  (0 = zero, 1 = a, 2 = b, 3 = c) and its sign. */
+static int ExpandOrbit ( int type, double a, double b, double v, double *x, double *y, double *z, double *w )
+{
+    double g[4] ;
+    int n = 0 ;
+    g[0] = 0.0 ;
+    switch ( type )
+    {
+        case 1: g[1] = 1.0 ; break ;
+        case 2: g[1] = 1.0 / sqrt ( 2.0 ) ; break ;
+        case 3: g[1] = 1.0 / sqrt ( 3.0 ) ; break ;
+        case 4: g[1] = a ; g[2] = sqrt ( 1.0 - 2.0 * a * a ) ; break ;
+        case 5: g[1] = a ; g[2] = sqrt ( 1.0 - a * a ) ; break ;
+        case 6: g[1] = a ; g[2] = b ; g[3] = sqrt ( 1.0 - a * a - b * b ) ; break ;
+    }
+    /* Each template lists (component slot, sign) for x, y, z; slots: 0 none, 1 g1, 2 g2, 3 g3; sign +1/-1. */
+    static const signed char T1[6][3]  = { {1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1} } ;
+    static const signed char T2[12][3] = { {0,1,1},{0,1,-1},{0,-1,1},{0,-1,-1},{1,0,1},{1,0,-1},{-1,0,1},{-1,0,-1},
+                                           {1,1,0},{1,-1,0},{-1,1,0},{-1,-1,0} } ;
+    static const signed char T3[8][3]  = { {1,1,1},{1,1,-1},{1,-1,1},{1,-1,-1},{-1,1,1},{-1,1,-1},{-1,-1,1},{-1,-1,-1} } ;
+    static const signed char T4[24][3] = { {1,1,2},{1,1,-2},{1,-1,2},{1,-1,-2},{-1,1,2},{-1,1,-2},{-1,-1,2},{-1,-1,-2},
+                                           {1,2,1},{1,-2,1},{1,2,-1},{1,-2,-1},{-1,2,1},{-1,-2,1},{-1,2,-1},{-1,-2,-1},
+                                           {2,1,1},{-2,1,1},{2,1,-1},{-2,1,-1},{2,-1,1},{-2,-1,1},{2,-1,-1},{-2,-1,-1} } ;
+    static const signed char T5[24][3] = { {1,2,0},{1,-2,0},{-1,2,0},{-1,-2,0},{2,1,0},{2,-1,0},{-2,1,0},{-2,-1,0},
+                                           {1,0,2},{1,0,-2},{-1,0,2},{-1,0,-2},{2,0,1},{2,0,-1},{-2,0,1},{-2,0,-1},
+                                           {0,1,2},{0,1,-2},{0,-1,2},{0,-1,-2},{0,2,1},{0,2,-1},{0,-2,1},{0,-2,-1} } ;
+    static const signed char T6[48][3] = { {1,2,3},{1,2,-3},{1,-2,3},{1,-2,-3},{-1,2,3},{-1,2,-3},{-1,-2,3},{-1,-2,-3},
+                                           {1,3,2},{1,3,-2},{1,-3,2},{1,-3,-2},{-1,3,2},{-1,3,-2},{-1,-3,2},{-1,-3,-2},
+                                           {2,1,3},{2,1,-3},{2,-1,3},{2,-1,-3},{-2,1,3},{-2,1,-3},{-2,-1,3},{-2,-1,-3},
+                                           {2,3,1},{2,3,-1},{2,-3,1},{2,-3,-1},{-2,3,1},{-2,3,-1},{-2,-3,1},{-2,-3,-1},
+                                           {3,1,2},{3,1,-2},{3,-1,2},{3,-1,-2},{-3,1,2},{-3,1,-2},{-3,-1,2},{-3,-1,-2},
+                                           {3,2,1},{3,2,-1},{3,-2,1},{3,-2,-1},{-3,2,1},{-3,2,-1},{-3,-2,1},{-3,-2,-1} } ;
+    const signed char ( *T )[3] = NULL ;
+    int count = 0 ;
+    switch ( type )
+    {
+        case 1: T = T1 ; count = 6  ; break ;
+        case 2: T = T2 ; count = 12 ; break ;
+        case 3: T = T3 ; count = 8  ; break ;
+        case 4: T = T4 ; count = 24 ; break ;
+        case 5: T = T5 ; count = 24 ; break ;
+        case 6: T = T6 ; count = 48 ; break ;
+    }
+    for ( n = 0 ; n < count ; n++ )
+    {
+        double c[3] ;
+        for ( int k = 0 ; k < 3 ; k++ )
+        {
+            int s = T[n][k] ;
+            c[k] = ( s == 0 ? 0.0 : ( s > 0 ? g[s] : -g[-s] ) ) ;
+        }
+        x[n] = c[0] ; y[n] = c[1] ; z[n] = c[2] ; w[n] = v ;
+    }
+    return count ;
+}
+
+/* Points of the rule with n points; returns the count (0 if there is no such rule). */
+int xqo_lebedev_points ( int n, double *x, double *y, double *z, double *w )
+{
+    int r, k, m = 0 ;
+    for ( r = 0 ; r < XQ_LEBEDEV_NRULES ; r++ ) if ( xq_lebedev_npoints[r] == n ) break ;
+    if ( r == XQ_LEBEDEV_NRULES ) return 0 ;
+    for ( k = xq_lebedev_first[r] ; k < xq_lebedev_first[r+1] ; k++ )
+    {
+        const xq_lebedev_orbit *o = &xq_lebedev_orbits[k] ;
+        m += ExpandOrbit ( o->type, o->a, o->b, o->v, x + m, y + m, z + m, w + m ) ;
+    }
+    return m ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Becke partition weight of point g for atom a (DFTGridWeights.c:43-80 for the pair data, :220-258 for the weight).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+typedef struct { int n ; const double *xyz ; double *invR ; double *aij ; } BeckePairs ;
+
+static void BeckePairs_Make ( BeckePairs *bp, int n, const double *xyz, const double *radii )
+{
+    int n2 = ( n * ( n - 1 ) ) / 2, ij = 0 ;
+    bp->n = n ; bp->xyz = xyz ;
+    bp->invR = ( double * ) malloc ( ( n2 > 0 ? n2 : 1 ) * sizeof ( double ) ) ;
+    bp->aij  = ( double * ) malloc ( ( n2 > 0 ? n2 : 1 ) * sizeof ( double ) ) ;
+    for ( int i = 0 ; i < n ; i++ )
+    {
+        for ( int j = 0 ; j < i ; j++, ij++ )
+        {
+            double dx = xyz[3*i] - xyz[3*j], dy = xyz[3*i+1] - xyz[3*j+1], dz = xyz[3*i+2] - xyz[3*j+2] ;
+            double chi = radii[i] / radii[j], u = ( chi - 1.0 ) / ( chi + 1.0 ), a = u / ( u * u - 1.0 ) ;
+            bp->invR[ij] = 1.0 / sqrt ( dx * dx + dy * dy + dz * dz ) ;
+            bp->aij [ij] = ( a > 0.5 ? 0.5 : ( a < -0.5 ? -0.5 : a ) ) ;
+        }
+    }
+}
+
+static double BeckeWeight ( const BeckePairs *bp, int atom, const double *g, double *P, double *r )
+{
+    int n = bp->n, ij = 0 ;
+    for ( int i = 0 ; i < n ; i++ )
+    {
+        double dx = bp->xyz[3*i] - g[0], dy = bp->xyz[3*i+1] - g[1], dz = bp->xyz[3*i+2] - g[2] ;
+        P[i] = 1.0 ;
+        r[i] = sqrt ( dx * dx + dy * dy + dz * dz ) ;
+    }
+    for ( int i = 0 ; i < n ; i++ )
+    {
+        for ( int j = 0 ; j < i ; j++, ij++ )
+        {
+            double mu = ( r[i] - r[j] ) * bp->invR[ij] ;
+            double nu = mu + bp->aij[ij] * ( 1.0 - mu * mu ) ;
+            double nu2 = nu * nu, term = nu, sum = 0.0 ;
+            for ( int t = 0 ; t <= 10 ; t++ ) { sum += BECKE_C[t] * term ; term *= nu2 ; }
+            P[i] *= ( 0.5 - BECKE_A * sum ) ;
+            P[j] *= ( 0.5 + BECKE_A * sum ) ;
+        }
+    }
+    double total = 0.0 ;
+    for ( int i = 0 ; i < n ; i++ ) total += P[i] ;
+    return P[atom] / total ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Basis sets.
+!  Spec (flat doubles), per element: Z, nShells, then per shell lLow, lHigh, nPrim, then per primitive: exponent, coefficient for
+!  each l in lLow..lHigh (primitive-normalised input, as the YAML files hold them).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+typedef struct {
+    int     atom, lLow, lHigh, nPrim, nCBF, nBasis, start ;
+    double *exps ;        /* nPrim                                    */
+    double *ccbf ;        /* nPrim x nCBF : unnormalised, with Cartesian factors */
+    double *c2s ;         /* nCBF x nBasis, or NULL for Cartesian sets */
+    int    *px, *py, *pz ;/* nCBF Cartesian powers                      */
+} OShell ;
+
+static double OddFact ( int n ) { double f = 1.0 ; for ( int i = 3 ; i <= n ; i += 2 ) f *= i ; return f ; }
+static double Fact    ( int n ) { double f = 1.0 ; for ( int i = 2 ; i <= n ; i++ ) f *= i ; return f ; }
+static double Binom   ( int n, int k ) { return ( k < 0 || k > n ) ? 0.0 : Fact ( n ) / ( Fact ( k ) * Fact ( n - k ) ) ; }
+
+/* Cartesian powers of angular momentum l in the reference order: for z = 0..l, for y = 0..l-z, x = l-y-z (GaussianBasis.c:13-22). */
+static int CartesianPowers ( int l, int *px, int *py, int *pz )
+{
+    int n = 0 ;
+    for ( int z = 0 ; z <= l ; z++ ) for ( int y = 0 ; y <= l - z ; y++, n++ ) { px[n] = l - y - z ; py[n] = y ; pz[n] = z ; }
+    return n ;
+}
+
+/* Transformation from normalised Cartesian to normalised real solid-harmonic Gaussians for one l: H.B. Schlegel and
+!  M.J. Frisch, Int. J. Quantum Chem. 54 (1995) 83, eq. 15, with real combinations (cos -> column 2m-1, sin -> 2m, m=0 -> 0)
+!  and the spherical order m = 0, 1c, 1s, 2c, 2s, ... used by the reference (GaussianBasis.c:23-27, 531-643).
+!  out is (l+1)(l+2)/2 x (2l+1), row-major, written into a larger matrix with row stride ld at (r0, c0). */
+static void C2SBlock ( int l, double *out, int ld, int r0, int c0 )
+{
+    int px[64], py[64], pz[64], nc = CartesianPowers ( l, px, py, pz ) ;
+    for ( int c = 0 ; c < nc ; c++ )
+    {
+        int x = px[c], y = py[c], z = pz[c] ;
+        for ( int m = 0 ; m <= l ; m++ )
+        {
+            int jj = l - m - z ;
+            if ( jj < 0 || ( jj % 2 ) != 0 ) continue ;
+            int j = jj / 2 ;
+            double s0 = 0.0, sc = 0.0, ss = 0.0 ;
+            for ( int i = 0 ; i <= ( l - m ) / 2 ; i++ )
+            {
+                if ( j > i ) continue ;
+                double t = Binom ( l, i ) * Binom ( i, j ) * ( ( i % 2 ) ? -1.0 : 1.0 ) * Fact ( 2 * l - 2 * i ) / Fact ( l - m - 2 * i ) ;
+                for ( int k = 0 ; k <= j ; k++ )
+                {
+                    int p = x - 2 * k ;           /* power of x drawn from (x + i y)^m */
+                    if ( p < 0 || p > m ) continue ;
+                    double u = t * Binom ( j, k ) * Binom ( m, p ) ;
+                    int q = m - p ;               /* power of (i y): even -> real part, odd -> imaginary part */
+                    double sign = ( ( ( q / 2 ) % 2 ) ? -1.0 : 1.0 ) ;
+                    if ( m == 0 ) s0 += u ;
+                    else if ( ( q % 2 ) == 0 ) sc += sqrt ( 2.0 ) * sign * u ;
+                    else                       ss += sqrt ( 2.0 ) * sign * u ;
+                }
+            }
+            double norm = sqrt ( Fact ( 2 * x ) * Fact ( 2 * y ) * Fact ( 2 * z ) * Fact ( l ) / ( Fact ( 2 * l ) * Fact ( x ) * Fact ( y ) * Fact ( z ) ) )
+                        * sqrt ( Fact ( l - m ) / Fact ( l + m ) ) / ( pow ( 2.0, l ) * Fact ( l ) ) ;
+            if ( m == 0 ) out[( r0 + c ) * ld + c0] = norm * s0 ;
+            else
+            {
+                out[( r0 + c ) * ld + c0 + 2 * m - 1] = norm * sc ;
+                out[( r0 + c ) * ld + c0 + 2 * m    ] = norm * ss ;
+            }
+        }
+    }
+}
+
+void xqo_c2s ( int lLow, int lHigh, double *out )
+{
+    int nc = 0, ns = 0 ;
+    for ( int l = lLow ; l <= lHigh ; l++ ) { nc += ( ( l + 1 ) * ( l + 2 ) ) / 2 ; ns += 2 * l + 1 ; }
+    memset ( out, 0, nc * ns * sizeof ( double ) ) ;
+    for ( int l = lLow, r0 = 0, c0 = 0 ; l <= lHigh ; l++ )
+    {
+        C2SBlock ( l, out, ns, r0, c0 ) ;
+        r0 += ( ( l + 1 ) * ( l + 2 ) ) / 2 ; c0 += 2 * l + 1 ;
+    }
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Handle.
+!---------------------------------------------------------------------------------------------------------------------------------*/
+typedef struct {
+    int     accuracy, nAtoms, spherical, nFunctions, nShells, nIDs, ids[4] ;
+    int    *Z ;
+    double *xyz ;
+    OShell *shells ;
+    /* grid */
+    int     nPoints, nBlocks ;
+    double *points ;                /* x, y, z, w per point */
+    int    *blockAtom, *blockStart, *blockCount ;
+    double  bfTol, rhoTol ;
+    /* statistics of the last quadrature */
+    double  bfValues ;              /* sum over blocks of kept functions x points */
+    long    blocksSkipped ;
+} OHandle ;
+
+/* Build the per-atom shells from the element spec; returns 0 or a Status-like code. */
+static int BuildBasis ( OHandle *h, int nSpec, const double *spec )
+{
+    /* Parse elements. */
+    int nEl = 0, elStart[128], elZ[128] ;
+    for ( int p = 0 ; p < nSpec ; )
+    {
+        if ( nEl >= 128 ) return 3 ;
+        elStart[nEl] = p ; elZ[nEl] = ( int ) spec[p] ;
+        int ns = ( int ) spec[p+1] ; p += 2 ;
+        for ( int s = 0 ; s < ns ; s++ )
+        {
+            int lLow = ( int ) spec[p], lHigh = ( int ) spec[p+1], np = ( int ) spec[p+2] ;
+            p += 3 + np * ( 1 + lHigh - lLow + 1 ) ;
+        }
+        nEl++ ;
+    }
+    /* Count shells. */
+    int total = 0 ;
+    for ( int a = 0 ; a < h->nAtoms ; a++ )
+    {
+        int e ;
+        for ( e = 0 ; e < nEl ; e++ ) if ( elZ[e] == h->Z[a] ) break ;
+        if ( e == nEl ) return 3 ;
+        total += ( int ) spec[elStart[e]+1] ;
+    }
+    h->shells  = ( OShell * ) calloc ( total, sizeof ( OShell ) ) ;
+    h->nShells = total ;
+    int s = 0, f = 0 ;
+    for ( int a = 0 ; a < h->nAtoms ; a++ )
+    {
+        int e ;
+        for ( e = 0 ; e < nEl ; e++ ) if ( elZ[e] == h->Z[a] ) break ;
+        int p = elStart[e] + 2, ns = ( int ) spec[elStart[e]+1] ;
+        for ( int k = 0 ; k < ns ; k++, s++ )
+        {
+            OShell *sh = &h->shells[s] ;
+            sh->atom = a ; sh->lLow = ( int ) spec[p] ; sh->lHigh = ( int ) spec[p+1] ; sh->nPrim = ( int ) spec[p+2] ; p += 3 ;
+            sh->nCBF = 0 ; sh->nBasis = 0 ;
+            for ( int l = sh->lLow ; l <= sh->lHigh ; l++ ) { sh->nCBF += ( ( l + 1 ) * ( l + 2 ) ) / 2 ; sh->nBasis += 2 * l + 1 ; }
+            if ( ! h->spherical ) sh->nBasis = sh->nCBF ;
+            sh->start = f ; f += sh->nBasis ;
+            sh->exps = ( double * ) malloc ( sh->nPrim * sizeof ( double ) ) ;
+            sh->ccbf = ( double * ) malloc ( sh->nPrim * sh->nCBF * sizeof ( double ) ) ;
+            sh->px = ( int * ) malloc ( sh->nCBF * sizeof ( int ) ) ;
+            sh->py = ( int * ) malloc ( sh->nCBF * sizeof ( int ) ) ;
+            sh->pz = ( int * ) malloc ( sh->nCBF * sizeof ( int ) ) ;
+            for ( int l = sh->lLow, c = 0 ; l <= sh->lHigh ; l++ ) c += CartesianPowers ( l, sh->px + c, sh->py + c, sh->pz + c ) ;
+            for ( int q = 0 ; q < sh->nPrim ; q++ )
+            {
+                double alpha = spec[p++] ;
+                sh->exps[q] = alpha ;
+                for ( int l = sh->lLow, c = 0 ; l <= sh->lHigh ; l++ )
+                {
+                    /* Unnormalise: c / sqrt( pi^(3/2) (2a)^(-3/2) prod_{i<=l} (2i-1)/(4a) )  (GaussianBasis.c:728-751). */
+                    double e2 = 2.0 * alpha, fac = pow ( M_PI, 1.5 ) / ( e2 * sqrt ( e2 ) ) ;
+                    for ( int i = 1 ; i <= l ; i++ ) fac *= ( double ) ( 2 * i - 1 ) / ( 2.0 * e2 ) ;
+                    double coef = spec[p++] / sqrt ( fac ) ;
+                    /* Cartesian factor sqrt((2l-1)!!/((2x-1)!!(2y-1)!!(2z-1)!!)) (GaussianBasis.c:448-501). */
+                    int nl = ( ( l + 1 ) * ( l + 2 ) ) / 2 ;
+                    for ( int i = 0 ; i < nl ; i++, c++ )
+                    {
+                        double m = sqrt ( OddFact ( 2 * l - 1 ) / ( OddFact ( 2 * sh->px[c] - 1 ) * OddFact ( 2 * sh->py[c] - 1 ) * OddFact ( 2 * sh->pz[c] - 1 ) ) ) ;
+                        sh->ccbf[q * sh->nCBF + c] = coef * m ;
+                    }
+                }
+            }
+            if ( h->spherical )
+            {
+                sh->c2s = ( double * ) malloc ( sh->nCBF * sh->nBasis * sizeof ( double ) ) ;
+                xqo_c2s ( sh->lLow, sh->lHigh, sh->c2s ) ;
+            }
+        }
+    }
+    h->nFunctions = f ;
+    return 0 ;
+}
+
+/* Values and gradients of all functions at nP points (GaussianBasisIntegrals_f1Op1.c:100-197 + GaussianBasisTransform.c:85-125):
+!  e0 = sum_p c e^{-a r^2}, e1 = -2 sum_p c a e^{-a r^2}; phi = x^i y^j z^k e0; d/dx phi = (i x^{i-1} e0 + x^{i+1} e1) y^j z^k;
+!  spherical = c2s^T Cartesian.  Output rows = functions, columns = points (row-major, ld = nP). */
+static void EvaluateBasis ( const OHandle *h, int nP, const double *pts, double *f, double *fx, double *fy, double *fz )
+{
+    double g[4][64], e0[64], e1[64] ;
+    for ( int p = 0 ; p < nP ; p++ )
+    {
+        for ( int s = 0 ; s < h->nShells ; s++ )
+        {
+            const OShell *sh = &h->shells[s] ;
+            double dx = pts[4*p] - h->xyz[3*sh->atom], dy = pts[4*p+1] - h->xyz[3*sh->atom+1], dz = pts[4*p+2] - h->xyz[3*sh->atom+2] ;
+            double r2 = dx * dx + dy * dy + dz * dz ;
+            double X0[8], Y0[8], Z0[8], X1[8], Y1[8], Z1[8] ;
+            X0[0] = Y0[0] = Z0[0] = 1.0 ; X1[0] = Y1[0] = Z1[0] = 0.0 ;
+            for ( int i = 1 ; i <= sh->lHigh + 1 ; i++ )
+            {
+                X0[i] = dx * X0[i-1] ; Y0[i] = dy * Y0[i-1] ; Z0[i] = dz * Z0[i-1] ;
+                X1[i] = i * X0[i-1] ; Y1[i] = i * Y0[i-1] ; Z1[i] = i * Z0[i-1] ;
+            }
+            for ( int c = 0 ; c < sh->nCBF ; c++ ) { e0[c] = 0.0 ; e1[c] = 0.0 ; }
+            for ( int q = 0 ; q < sh->nPrim ; q++ )
+            {
+                double ex = exp ( - sh->exps[q] * r2 ) ;
+                for ( int c = 0 ; c < sh->nCBF ; c++ )
+                {
+                    e0[c] += sh->ccbf[q * sh->nCBF + c] * ex ;
+                    e1[c] += sh->ccbf[q * sh->nCBF + c] * sh->exps[q] * ex ;
+                }
+            }
+            for ( int c = 0 ; c < sh->nCBF ; c++ )
+            {
+                int i = sh->px[c], j = sh->py[c], k = sh->pz[c] ;
+                double E1 = -2.0 * e1[c] ;
+                g[0][c] = X0[i] * Y0[j] * Z0[k] * e0[c] ;
+                g[1][c] = ( X1[i] * e0[c] + X0[i+1] * E1 ) * Y0[j] * Z0[k] ;
+                g[2][c] = ( Y1[j] * e0[c] + Y0[j+1] * E1 ) * X0[i] * Z0[k] ;
+                g[3][c] = ( Z1[k] * e0[c] + Z0[k+1] * E1 ) * X0[i] * Y0[j] ;
+            }
+            double *out[4] = { f, fx, fy, fz } ;
+            for ( int d = 0 ; d < 4 ; d++ )
+            {
+                for ( int b = 0 ; b < sh->nBasis ; b++ )
+                {
+                    double v ;
+                    if ( sh->c2s == NULL ) v = g[d][b] ;
+                    else { v = 0.0 ; for ( int c = 0 ; c < sh->nCBF ; c++ ) v += sh->c2s[c * sh->nBasis + b] * g[d][c] ; }
+                    out[d][( sh->start + b ) * nP + p] = v ;
+                }
+            }
+        }
+    }
+}
+
+/* Keep rows with max_p |f| > tol, compacting in place; returns the kept count (GridFunctionDataBlock.c:130-185). */
+static int FilterRows ( int nF, int nP, double tol, int *idx, double *f, double *fx, double *fy, double *fz )
+{
+    int n = 0 ;
+    for ( int m = 0 ; m < nF ; m++ )
+    {
+        double mx = 0.0 ;
+        for ( int p = 0 ; p < nP ; p++ ) { double a = fabs ( f[m * nP + p] ) ; if ( a > mx ) mx = a ; }
+        if ( mx > tol )
+        {
+            if ( m != n )
+            {
+                memcpy ( f  + n * nP, f  + m * nP, nP * sizeof ( double ) ) ;
+                memcpy ( fx + n * nP, fx + m * nP, nP * sizeof ( double ) ) ;
+                memcpy ( fy + n * nP, fy + m * nP, nP * sizeof ( double ) ) ;
+                memcpy ( fz + n * nP, fz + m * nP, nP * sizeof ( double ) ) ;
+            }
+            idx[n++] = m ;
+        }
+    }
+    return n ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Functionals, unpolarised, as libxc 1.2.0 evaluates them for pDynamo3 (cxc/work_gga_x.i:37-183, cxc/work_gga_c.i:25-245).
+! CBRT is pow(x, 1/3): the vendored libxc's config.h is not included, so xc_config.h falls back to pow (cxc/xc_config.h).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+# define XC_MIN_DENS 5.0e-13
+# define XC_MIN_GRAD 5.0e-13
+# define X_FACTOR_C  0.9305257363491000250020102180716672510262
+# define X2S         0.1282782438530421943003109254455883701296
+# define CBRT2       1.259921049894873164767210607278228350570
+static double Cbrt ( double x ) { return pow ( x, 1.0 / 3.0 ) ; }
+
+/* Becke 88 enhancement factor F(x) = 1 + b x^2/(X_FACTOR_C (1 + 6 b x asinh x)), b = 0.0042 (Becke 1988; cxc/gga_x_b88.c:48,109-148). */
+static void B88 ( double x, double *F, double *dF )
+{
+    const double beta = 0.0042, gamma = 6.0 ;
+    double f1 = beta / X_FACTOR_C * x * x, f2 = 1.0 + gamma * beta * x * asinh ( x ) ;
+    double df1 = 2.0 * beta / X_FACTOR_C * x, df2 = gamma * beta * ( asinh ( x ) + x / sqrt ( 1.0 + x * x ) ) ;
+    *F  = 1.0 + f1 / f2 ;
+    *dF = ( df1 * f2 - f1 * df2 ) / ( f2 * f2 ) ;
+}
+
+/* PBE exchange enhancement F(s) = 1 + k - k^2/(k + mu s^2), s = X2S x (Perdew, Burke, Ernzerhof 1996; cxc/gga_x_pbe.c:48,64,128-162). */
+static void PBEX ( double x, double *F, double *dF )
+{
+    const double kappa = 0.8040, mu = 0.2195149727645171 ;
+    double s = X2S * x, f0 = kappa + mu * s * s ;
+    *F  = 1.0 + kappa * ( 1.0 - kappa / f0 ) ;
+    *dF = X2S * kappa * kappa * ( 2.0 * mu * s ) / ( f0 * f0 ) ;
+}
+
+/* GGA exchange per point: eps per particle (zk = 2 (-X_FACTOR_C) (rho/2)^(4/3) F(x) / rho), vrho and vsigma. */
+static void ExchangeGGA ( int which, double rho, double sigma, double *eps, double *vr, double *vs )
+{
+    *eps = 0.0 ; *vr = 0.0 ; *vs = 0.0 ;
+    if ( rho < XC_MIN_DENS ) return ;
+    double ds = rho / 2.0, gdm = sqrt ( sigma ) / 2.0 ;
+    if ( gdm < XC_MIN_GRAD ) gdm = XC_MIN_GRAD ;
+    double rl = pow ( ds, 4.0 / 3.0 ), x = gdm / pow ( ds, 4.0 / 3.0 ), F, dF ;
+    if ( which == 0 ) B88 ( x, &F, &dF ) ; else PBEX ( x, &F, &dF ) ;
+    *eps = ( 2.0 * ( - X_FACTOR_C ) * rl * F ) / rho ;
+    *vr  = ( - X_FACTOR_C ) * ( rl / ds ) * ( ( 4.0 / 3.0 ) * F - ( 4.0 / 3.0 ) * dF * x ) ;
+    if ( gdm > XC_MIN_GRAD ) *vs = 2.0 * ( - X_FACTOR_C ) * rl * ( dF * x / ( 2.0 * sigma ) ) ;
+}
+
+/* LYP correlation at zeta = 0 in the Miehlich et al. (1989) form, variables rs, xt, xs = 2^(1/3) xt
+!  (Lee, Yang, Parr 1988; cxc/gga_c_lyp.c:40,65-223).  Returns f, df/drs, df/dxt, df/dxs (per spin channel). */
+static void LYP ( double rs, double xt, double xs, double *f, double *frs, double *fxt, double *fxs )
+{
+    const double A = 0.04918, B = 0.132 ;
+    double crs = Cbrt ( 4.0 * M_PI / 3.0 ), Cf = 3.0 * pow ( 3.0 * M_PI * M_PI, 2.0 / 3.0 ) / 10.0 ;
+    double c = 0.2533 * crs, d = 0.349 * crs ;
+    double xt2 = xt * xt, xs2 = xs * xs ;
+    double opd = 1.0 / ( 1.0 + d * rs ), om = B * exp ( - c * rs ) * opd, de = ( c + d * opd ) * rs ;
+    double a6 = 1.0 / pow ( 2.0, 8.0 / 3.0 ), a4 = a6 / 4.0, a5 = a4 / ( 9.0 * 2.0 ) ;
+    double t1 = - 1.0 / ( 1.0 + d * rs ) ;
+    double t2 = - xt2 * ( ( 47.0 - 7.0 * de ) / ( 4.0 * 18.0 ) - 2.0 / 3.0 ) ;
+    double t3 = - Cf / 2.0 * 2.0 ;
+    double t4 = a4 * ( 5.0 / 2.0 - de / 18.0 ) * ( 2.0 * xs2 ) ;
+    double t5 = a5 * ( de - 11.0 ) * ( 2.0 * xs2 ) ;
+    double t6 = - a6 * ( 2.0 / 3.0 * ( 2.0 * xs2 ) - xs2 / 4.0 - xs2 / 4.0 ) ;
+    double tt = t2 + t3 + t4 + t5 + t6 ;
+    *f = A * ( t1 + om * tt ) ;
+    double dom = - om * ( c + d * opd ), dde = c + d * opd * opd ;
+    double dt1 = - d * t1 / ( 1.0 + d * rs ) ;
+    double dt2 = xt2 * dde * 7.0 / ( 4.0 * 18.0 ) ;
+    double dt4 = - a4 * dde / 18.0 * ( 2.0 * xs2 ) ;
+    double dt5 = a5 * dde * ( 2.0 * xs2 ) ;
+    *frs = A * ( dt1 + dom * tt + om * ( dt2 + dt4 + dt5 ) ) ;
+    *fxt = - 2.0 * A * om * xt * ( ( 47.0 - 7.0 * de ) / ( 4.0 * 18.0 ) - 2.0 / 3.0 ) ;
+    *fxs = A * om * 2.0 * xs * ( a4 * ( 5.0 / 2.0 - de / 18.0 ) + a5 * ( de - 11.0 ) - a6 * ( 2.0 / 3.0 - 1.0 / 4.0 ) ) ;
+}
+
+/* PW92 correlation per particle of the unpolarised gas, "modified" parameter set used by PBE
+!  (Perdew & Wang 1992, eq. 10 and A5; cxc/lda_c_pw.c:60-236 with the PW_MOD constants). */
+static void PW92Mod ( double rs, double *ec, double *decdrs )
+{
+    const double a = 0.0310907, alpha = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294 ;
+    double srs = sqrt ( rs ) ;
+    double q0 = - 2.0 * a * ( 1.0 + alpha * rs ) ;
+    double q1 = 2.0 * a * ( b1 * srs + b2 * rs + b3 * srs * rs + b4 * pow ( rs, 2.0 ) ) ;
+    double q2 = log ( 1.0 + 1.0 / q1 ) ;
+    *ec = q0 * q2 ;
+    double dq0 = - 2.0 * a * alpha ;
+    double dq1 = a * ( b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 2.0 * 2.0 * b4 * pow ( rs, 1.0 ) ) ;
+    *decdrs = dq0 * q2 - q0 * dq1 / ( q1 * ( 1.0 + q1 ) ) ;
+}
+
+/* PBE correlation at zeta = 0 (phi = 1): f = ec + H(t, A), t = xt / (4 2^(1/3) sqrt(rs))
+!  (Perdew, Burke, Ernzerhof 1996, eqs. 7-8; cxc/gga_c_pbe.c:40,75,80-264). */
+static void PBEC ( double rs, double xt, double *f, double *frs, double *fxt )
+{
+    const double beta = 0.06672455060314922 ;
+    const double gam = ( 1.0 - log ( 2.0 ) ) / ( M_PI * M_PI ) ;
+    double ec, dec ;
+    PW92Mod ( rs, &ec, &dec ) ;
+    double tconv = 4.0 * CBRT2, srs = sqrt ( rs ), t = xt / ( tconv * srs ) ;
+    /* eq. 8 */
+    double e1 = ec / gam, e2 = exp ( - e1 ), e3 = e2 - 1.0 ;
+    double Aa = beta / ( gam * e3 ) ;
+    double dx = Aa * e2 / e3, dAdec = dx / gam ;
+    /* eq. 7 */
+    double t2 = t * t, g1 = t2 + Aa * t2 * t2, g3 = 1.0 + Aa * g1, g2 = beta * g1 / ( gam * g3 ) ;
+    double H = gam * log ( 1.0 + g2 ) ;
+    double dg1dt = t * ( 2.0 + 4.0 * Aa * t2 ), dg2dt = beta / ( gam * g3 * g3 ) * dg1dt ;
+    double dHdt = gam * dg2dt / ( 1.0 + g2 ) ;
+    double dg2dA = beta / ( gam * g3 * g3 ) * ( t2 * t2 - g1 * g1 ) ;
+    double dHdA = gam * dg2dA / ( 1.0 + g2 ) ;
+    *f = ec + H ;
+    double dfdec = 1.0 + dHdA * dAdec ;
+    double dtdrs = - xt / ( 2.0 * tconv * rs * srs ) ;
+    *frs = dfdec * dec + dHdt * dtdrs ;
+    *fxt = dHdt * ( t / xt ) ;
+}
+
+/* GGA correlation per point (work_gga_c.i, unpolarised branch): eps per particle, vrho, vsigma. */
+static void CorrelationGGA ( int which, double rho, double sigma, double *eps, double *vr, double *vs )
+{
+    *eps = 0.0 ; *vr = 0.0 ; *vs = 0.0 ;
+    double dens = ( rho > 0.0 ? rho : 0.0 ) ;
+    if ( dens < XC_MIN_DENS ) return ;
+    double rs = Cbrt ( 3.0 / ( 4.0 * M_PI * dens ) ) ;
+    double st = ( sigma > XC_MIN_GRAD * XC_MIN_GRAD ? sigma : XC_MIN_GRAD * XC_MIN_GRAD ) ;
+    double xt = sqrt ( st ) / pow ( dens, 4.0 / 3.0 ), xs = Cbrt ( 2.0 ) * xt ;
+    double f, frs, fxt, fxs = 0.0 ;
+    if ( which == 0 ) LYP ( rs, xt, xs, &f, &frs, &fxt, &fxs ) ;
+    else              PBEC ( rs, xt, &f, &frs, &fxt ) ;
+    double drs = - rs / ( 3.0 * dens ), dxt = - 4.0 * xt / ( 3.0 * dens ), dxtds = xt / ( 2.0 * st ) ;
+    *eps = f ;
+    *vr  = f + dens * ( frs * drs + fxt * dxt ) ;
+    *vs  = dens * fxt * dxtds ;
+    *vr += 2.0 * dens * fxs * ( CBRT2 * dxt ) ;
+    *vs += 2.0 * dens * fxs * ( CBRT2 * dxtds ) ;
+}
+
+/* Functional IDs (pMolecule/QCModel/DFTDefinitions.py:49,54,78-79): 101 PBE-x, 106 B88, 130 PBE-c, 131 LYP. */
+static int ValidID ( int id ) { return id == 101 || id == 106 || id == 130 || id == 131 ; }
+
+/* eps (per particle, summed over functionals), vrho and vsigma for n points (DFTFunctionalModel.c:108-154). */
+int xqo_functional ( int nIDs, const int *ids, int n, const double *rho, const double *sigma, double *eps, double *vr, double *vs )
+{
+    for ( int f = 0 ; f < nIDs ; f++ ) if ( ! ValidID ( ids[f] ) ) return 3 ;
+    for ( int p = 0 ; p < n ; p++ )
+    {
+        double e = 0.0, a = 0.0, b = 0.0 ;
+        for ( int f = 0 ; f < nIDs ; f++ )
+        {
+            double ed = 0.0, v1 = 0.0, v2 = 0.0 ;
+            switch ( ids[f] )
+            {
+                case 106: ExchangeGGA    ( 0, rho[p], sigma[p], &ed, &v1, &v2 ) ; break ;
+                case 101: ExchangeGGA    ( 1, rho[p], sigma[p], &ed, &v1, &v2 ) ; break ;
+                case 131: CorrelationGGA ( 0, rho[p], sigma[p], &ed, &v1, &v2 ) ; break ;
+                case 130: CorrelationGGA ( 1, rho[p], sigma[p], &ed, &v1, &v2 ) ; break ;
+            }
+            e += ed ; a += v1 ; b += v2 ;
+        }
+        eps[p] = e ; vr[p] = a ; vs[p] = b ;
+    }
+    return 0 ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Setup: basis, functional and grid (DFTGrid.c:105-263).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+void xqo_free ( void *vh ) ;
+
+void *xqo_setup ( int accuracy, int nAtoms, const int *Z, const double *xyzBohr, int nSpec, const double *spec, int spherical,
+                  int nIDs, const int *ids, int *status )
+{
+    *status = 0 ;
+    if ( nAtoms <= 0 || accuracy < 0 || accuracy > 4 || nIDs < 1 || nIDs > 4 ) { *status = 3 ; return NULL ; }
+    for ( int f = 0 ; f < nIDs ; f++ ) if ( ! ValidID ( ids[f] ) ) { *status = 3 ; return NULL ; }
+    OHandle *h = ( OHandle * ) calloc ( 1, sizeof ( OHandle ) ) ;
+    h->accuracy = accuracy ; h->nAtoms = nAtoms ; h->spherical = spherical ; h->nIDs = nIDs ;
+    for ( int f = 0 ; f < nIDs ; f++ ) h->ids[f] = ids[f] ;
+    h->Z   = ( int    * ) malloc ( nAtoms * sizeof ( int ) ) ;
+    h->xyz = ( double * ) malloc ( 3 * nAtoms * sizeof ( double ) ) ;
+    memcpy ( h->Z, Z, nAtoms * sizeof ( int ) ) ;
+    memcpy ( h->xyz, xyzBohr, 3 * nAtoms * sizeof ( double ) ) ;
+    h->bfTol = BF_TOL[accuracy] ; h->rhoTol = RHO_TOL[accuracy] ;
+    if ( ( *status = BuildBasis ( h, nSpec, spec ) ) != 0 ) { xqo_free ( h ) ; return NULL ; }
+    /* Grid. */
+    double *radii = ( double * ) malloc ( nAtoms * sizeof ( double ) ) ;
+    for ( int a = 0 ; a < nAtoms ; a++ ) radii[a] = BraggRadius ( Z[a] ) ;
+    BeckePairs bp ;
+    BeckePairs_Make ( &bp, nAtoms, h->xyz, radii ) ;
+    double *P = ( double * ) malloc ( nAtoms * sizeof ( double ) ), *R = ( double * ) malloc ( nAtoms * sizeof ( double ) ) ;
+    int cap = 0 ;
+    for ( int a = 0 ; a < nAtoms ; a++ )
+    {
+        int nR, lMax ; double rMax ;
+        AtomParameters ( accuracy, Z[a], &nR, &lMax, &rMax ) ;
+        cap += nR * xqo_lebedev_size_for_l ( lMax ) ;
+    }
+    h->points     = ( double * ) malloc ( 4 * ( size_t ) ( cap > 0 ? cap : 1 ) * sizeof ( double ) ) ;
+    h->blockAtom  = ( int * ) malloc ( ( cap / 1 + 1 ) * sizeof ( int ) ) ;
+    h->blockStart = ( int * ) malloc ( ( cap / 1 + 1 ) * sizeof ( int ) ) ;
+    h->blockCount = ( int * ) malloc ( ( cap / 1 + 1 ) * sizeof ( int ) ) ;
+    double ax[5810], ay[5810], az[5810], aw[5810] ;
+    for ( int a = 0 ; a < nAtoms ; a++ )
+    {
+        int nR, lMax ; double rMax ;
+        AtomParameters ( accuracy, Z[a], &nR, &lMax, &rMax ) ;
+        int nAngMax = xqo_lebedev_size_for_l ( lMax ), nAngMin = xqo_lebedev_size_for_l ( MIN_L ) ;
+        double range = BraggRadius ( Z[a] ), rCut = R_CUT_FACTOR * range ;
+        double *rr = ( double * ) malloc ( nR * sizeof ( double ) ), *wr = ( double * ) malloc ( nR * sizeof ( double ) ) ;
+        RadialPoints ( nR, range, rr, wr ) ;
+        int first = h->nPoints, lOld = -1, nA = 0 ;
+        for ( int ir = 0 ; ir < nR ; ir++ )
+        {
+            if ( rr[ir] > rMax ) break ;
+            int l ;
+            if ( rr[ir] > rCut ) l = lMax ;
+            else
+            {
+                int n = ( int ) ceil ( ( double ) nAngMax * rr[ir] / rCut ) ;
+                l = ( n < nAngMin ? MIN_L : xqo_lebedev_l_for_size ( n ) ) ;
+            }
+            if ( l != lOld ) { nA = xqo_lebedev_points ( xqo_lebedev_size_for_l ( l ), ax, ay, az, aw ) ; lOld = l ; }
+            double wf = 4.0 * M_PI * wr[ir] ;
+            for ( int k = 0 ; k < nA ; k++ )
+            {
+                double g[3] = { ax[k] * rr[ir] + h->xyz[3*a], ay[k] * rr[ir] + h->xyz[3*a+1], az[k] * rr[ir] + h->xyz[3*a+2] } ;
+                double w = wf * aw[k] * BeckeWeight ( &bp, a, g, P, R ) ;
+                if ( fabs ( w ) > W_TOL )
+                {
+                    double *q = h->points + 4 * ( size_t ) h->nPoints ;
+                    q[0] = g[0] ; q[1] = g[1] ; q[2] = g[2] ; q[3] = w ;
+                    h->nPoints++ ;
+                }
+            }
+        }
+        free ( rr ) ; free ( wr ) ;
+        /* Consecutive blocks of at most BLOCK_SIZE points per atom (DFTGrid.c:213-238). */
+        for ( int s = first ; s < h->nPoints ; s += BLOCK_SIZE )
+        {
+            h->blockAtom [h->nBlocks] = a ;
+            h->blockStart[h->nBlocks] = s ;
+            h->blockCount[h->nBlocks] = ( h->nPoints - s < BLOCK_SIZE ? h->nPoints - s : BLOCK_SIZE ) ;
+            h->nBlocks++ ;
+        }
+    }
+    free ( P ) ; free ( R ) ; free ( radii ) ; free ( bp.invR ) ; free ( bp.aij ) ;
+    return ( void * ) h ;
+}
+
+int xqo_npoints    ( void *vh ) { return ( ( OHandle * ) vh )->nPoints    ; }
+int xqo_nblocks    ( void *vh ) { return ( ( OHandle * ) vh )->nBlocks    ; }
+int xqo_nfunctions ( void *vh ) { return ( ( OHandle * ) vh )->nFunctions ; }
+double xqo_bf_values ( void *vh ) { return ( ( OHandle * ) vh )->bfValues ; }
+
+void xqo_grid ( void *vh, double *xyzw, int *blockAtom, int *blockCount )
+{
+    OHandle *h = ( OHandle * ) vh ;
+    memcpy ( xyzw, h->points, 4 * ( size_t ) h->nPoints * sizeof ( double ) ) ;
+    for ( int b = 0 ; b < h->nBlocks ; b++ ) { blockAtom[b] = h->blockAtom[b] ; blockCount[b] = h->blockCount[b] ; }
+}
+
+/* Basis values of one block after the bfTol filter (test access). */
+int xqo_block_values ( void *vh, int block, int *indices, double *f, double *fx, double *fy, double *fz )
+{
+    OHandle *h = ( OHandle * ) vh ;
+    int nP = h->blockCount[block], nF = h->nFunctions ;
+    double *buf = ( double * ) malloc ( 4 * ( size_t ) nF * nP * sizeof ( double ) ) ;
+    EvaluateBasis ( h, nP, h->points + 4 * ( size_t ) h->blockStart[block], buf, buf + nF * nP, buf + 2 * nF * nP, buf + 3 * nF * nP ) ;
+    int nb = FilterRows ( nF, nP, h->bfTol, indices, buf, buf + nF * nP, buf + 2 * nF * nP, buf + 3 * nF * nP ) ;
+    memcpy ( f , buf             , nb * nP * sizeof ( double ) ) ;
+    memcpy ( fx, buf + nF * nP    , nb * nP * sizeof ( double ) ) ;
+    memcpy ( fy, buf + 2 * nF * nP, nb * nP * sizeof ( double ) ) ;
+    memcpy ( fz, buf + 3 * nF * nP, nb * nP * sizeof ( double ) ) ;
+    free ( buf ) ;
+    return nb ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Quadrature (DFTIntegrator.c:159-509, restricted closed shell, Fock only).
+!   rho_p = sum_mn phi_mp P_mn phi_np (X = P_red Phi, :838-839); grad rho = 2 sum_m grad phi_mp X_mp; sigma = |grad rho|^2 (:843-850);
+!   skip the block if max rho <= rhoTol (:332); vrho, vsigma *= w (:336-338); E += sum w eps rho, N += sum w rho (:350-352);
+!   V_mn += sum_p phi_mp vrho_p phi_np + sum_p (phi_mp Z_np + Z_mp phi_np), Z = 2 vsigma (grad rho . grad phi) (:517-618).
+!   V is packed lower triangle (i >= j, index i(i+1)/2 + j) and is incremented, not overwritten.
+!---------------------------------------------------------------------------------------------------------------------------------*/
+int xqo_integrate ( void *vh, const double *Ppacked, int nThreads, double *eXC, double *nE, double *Vpacked, double *seconds )
+{
+    OHandle *h = ( OHandle * ) vh ;
+    int nF = h->nFunctions ;
+    size_t nPacked = ( ( size_t ) nF * ( nF + 1 ) ) / 2 ;
+    struct timespec t0, t1 ;
+    clock_gettime ( CLOCK_MONOTONIC, &t0 ) ;
+    double eTot = 0.0, nTot = 0.0, bfv = 0.0 ;
+    long skipped = 0 ;
+    if ( nThreads < 1 ) nThreads = 1 ;
+# ifdef _OPENMP
+    omp_set_num_threads ( nThreads ) ;
+# endif
+    /* Each thread accumulates its own V, then the threads' V are added in a fixed order. */
+    double *Vt = ( double * ) calloc ( ( size_t ) nThreads * nPacked, sizeof ( double ) ) ;
+    double *eT = ( double * ) calloc ( nThreads, sizeof ( double ) ), *nT = ( double * ) calloc ( nThreads, sizeof ( double ) ) ;
+    double *bT = ( double * ) calloc ( nThreads, sizeof ( double ) ) ;
+    long   *sT = ( long * ) calloc ( nThreads, sizeof ( long ) ) ;
+    # pragma omp parallel num_threads(nThreads)
+    {
+        int tid = 0 ;
+# ifdef _OPENMP
+        tid = omp_get_thread_num ( ) ;
+# endif
+        double *V = Vt + ( size_t ) tid * nPacked ;
+        double *buf = ( double * ) malloc ( 4 * ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *X   = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *Z   = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *Pr  = ( double * ) malloc ( ( size_t ) nF * nF * sizeof ( double ) ) ;
+        int    *idx = ( int * ) malloc ( nF * sizeof ( int ) ) ;
+        double rho[BLOCK_SIZE], gx[BLOCK_SIZE], gy[BLOCK_SIZE], gz[BLOCK_SIZE], sig[BLOCK_SIZE], eps[BLOCK_SIZE], vr[BLOCK_SIZE], vs[BLOCK_SIZE] ;
+        # pragma omp for schedule(dynamic,1)
+        for ( int b = 0 ; b < h->nBlocks ; b++ )
+        {
+            int nP = h->blockCount[b] ;
+            const double *pts = h->points + 4 * ( size_t ) h->blockStart[b] ;
+            double *f = buf, *fx = buf + ( size_t ) nF * nP, *fy = buf + 2 * ( size_t ) nF * nP, *fz = buf + 3 * ( size_t ) nF * nP ;
+            EvaluateBasis ( h, nP, pts, f, fx, fy, fz ) ;
+            int nb = FilterRows ( nF, nP, h->bfTol, idx, f, fx, fy, fz ) ;
+            if ( nb <= 0 ) continue ;
+            bT[tid] += ( double ) nb * nP ;
+            /* Reduced density. */
+            for ( int i = 0 ; i < nb ; i++ )
+                for ( int j = 0 ; j <= i ; j++ )
+                {
+                    int m = idx[i], n = idx[j] ;
+                    double v = Ppacked[( ( size_t ) m * ( m + 1 ) ) / 2 + n] ;
+                    Pr[i * nb + j] = v ; Pr[j * nb + i] = v ;
+                }
+            /* X = P_red f. */
+            for ( int i = 0 ; i < nb ; i++ )
+                for ( int p = 0 ; p < nP ; p++ )
+                {
+                    double s = 0.0 ;
+                    for ( int k = 0 ; k < nb ; k++ ) s += Pr[i * nb + k] * f[k * nP + p] ;
+                    X[i * nP + p] = s ;
+                }
+            double mx = 0.0 ;
+            for ( int p = 0 ; p < nP ; p++ )
+            {
+                double r = 0.0, ax = 0.0, ay = 0.0, az = 0.0 ;
+                for ( int i = 0 ; i < nb ; i++ )
+                {
+                    double xv = X[i * nP + p] ;
+                    r  += f [i * nP + p] * xv ;
+                    ax += fx[i * nP + p] * xv ;
+                    ay += fy[i * nP + p] * xv ;
+                    az += fz[i * nP + p] * xv ;
+                }
+                rho[p] = r ;
+                sig[p] = 4.0 * ( ax * ax + ay * ay + az * az ) ;
+                gx[p] = 2.0 * ax ; gy[p] = 2.0 * ay ; gz[p] = 2.0 * az ;
+                if ( fabs ( r ) > mx ) mx = fabs ( r ) ;
+            }
+            if ( mx <= h->rhoTol ) { sT[tid]++ ; continue ; }
+            xqo_functional ( h->nIDs, h->ids, nP, rho, sig, eps, vr, vs ) ;
+            for ( int p = 0 ; p < nP ; p++ )
+            {
+                double w = pts[4 * p + 3] ;
+                vr[p] *= w ; vs[p] *= w ;
+                eT[tid] += eps[p] * rho[p] * w ;
+                nT[tid] += rho[p] * w ;
+            }
+            /* Z = 2 vsigma (grad rho . grad phi). */
+            for ( int i = 0 ; i < nb ; i++ )
+                for ( int p = 0 ; p < nP ; p++ )
+                    Z[i * nP + p] = 2.0 * vs[p] * ( fx[i * nP + p] * gx[p] + fy[i * nP + p] * gy[p] + fz[i * nP + p] * gz[p] ) ;
+            for ( int i = 0 ; i < nb ; i++ )
+            {
+                int m = idx[i] ;
+                for ( int j = 0 ; j <= i ; j++ )
+                {
+                    int n = idx[j] ;
+                    double s = 0.0 ;
+                    for ( int p = 0 ; p < nP ; p++ )
+                        s += f[i * nP + p] * vr[p] * f[j * nP + p] + f[i * nP + p] * Z[j * nP + p] + Z[i * nP + p] * f[j * nP + p] ;
+                    V[( ( size_t ) m * ( m + 1 ) ) / 2 + n] += s ;
+                }
+            }
+        }
+        free ( buf ) ; free ( X ) ; free ( Z ) ; free ( Pr ) ; free ( idx ) ;
+    }
+    for ( int t = 0 ; t < nThreads ; t++ )
+    {
+        eTot += eT[t] ; nTot += nT[t] ; bfv += bT[t] ; skipped += sT[t] ;
+        double *V = Vt + ( size_t ) t * nPacked ;
+        for ( size_t k = 0 ; k < nPacked ; k++ ) Vpacked[k] += V[k] ;
+    }
+    free ( Vt ) ; free ( eT ) ; free ( nT ) ; free ( bT ) ; free ( sT ) ;
+    clock_gettime ( CLOCK_MONOTONIC, &t1 ) ;
+    *eXC = eTot ; *nE = nTot ;
+    h->bfValues = bfv ; h->blocksSkipped = skipped ;
+    if ( seconds != NULL ) *seconds = ( t1.tv_sec - t0.tv_sec ) + 1.0e-9 * ( t1.tv_nsec - t0.tv_nsec ) ;
+    return 0 ;
+}
+
+void xqo_free ( void *vh )
+{
+    OHandle *h = ( OHandle * ) vh ;
+    if ( h == NULL ) return ;
+    for ( int s = 0 ; s < h->nShells ; s++ )
+    {
+        free ( h->shells[s].exps ) ; free ( h->shells[s].ccbf ) ; free ( h->shells[s].c2s ) ;
+        free ( h->shells[s].px ) ; free ( h->shells[s].py ) ; free ( h->shells[s].pz ) ;
+    }
+    free ( h->shells ) ; free ( h->Z ) ; free ( h->xyz ) ; free ( h->points ) ;
+    free ( h->blockAtom ) ; free ( h->blockStart ) ; free ( h->blockCount ) ;
+    free ( h ) ;
+}
