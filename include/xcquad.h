@@ -1,0 +1,74 @@
+/*==================================================================================================================================
+! xcquad.h -- C-ABI of the MI355X exchange-correlation quadrature library (libxcquad.so).
+!
+! Drop-in for the C entry points behind pDynamo3's DFTGridIntegrator (pMolecule/QCModel/DFTGridIntegrator.py:25-93):
+!   xq_grid_build        <- DFTGrid_Construct         (pMolecule/QCModel/__extensions__/cinclude/DFTGrid.h:59-62; DFTGrid.c:105)
+!   xq_grid_counts       <- DFTGrid_NumberOfPoints / _NumberOfRecords (DFTGrid.h:74-75)
+!   xq_grid_points       <- the grid point records (DFTGrid.h:28-34: atom, coordinates3, weights)
+!   xq_estimated_points  <- DFTGrid_EstimatedPoints   (DFTGrid.h:67-69; DFTGrid.c:301-356)
+!   xq_basis_set         <- the GaussianBasisContainer argument of DFTIntegrator_Integrate, built as
+!                           GaussianBasis.pyx:114-137 + GaussianBasis_Finalize (GaussianBasis.c:391-398) build it
+!   xq_integrate         <- DFTIntegrator_Integrate   (DFTIntegrator.h:55-68; DFTIntegrator.c:159-509), restricted
+!                           closed shell, Fock only (gradients3 = NULL)
+!   xq_estimate_bytes    <- DFTGridIntegrator.EstimateMemory (DFTGridIntegrator.py:31-39), for this library's HBM layout
+!   xq_free              <- DFTGrid_Deallocate        (DFTGrid.h:63-64)
+!
+! Conventions (as in the reference):
+!   - Coordinates are bohr, n x 3 row-major FP64 (Coordinates3).  Matrices are packed lower triangles, entry (i >= j) at
+!     i(i+1)/2 + j, FP64 (SymmetricMatrix.h:43-46).
+!   - xq_integrate INCREMENTS V (the reference's Fock matrix is not initialised by the integrator, DFTIntegrator.pyx:9) and
+!     OVERWRITES *eXC and *nElectrons (DFTIntegrator.c:174-175, 505-506).
+!   - Return values are the reference's Status codes (pCore/__extensions__/cinclude/Status.h:24-33): 0 OK, 1 algorithm error
+!     (used here for a HIP runtime failure), 3 invalid argument, 6 non-conformable arrays, 7 out of memory.  The caller turns
+!     any non-zero code into an error, as DFTIntegrator.pyx:72 does.  xq_last_error() describes the last failure.
+!   - All pointers are host pointers; the caller owns them.  The handle owns its device memory (grid, basis values,
+!     work arrays), which stays resident across xq_integrate calls.  Calls on one handle must not overlap.
+!=================================================================================================================================*/
+# ifndef XCQUAD_H
+# define XCQUAD_H
+
+# include <stddef.h>
+
+# ifdef __cplusplus
+extern "C" {
+# endif
+
+typedef struct xq_grid xq_grid ;
+
+/* Precision of the basis-value storage and of the two contractions (X = P Phi, V = Phi W^T + W Phi^T). */
+enum {
+    XQ_PRECISION_FP64  = 0 ,  /* FP64 storage, FP64 arithmetic                                                 */
+    XQ_PRECISION_MIXED = 1    /* FP32 storage and FP32 contractions; FP64 weights, functional and reductions     */
+} ;
+
+/* Grid accuracy: 0 VeryLow, 1 Low, 2 Medium, 3 High, 4 VeryHigh (DFTGrid.h:16-25). */
+int         xq_grid_build       ( int accuracy, int nAtoms, const int *atomicNumbers, const double *xyzBohr, xq_grid **grid ) ;
+int         xq_grid_counts      ( const xq_grid *grid, int *nPoints, int *nBlocks ) ;
+int         xq_grid_points      ( const xq_grid *grid, double *xyzw, int *blockAtom, int *blockCount ) ;
+int         xq_estimated_points ( int accuracy, int nAtoms, const int *atomicNumbers, long long *nPoints ) ;
+
+/* Basis: element specification (flat FP64), per element Z, nShells, then per shell lLow, lHigh, nPrimitives, then per
+!  primitive the exponent and one primitive-normalised coefficient per l in lLow..lHigh.  spherical != 0 selects real solid
+!  harmonics (the reference default), 0 Cartesian functions.  Evaluates and stores the screened basis values and gradients of
+!  every grid block on the device (the reference's inCore mode).  precision selects the storage (XQ_PRECISION_*). */
+int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, int spherical, int precision, int *nFunctions ) ;
+
+/* One quadrature.  ids: libxc functional IDs (101 PBE-x, 106 B88, 130 PBE-c, 131 LYP; pMolecule/QCModel/DFTDefinitions.py).
+!  densityP: packed total density (n(n+1)/2).  fockV: packed, incremented by V_xc.  Statistics (optional, may be NULL):
+!  stats[0] = sum over blocks of kept functions x points ("Quadrature BF Values"), stats[1] = blocks skipped by the density
+!  tolerance, stats[2] = device time of the quadrature kernels in ms. */
+int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
+                                  double *nElectrons, double *fockV, double *stats ) ;
+
+/* Device bytes this library holds for a molecule with the given grid size, function count and mean kept functions per
+!  block (admission control for the request loop). */
+int         xq_estimate_bytes   ( long long nPoints, int nFunctions, double meanKeptFunctions, int precision, long long *bytes ) ;
+int         xq_device_bytes     ( const xq_grid *grid, long long *bytes ) ;
+
+void        xq_free             ( xq_grid *grid ) ;
+const char *xq_last_error       ( void ) ;
+
+# ifdef __cplusplus
+}
+# endif
+# endif

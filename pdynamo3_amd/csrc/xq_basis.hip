@@ -1,0 +1,512 @@
+// Gaussian basis set-up and block-wise basis values and gradients on the device.
+//
+// Host side: the element specification is turned into per-atom shells with unnormalised primitive coefficients and the
+// Cartesian factors of GaussianBasis_UnnormalizePrimitives / _NormalizePrimitiveCCBF (pMolecule/QCModel/GaussianBases/
+// __extensions__/csource/GaussianBasis.c:448-501, 728-751), and the Cartesian -> real solid harmonic transformation of
+// Schlegel & Frisch (Int. J. Quantum Chem. 54 (1995) 83, eq. 15), with the reference's function order (GaussianBasis.c:13-27).
+//
+// Device side, once per geometry ("in core", DFTIntegrator.c:209-210,491):
+//   1. count kernel, one work-group per grid block: bounding sphere of the block's points, a rigorous per-shell upper bound on
+//      |phi| over the sphere (shells that cannot reach bfTol are skipped), exact values of the remaining functions, and the
+//      reference's filter max_p |phi_m(p)| > bfTol (GridFunctionDataBlock.c:130-185) -> ascending kept-function list.
+//   2. eval kernel, one work-group per block: values and gradients of the kept functions (GaussianBasisIntegrals_f1Op1.c:
+//      100-197 and GaussianBasisTransform.c:85-125), stored function-major in rows of 128 points (coalesced stores).
+// The kept sets are exactly the reference's: screening removes only functions whose bound is below the tolerance.
+#include <cmath>
+#include <cstring>
+#include <hip/hip_runtime.h>
+
+#include "xq_internal.h"
+
+namespace xq {
+
+namespace {
+
+constexpr int kMaxShellsLDS = 16384;
+
+// Cartesian powers in the reference order (for z = 0..l, for y = 0..l-z, x = l-y-z), for l = 0..3, concatenated.
+__host__ __device__ constexpr int cart_offset(int l) { return l * (l + 1) * (l + 2) / 6; }
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+struct CartPow {
+    int x[20], y[20], z[20];
+};
+__host__ __device__ constexpr CartPow make_cart_pow() {
+    CartPow p{};
+    int n = 0;
+    for (int l = 0; l <= 3; l++)
+        for (int z = 0; z <= l; z++)
+            for (int y = 0; y <= l - z; y++) {
+                p.x[n] = l - y - z;
+                p.y[n] = y;
+                p.z[n] = z;
+                n++;
+            }
+    return p;
+}
+constexpr CartPow kCart = make_cart_pow();
+
+double odd_fact(int n) {
+    double f = 1.0;
+    for (int i = 3; i <= n; i += 2) f *= i;
+    return f;
+}
+double fact(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; i++) f *= i;
+    return f;
+}
+double binom(int n, int k) { return (k < 0 || k > n) ? 0.0 : fact(n) / (fact(k) * fact(n - k)); }
+
+// c2s block for one l (normalised Cartesian -> normalised real solid harmonics), written at (r0, c0) of a row-major matrix
+// with ns columns.  Spherical order m = 0, 1c, 1s, 2c, 2s, ...
+void c2s_block(int l, double *out, int ns, int r0, int c0) {
+    const int off = cart_offset(l);
+    for (int c = 0; c < ncart(l); c++) {
+        const int x = kCart.x[off + c], y = kCart.y[off + c], z = kCart.z[off + c];
+        for (int m = 0; m <= l; m++) {
+            const int jj = l - m - z;
+            if (jj < 0 || (jj % 2) != 0) continue;
+            const int j = jj / 2;
+            double s0 = 0.0, sc = 0.0, ss = 0.0;
+            for (int i = 0; i <= (l - m) / 2; i++) {
+                if (j > i) continue;
+                const double t = binom(l, i) * binom(i, j) * ((i % 2) ? -1.0 : 1.0) * fact(2 * l - 2 * i) / fact(l - m - 2 * i);
+                for (int k = 0; k <= j; k++) {
+                    const int p = x - 2 * k;  // power of x taken from (x + i y)^m
+                    if (p < 0 || p > m) continue;
+                    const double u = t * binom(j, k) * binom(m, p);
+                    const int q = m - p;      // power of (i y)
+                    const double sign = ((q / 2) % 2) ? -1.0 : 1.0;
+                    if (m == 0) s0 += u;
+                    else if ((q % 2) == 0) sc += std::sqrt(2.0) * sign * u;
+                    else ss += std::sqrt(2.0) * sign * u;
+                }
+            }
+            const double norm = std::sqrt(fact(2 * x) * fact(2 * y) * fact(2 * z) * fact(l) / (fact(2 * l) * fact(x) * fact(y) * fact(z))) *
+                                std::sqrt(fact(l - m) / fact(l + m)) / (std::pow(2.0, l) * fact(l));
+            if (m == 0) out[(r0 + c) * ns + c0] = norm * s0;
+            else {
+                out[(r0 + c) * ns + c0 + 2 * m - 1] = norm * sc;
+                out[(r0 + c) * ns + c0 + 2 * m] = norm * ss;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------------------
+// Device evaluation of one shell at one point.  Template over (lLow, lHigh, spherical) so that every array index is a
+// compile-time constant and stays in registers.
+// ------------------------------------------------------------------------------------------------------------------------
+template <int LL, int LH>
+struct ShellDims {
+    static constexpr int nCBF = cart_offset(LH + 1) - cart_offset(LL);
+    static constexpr int nSph = (LH + 1) * (LH + 1) - LL * LL;
+};
+
+template <int LL, int LH, bool SPH, bool GRAD>
+__device__ __forceinline__ void shell_eval(const xq::DevShell &sh, const double *__restrict__ coef,
+                                           const double *__restrict__ trans, double dx, double dy, double dz, double *v,
+                                           double *vx, double *vy, double *vz) {
+    constexpr int NC = ShellDims<LL, LH>::nCBF;
+    constexpr int NB = SPH ? ShellDims<LL, LH>::nSph : NC;
+    constexpr int C0 = cart_offset(LL);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double e0[NC], e1[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) { e0[c] = 0.0; e1[c] = 0.0; }
+    const double *ex = coef + sh.coef, *cc = coef + sh.coef + sh.nPrim;
+    for (int q = 0; q < sh.nPrim; q++) {
+        const double a = ex[q], e = exp(-a * r2);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double k = cc[q * NC + c];
+            e0[c] += k * e;
+            e1[c] += k * a * e;
+        }
+    }
+    double X0[LH + 2], Y0[LH + 2], Z0[LH + 2];
+    X0[0] = Y0[0] = Z0[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i <= LH + 1; i++) { X0[i] = dx * X0[i - 1]; Y0[i] = dy * Y0[i - 1]; Z0[i] = dz * Z0[i - 1]; }
+    double g0[NC], gx[NC], gy[NC], gz[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int i = kCart.x[C0 + c], j = kCart.y[C0 + c], k = kCart.z[C0 + c];
+        g0[c] = X0[i] * Y0[j] * Z0[k] * e0[c];
+        if (GRAD) {
+            const double E1 = -2.0 * e1[c];
+            const double xi1 = i > 0 ? i * X0[i > 0 ? i - 1 : 0] : 0.0;
+            const double yj1 = j > 0 ? j * Y0[j > 0 ? j - 1 : 0] : 0.0;
+            const double zk1 = k > 0 ? k * Z0[k > 0 ? k - 1 : 0] : 0.0;
+            gx[c] = (xi1 * e0[c] + X0[i + 1] * E1) * Y0[j] * Z0[k];
+            gy[c] = (yj1 * e0[c] + Y0[j + 1] * E1) * X0[i] * Z0[k];
+            gz[c] = (zk1 * e0[c] + Z0[k + 1] * E1) * X0[i] * Y0[j];
+        }
+    }
+    if (SPH) {
+        const double *T = trans + sh.trans;
+#pragma unroll
+        for (int s = 0; s < NB; s++) {
+            double a = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const double t = T[c * NB + s];
+                a += t * g0[c];
+                if (GRAD) { bx += t * gx[c]; by += t * gy[c]; bz += t * gz[c]; }
+            }
+            v[s] = a;
+            if (GRAD) { vx[s] = bx; vy[s] = by; vz[s] = bz; }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NB; s++) {
+            v[s] = g0[s];
+            if (GRAD) { vx[s] = gx[s]; vy[s] = gy[s]; vz[s] = gz[s]; }
+        }
+    }
+}
+
+// Dispatch on the shell type; calls body(nBasis, v, vx, vy, vz).  Supported: S, P, D, F, SP.
+template <bool GRAD, typename Body>
+__device__ __forceinline__ void with_shell(const xq::DevShell &sh, bool sph, const double *coef, const double *trans,
+                                           double dx, double dy, double dz, Body body) {
+    double v[16], vx[16], vy[16], vz[16];
+    const int key = sh.lLow * 4 + sh.lHigh;
+    if (sph) {
+        switch (key) {
+            case 0: shell_eval<0, 0, true, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(1, v, vx, vy, vz); break;
+            case 5: shell_eval<1, 1, true, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(3, v, vx, vy, vz); break;
+            case 1: shell_eval<0, 1, true, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(4, v, vx, vy, vz); break;
+            case 10: shell_eval<2, 2, true, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(5, v, vx, vy, vz); break;
+            case 15: shell_eval<3, 3, true, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(7, v, vx, vy, vz); break;
+            default: break;
+        }
+    } else {
+        switch (key) {
+            case 0: shell_eval<0, 0, false, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(1, v, vx, vy, vz); break;
+            case 5: shell_eval<1, 1, false, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(3, v, vx, vy, vz); break;
+            case 1: shell_eval<0, 1, false, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(4, v, vx, vy, vz); break;
+            case 10: shell_eval<2, 2, false, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(6, v, vx, vy, vz); break;
+            case 15: shell_eval<3, 3, false, GRAD>(sh, coef, trans, dx, dy, dz, v, vx, vy, vz); body(10, v, vx, vy, vz); break;
+            default: break;
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Upper bound of max_{r >= d} r^L exp(-a r^2).
+__device__ __forceinline__ double radial_bound(int L, double a, double d) {
+    if (L == 0) return exp(-a * d * d);
+    const double rs = sqrt(L / (2.0 * a)), r = d > rs ? d : rs;
+    double p = 1.0;
+    for (int i = 0; i < L; i++) p *= r;
+    return p * exp(-a * r * r);
+}
+
+// Work-group of kBlockSize threads, one thread per point.  Writes the kept functions of block b to tmpIdx[b * nF ...] and
+// their count to counts[b].
+__global__ void __launch_bounds__(kBlockSize) basis_count_kernel(
+    int nBlocks, const xq::DevBlock *__restrict__ blocks, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, int nShells, const xq::DevShell *__restrict__ shells, const double *__restrict__ coef,
+    const double *__restrict__ trans, int spherical, int nF, double bfTol, int *__restrict__ tmpIdx,
+    int *__restrict__ counts) {
+    __shared__ unsigned char cand[kMaxShellsLDS];
+    __shared__ double red[4][2];
+    __shared__ int nKeptS;
+    const int b = blockIdx.x;
+    const xq::DevBlock blk = blocks[b];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const bool valid = t < blk.nPoints;
+    const double x = valid ? px[blk.pStart + t] : 0.0, y = valid ? py[blk.pStart + t] : 0.0,
+                 z = valid ? pz[blk.pStart + t] : 0.0;
+    // Bounding sphere: centroid and largest distance.
+    double sx = wave_sum(x), sy = wave_sum(y), sz = wave_sum(z);
+    if (lane == 0) { red[wave][0] = sx; red[wave][1] = sy; red[wave + 2][0] = sz; }
+    __syncthreads();
+    const double inv = 1.0 / blk.nPoints;
+    const double cx = (red[0][0] + red[1][0]) * inv, cy = (red[0][1] + red[1][1]) * inv, cz = (red[2][0] + red[3][0]) * inv;
+    __syncthreads();
+    double rr = valid ? sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz)) : 0.0;
+    rr = wave_max(rr);
+    if (lane == 0) red[wave][0] = rr;
+    __syncthreads();
+    const double radius = fmax(red[0][0], red[1][0]);
+    // Candidate shells.
+    for (int s = t; s < nShells; s += blockDim.x) {
+        const xq::DevShell &sh = shells[s];
+        const double dd = sqrt((sh.x - cx) * (sh.x - cx) + (sh.y - cy) * (sh.y - cy) + (sh.z - cz) * (sh.z - cz)) - radius;
+        const double d = dd > 0.0 ? dd : 0.0;
+        double bound = 0.0;
+        const double *ex = coef + sh.coef;
+        for (int q = 0; q < sh.nPrim; q++)
+            for (int L = sh.lLow; L <= sh.lHigh; L++) bound += sh.bound[q] * radial_bound(L, ex[q], d);
+        cand[s] = bound * (1.0 + 1.0e-12) > bfTol ? 1 : 0;
+    }
+    if (t == 0) nKeptS = 0;
+    __syncthreads();
+    int *out = tmpIdx + static_cast<size_t>(b) * nF;
+    for (int s = 0; s < nShells; s++) {
+        if (!cand[s]) continue;
+        const xq::DevShell sh = shells[s];
+        with_shell<false>(sh, spherical != 0, coef, trans, x - sh.x, y - sh.y, z - sh.z,
+                          [&](int nB, const double *v, const double *, const double *, const double *) {
+                              for (int k = 0; k < nB; k++) {
+                                  double m = wave_max(valid ? fabs(v[k]) : 0.0);
+                                  if (lane == 0) red[wave][0] = m;
+                                  __syncthreads();
+                                  if (t == 0 && fmax(red[0][0], red[1][0]) > bfTol) out[nKeptS++] = sh.fStart + k;
+                                  __syncthreads();
+                              }
+                          });
+    }
+    if (t == 0) counts[b] = nKeptS;
+}
+
+// Values and gradients of the kept functions of block b into rows [vOffset, vOffset + nKept) of f, fx, fy, fz (row stride
+// kBlockSize; entries past the block's points are zero).  T is the storage type.
+template <typename T>
+__global__ void __launch_bounds__(kBlockSize) basis_eval_kernel(
+    const xq::DevBlock *__restrict__ blocks, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, const xq::DevShell *__restrict__ shells, const int *__restrict__ shellOf,
+    const double *__restrict__ coef, const double *__restrict__ trans, int spherical, int nF,
+    const int *__restrict__ tmpIdx, int *__restrict__ idx, T *__restrict__ f, T *__restrict__ fx, T *__restrict__ fy,
+    T *__restrict__ fz) {
+    const int b = blockIdx.x;
+    const xq::DevBlock blk = blocks[b];
+    const int t = threadIdx.x;
+    const bool valid = t < blk.nPoints;
+    const double x = valid ? px[blk.pStart + t] : 0.0, y = valid ? py[blk.pStart + t] : 0.0,
+                 z = valid ? pz[blk.pStart + t] : 0.0;
+    const int *kept = tmpIdx + static_cast<size_t>(b) * nF;
+    for (int k = t; k < blk.nKept; k += blockDim.x) idx[blk.iOffset + k] = kept[k];
+    int k = 0;
+    while (k < blk.nKept) {
+        const xq::DevShell sh = shells[shellOf[kept[k]]];
+        with_shell<true>(sh, spherical != 0, coef, trans, x - sh.x, y - sh.y, z - sh.z,
+                         [&](int nB, const double *v, const double *vx, const double *vy, const double *vz) {
+                             for (int j = 0; j < nB; j++) {
+                                 if (k < blk.nKept && kept[k] == sh.fStart + j) {
+                                     const size_t o = static_cast<size_t>(blk.vOffset + k) * kBlockSize + t;
+                                     f[o] = static_cast<T>(valid ? v[j] : 0.0);
+                                     fx[o] = static_cast<T>(valid ? vx[j] : 0.0);
+                                     fy[o] = static_cast<T>(valid ? vy[j] : 0.0);
+                                     fz[o] = static_cast<T>(valid ? vz[j] : 0.0);
+                                     k++;
+                                 }
+                             }
+                         });
+    }
+}
+
+}  // namespace
+
+void free_basis(xq_grid *g) {
+    dev_free(&g->d_shells);
+    dev_free(&g->d_coef);
+    dev_free(&g->d_trans);
+    dev_free(&g->d_pow);
+    dev_free(&g->d_idx);
+    if (g->d_f) { (void)hipFree(g->d_f); g->d_f = nullptr; }
+    if (g->d_fx) { (void)hipFree(g->d_fx); g->d_fx = nullptr; }
+    if (g->d_fy) { (void)hipFree(g->d_fy); g->d_fy = nullptr; }
+    if (g->d_fz) { (void)hipFree(g->d_fz); g->d_fz = nullptr; }
+    g->hasBasis = false;
+}
+
+int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int precision) {
+    if (precision != XQ_PRECISION_FP64 && precision != XQ_PRECISION_MIXED) {
+        set_error("unknown precision mode");
+        return kInvalidArgument;
+    }
+    // Parse the element specification.
+    struct Element { int z, start, nShells; };
+    std::vector<Element> elements;
+    for (int p = 0; p < nSpec;) {
+        if (p + 2 > nSpec) { set_error("truncated basis specification"); return kInvalidArgument; }
+        Element e{static_cast<int>(spec[p]), p + 2, static_cast<int>(spec[p + 1])};
+        p += 2;
+        for (int s = 0; s < e.nShells; s++) {
+            if (p + 3 > nSpec) { set_error("truncated basis specification"); return kInvalidArgument; }
+            const int lLow = static_cast<int>(spec[p]), lHigh = static_cast<int>(spec[p + 1]), np = static_cast<int>(spec[p + 2]);
+            if (lLow < 0 || lHigh < lLow || lHigh > kMaxL || np < 1 || np > kMaxPrim) {
+                set_error("unsupported shell in basis specification");
+                return kInvalidArgument;
+            }
+            const int key = lLow * 4 + lHigh;
+            if (!(key == 0 || key == 5 || key == 1 || key == 10 || key == 15)) {
+                set_error("unsupported shell type (supported: S, P, D, F, SP)");
+                return kInvalidArgument;
+            }
+            p += 3 + np * (1 + lHigh - lLow + 1);
+        }
+        if (p > nSpec) { set_error("truncated basis specification"); return kInvalidArgument; }
+        elements.push_back(e);
+    }
+    std::vector<DevShell> shells;
+    std::vector<double> coef, trans;
+    std::vector<int> shellOf;
+    int f = 0;
+    for (int a = 0; a < g->nAtoms; a++) {
+        const Element *el = nullptr;
+        for (const Element &e : elements) if (e.z == g->Z[a]) el = &e;
+        if (el == nullptr) { set_error("no basis for an atom's element"); return kInvalidArgument; }
+        int p = el->start;
+        for (int s = 0; s < el->nShells; s++) {
+            DevShell sh{};
+            sh.x = g->xyz[3 * a]; sh.y = g->xyz[3 * a + 1]; sh.z = g->xyz[3 * a + 2];
+            sh.atom = a;
+            sh.lLow = static_cast<int>(spec[p]); sh.lHigh = static_cast<int>(spec[p + 1]); sh.nPrim = static_cast<int>(spec[p + 2]);
+            p += 3;
+            sh.nCBF = 0; sh.nBasis = 0;
+            for (int l = sh.lLow; l <= sh.lHigh; l++) { sh.nCBF += ncart(l); sh.nBasis += 2 * l + 1; }
+            if (!spherical) sh.nBasis = sh.nCBF;
+            sh.fStart = f;
+            sh.coef = static_cast<int>(coef.size());
+            sh.pow = 0;
+            std::vector<double> ex(sh.nPrim), cc(static_cast<size_t>(sh.nPrim) * sh.nCBF);
+            for (int q = 0; q < sh.nPrim; q++) {
+                const double alpha = spec[p++];
+                ex[q] = alpha;
+                for (int l = sh.lLow, c = 0; l <= sh.lHigh; l++) {
+                    // Unnormalise (GaussianBasis.c:728-751).
+                    const double e2 = 2.0 * alpha;
+                    double fac = std::pow(M_PI, 1.5) / (e2 * std::sqrt(e2));
+                    for (int i = 1; i <= l; i++) fac *= static_cast<double>(2 * i - 1) / (2.0 * e2);
+                    const double k = spec[p++] / std::sqrt(fac);
+                    // Cartesian factors (GaussianBasis.c:448-501).
+                    const int off = cart_offset(l);
+                    for (int i = 0; i < ncart(l); i++, c++) {
+                        const double m = std::sqrt(odd_fact(2 * l - 1) / (odd_fact(2 * kCart.x[off + i] - 1) *
+                                                                         odd_fact(2 * kCart.y[off + i] - 1) *
+                                                                         odd_fact(2 * kCart.z[off + i] - 1)));
+                        cc[static_cast<size_t>(q) * sh.nCBF + c] = k * m;
+                    }
+                }
+            }
+            coef.insert(coef.end(), ex.begin(), ex.end());
+            coef.insert(coef.end(), cc.begin(), cc.end());
+            double K = 1.0;
+            if (spherical) {
+                sh.trans = static_cast<int>(trans.size());
+                std::vector<double> T(static_cast<size_t>(sh.nCBF) * sh.nBasis, 0.0);
+                for (int l = sh.lLow, r0 = 0, c0 = 0; l <= sh.lHigh; l++) {
+                    c2s_block(l, T.data(), sh.nBasis, r0, c0);
+                    r0 += ncart(l);
+                    c0 += 2 * l + 1;
+                }
+                K = 0.0;
+                for (int sI = 0; sI < sh.nBasis; sI++) {
+                    double col = 0.0;
+                    for (int c = 0; c < sh.nCBF; c++) col += std::fabs(T[static_cast<size_t>(c) * sh.nBasis + sI]);
+                    K = std::max(K, col);
+                }
+                trans.insert(trans.end(), T.begin(), T.end());
+            } else sh.trans = -1;
+            for (int q = 0; q < sh.nPrim; q++) {
+                double amax = 0.0;
+                for (int c = 0; c < sh.nCBF; c++) amax = std::max(amax, std::fabs(cc[static_cast<size_t>(q) * sh.nCBF + c]));
+                sh.bound[q] = amax * K;
+            }
+            for (int j = 0; j < sh.nBasis; j++) shellOf.push_back(static_cast<int>(shells.size()));
+            f += sh.nBasis;
+            shells.push_back(sh);
+        }
+    }
+    if (shells.size() > static_cast<size_t>(kMaxShellsLDS)) { set_error("too many shells"); return kInvalidArgument; }
+    free_basis(g);
+    free_work(g);
+    g->nFunctions = f;
+    g->nShells = static_cast<int>(shells.size());
+    g->spherical = spherical ? 1 : 0;
+    g->precision = precision;
+    hipStream_t s = g->stream;
+    XQ_TRY(dev_alloc(g, &g->d_shells, shells.size()));
+    XQ_TRY(dev_alloc(g, &g->d_coef, coef.size()));
+    XQ_TRY(dev_alloc(g, &g->d_trans, trans.size()));
+    XQ_CHECK(hipMemcpyAsync(g->d_shells, shells.data(), shells.size() * sizeof(DevShell), hipMemcpyHostToDevice, s));
+    XQ_CHECK(hipMemcpyAsync(g->d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    if (!trans.empty())
+        XQ_CHECK(hipMemcpyAsync(g->d_trans, trans.data(), trans.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    // Pass 1: kept functions per block.
+    int *tmpIdx = nullptr, *counts = nullptr, *d_shellOf = nullptr;
+    const long long base = g->deviceBytes;
+    int status = kOK;
+    do {
+        if ((status = dev_alloc(g, &tmpIdx, static_cast<size_t>(g->nBlocks) * f)) != kOK) break;
+        if ((status = dev_alloc(g, &counts, g->nBlocks)) != kOK) break;
+        if ((status = dev_alloc(g, &d_shellOf, shellOf.size())) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_shellOf, shellOf.data(), shellOf.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                            "copy shellOf")) != kOK) break;
+        if (g->nBlocks > 0) {
+            hipLaunchKernelGGL(basis_count_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->nBlocks, g->d_blocks, g->d_x,
+                               g->d_y, g->d_z, g->nShells, g->d_shells, g->d_coef, g->d_trans, g->spherical, f, g->bfTol,
+                               tmpIdx, counts);
+            if ((status = check(hipGetLastError(), "basis_count_kernel")) != kOK) break;
+        }
+        std::vector<int> h_counts(g->nBlocks > 0 ? g->nBlocks : 1, 0);
+        if (g->nBlocks > 0 &&
+            (status = check(hipMemcpyAsync(h_counts.data(), counts, g->nBlocks * sizeof(int), hipMemcpyDeviceToHost, s),
+                            "copy counts")) != kOK) break;
+        if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
+        long long rows = 0;
+        double bfv = 0.0;
+        for (int b = 0; b < g->nBlocks; b++) {
+            g->blocks[b].nKept = h_counts[b];
+            g->blocks[b].vOffset = rows;
+            g->blocks[b].iOffset = static_cast<int>(rows);
+            rows += h_counts[b];
+            bfv += static_cast<double>(h_counts[b]) * g->blocks[b].nPoints;
+        }
+        g->nValueRows = rows;
+        g->bfValues = bfv;
+        if ((status = check(hipMemcpyAsync(g->d_blocks, g->blocks.data(), g->nBlocks * sizeof(DevBlock), hipMemcpyHostToDevice, s),
+                            "copy blocks")) != kOK) break;
+        const size_t nVal = static_cast<size_t>(rows > 0 ? rows : 1) * kBlockSize;
+        const size_t elt = precision == XQ_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+        if ((status = dev_alloc(g, &g->d_idx, static_cast<size_t>(rows > 0 ? rows : 1))) != kOK) break;
+        void **arrays[4] = {&g->d_f, &g->d_fx, &g->d_fy, &g->d_fz};
+        for (void **a : arrays) {
+            hipError_t e = hipMalloc(a, nVal * elt);
+            if (e != hipSuccess) { set_error("hipMalloc failed for basis values"); status = kOutOfMemory; break; }
+            g->deviceBytes += static_cast<long long>(nVal * elt);
+        }
+        if (status != kOK) break;
+        if (g->nBlocks > 0) {
+            if (precision == XQ_PRECISION_FP64)
+                hipLaunchKernelGGL(basis_eval_kernel<double>, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, g->d_x,
+                                   g->d_y, g->d_z, g->d_shells, d_shellOf, g->d_coef, g->d_trans, g->spherical, f, tmpIdx,
+                                   g->d_idx, static_cast<double *>(g->d_f), static_cast<double *>(g->d_fx),
+                                   static_cast<double *>(g->d_fy), static_cast<double *>(g->d_fz));
+            else
+                hipLaunchKernelGGL(basis_eval_kernel<float>, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, g->d_x,
+                                   g->d_y, g->d_z, g->d_shells, d_shellOf, g->d_coef, g->d_trans, g->spherical, f, tmpIdx,
+                                   g->d_idx, static_cast<float *>(g->d_f), static_cast<float *>(g->d_fx),
+                                   static_cast<float *>(g->d_fy), static_cast<float *>(g->d_fz));
+            if ((status = check(hipGetLastError(), "basis_eval_kernel")) != kOK) break;
+        }
+        status = check(hipStreamSynchronize(s), "basis sync");
+    } while (false);
+    const long long temp = static_cast<long long>(sizeof(int)) * (static_cast<long long>(g->nBlocks) * f + g->nBlocks +
+                                                                 static_cast<long long>(shellOf.size()));
+    dev_free(&tmpIdx);
+    dev_free(&counts);
+    dev_free(&d_shellOf);
+    g->deviceBytes -= temp > 0 ? temp : 0;
+    (void)base;
+    if (status != kOK) { free_basis(g); return status; }
+    g->hasBasis = true;
+    return kOK;
+}
+
+}  // namespace xq

@@ -1,0 +1,447 @@
+// Grid construction on the device: Euler-Maclaurin radial shells x pruned Lebedev-Laikov spheres around every atom, Becke
+// partition weights in FP64, the |w| > 1e-30 cull and per-atom blocks of at most 128 points.
+//
+// Follows DFTGrid_Construct (pMolecule/QCModel/__extensions__/csource/DFTGrid.c:105-263) with DFTGrid_Atom_Parameters
+// (:474-523), DFTGrid_Radial_Points (:562-575), the Lebedev size <-> degree maps (pScientific/Quadrature/__extensions__/
+// csource/Lebedev.c:62-193), the octahedral orbit expansion order (Lebedev.c:1565-1719) and the Becke weight
+// (DFTGridWeights.c:43-80, 220-258).
+//
+// Device layout: every radial shell (atom, radius) is one work-group; each thread computes the Becke weight of one angular
+// point in FP64.  Candidate points of one atom are contiguous and ordered (radial shell, angular index) exactly as the
+// reference orders them, so the order-preserving cull reproduces the reference's point and block order.
+//
+// Becke weight: P_A / sum_i P_i with P_i = prod_{j != i} s(nu_ij).  P_A (the point's own atom) is formed in full.  For
+// i != A the product is formed factor by factor, starting with the factor from atom A, and stops once it falls below
+// 2^-64 P_A: all later factors lie in [0, 1], so the dropped part of sum_i P_i is below nAtoms * 2^-64 of P_A <= the sum.
+// This turns the reference's O(nAtoms^2) pair loop per point into O(nAtoms) for points that sit in their own atom's cell.
+#include <cmath>
+#include <cstring>
+#include <hip/hip_runtime.h>
+
+#include "lebedev_tables.h"
+#include "xq_internal.h"
+
+namespace xq {
+
+namespace {
+
+// Bragg radii in Angstrom by atomic number (DFTGrid.c:29-40).
+const double kBragg[120] = {
+    0.75, 0.35, 0.35, 1.45, 1.05, 0.85, 0.70, 0.65, 0.60, 0.50, 0.50, 1.80, 1.50, 1.25, 1.10, 1.00, 1.00, 1.00, 1.00, 2.20,
+    1.80, 1.60, 1.40, 1.35, 1.40, 1.40, 1.40, 1.35, 1.35, 1.35, 1.35, 1.30, 1.25, 1.15, 1.15, 1.15, 1.15, 2.35, 2.00, 1.80,
+    1.55, 1.45, 1.45, 1.35, 1.30, 1.35, 1.40, 1.60, 1.55, 1.55, 1.45, 1.45, 1.40, 1.40, 1.40, 2.60, 2.15, 1.95, 1.85, 1.85,
+    1.85, 1.85, 1.85, 1.85, 1.80, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75, 1.55, 1.45, 1.35, 1.35, 1.30, 1.35, 1.35, 1.35,
+    1.50, 1.90, 1.80, 1.60, 1.90, 1.90, 1.90, 2.60, 2.15, 1.95, 1.80, 1.80, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75, 1.75,
+    1.75, 1.75, 1.75, 1.75, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55, 1.55};
+const double kAngstromToBohr = 1.0e-10 / 5.291772083e-11;   // Units.h:35, Constants.h:15
+const double kBFTol[5] = {1.0e-8, 1.0e-9, 1.0e-10, 1.0e-12, 1.0e-15};    // DFTGrid.c:43
+const double kRhoTol[5] = {1.0e-13, 1.0e-14, 1.0e-15, 1.0e-17, 1.0e-20}; // DFTGrid.c:46
+constexpr int kMinL = 9;                 // DFTGrid.c:101
+constexpr double kRadialCutFactor = 0.2; // DFTGrid.c:102
+constexpr double kWeightTol = 1.0e-30;   // DFTGrid.c:103
+
+void atom_parameters(int accuracy, int z, int *nR, int *lMax, double *rMax) {
+    switch (accuracy) {
+        case 0: *nR = z < 11 ? 21 : z < 19 ? 42 : z < 37 ? 75 : 84; *rMax = 20.0; *lMax = 23; break;
+        case 1: *nR = z < 11 ? 35 : z < 19 ? 70 : z < 37 ? 95 : 104; *rMax = 25.0; *lMax = 35; break;
+        case 2: *nR = z < 11 ? 49 : z < 19 ? 88 : z < 37 ? 112 : 123; *rMax = 30.0; *lMax = z < 11 ? 35 : 41; break;
+        case 3:
+            *nR = z < 11 ? 70 : z < 19 ? 123 : z < 37 ? 130 : 155; *rMax = 35.0;
+            *lMax = z < 11 ? 41 : z < 19 ? 47 : z < 89 ? 53 : 59;
+            break;
+        default: *nR = z < 11 ? 100 : z < 19 ? 125 : z < 37 ? 160 : 205; *rMax = 35.0; *lMax = 65; break;
+    }
+}
+
+double bragg_radius(int z) {
+    if (z < 0) z = 0;
+    if (z >= 120) z = 119;
+    return kAngstromToBohr * kBragg[z];
+}
+
+// Number of points of the smallest rule integrating degree l (Lebedev.c:134-193).
+int lebedev_size_for_l(int l) {
+    static const int small[16] = {6, 6, 14, 26, 38, 50, 74, 86, 110, 146, 170, 194, 230, 266, 302, 350};
+    static const int large[22] = {0, 0, 0, 0, 0, 434, 590, 770, 974, 1202, 1454, 1730, 2030, 2354, 2702, 3074, 3470,
+                                  3890, 4334, 4802, 5294, 5810};
+    if (l < 3) return 6;
+    if (l <= 31) return small[l / 2];
+    if (l <= 131) return large[l / 6];
+    return 0;
+}
+
+// Degree of the smallest rule with at least n points (Lebedev.c:62-98).
+int lebedev_l_for_size(int n) {
+    static const int ls[XQ_LEBEDEV_NRULES] = {3, 5, 7, 9, 11, 13, 15, 17, 19, 21, 23, 25, 27, 29, 31, 35,
+                                              41, 47, 53, 59, 65, 71, 77, 83, 89, 95, 101, 107, 113, 119, 125, 131};
+    for (int k = 0; k < XQ_LEBEDEV_NRULES; k++)
+        if (n <= xq_lebedev_npoints[k]) return ls[k];
+    return 0;
+}
+
+// Expand rule k into unit vectors and weights, in the orbit order of Lebedev.c:1565-1719.  Each template row names, for x, y
+// and z, the generator slot (1 = a, 2 = b, 3 = c, 0 = zero) and its sign.
+int lebedev_expand(int k, double *out /* x, y, z, w per point */) {
+    static const signed char T1[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    static const signed char T2[12][3] = {{0, 1, 1}, {0, 1, -1}, {0, -1, 1}, {0, -1, -1}, {1, 0, 1}, {1, 0, -1},
+                                          {-1, 0, 1}, {-1, 0, -1}, {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}};
+    static const signed char T3[8][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {1, -1, -1},
+                                         {-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+    static const signed char T4[24][3] = {{1, 1, 2}, {1, 1, -2}, {1, -1, 2}, {1, -1, -2}, {-1, 1, 2}, {-1, 1, -2},
+                                          {-1, -1, 2}, {-1, -1, -2}, {1, 2, 1}, {1, -2, 1}, {1, 2, -1}, {1, -2, -1},
+                                          {-1, 2, 1}, {-1, -2, 1}, {-1, 2, -1}, {-1, -2, -1}, {2, 1, 1}, {-2, 1, 1},
+                                          {2, 1, -1}, {-2, 1, -1}, {2, -1, 1}, {-2, -1, 1}, {2, -1, -1}, {-2, -1, -1}};
+    static const signed char T5[24][3] = {{1, 2, 0}, {1, -2, 0}, {-1, 2, 0}, {-1, -2, 0}, {2, 1, 0}, {2, -1, 0},
+                                          {-2, 1, 0}, {-2, -1, 0}, {1, 0, 2}, {1, 0, -2}, {-1, 0, 2}, {-1, 0, -2},
+                                          {2, 0, 1}, {2, 0, -1}, {-2, 0, 1}, {-2, 0, -1}, {0, 1, 2}, {0, 1, -2},
+                                          {0, -1, 2}, {0, -1, -2}, {0, 2, 1}, {0, 2, -1}, {0, -2, 1}, {0, -2, -1}};
+    static const signed char T6[48][3] = {
+        {1, 2, 3}, {1, 2, -3}, {1, -2, 3}, {1, -2, -3}, {-1, 2, 3}, {-1, 2, -3}, {-1, -2, 3}, {-1, -2, -3},
+        {1, 3, 2}, {1, 3, -2}, {1, -3, 2}, {1, -3, -2}, {-1, 3, 2}, {-1, 3, -2}, {-1, -3, 2}, {-1, -3, -2},
+        {2, 1, 3}, {2, 1, -3}, {2, -1, 3}, {2, -1, -3}, {-2, 1, 3}, {-2, 1, -3}, {-2, -1, 3}, {-2, -1, -3},
+        {2, 3, 1}, {2, 3, -1}, {2, -3, 1}, {2, -3, -1}, {-2, 3, 1}, {-2, 3, -1}, {-2, -3, 1}, {-2, -3, -1},
+        {3, 1, 2}, {3, 1, -2}, {3, -1, 2}, {3, -1, -2}, {-3, 1, 2}, {-3, 1, -2}, {-3, -1, 2}, {-3, -1, -2},
+        {3, 2, 1}, {3, 2, -1}, {3, -2, 1}, {3, -2, -1}, {-3, 2, 1}, {-3, 2, -1}, {-3, -2, 1}, {-3, -2, -1}};
+    int m = 0;
+    for (int o = xq_lebedev_first[k]; o < xq_lebedev_first[k + 1]; o++) {
+        const xq_lebedev_orbit &orb = xq_lebedev_orbits[o];
+        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        const signed char(*T)[3] = nullptr;
+        int count = 0;
+        switch (orb.type) {
+            case 1: g[1] = 1.0; T = T1; count = 6; break;
+            case 2: g[1] = 1.0 / std::sqrt(2.0); T = T2; count = 12; break;
+            case 3: g[1] = 1.0 / std::sqrt(3.0); T = T3; count = 8; break;
+            case 4: g[1] = orb.a; g[2] = std::sqrt(1.0 - 2.0 * orb.a * orb.a); T = T4; count = 24; break;
+            case 5: g[1] = orb.a; g[2] = std::sqrt(1.0 - orb.a * orb.a); T = T5; count = 24; break;
+            case 6: g[1] = orb.a; g[2] = orb.b; g[3] = std::sqrt(1.0 - orb.a * orb.a - orb.b * orb.b); T = T6; count = 48; break;
+        }
+        for (int n = 0; n < count; n++, m++) {
+            for (int c = 0; c < 3; c++) {
+                int s = T[n][c];
+                out[4 * m + c] = s == 0 ? 0.0 : (s > 0 ? g[s] : -g[-s]);
+            }
+            out[4 * m + 3] = orb.v;
+        }
+    }
+    return m;
+}
+
+// Becke cell-function constants (DFTGridWeights.c:21-35).
+__constant__ double c_beckeC[11] = {1.000000000000000e+00, -3.333333333333333e+00, 9.000000000000000e+00,
+                                    -17.142857142857142e+00, 23.333333333333332e+00, -22.909090909090910e+00,
+                                    16.153846153846153e+00, -8.000000000000000e+00, 2.647058823529412e+00,
+                                    -0.526315789473684e+00, 0.047619047619048e+00};
+constexpr double kBeckeA = 1.850069046020527e+00;
+
+struct RadialShell {
+    int atom;
+    int rule;        // first unit vector of the Lebedev rule in the device rule array
+    int nAng;
+    int out;         // first candidate point of the shell
+    double r;
+    double wr;       // 4 pi w_r
+};
+
+__device__ __forceinline__ double becke_poly(double nu) {
+    double nu2 = nu * nu, term = nu, sum = 0.0;
+#pragma unroll
+    for (int t = 0; t <= 10; t++) {
+        sum += c_beckeC[t] * term;
+        term *= nu2;
+    }
+    return sum;
+}
+
+// Factor of P_k from the pair (k, m), k != m: the pair data is stored for the ordered pair (max, min).
+__device__ __forceinline__ double becke_factor(int k, int m, double rk, double rm, const double *invR, const double *aij) {
+    int hi = k > m ? k : m, lo = k > m ? m : k;
+    int ij = (hi * (hi - 1)) / 2 + lo;
+    double rhi = k > m ? rk : rm, rlo = k > m ? rm : rk;
+    double mu = (rhi - rlo) * invR[ij];
+    double nu = mu + aij[ij] * (1.0 - mu * mu);
+    double s = kBeckeA * becke_poly(nu);
+    return k > m ? (0.5 - s) : (0.5 + s);
+}
+
+__device__ __forceinline__ double dist(const double *xyz, int i, double px, double py, double pz) {
+    double dx = xyz[3 * i] - px, dy = xyz[3 * i + 1] - py, dz = xyz[3 * i + 2] - pz;
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell *__restrict__ shells,
+                                                               const double *__restrict__ rules,
+                                                               const double *__restrict__ xyz, int nAtoms,
+                                                               const double *__restrict__ invR,
+                                                               const double *__restrict__ aij,
+                                                               double *__restrict__ cx, double *__restrict__ cy,
+                                                               double *__restrict__ cz, double *__restrict__ cw,
+                                                               int *__restrict__ keep) {
+    const RadialShell sh = shells[blockIdx.x];
+    const int A = sh.atom;
+    const double ax = xyz[3 * A], ay = xyz[3 * A + 1], az = xyz[3 * A + 2];
+    for (int k = threadIdx.x; k < sh.nAng; k += blockDim.x) {
+        const double *u = rules + 4 * (sh.rule + k);
+        const double px = u[0] * sh.r + ax, py = u[1] * sh.r + ay, pz = u[2] * sh.r + az;
+        const double rA = dist(xyz, A, px, py, pz);
+        // P_A in full.
+        double PA = 1.0;
+        for (int j = 0; j < nAtoms; j++) {
+            if (j == A) continue;
+            PA *= becke_factor(A, j, rA, dist(xyz, j, px, py, pz), invR, aij);
+        }
+        const double floor = PA * 0x1p-64;
+        double sum = PA;
+        for (int i = 0; i < nAtoms; i++) {
+            if (i == A) continue;
+            const double ri = dist(xyz, i, px, py, pz);
+            double Pi = becke_factor(i, A, ri, rA, invR, aij);
+            for (int j = 0; j < nAtoms && fabs(Pi) >= floor; j++) {
+                if (j == i || j == A) continue;
+                Pi *= becke_factor(i, j, ri, dist(xyz, j, px, py, pz), invR, aij);
+            }
+            if (fabs(Pi) >= floor) sum += Pi;
+        }
+        const double w = sh.wr * u[3] * (PA / sum);
+        const int o = sh.out + k;
+        cx[o] = px;
+        cy[o] = py;
+        cz[o] = pz;
+        cw[o] = w;
+        keep[o] = fabs(w) > kWeightTol ? 1 : 0;
+    }
+}
+
+// Order-preserving compaction of the kept candidates: dst = exclusive prefix sum of keep.
+__global__ void compact_points_kernel(int n, const int *__restrict__ keep, const int *__restrict__ dst,
+                                      const double *__restrict__ cx, const double *__restrict__ cy,
+                                      const double *__restrict__ cz, const double *__restrict__ cw,
+                                      double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                                      double *__restrict__ w) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && keep[i]) {
+        int d = dst[i];
+        x[d] = cx[i];
+        y[d] = cy[i];
+        z[d] = cz[i];
+        w[d] = cw[i];
+    }
+}
+
+// Single work-group exclusive scan (n up to a few million: each thread scans a contiguous chunk).
+__global__ void __launch_bounds__(1024) exclusive_scan_kernel(int n, const int *__restrict__ in, int *__restrict__ out,
+                                                              int *__restrict__ total) {
+    __shared__ int partial[1024];
+    const int t = threadIdx.x, T = blockDim.x;
+    const int chunk = (n + T - 1) / T;
+    const int lo = t * chunk, hi = min(n, lo + chunk);
+    int s = 0;
+    for (int i = lo; i < hi; i++) s += in[i];
+    partial[t] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the per-thread sums.
+    for (int off = 1; off < T; off <<= 1) {
+        int v = t >= off ? partial[t - off] : 0;
+        __syncthreads();
+        partial[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? partial[t - 1] : 0;
+    for (int i = lo; i < hi; i++) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (t == T - 1) *total = partial[T - 1];
+}
+
+}  // namespace
+
+int build_grid(xq_grid *g) {
+    const int nA = g->nAtoms;
+    g->bfTol = kBFTol[g->accuracy];
+    g->rhoTol = kRhoTol[g->accuracy];
+    // Lebedev rules used by this grid, expanded once.
+    std::vector<int> ruleFirst(XQ_LEBEDEV_NRULES, -1);
+    std::vector<double> rules;
+    auto rule_of = [&](int l) -> int {
+        int n = lebedev_size_for_l(l), k = 0;
+        while (k < XQ_LEBEDEV_NRULES && xq_lebedev_npoints[k] != n) k++;
+        if (ruleFirst[k] < 0) {
+            ruleFirst[k] = static_cast<int>(rules.size() / 4);
+            rules.resize(rules.size() + 4 * static_cast<size_t>(n));
+            lebedev_expand(k, rules.data() + 4 * static_cast<size_t>(ruleFirst[k]));
+        }
+        return k;
+    };
+    // Radial shells, per atom in order (DFTGrid.c:138-211).
+    std::vector<RadialShell> shells;
+    std::vector<int> atomFirst(nA + 1, 0);
+    int nCand = 0;
+    for (int a = 0; a < nA; a++) {
+        int nR, lMax;
+        double rMax;
+        atom_parameters(g->accuracy, g->Z[a], &nR, &lMax, &rMax);
+        const int nAngMax = lebedev_size_for_l(lMax), nAngMin = lebedev_size_for_l(kMinL);
+        const double range = bragg_radius(g->Z[a]), rCut = kRadialCutFactor * range;
+        const double d = 1.0 + static_cast<double>(nR);
+        atomFirst[a] = nCand;
+        for (int i = 0; i < nR; i++) {
+            const double x = static_cast<double>(i + 1) / d;
+            const double r = range * std::pow(x / (1.0 - x), 2.0);
+            const double wr = 2.0 * std::pow(range, 3.0) * std::pow(x, 5.0) / std::pow(1.0 - x, 7.0) / d;
+            if (r > rMax) break;
+            int l;
+            if (r > rCut) l = lMax;
+            else {
+                int n = static_cast<int>(std::ceil(static_cast<double>(nAngMax) * r / rCut));
+                l = n < nAngMin ? kMinL : lebedev_l_for_size(n);
+            }
+            const int k = rule_of(l);
+            RadialShell s;
+            s.atom = a;
+            s.rule = ruleFirst[k];
+            s.nAng = xq_lebedev_npoints[k];
+            s.out = nCand;
+            s.r = r;
+            s.wr = 4.0 * M_PI * wr;
+            shells.push_back(s);
+            nCand += s.nAng;
+        }
+    }
+    atomFirst[nA] = nCand;
+    // Becke pair data (DFTGridWeights.c:43-80): 1/R_ij and the size adjustment a_ij, for i > j.
+    const size_t nPairs = nA > 1 ? static_cast<size_t>(nA) * (nA - 1) / 2 : 1;
+    std::vector<double> invR(nPairs, 0.0), aij(nPairs, 0.0);
+    for (int i = 0, ij = 0; i < nA; i++) {
+        for (int j = 0; j < i; j++, ij++) {
+            double dx = g->xyz[3 * i] - g->xyz[3 * j], dy = g->xyz[3 * i + 1] - g->xyz[3 * j + 1],
+                   dz = g->xyz[3 * i + 2] - g->xyz[3 * j + 2];
+            double chi = bragg_radius(g->Z[i]) / bragg_radius(g->Z[j]);
+            double u = (chi - 1.0) / (chi + 1.0), a = u / (u * u - 1.0);
+            invR[ij] = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+            aij[ij] = a > 0.5 ? 0.5 : (a < -0.5 ? -0.5 : a);
+        }
+    }
+    // Device work.
+    RadialShell *d_shells = nullptr;
+    double *d_rules = nullptr, *d_xyz = nullptr, *d_invR = nullptr, *d_aij = nullptr;
+    double *cx = nullptr, *cy = nullptr, *cz = nullptr, *cw = nullptr;
+    int *keep = nullptr, *dst = nullptr, *total = nullptr;
+    int status = kOK;
+    auto cleanup = [&]() {
+        dev_free(&d_shells); dev_free(&d_rules); dev_free(&d_xyz); dev_free(&d_invR); dev_free(&d_aij);
+        dev_free(&cx); dev_free(&cy); dev_free(&cz); dev_free(&cw); dev_free(&keep); dev_free(&dst); dev_free(&total);
+    };
+    const long long before = g->deviceBytes;
+    do {
+        if ((status = dev_alloc(g, &d_shells, shells.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_rules, rules.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_xyz, g->xyz.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_invR, nPairs)) != kOK) break;
+        if ((status = dev_alloc(g, &d_aij, nPairs)) != kOK) break;
+        if ((status = dev_alloc(g, &cx, nCand)) != kOK) break;
+        if ((status = dev_alloc(g, &cy, nCand)) != kOK) break;
+        if ((status = dev_alloc(g, &cz, nCand)) != kOK) break;
+        if ((status = dev_alloc(g, &cw, nCand)) != kOK) break;
+        if ((status = dev_alloc(g, &keep, nCand)) != kOK) break;
+        if ((status = dev_alloc(g, &dst, nCand)) != kOK) break;
+        if ((status = dev_alloc(g, &total, 1)) != kOK) break;
+        hipStream_t s = g->stream;
+        if ((status = check(hipMemcpyAsync(d_shells, shells.data(), shells.size() * sizeof(RadialShell),
+                                           hipMemcpyHostToDevice, s), "copy shells")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_rules, rules.data(), rules.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy rules")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_xyz, g->xyz.data(), g->xyz.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy xyz")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_invR, invR.data(), nPairs * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy invR")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_aij, aij.data(), nPairs * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy aij")) != kOK) break;
+        if (!shells.empty()) {
+            hipLaunchKernelGGL(candidate_points_kernel, dim3(shells.size()), dim3(256), 0, s, d_shells, d_rules, d_xyz, nA,
+                               d_invR, d_aij, cx, cy, cz, cw, keep);
+            if ((status = check(hipGetLastError(), "candidate_points_kernel")) != kOK) break;
+            hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, s, nCand, keep, dst, total);
+            if ((status = check(hipGetLastError(), "exclusive_scan_kernel")) != kOK) break;
+        }
+        // Per-atom kept counts from the scan at the atom boundaries.
+        std::vector<int> scanned(nCand > 0 ? nCand : 1, 0), kept(nCand > 0 ? nCand : 1, 0);
+        int nKept = 0;
+        if (nCand > 0) {
+            if ((status = check(hipMemcpyAsync(scanned.data(), dst, nCand * sizeof(int), hipMemcpyDeviceToHost, s),
+                                "copy scan")) != kOK) break;
+            if ((status = check(hipMemcpyAsync(kept.data(), keep, nCand * sizeof(int), hipMemcpyDeviceToHost, s),
+                                "copy keep")) != kOK) break;
+            if ((status = check(hipMemcpyAsync(&nKept, total, sizeof(int), hipMemcpyDeviceToHost, s), "copy total")) != kOK)
+                break;
+            if ((status = check(hipStreamSynchronize(s), "grid sync")) != kOK) break;
+        }
+        g->nPoints = nKept;
+        if ((status = dev_alloc(g, &g->d_x, nKept)) != kOK) break;
+        if ((status = dev_alloc(g, &g->d_y, nKept)) != kOK) break;
+        if ((status = dev_alloc(g, &g->d_z, nKept)) != kOK) break;
+        if ((status = dev_alloc(g, &g->d_w, nKept)) != kOK) break;
+        if (nCand > 0) {
+            hipLaunchKernelGGL(compact_points_kernel, dim3((nCand + 255) / 256), dim3(256), 0, s, nCand, keep, dst, cx, cy,
+                               cz, cw, g->d_x, g->d_y, g->d_z, g->d_w);
+            if ((status = check(hipGetLastError(), "compact_points_kernel")) != kOK) break;
+        }
+        // Blocks of at most kBlockSize consecutive points per atom.
+        g->blocks.clear();
+        for (int a = 0; a < nA; a++) {
+            const int c0 = atomFirst[a], c1 = atomFirst[a + 1];
+            if (c1 <= c0) continue;
+            const int p0 = scanned[c0];
+            const int p1 = scanned[c1 - 1] + kept[c1 - 1];
+            for (int p = p0; p < p1; p += kBlockSize) {
+                DevBlock b;
+                b.atom = a;
+                b.pStart = p;
+                b.nPoints = (p1 - p) < kBlockSize ? (p1 - p) : kBlockSize;
+                b.nKept = 0;
+                b.vOffset = 0;
+                b.iOffset = 0;
+                b.pad = 0;
+                g->blocks.push_back(b);
+            }
+        }
+        g->nBlocks = static_cast<int>(g->blocks.size());
+        if ((status = dev_alloc(g, &g->d_blocks, g->blocks.size())) != kOK) break;
+        if (!g->blocks.empty() &&
+            (status = check(hipMemcpyAsync(g->d_blocks, g->blocks.data(), g->blocks.size() * sizeof(DevBlock),
+                                           hipMemcpyHostToDevice, s), "copy blocks")) != kOK) break;
+        status = check(hipStreamSynchronize(s), "grid sync");
+    } while (false);
+    cleanup();
+    // Only the kept points and the block table stay resident.
+    g->deviceBytes = before + static_cast<long long>(4 * sizeof(double) * (size_t)(g->nPoints > 0 ? g->nPoints : 1) +
+                                                     sizeof(DevBlock) * (g->blocks.empty() ? 1 : g->blocks.size()));
+    return status;
+}
+
+// Estimated point count without the cull (DFTGrid_EstimatedPoints, DFTGrid.c:301-356).
+long long estimated_points(int accuracy, int nAtoms, const int *Z) {
+    long long p = 0;
+    for (int a = 0; a < nAtoms; a++) {
+        int nR, lMax;
+        double rMax;
+        atom_parameters(accuracy, Z[a], &nR, &lMax, &rMax);
+        const int nAngMax = lebedev_size_for_l(lMax), nAngMin = lebedev_size_for_l(kMinL);
+        const double range = bragg_radius(Z[a]), rCut = kRadialCutFactor * range, d = 1.0 + nR;
+        for (int i = 0; i < nR; i++) {
+            const double x = static_cast<double>(i + 1) / d;
+            const double r = range * std::pow(x / (1.0 - x), 2.0);
+            if (r > rMax) break;
+            int l;
+            if (r > rCut) l = lMax;
+            else {
+                int n = static_cast<int>(std::ceil(static_cast<double>(nAngMax) * r / rCut));
+                l = n < nAngMin ? kMinL : lebedev_l_for_size(n);
+            }
+            p += lebedev_size_for_l(l);
+        }
+    }
+    return p;
+}
+
+}  // namespace xq

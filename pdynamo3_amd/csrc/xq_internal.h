@@ -1,0 +1,138 @@
+// Internal definitions of libxcquad: the handle, device data layout and error handling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/xcquad.h"
+
+namespace xq {
+
+// Status codes (pCore/__extensions__/cinclude/Status.h:24-33).
+enum Status { kOK = 0, kAlgorithmError = 1, kInvalidArgument = 3, kNonConformable = 6, kOutOfMemory = 7 };
+
+// Points per grid block (DFTGrid.c:82); block rows of basis values are stored with this stride.
+constexpr int kBlockSize = 128;
+// Largest shell supported by the evaluation kernels: up to l = 3 in one shell (SPDF) -> 20 Cartesian functions.
+constexpr int kMaxCBF = 20;
+constexpr int kMaxPrim = 16;
+constexpr int kMaxL = 3;
+
+void set_error(const std::string &message);
+
+// Shell description on the device (one per shell of every atom).
+struct DevShell {
+    double x, y, z;            // centre (bohr)
+    int atom;
+    int lLow, lHigh;
+    int nPrim, nCBF, nBasis;
+    int fStart;                // index of the shell's first function
+    int coef;                  // offset into the coefficient array: exponents[nPrim], then ccbf[nPrim][nCBF]
+    int trans;                 // offset into the c2s array (nCBF x nBasis), -1 for Cartesian
+    int pow;                   // offset into the Cartesian power array (3 ints per Cartesian function)
+    double bound[kMaxPrim];    // screening prefactor per primitive: max_c |ccbf_qc| * max_s sum_c |c2s_cs|
+};
+
+// One grid block: at most kBlockSize consecutive points of one atom (DFTGrid.c:213-238).
+struct DevBlock {
+    int atom;
+    int pStart;                // first point
+    int nPoints;
+    int nKept;                 // functions kept by the |phi| > bfTol filter
+    long long vOffset;         // offset of the block's first value row (rows of kBlockSize entries)
+    int iOffset;               // offset of the block's kept-function indices
+    int pad;
+};
+
+}  // namespace xq
+
+// The handle.  Device arrays are prefixed d_.
+struct xq_grid {
+    int accuracy = 2;
+    int nAtoms = 0;
+    std::vector<int> Z;
+    std::vector<double> xyz;
+    double bfTol = 1.0e-10, rhoTol = 1.0e-15;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // Grid.
+    int nPoints = 0, nBlocks = 0;
+    std::vector<xq::DevBlock> blocks;     // host copy
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_w = nullptr;
+    xq::DevBlock *d_blocks = nullptr;
+
+    // Basis.
+    int nFunctions = 0, nShells = 0, spherical = 1, precision = 0;
+    bool hasBasis = false;
+    xq::DevShell *d_shells = nullptr;
+    double *d_coef = nullptr, *d_trans = nullptr;
+    int *d_pow = nullptr;
+    long long nValueRows = 0;             // sum over blocks of kept functions
+    int *d_idx = nullptr;                 // kept-function indices, per block
+    void *d_f = nullptr, *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;   // values and gradients, rows of kBlockSize
+    double bfValues = 0.0;                // sum over blocks of kept functions x points
+
+    // Work arrays for a quadrature.
+    double *d_P = nullptr, *d_V = nullptr, *d_packed = nullptr;   // dense n x n P and V, packed staging
+    float *d_P32 = nullptr;
+    double *d_rho = nullptr;              // per point: rho, grad x, grad y, grad z
+    double *d_vrs = nullptr;              // per point: w vrho, w vsigma
+    double *d_blockEN = nullptr;          // per block: E_xc and N_e partial sums
+    int *d_skip = nullptr;                // per block: 1 if skipped by the density tolerance
+    std::vector<double> h_blockEN;
+    std::vector<int> h_skip;
+    long long deviceBytes = 0;
+};
+
+namespace xq {
+
+// Allocation helpers that account device bytes on the handle.
+template <typename T>
+int dev_alloc(xq_grid *g, T **ptr, size_t count) {
+    *ptr = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(ptr), count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        *ptr = nullptr;
+        return kOutOfMemory;
+    }
+    g->deviceBytes += static_cast<long long>(count * sizeof(T));
+    return kOK;
+}
+
+template <typename T>
+void dev_free(T **ptr) {
+    if (*ptr != nullptr) {
+        (void)hipFree(reinterpret_cast<void *>(*ptr));
+        *ptr = nullptr;
+    }
+}
+
+int check(hipError_t e, const char *what);
+
+// Stages.
+int build_grid(xq_grid *g);
+int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int precision);
+int integrate(xq_grid *g, int nIDs, const int *ids, const double *P, double *eXC, double *nE, double *V, double *stats);
+void free_basis(xq_grid *g);
+void free_work(xq_grid *g);
+
+}  // namespace xq
+
+#define XQ_CHECK(call)                                   \
+    do {                                                 \
+        int _s = xq::check((call), #call);               \
+        if (_s != xq::kOK) return _s;                    \
+    } while (0)
+
+#define XQ_TRY(call)                                     \
+    do {                                                 \
+        int _s = (call);                                 \
+        if (_s != xq::kOK) return _s;                    \
+    } while (0)

@@ -1,0 +1,192 @@
+"""Host-side mirror of pDynamo3's DFT grid integrator on top of libxcquad.
+
+QuadratureGrid wraps one device-resident grid (points, weights, screened basis values); DFTGridIntegrator keeps the
+reference's plugin interface (pMolecule/QCModel/DFTGridIntegrator.py:17-93): BuildGrid(target), EstimateMemory(qcModel,
+qcState), Fock(target), Gradients(target), SummaryItems(), with the same report keys and the same error behaviour (any
+non-zero status raises QCModelError("Error in DFT integrator."), DFTIntegrator.pyx:72).
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .basis import element_spec, load_basis, number_of_functions
+from .definitions import DFTGridAccuracy, functional_ids
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+class QCModelError(Exception):
+    """Error raised by the quadrature (pMolecule/QCModel/QCModelError.py)."""
+
+
+def _check(status, what):
+    if status != 0:
+        raise QCModelError("Error in DFT integrator ({:s}: status {:d}, {:s}).".format(what, status, _lib.last_error()))
+
+
+class QuadratureGrid:
+    """One molecule's grid and basis values on the device."""
+
+    def __init__(self, atomic_numbers, xyz_bohr, accuracy=DFTGridAccuracy.Medium):
+        self.atomic_numbers = _lib.i32(atomic_numbers)
+        self.xyz = _lib.f64(xyz_bohr).reshape(-1, 3)
+        self.accuracy = DFTGridAccuracy(accuracy)
+        self._h = ctypes.c_void_p()
+        _check(_lib.lib.xq_grid_build(int(self.accuracy), len(self.atomic_numbers), _lib.iptr(self.atomic_numbers),
+                                      _lib.dptr(self.xyz), ctypes.byref(self._h)), "grid build")
+        n_p, n_b = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.lib.xq_grid_counts(self._h, ctypes.byref(n_p), ctypes.byref(n_b)), "grid counts")
+        self.number_of_points, self.number_of_blocks = n_p.value, n_b.value
+        self.number_of_functions = 0
+        self.precision = None
+
+    def points(self):
+        """(x, y, z, w) per point [G, 4], block atoms [B] and block sizes [B]."""
+        xyzw = np.zeros((self.number_of_points, 4))
+        ba = np.zeros(self.number_of_blocks, dtype=np.int32)
+        bc = np.zeros(self.number_of_blocks, dtype=np.int32)
+        _check(_lib.lib.xq_grid_points(self._h, _lib.dptr(xyzw), _lib.iptr(ba), _lib.iptr(bc)), "grid points")
+        return xyzw, ba, bc
+
+    def set_basis(self, spec, spherical=True, precision="fp64"):
+        """Evaluate and store the screened basis values (precision 'fp64' or 'mixed')."""
+        spec = _lib.f64(spec)
+        nf = ctypes.c_int()
+        _check(_lib.lib.xq_basis_set(self._h, spec.size, _lib.dptr(spec), int(bool(spherical)), PRECISIONS[precision],
+                                     ctypes.byref(nf)), "basis")
+        self.number_of_functions = nf.value
+        self.precision = precision
+
+    def integrate(self, ids, density, fock=None):
+        """One quadrature: returns (E_xc, N_e, stats); fock (packed, float64) is incremented in place when given."""
+        ids = _lib.i32(ids)
+        P = _lib.f64(density)
+        n = self.number_of_functions
+        if P.size != n * (n + 1) // 2:
+            raise QCModelError("Density has {:d} entries; {:d} expected.".format(P.size, n * (n + 1) // 2))
+        V = fock if fock is not None else np.zeros_like(P)
+        if V.dtype != np.float64 or not V.flags.c_contiguous or V.size != P.size:
+            raise QCModelError("Fock matrix must be a contiguous float64 array of the density's size.")
+        e, ne = ctypes.c_double(), ctypes.c_double()
+        stats = np.zeros(3)
+        _check(_lib.lib.xq_integrate(self._h, ids.size, _lib.iptr(ids), _lib.dptr(P), ctypes.byref(e), ctypes.byref(ne),
+                                     _lib.dptr(V), _lib.dptr(stats)), "integrate")
+        return e.value, ne.value, {"bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2]}
+
+    def device_bytes(self):
+        b = ctypes.c_longlong()
+        _check(_lib.lib.xq_device_bytes(self._h, ctypes.byref(b)), "device bytes")
+        return b.value
+
+    def close(self):
+        if self._h:
+            _lib.lib.xq_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def estimated_points(accuracy, atomic_numbers):
+    """DFTGrid_EstimatedPoints (DFTGrid.c:301-356): grid size before the weight cull."""
+    z = _lib.i32(atomic_numbers)
+    n = ctypes.c_longlong()
+    _check(_lib.lib.xq_estimated_points(int(accuracy), z.size, _lib.iptr(z), ctypes.byref(n)), "estimated points")
+    return n.value
+
+
+def estimate_bytes(number_of_points, number_of_functions, mean_kept_functions, precision="fp64"):
+    """Device bytes for a job of the given size (admission control)."""
+    b = ctypes.c_longlong()
+    _check(_lib.lib.xq_estimate_bytes(int(number_of_points), int(number_of_functions), float(mean_kept_functions),
+                                      PRECISIONS[precision], ctypes.byref(b)), "estimate bytes")
+    return b.value
+
+
+# ---------------------------------------------------------------------------------------------------------------------------------
+# The plugin interface.
+# ---------------------------------------------------------------------------------------------------------------------------------
+class OrbitalBasis:
+    """The orbital basis of a QC state: a basis label, the per-atom element specification and its shape."""
+
+    def __init__(self, label, atomic_numbers, spherical=True):
+        self.label = label
+        self.spherical = bool(spherical)
+        self._basis = load_basis(label)
+        self.spec = element_spec(self._basis, atomic_numbers)
+        self.n = number_of_functions(self._basis, atomic_numbers, spherical)
+
+    def __len__(self):
+        return self.n
+
+
+class FunctionalModel:
+    """Functional by name ('blyp', 'pbe'): libxc IDs; order 1 and sigma for the GGAs (DFTFunctionalModel.c:183-186)."""
+
+    def __init__(self, name):
+        self.name = name
+        self.ids = functional_ids(name)
+        self.order = 1
+        self.hasSigma = True
+
+
+class DFTGridIntegrator:
+    """Drop-in counterpart of pMolecule/QCModel/DFTGridIntegrator.DFTGridIntegrator on the GPU.
+
+    accuracy  : DFTGridAccuracy (default Medium, DFTGridIntegrator.py:22).
+    inCore    : accepted for interface parity; basis values always stay on the device across calls.
+    precision : 'fp64' or 'mixed' (basis-value storage and contraction precision).
+    """
+
+    def __init__(self, accuracy=DFTGridAccuracy.Medium, inCore=True, precision="fp64"):
+        self.accuracy = DFTGridAccuracy(accuracy)
+        self.inCore = inCore
+        self.precision = precision
+
+    def BuildGrid(self, target):
+        """Build the grid (DFTGridIntegrator.py:25-29)."""
+        grid = QuadratureGrid(target.qcState.atomicNumbers, target.scratch.qcCoordinates3AU, self.accuracy)
+        target.scratch.dftGrid = grid
+        target.scratch.qcEnergyReport["Quadrature Points"] = grid.number_of_points
+
+    def EstimateMemory(self, qcModel, qcState):
+        """Device bytes for this job (DFTGridIntegrator.py:31-39; sized for this library's layout and an unscreened basis)."""
+        p = estimated_points(self.accuracy, qcState.atomicNumbers)
+        n = len(qcState.orbitalBases)
+        return estimate_bytes(p, n, float(n), self.precision)
+
+    def Fock(self, target):
+        """Add V_xc into the Fock matrix and return E_xc (DFTGridIntegrator.py:41-70)."""
+        scratch = target.scratch
+        if not target.electronicState.isSpinRestricted:
+            raise QCModelError("Error in DFT integrator (spin-unrestricted densities are not supported).")
+        grid = scratch.dftGrid
+        bases = target.qcState.orbitalBases
+        if grid.number_of_functions == 0 or grid.precision != self.precision:
+            grid.set_basis(bases.spec, bases.spherical, self.precision)
+        eQuad, rhoQuad, stats = grid.integrate(target.qcModel.functionalModel.ids, scratch.onePDMP.density,
+                                               scratch.onePDMP.fock)
+        scratch.qcEnergyReport["Quadrature Electron Count"] = (rhoQuad, "{:.3f}")
+        scratch.qcEnergyReport["Quadrature Energy"] = eQuad
+        n = stats["bf_values"]
+        o = float(len(bases))
+        scratch.qcEnergyReport["Quadrature BF Values"] = n
+        scratch.qcEnergyReport["Quadrature BF Sparsity (%)"] = ((1.0 - n / (grid.number_of_points * o)) * 100.0, "{:.1f}")
+        scratch.qcEnergyReport["Quadrature BF Storage (GB)"] = (grid.device_bytes() / 1.0e9, "{:.3f}")
+        return eQuad
+
+    def Gradients(self, target):
+        """XC nuclear gradients (DFTGridIntegrator.py:72-88) are not provided by this library."""
+        if getattr(target.scratch, "doGradients", False):
+            raise QCModelError("Error in DFT integrator (gradients are not available on this path).")
+
+    def SummaryItems(self):
+        """Summary items (DFTGridIntegrator.py:90-93)."""
+        return [("Grid Accuracy", "{:s}".format(self.accuracy.name)),
+                ("Save Grid Function Data", "{:s}".format(repr(self.inCore))),
+                ("Quadrature Precision", self.precision)]
