@@ -1,0 +1,172 @@
+#!/usr/bin/env python
+"""XC quadrature throughput on one MI355X: grid points per second of the mixed-precision path and of the FP64 path.
+
+One step = one quadrature (xq_integrate: density and gradient, GGA functional, E_xc, N_e and V_xc) over every grid point of
+the workload, with the grid, the screened basis values and the density already on the device.  `value` is grid points per
+second of the mixed-precision path from device time; the FP64 path, the mixed path's E_xc / V_xc error against it, the
+host wall time per call (P upload and V download over PCIe included) and the per-call latency percentiles are reported beside
+it.  The CPU baseline (rank 0 only) times the reference's own C quadrature (compiled from its sources into
+oracle/_ref/libxcref.so), single-threaded, on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload h2o16|h2o64|h2o128] [--cpu-baseline ...]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # name: (waters, cube in Angstrom, seed, functional) -- BASELINE configurations 1, 2 and 4.
+    "h2o16": (16, 9.0, 1, "pbe"),
+    "h2o64": (64, 14.0, 2, "blyp"),
+    "h2o128": (128, 18.0, 4, "pbe"),
+}
+METRIC = "XC grid pts/s mixed vs FP64 at |dExc|<=1e-6 Eh; p50/p99 SCF-request latency"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="h2o64", choices=sorted(WORKLOADS))
+    p.add_argument("--spherical", action="store_true", help="spherical d (reference default); default Cartesian as in BASELINE")
+    p.add_argument("--cpu-baseline", default="reference", choices=["reference", "port", "none"])
+    p.add_argument("--cpu-sample-waters", type=int, default=8)
+    p.add_argument("--skip-fp64", action="store_true")
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_leg(grid, ids, P, precision, steps, warmup, spec, spherical):
+    t0 = time.perf_counter()
+    grid.set_basis(spec, spherical, precision)
+    basis_s = time.perf_counter() - t0
+    V = np.zeros_like(P)
+    for _ in range(warmup):
+        V[:] = 0.0
+        grid.integrate(ids, P, V)
+    dev, rho, fock, wall = [], [], [], []
+    E = N = None
+    for _ in range(steps):
+        V[:] = 0.0
+        t = time.perf_counter()
+        E, N, st = grid.integrate(ids, P, V)
+        wall.append((time.perf_counter() - t) * 1e3)
+        dev.append(st["device_ms"])
+        rho.append(st["rho_ms"])
+        fock.append(st["fock_ms"])
+    return {"E": E, "N": N, "V": V.copy(), "device_ms": np.asarray(dev), "rho_ms": np.asarray(rho),
+            "fock_ms": np.asarray(fock), "wall_ms": np.asarray(wall), "stats": st, "basis_s": basis_s,
+            "device_bytes": grid.device_bytes()}
+
+
+def cpu_baseline(kind, n_waters, functional, spherical):
+    """Single-threaded CPU quadrature of a (H2O)n sample cut from the same generator; points/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from pdynamo3_amd import basis, densities, molecules
+    from pdynamo3_amd.definitions import functional_ids
+    lib = oracle.reference() if kind == "reference" else oracle.oracle()
+    Z, xa, rots, _ = molecules.water_cluster(n_waters, 9.0 * (n_waters / 16.0) ** (1.0 / 3.0), 1)
+    xyz = molecules.to_bohr(xa)
+    spec = basis.element_spec(basis.load_basis(), Z)
+    P = densities.cluster_density(rots, spherical=spherical)
+    h = lib.setup(2, Z, xyz, spec, spherical, functional_ids(functional))
+    # The reference's first call evaluates and caches the basis values (inCore); time the second, as the GPU does.
+    arg = 1 if kind == "reference" else 1
+    h.integrate(P, arg)
+    _, _, _, sec = h.integrate(P, arg)
+    return {"value": h.npoints / sec, "unit": "grid points/s", "cores": 1, "kind": kind,
+            "sample": "(H2O){:d} cluster, same generator, {:s}, Medium grid ({:d} points), one in-core quadrature call, "
+                      "{:.2f} s".format(n_waters, functional.upper(), h.npoints, sec)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the quadrature runs on one "
+              "GPU (the sharding design is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+    from pdynamo3_amd import basis, densities, integrator, molecules
+    from pdynamo3_amd.definitions import functional_ids
+    n_w, cube, seed, functional = WORKLOADS[args.workload]
+    spherical = bool(args.spherical)
+    Z, xa, rots, _ = molecules.water_cluster(n_w, cube, seed)
+    xyz = molecules.to_bohr(xa)
+    b = basis.load_basis("6-31g_st")
+    spec = basis.element_spec(b, Z)
+    n = basis.number_of_functions(b, Z, spherical)
+    P = densities.cluster_density(rots, spherical=spherical)
+    ids = functional_ids(functional)
+    log("workload", args.workload, "atoms", len(Z), "functions", n)
+    t0 = time.perf_counter()
+    grid = integrator.QuadratureGrid(Z, xyz, 2)
+    grid_s = time.perf_counter() - t0
+    G = grid.number_of_points
+    log("grid", G, "points", grid.number_of_blocks, "blocks", "{:.3f} s".format(grid_s))
+    fp64 = None
+    if not args.skip_fp64:
+        fp64 = run_leg(grid, ids, P, "fp64", args.steps, args.warmup, spec, spherical)
+        log("fp64: E", fp64["E"], "N", fp64["N"], "device ms", np.mean(fp64["device_ms"]), "rho", np.mean(fp64["rho_ms"]),
+            "fock", np.mean(fp64["fock_ms"]))
+    mixed = run_leg(grid, ids, P, "mixed", args.steps, args.warmup, spec, spherical)
+    log("mixed: E", mixed["E"], "N", mixed["N"], "device ms", np.mean(mixed["device_ms"]), "rho",
+        np.mean(mixed["rho_ms"]), "fock", np.mean(mixed["fock_ms"]))
+    st = mixed["stats"]
+    ms = float(np.mean(mixed["device_ms"]))
+    value = G / (ms * 1e-3)
+    # Roofline of the dominant kernel (largest mean device time).
+    flops = 2.0 * st["bf_squares"]
+    kernels = {"rho_functional_kernel": float(np.mean(mixed["rho_ms"])), "fock_kernel": float(np.mean(mixed["fock_ms"]))}
+    dom = max(kernels, key=kernels.get)
+    achieved = flops / (kernels[dom] * 1e-3) / 1e12
+    roofline = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "algorithmic": "2 * sum_b nb_b^2 G_b = {:.4g} flop per launch".format(flops),
+                "kernel_ms": kernels}
+    out = {"metric": METRIC, "value": value, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32-storage/fp64-accumulate", "data": "synthetic",
+           "config": {"workload": "(H2O){:d} cluster, {:.0f} A cube, seed {:d}, 6-31G* {:s} ({:d} functions), {:s}, "
+                                  "Medium grid ({:d} points)".format(n_w, cube, seed,
+                                                                     "spherical" if spherical else "Cartesian", n,
+                                                                     functional.upper(), G),
+                      "parallelism": "single GPU"},
+           "grid_points": G, "grid_build_s": grid_s, "basis_eval_s": mixed["basis_s"],
+           "host_wall_ms_per_call_pcie_inclusive": float(np.mean(mixed["wall_ms"])),
+           "latency_ms_p50": float(np.percentile(mixed["wall_ms"], 50)),
+           "latency_ms_p99": float(np.percentile(mixed["wall_ms"], 99)),
+           "E_xc": mixed["E"], "N_e": mixed["N"], "device_bytes": mixed["device_bytes"], "roofline": roofline}
+    if fp64 is not None:
+        ms64 = float(np.mean(fp64["device_ms"]))
+        out["fp64_value"] = G / (ms64 * 1e-3)
+        out["fp64_ms_per_step"] = ms64
+        out["speedup_mixed_vs_fp64"] = ms64 / ms
+        out["dExc_mixed_vs_fp64"] = mixed["E"] - fp64["E"]
+        out["max_dVxc_mixed_vs_fp64"] = float(np.max(np.abs(mixed["V"] - fp64["V"])))
+    if args.cpu_baseline != "none":
+        try:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_baseline, args.cpu_sample_waters, functional, spherical)
+        except Exception as exc:  # the bench line is still printed; the failure is named
+            out["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    print(json.dumps(out), flush=True)
+    grid.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -38,7 +38,7 @@ typedef struct xq_grid xq_grid ;
 /* Precision of the basis-value storage and of the two contractions (X = P Phi, V = Phi W^T + W Phi^T). */
 enum {
     XQ_PRECISION_FP64  = 0 ,  /* FP64 storage, FP64 arithmetic                                                 */
-    XQ_PRECISION_MIXED = 1    /* FP32 storage and FP32 contractions; FP64 weights, functional and reductions     */
+    XQ_PRECISION_MIXED = 1    /* FP32 storage of basis values; FP64 P, accumulation, functional and reductions   */
 } ;
 
 /* Grid accuracy: 0 VeryLow, 1 Low, 2 Medium, 3 High, 4 VeryHigh (DFTGrid.h:16-25). */
@@ -54,9 +54,12 @@ int         xq_estimated_points ( int accuracy, int nAtoms, const int *atomicNum
 int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, int spherical, int precision, int *nFunctions ) ;
 
 /* One quadrature.  ids: libxc functional IDs (101 PBE-x, 106 B88, 130 PBE-c, 131 LYP; pMolecule/QCModel/DFTDefinitions.py).
-!  densityP: packed total density (n(n+1)/2).  fockV: packed, incremented by V_xc.  Statistics (optional, may be NULL):
-!  stats[0] = sum over blocks of kept functions x points ("Quadrature BF Values"), stats[1] = blocks skipped by the density
-!  tolerance, stats[2] = device time of the quadrature kernels in ms. */
+!  densityP: packed total density (n(n+1)/2).  fockV: packed, incremented by V_xc.  Statistics (optional, may be NULL, else
+!  XQ_NSTATS doubles): [0] sum over blocks of kept functions x points ("Quadrature BF Values"), [1] blocks skipped by the
+!  density tolerance, [2] device ms from P expansion to packed V, [3] device ms of the density/functional kernel, [4] device
+!  ms of the Fock kernel, [5] sum over blocks of kept functions^2 x points (the contractions' algorithmic flops are
+!  2 [5] each). */
+# define XQ_NSTATS 8
 int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
                                   double *nElectrons, double *fockV, double *stats ) ;
 

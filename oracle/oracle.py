@@ -139,6 +139,31 @@ class _Handle:
             self.h = None
 
 
+def lebedev_points(n):
+    """Unit vectors and weights of the oracle's Lebedev-Laikov rule with n points ([n, 3], [n])."""
+    lib = oracle().lib
+    fn = lib.xqo_lebedev_points
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+    x, y, z, w = (np.zeros(n) for _ in range(4))
+    m = fn(n, _d(x), _d(y), _d(z), _d(w))
+    if m != n:
+        raise ValueError("no Lebedev rule with {:d} points".format(n))
+    return np.stack([x, y, z], axis=1), w
+
+
+def basis_at(handle, xyz):
+    """All basis functions and gradients of an oracle handle at arbitrary points xyz [K, 3]: four [nF, K] arrays."""
+    fn = handle.lib.lib.xqo_basis_at
+    fn.restype = None
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    k, n = xyz.shape[0], handle.nfunctions
+    out = [np.zeros(n * k) for _ in range(4)]
+    fn(handle.h, k, _d(xyz), *[_d(o) for o in out])
+    return [o.reshape(n, k) for o in out]
+
+
 _oracle = None
 _reference = None
 

@@ -883,6 +883,16 @@ int xqo_integrate ( void *vh, const double *Ppacked, int nThreads, double *eXC, 
     return 0 ;
 }
 
+/* All functions and their gradients at n arbitrary points xyz[n x 3] (test access); outputs nFunctions x n row-major. */
+void xqo_basis_at ( void *vh, int n, const double *xyz, double *f, double *fx, double *fy, double *fz )
+{
+    OHandle *h = ( OHandle * ) vh ;
+    double *pts = ( double * ) malloc ( 4 * ( size_t ) ( n > 0 ? n : 1 ) * sizeof ( double ) ) ;
+    for ( int p = 0 ; p < n ; p++ ) { pts[4*p] = xyz[3*p] ; pts[4*p+1] = xyz[3*p+1] ; pts[4*p+2] = xyz[3*p+2] ; pts[4*p+3] = 0.0 ; }
+    EvaluateBasis ( h, n, pts, f, fx, fy, fz ) ;
+    free ( pts ) ;
+}
+
 void xqo_free ( void *vh )
 {
     OHandle *h = ( OHandle * ) vh ;

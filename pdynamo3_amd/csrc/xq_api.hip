@@ -48,6 +48,9 @@ int xq_grid_build(int accuracy, int nAtoms, const int *atomicNumbers, const doub
     int status = check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (status == kOK) status = check(hipEventCreate(&g->ev0), "hipEventCreate");
     if (status == kOK) status = check(hipEventCreate(&g->ev1), "hipEventCreate");
+    if (status == kOK) status = check(hipEventCreate(&g->evRho0), "hipEventCreate");
+    if (status == kOK) status = check(hipEventCreate(&g->evRho1), "hipEventCreate");
+    if (status == kOK) status = check(hipEventCreate(&g->evFock1), "hipEventCreate");
     if (status == kOK) status = build_grid(g);
     if (status != kOK) { xq_free(g); return status; }
     *grid = g;
@@ -143,6 +146,9 @@ void xq_free(xq_grid *grid) {
     dev_free(&grid->d_blocks);
     if (grid->ev0) (void)hipEventDestroy(grid->ev0);
     if (grid->ev1) (void)hipEventDestroy(grid->ev1);
+    if (grid->evRho0) (void)hipEventDestroy(grid->evRho0);
+    if (grid->evRho1) (void)hipEventDestroy(grid->evRho1);
+    if (grid->evFock1) (void)hipEventDestroy(grid->evFock1);
     if (grid->stream) (void)hipStreamDestroy(grid->stream);
     delete grid;
 }

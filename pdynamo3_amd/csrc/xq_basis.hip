@@ -460,8 +460,9 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
                             "copy counts")) != kOK) break;
         if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
         long long rows = 0;
-        double bfv = 0.0;
+        double bfv = 0.0, bfs = 0.0;
         for (int b = 0; b < g->nBlocks; b++) {
+            bfs += static_cast<double>(h_counts[b]) * h_counts[b] * g->blocks[b].nPoints;
             g->blocks[b].nKept = h_counts[b];
             g->blocks[b].vOffset = rows;
             g->blocks[b].iOffset = static_cast<int>(rows);
@@ -470,6 +471,7 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
         }
         g->nValueRows = rows;
         g->bfValues = bfv;
+        g->bfSquares = bfs;
         if ((status = check(hipMemcpyAsync(g->d_blocks, g->blocks.data(), g->nBlocks * sizeof(DevBlock), hipMemcpyHostToDevice, s),
                             "copy blocks")) != kOK) break;
         const size_t nVal = static_cast<size_t>(rows > 0 ? rows : 1) * kBlockSize;

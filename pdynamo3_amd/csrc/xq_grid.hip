@@ -21,6 +21,10 @@
 #include "lebedev_tables.h"
 #include "xq_internal.h"
 
+// No fused multiply-add in this file: the cull |w| > 1e-30 depends on Becke cell factors that cancel to exactly zero near
+// nuclei, so point coordinates, distances and the cell polynomial are rounded operation by operation as in the reference.
+#pragma clang fp contract(off)
+
 namespace xq {
 
 namespace {

@@ -61,13 +61,6 @@ __global__ void zero_kernel(size_t n, double *__restrict__ a) {
         a[e] = 0.0;
 }
 
-template <typename T>
-__device__ __forceinline__ T ld_p(const double *P, const float *P32, size_t o);
-template <>
-__device__ __forceinline__ double ld_p<double>(const double *P, const float *, size_t o) { return P[o]; }
-template <>
-__device__ __forceinline__ float ld_p<float>(const double *, const float *P32, size_t o) { return P32[o]; }
-
 // 256 threads: ty = t / 16 owns X rows ty + 16 i (i < 4), tx = t % 16 owns points tx + 16 j (j < 8).
 template <typename T, typename A>
 __global__ void __launch_bounds__(256) rho_functional_kernel(
@@ -105,7 +98,7 @@ __global__ void __launch_bounds__(256) rho_functional_kernel(
                 const int r = e / kKC, c = e % kKC;
                 A v = A(0);
                 if (I + r < nb && K + c < nb)
-                    v = static_cast<A>(ld_p<T>(P, P32, static_cast<size_t>(idx[I + r]) * n + idx[K + c]));
+                    v = static_cast<A>(P[static_cast<size_t>(idx[I + r]) * n + idx[K + c]]);
                 Ps[c][r] = v;
             }
             // Phi rows K.. (32 x 128).
@@ -311,11 +304,14 @@ void launch_quadrature(xq_grid *g, const FunctionalSet &fs) {
     hipStream_t s = g->stream;
     const T *f = static_cast<const T *>(g->d_f), *fx = static_cast<const T *>(g->d_fx),
             *fy = static_cast<const T *>(g->d_fy), *fz = static_cast<const T *>(g->d_fz);
+    (void)hipEventRecord(g->evRho0, s);
     hipLaunchKernelGGL((rho_functional_kernel<T, A>), dim3(g->nBlocks), dim3(256), 0, s, g->d_blocks, g->d_idx, f, fx, fy,
                        fz, g->nFunctions, g->d_P, g->d_P32, g->d_w, fs, g->rhoTol, g->d_rho, g->d_vrs, g->d_blockEN,
                        g->d_skip);
+    (void)hipEventRecord(g->evRho1, s);
     hipLaunchKernelGGL((fock_kernel<T, A>), dim3(g->nBlocks), dim3(256), 0, s, g->d_blocks, g->d_idx, f, fx, fy, fz,
                        g->nFunctions, g->d_rho, g->d_vrs, g->d_skip, g->d_V);
+    (void)hipEventRecord(g->evFock1, s);
 }
 
 }  // namespace
@@ -337,7 +333,6 @@ static int ensure_work(xq_grid *g) {
     XQ_TRY(dev_alloc(g, &g->d_P, n * n));
     XQ_TRY(dev_alloc(g, &g->d_V, n * n));
     XQ_TRY(dev_alloc(g, &g->d_packed, n * (n + 1) / 2));
-    if (g->precision == XQ_PRECISION_MIXED) XQ_TRY(dev_alloc(g, &g->d_P32, n * n));
     XQ_TRY(dev_alloc(g, &g->d_rho, 4 * static_cast<size_t>(g->nPoints)));
     XQ_TRY(dev_alloc(g, &g->d_vrs, 2 * static_cast<size_t>(g->nPoints)));
     XQ_TRY(dev_alloc(g, &g->d_blockEN, 2 * static_cast<size_t>(g->nBlocks)));
@@ -371,7 +366,7 @@ int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
     XQ_CHECK(hipGetLastError());
     if (g->nBlocks > 0) {
         if (g->precision == XQ_PRECISION_FP64) launch_quadrature<double, double>(g, fs);
-        else launch_quadrature<float, float>(g, fs);
+        else launch_quadrature<float, double>(g, fs);
         XQ_CHECK(hipGetLastError());
     }
     const int pgrid = static_cast<int>(std::min<size_t>((nPacked + 255) / 256, 16384));
@@ -397,11 +392,18 @@ int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
     *eXC = e;
     *nE = ne;
     if (stats != nullptr) {
-        float ms = 0.0f;
+        float ms = 0.0f, msRho = 0.0f, msFock = 0.0f;
         (void)hipEventElapsedTime(&ms, g->ev0, g->ev1);
+        if (g->nBlocks > 0) {
+            (void)hipEventElapsedTime(&msRho, g->evRho0, g->evRho1);
+            (void)hipEventElapsedTime(&msFock, g->evRho1, g->evFock1);
+        }
         stats[0] = g->bfValues;
         stats[1] = static_cast<double>(skipped);
         stats[2] = static_cast<double>(ms);
+        stats[3] = static_cast<double>(msRho);
+        stats[4] = static_cast<double>(msFock);
+        stats[5] = g->bfSquares;
     }
     return kOK;
 }

@@ -58,7 +58,7 @@ struct xq_grid {
     std::vector<double> xyz;
     double bfTol = 1.0e-10, rhoTol = 1.0e-15;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evRho0 = nullptr, evRho1 = nullptr, evFock1 = nullptr;
 
     // Grid.
     int nPoints = 0, nBlocks = 0;
@@ -76,6 +76,7 @@ struct xq_grid {
     int *d_idx = nullptr;                 // kept-function indices, per block
     void *d_f = nullptr, *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;   // values and gradients, rows of kBlockSize
     double bfValues = 0.0;                // sum over blocks of kept functions x points
+    double bfSquares = 0.0;               // sum over blocks of kept functions^2 x points
 
     // Work arrays for a quadrature.
     double *d_P = nullptr, *d_V = nullptr, *d_packed = nullptr;   // dense n x n P and V, packed staging

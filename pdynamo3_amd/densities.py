@@ -1,0 +1,131 @@
+"""Synthetic density matrices for water clusters: rotated copies of a water monomer density, block-diagonal over waters.
+
+A water cluster's density is taken as the superposition of rigidly rotated monomer densities.  For water k with rotation R_k
+(atoms at R_k a + c_k), the monomer density rho(y) = sum_mn P_mn phi_m(y) phi_n(y) becomes rho_k(r) = rho(R_k^T (r - c_k)),
+whose matrix in the cluster basis is P_k = D_k^T P D_k, where D_k acts on each shell's angular functions:
+A(R_k^T u) = D_k A(u).  D_k is fitted on random unit vectors (the angular functions of one shell span an invariant space, so
+the fit is exact to rounding).
+
+The monomer density itself is a committed fixture (tests/golden/water_density_<basis>_<sph|cart>.json).
+"""
+
+import json
+import math
+import os
+
+import numpy as np
+
+from .basis import load_basis
+
+_FIXTURES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden")
+
+
+def _cart_powers(l):
+    return [(l - y - z, y, z) for z in range(l + 1) for y in range(l - z + 1)]
+
+
+def _odd_fact(n):
+    f = 1.0
+    for i in range(3, n + 1, 2):
+        f *= i
+    return f
+
+
+def c2s(l):
+    """Normalised Cartesian -> real solid harmonic transformation for one l ((l+1)(l+2)/2 x (2l+1)), spherical order
+    m = 0, 1c, 1s, 2c, 2s, ... (Schlegel & Frisch, Int. J. Quantum Chem. 54 (1995) 83, eq. 15)."""
+    F = math.factorial
+    pw = _cart_powers(l)
+    out = np.zeros((len(pw), 2 * l + 1))
+    for c, (x, y, z) in enumerate(pw):
+        for m in range(l + 1):
+            jj = l - m - z
+            if jj < 0 or jj % 2:
+                continue
+            j = jj // 2
+            s0 = sc = ss = 0.0
+            for i in range((l - m) // 2 + 1):
+                if j > i:
+                    continue
+                t = math.comb(l, i) * math.comb(i, j) * (-1) ** i * F(2 * l - 2 * i) / F(l - m - 2 * i)
+                for k in range(j + 1):
+                    p = x - 2 * k
+                    if p < 0 or p > m:
+                        continue
+                    u = t * math.comb(j, k) * math.comb(m, p)
+                    q = m - p
+                    sign = -1.0 if (q // 2) % 2 else 1.0
+                    if m == 0:
+                        s0 += u
+                    elif q % 2 == 0:
+                        sc += math.sqrt(2.0) * sign * u
+                    else:
+                        ss += math.sqrt(2.0) * sign * u
+            norm = math.sqrt(F(2 * x) * F(2 * y) * F(2 * z) * F(l) / (F(2 * l) * F(x) * F(y) * F(z))) * \
+                math.sqrt(F(l - m) / F(l + m)) / (2 ** l * F(l))
+            if m == 0:
+                out[c, 0] = norm * s0
+            else:
+                out[c, 2 * m - 1] = norm * sc
+                out[c, 2 * m] = norm * ss
+    return out
+
+
+def _angular(l, u, spherical):
+    """Angular functions of one l at unit vectors u [3, K]: normalised Cartesians (factor sqrt((2l-1)!!/prod (2x-1)!!)),
+    or real solid harmonics."""
+    rows = []
+    for (x, y, z) in _cart_powers(l):
+        m = math.sqrt(_odd_fact(2 * l - 1) / (_odd_fact(2 * x - 1) * _odd_fact(2 * y - 1) * _odd_fact(2 * z - 1)))
+        rows.append(m * u[0] ** x * u[1] ** y * u[2] ** z)
+    A = np.asarray(rows)
+    return c2s(l).T @ A if spherical else A
+
+
+def shell_rotation(l, R, spherical, samples=64, seed=0):
+    """D with A(R^T u) = D A(u) for the functions of angular momentum l."""
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(3, samples))
+    u /= np.linalg.norm(u, axis=0)
+    A = _angular(l, u, spherical)
+    B = _angular(l, R.T @ u, spherical)
+    return B @ np.linalg.pinv(A)
+
+
+def monomer_rotation(R, basis, spherical, atomic_numbers=(8, 1, 1)):
+    """Block-diagonal D over the monomer's functions (atom order O, H, H)."""
+    blocks = []
+    for z in atomic_numbers:
+        for shell in basis["elements"][str(z)]["shells"]:
+            for l in range(shell["lLow"], shell["lHigh"] + 1):
+                blocks.append(shell_rotation(l, R, spherical))
+    n = sum(b.shape[0] for b in blocks)
+    D = np.zeros((n, n))
+    o = 0
+    for b in blocks:
+        D[o:o + b.shape[0], o:o + b.shape[0]] = b
+        o += b.shape[0]
+    return D
+
+
+def load_monomer_density(label="6-31g_st", spherical=True, path=None):
+    """The committed monomer density (dense n x n) and its description."""
+    if path is None:
+        path = os.path.join(_FIXTURES, "water_density_{:s}_{:s}.json".format(label, "sph" if spherical else "cart"))
+    with open(path) as f:
+        data = json.load(f)
+    return np.asarray(data["P"], dtype=np.float64), data
+
+
+def cluster_density(rotations, label="6-31g_st", spherical=True):
+    """Packed (lower triangle) density of a water cluster from the monomer density and the per-water rotations."""
+    basis = load_basis(label)
+    P1, _ = load_monomer_density(label, spherical)
+    m = P1.shape[0]
+    n = m * len(rotations)
+    P = np.zeros((n, n))
+    for k, R in enumerate(rotations):
+        D = monomer_rotation(np.asarray(R), basis, spherical)
+        P[k * m:(k + 1) * m, k * m:(k + 1) * m] = D.T @ P1 @ D
+    P = 0.5 * (P + P.T)
+    return np.ascontiguousarray(P[np.tril_indices(n)])

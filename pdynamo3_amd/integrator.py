@@ -70,10 +70,11 @@ class QuadratureGrid:
         if V.dtype != np.float64 or not V.flags.c_contiguous or V.size != P.size:
             raise QCModelError("Fock matrix must be a contiguous float64 array of the density's size.")
         e, ne = ctypes.c_double(), ctypes.c_double()
-        stats = np.zeros(3)
+        stats = np.zeros(8)
         _check(_lib.lib.xq_integrate(self._h, ids.size, _lib.iptr(ids), _lib.dptr(P), ctypes.byref(e), ctypes.byref(ne),
                                      _lib.dptr(V), _lib.dptr(stats)), "integrate")
-        return e.value, ne.value, {"bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2]}
+        return e.value, ne.value, {"bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2],
+                                   "rho_ms": stats[3], "fock_ms": stats[4], "bf_squares": stats[5]}
 
     def device_bytes(self):
         b = ctypes.c_longlong()

@@ -1,0 +1,132 @@
+"""Writes the golden vectors of tests/golden/ from the compiled reference path (oracle/_ref/libxcref.so).
+
+The reference's C sources are compiled where they lie under the reference tree by `make -C oracle ref`; this script calls
+them through oracle/ref_driver.c and stores inputs and outputs only (numbers, no reference code):
+
+  reference_water.json      water: grid size, block structure, (x, y, z, w) checksums, the first 64 grid points; per
+                            basis flavour (spherical / Cartesian) and functional (BLYP / PBE): E_xc, N_e and packed V_xc for
+                            the EHT monomer density, and the kept functions and values of three blocks.
+  reference_cluster.json    (H2O)4 (6 A cube, seed 5): grid size and checksums, E_xc, N_e and packed V_xc (PBE, spherical);
+                            (H2O)16 (9 A cube, seed 1, the benchmark cluster): grid size, E_xc and N_e (PBE, Cartesian).
+  reference_functionals.json  libxc-1.2.0 (as vendored) eps, vrho, vsigma of BLYP and PBE on bulk and threshold (rho, sigma).
+  reference_c2s.json        the reference's own Cartesian -> spherical fixture (examples/pMolecule/data/yaml/
+                            C2STransformations_0to6.yaml, read with SafeLoader) converted to JSON.
+
+Run: make -C oracle ref && python tests/golden/make_golden.py
+"""
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle  # noqa: E402
+from pdynamo3_amd import basis, densities, molecules  # noqa: E402
+
+REFERENCE = os.environ.get("PDYNAMO3_REFERENCE", "/root/reference")
+BLYP, PBE = [106, 131], [101, 130]
+
+
+def checksums(xyzw):
+    return {"sum_x": float(np.sum(xyzw[:, 0])), "sum_y": float(np.sum(xyzw[:, 1])), "sum_z": float(np.sum(xyzw[:, 2])),
+            "sum_w": float(np.sum(xyzw[:, 3])), "sum_abs_w": float(np.sum(np.abs(xyzw[:, 3]))),
+            "sum_w_r2": float(np.sum(xyzw[:, 3] * np.sum(xyzw[:, :3] ** 2, axis=1)))}
+
+
+def water():
+    ref = oracle.reference()
+    b = basis.load_basis()
+    Z, xyz = molecules.water()
+    spec = basis.element_spec(b, Z)
+    out = {"Z": Z.tolist(), "xyz_bohr": xyz.tolist(), "accuracy": 2, "cases": []}
+    h = ref.setup(2, Z, xyz, spec, True, BLYP)
+    xyzw, ba, bc = h.grid()
+    out["npoints"] = h.npoints
+    out["nblocks"] = h.nblocks
+    out["block_atom"] = ba.tolist()
+    out["block_count"] = bc.tolist()
+    out["checksums"] = checksums(xyzw)
+    out["first_points"] = xyzw[:64].tolist()
+    for spherical in (True, False):
+        P1, _ = densities.load_monomer_density(spherical=spherical)
+        P = P1[np.tril_indices(P1.shape[0])]
+        for name, ids in (("blyp", BLYP), ("pbe", PBE)):
+            h = ref.setup(2, Z, xyz, spec, spherical, ids)
+            e, ne, V, _ = h.integrate(P, 0)
+            case = {"spherical": spherical, "functional": name, "E_xc": e, "N_e": ne, "V": V.tolist(), "blocks": []}
+            if name == "blyp":
+                for blk in (0, 40, 200):
+                    idx, f, fx, fy, fz = h.block_values(blk)
+                    case["blocks"].append({"block": blk, "indices": idx.tolist(), "f": f.tolist(), "fx": fx.tolist(),
+                                           "fy": fy.tolist(), "fz": fz.tolist()})
+            out["cases"].append(case)
+    with open(os.path.join(HERE, "reference_water.json"), "w") as f:
+        json.dump(out, f)
+
+
+def cluster():
+    ref = oracle.reference()
+    b = basis.load_basis()
+    out = {}
+    Z, xa, rots, _ = molecules.water_cluster(4, 6.0, 5)
+    xyz = molecules.to_bohr(xa)
+    spec = basis.element_spec(b, Z)
+    P = densities.cluster_density(rots, spherical=True)
+    h = ref.setup(2, Z, xyz, spec, True, PBE)
+    xyzw, ba, bc = h.grid()
+    e, ne, V, _ = h.integrate(P, 0)
+    out["h2o4"] = {"waters": 4, "cube": 6.0, "seed": 5, "spherical": True, "functional": "pbe", "npoints": h.npoints,
+                   "nblocks": h.nblocks, "checksums": checksums(xyzw), "E_xc": e, "N_e": ne, "V": V.tolist()}
+    print("h2o4", h.npoints, e, ne, flush=True)
+    Z, xa, rots, _ = molecules.water_cluster(16, 9.0, 1)
+    xyz = molecules.to_bohr(xa)
+    spec = basis.element_spec(b, Z)
+    P = densities.cluster_density(rots, spherical=False)
+    h = ref.setup(2, Z, xyz, spec, False, PBE)
+    xyzw, _, _ = h.grid()
+    e, ne, V, sec = h.integrate(P, 0)
+    out["h2o16"] = {"waters": 16, "cube": 9.0, "seed": 1, "spherical": False, "functional": "pbe", "npoints": h.npoints,
+                    "nblocks": h.nblocks, "checksums": checksums(xyzw), "E_xc": e, "N_e": ne,
+                    "V_norm": float(np.linalg.norm(V)), "V_trace_like": float(np.sum(V)), "seconds": sec}
+    print("h2o16", h.npoints, e, ne, sec, flush=True)
+    with open(os.path.join(HERE, "reference_cluster.json"), "w") as f:
+        json.dump(out, f)
+
+
+def functionals():
+    ref = oracle.reference()
+    rho = [5.0e-13, 4.9e-13, 1.0e-12, 1.0e-10, 1.0e-6, 1.0e-3, 0.1, 1.0, 10.0, 300.0, 1.0e-3, 1.0, 1.0, 0.5, 2.0e-5]
+    sigma = [0.0, 1.0e-20, 1.0e-30, 1.0e-18, 1.0e-9, 1.0e-5, 0.01, 1.0, 100.0, 1.0e5, 0.0, 2.5e-25, 1.0e-26, 7.3, 4.0e-8]
+    out = {"rho": rho, "sigma": sigma}
+    for name, ids in (("blyp", BLYP), ("pbe", PBE), ("b88", [106]), ("lyp", [131]), ("pbex", [101]), ("pbec", [130])):
+        e, vr, vs = ref.functional(ids, np.asarray(rho), np.asarray(sigma))
+        out[name] = {"ids": ids, "eps": e.tolist(), "vrho": vr.tolist(), "vsigma": vs.tolist()}
+    with open(os.path.join(HERE, "reference_functionals.json"), "w") as f:
+        json.dump(out, f)
+
+
+def c2s():
+    import yaml
+    path = os.path.join(REFERENCE, "examples", "pMolecule", "data", "yaml", "C2STransformations_0to6.yaml")
+    with open(path) as f:
+        data = yaml.load(f, Loader=yaml.SafeLoader)
+    out = {"source": "examples/pMolecule/data/yaml/C2STransformations_0to6.yaml"}
+    ref = oracle.reference()
+    for l in range(0, 7):
+        out["reference_driver_l{:d}".format(l)] = ref.c2s(l, l).tolist()
+    out["yaml"] = json.loads(json.dumps(data, default=str))
+    with open(os.path.join(HERE, "reference_c2s.json"), "w") as f:
+        json.dump(out, f)
+
+
+if __name__ == "__main__":
+    functionals()
+    c2s()
+    water()
+    cluster()

@@ -83,3 +83,54 @@ def test_cluster4_matches_oracle(oracle_lib, precision):
     print("dE", e - e0, "dN", ne - n0, "max dV", np.max(np.abs(V - V0)), stats)
     assert abs(e - e0) <= tolE
     assert np.max(np.abs(V - V0)) <= tolV
+
+
+def _golden(name):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)) as f:
+        return json.load(f)
+
+
+def _cluster(waters, cube, seed, spherical):
+    from pdynamo3_amd import basis, densities, molecules
+    Z, xa, rots, _ = molecules.water_cluster(waters, cube, seed)
+    xyz = molecules.to_bohr(xa)
+    spec = basis.element_spec(basis.load_basis(), Z)
+    return Z, xyz, spec, densities.cluster_density(rots, spherical=spherical)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_h2o16_matches_compiled_reference(precision):
+    """The benchmark cluster (H2O)16 (PBE, Cartesian, 304 functions) against the reference C quadrature's E_xc and N_e."""
+    I = _integrator()
+    c = _golden("reference_cluster.json")["h2o16"]
+    Z, xyz, spec, P = _cluster(16, 9.0, 1, False)
+    g = I.QuadratureGrid(Z, xyz, 2)
+    assert g.number_of_points == c["npoints"]
+    assert g.number_of_blocks == c["nblocks"]
+    g.set_basis(spec, False, precision)
+    V = np.zeros_like(P)
+    e, ne, _ = g.integrate(ids("pbe"), P, V)
+    tol = 1e-9 if precision == "fp64" else 1e-6
+    print(precision, "dE", e - c["E_xc"], "dN", ne - c["N_e"], "dVnorm", np.linalg.norm(V) - c["V_norm"])
+    assert abs(e - c["E_xc"]) <= tol
+    assert abs(ne - c["N_e"]) <= tol
+    assert abs(np.linalg.norm(V) - c["V_norm"]) <= (1e-9 if precision == "fp64" else 1e-5)
+
+
+def test_h2o64_mixed_against_fp64():
+    """Mixed-precision quadrature of (H2O)64 (BLYP, Cartesian): |dE_xc| <= 1e-6 Eh and max|dV| <= 1e-7 against FP64."""
+    I = _integrator()
+    Z, xyz, spec, P = _cluster(64, 14.0, 2, False)
+    g = I.QuadratureGrid(Z, xyz, 2)
+    g.set_basis(spec, False, "fp64")
+    V0 = np.zeros_like(P)
+    e0, n0, _ = g.integrate(ids("blyp"), P, V0)
+    g.set_basis(spec, False, "mixed")
+    V = np.zeros_like(P)
+    e, ne, _ = g.integrate(ids("blyp"), P, V)
+    dv = float(np.max(np.abs(V - V0)))
+    print("dE", e - e0, "dN", ne - n0, "max dV", dv)
+    assert abs(e - e0) <= 1e-6
+    assert dv <= 1e-7

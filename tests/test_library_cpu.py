@@ -1,0 +1,68 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point of include/xcquad.h, and answers the host-only
+calls (estimated points, memory estimate, argument errors with the reference's Status codes)."""
+
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "xcquad.h")).read()
+    return sorted(set(re.findall(r"\b(xq_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pdynamo3_amd import _lib
+    names = declared_symbols()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(_lib.lib, name), name
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_estimated_points_match_reference_count():
+    from pdynamo3_amd import integrator
+    # DFTGrid_EstimatedPoints for water on the Medium grid: O 13,450 + 2 x H 14,318 (no weight cull).
+    assert integrator.estimated_points(2, [8, 1, 1]) == 42086
+    assert integrator.estimated_points(2, [8, 1, 1] * 16) == 16 * 42086
+
+
+def test_argument_errors_return_status_codes():
+    from pdynamo3_amd import _lib
+    h = ctypes.c_void_p()
+    z = np.array([8, 1, 1], dtype=np.int32)
+    x = np.zeros(9)
+    assert _lib.lib.xq_grid_build(7, 3, _lib.iptr(z), _lib.dptr(x), ctypes.byref(h)) == 3
+    assert "invalid" in _lib.last_error()
+    assert _lib.lib.xq_grid_build(2, 0, _lib.iptr(z), _lib.dptr(x), ctypes.byref(h)) == 3
+    bad = np.array([8, 0, 1], dtype=np.int32)
+    assert _lib.lib.xq_grid_build(2, 3, _lib.iptr(bad), _lib.dptr(x), ctypes.byref(h)) == 3
+    nanx = x.copy()
+    nanx[4] = np.nan
+    assert _lib.lib.xq_grid_build(2, 3, _lib.iptr(z), _lib.dptr(nanx), ctypes.byref(h)) == 3
+    e, n = ctypes.c_double(1.0), ctypes.c_double(1.0)
+    ids = np.array([106, 131], dtype=np.int32)
+    assert _lib.lib.xq_integrate(None, 2, _lib.iptr(ids), _lib.dptr(x), ctypes.byref(e), ctypes.byref(n),
+                                 _lib.dptr(x), None) == 3
+
+
+def test_memory_estimate_scales():
+    from pdynamo3_amd import integrator
+    small = integrator.estimate_bytes(42086, 19, 19.0, "fp64")
+    big = integrator.estimate_bytes(16 * 42086, 304, 200.0, "fp64")
+    mixed = integrator.estimate_bytes(16 * 42086, 304, 200.0, "mixed")
+    assert 0 < small < big
+    assert mixed < big
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
