@@ -131,7 +131,7 @@ def main():
     value = G / (ms * 1e-3)
     # Roofline of the dominant kernel (largest mean device time).
     flops = 2.0 * st["bf_squares"]
-    kernels = {"rho_functional_kernel": float(np.mean(mixed["rho_ms"])), "fock_kernel": float(np.mean(mixed["fock_ms"]))}
+    kernels = {"density_kernel": float(np.mean(mixed["rho_ms"])), "fock_kernel": float(np.mean(mixed["fock_ms"]))}
     dom = max(kernels, key=kernels.get)
     achieved = flops / (kernels[dom] * 1e-3) / 1e12
     roofline = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",

@@ -38,7 +38,7 @@ typedef struct xq_grid xq_grid ;
 /* Precision of the basis-value storage and of the two contractions (X = P Phi, V = Phi W^T + W Phi^T). */
 enum {
     XQ_PRECISION_FP64  = 0 ,  /* FP64 storage, FP64 arithmetic                                                 */
-    XQ_PRECISION_MIXED = 1    /* FP32 storage of basis values; FP64 P, accumulation, functional and reductions   */
+    XQ_PRECISION_MIXED = 1    /* int8-digit basis values and P on int8 MFMA, exact int32 sums; FP64 functional   */
 } ;
 
 /* Grid accuracy: 0 VeryLow, 1 Low, 2 Medium, 3 High, 4 VeryHigh (DFTGrid.h:16-25). */

@@ -312,6 +312,7 @@ __global__ void __launch_bounds__(kBlockSize) basis_eval_kernel(
 }  // namespace
 
 void free_basis(xq_grid *g) {
+    free_mixed(g);
     dev_free(&g->d_shells);
     dev_free(&g->d_coef);
     dev_free(&g->d_trans);
@@ -475,7 +476,7 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
         if ((status = check(hipMemcpyAsync(g->d_blocks, g->blocks.data(), g->nBlocks * sizeof(DevBlock), hipMemcpyHostToDevice, s),
                             "copy blocks")) != kOK) break;
         const size_t nVal = static_cast<size_t>(rows > 0 ? rows : 1) * kBlockSize;
-        const size_t elt = precision == XQ_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+        const size_t elt = sizeof(double);
         if ((status = dev_alloc(g, &g->d_idx, static_cast<size_t>(rows > 0 ? rows : 1))) != kOK) break;
         void **arrays[4] = {&g->d_f, &g->d_fx, &g->d_fy, &g->d_fz};
         for (void **a : arrays) {
@@ -485,17 +486,22 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
         }
         if (status != kOK) break;
         if (g->nBlocks > 0) {
-            if (precision == XQ_PRECISION_FP64)
-                hipLaunchKernelGGL(basis_eval_kernel<double>, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, g->d_x,
-                                   g->d_y, g->d_z, g->d_shells, d_shellOf, g->d_coef, g->d_trans, g->spherical, f, tmpIdx,
-                                   g->d_idx, static_cast<double *>(g->d_f), static_cast<double *>(g->d_fx),
-                                   static_cast<double *>(g->d_fy), static_cast<double *>(g->d_fz));
-            else
-                hipLaunchKernelGGL(basis_eval_kernel<float>, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, g->d_x,
-                                   g->d_y, g->d_z, g->d_shells, d_shellOf, g->d_coef, g->d_trans, g->spherical, f, tmpIdx,
-                                   g->d_idx, static_cast<float *>(g->d_f), static_cast<float *>(g->d_fx),
-                                   static_cast<float *>(g->d_fy), static_cast<float *>(g->d_fz));
+            hipLaunchKernelGGL(basis_eval_kernel<double>, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, g->d_x,
+                               g->d_y, g->d_z, g->d_shells, d_shellOf, g->d_coef, g->d_trans, g->spherical, f, tmpIdx,
+                               g->d_idx, static_cast<double *>(g->d_f), static_cast<double *>(g->d_fx),
+                               static_cast<double *>(g->d_fy), static_cast<double *>(g->d_fz));
             if ((status = check(hipGetLastError(), "basis_eval_kernel")) != kOK) break;
+        }
+        if (precision == XQ_PRECISION_MIXED) {
+            // Digit planes and FP32 gradients replace the FP64 values.
+            if ((status = prepare_mixed(g, static_cast<double *>(g->d_f), static_cast<double *>(g->d_fx),
+                                        static_cast<double *>(g->d_fy), static_cast<double *>(g->d_fz))) != kOK) break;
+            if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
+            for (void **a : arrays) {
+                (void)hipFree(*a);
+                *a = nullptr;
+                g->deviceBytes -= static_cast<long long>(nVal * elt);
+            }
         }
         status = check(hipStreamSynchronize(s), "basis sync");
     } while (false);

@@ -316,6 +316,9 @@ void launch_quadrature(xq_grid *g, const FunctionalSet &fs) {
 
 }  // namespace
 
+void launch_fused_f64(xq_grid *g, const FunctionalSet &fs);
+int launch_fused_mixed(xq_grid *g, const FunctionalSet &fs);
+
 void free_work(xq_grid *g) {
     dev_free(&g->d_P);
     dev_free(&g->d_P32);
@@ -365,8 +368,14 @@ int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
     hipLaunchKernelGGL(zero_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, n * n, g->d_V);
     XQ_CHECK(hipGetLastError());
     if (g->nBlocks > 0) {
-        if (g->precision == XQ_PRECISION_FP64) launch_quadrature<double, double>(g, fs);
-        else launch_quadrature<float, double>(g, fs);
+        if (g->precision == XQ_PRECISION_FP64) {
+            (void)hipEventRecord(g->evRho0, s);
+            launch_fused_f64(g, fs);
+            (void)hipEventRecord(g->evRho1, s);
+            (void)hipEventRecord(g->evFock1, s);
+        } else {
+            XQ_TRY(launch_fused_mixed(g, fs));
+        }
         XQ_CHECK(hipGetLastError());
     }
     const int pgrid = static_cast<int>(std::min<size_t>((nPacked + 255) / 256, 16384));

@@ -75,6 +75,13 @@ struct xq_grid {
     long long nValueRows = 0;             // sum over blocks of kept functions
     int *d_idx = nullptr;                 // kept-function indices, per block
     void *d_f = nullptr, *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;   // values and gradients, rows of kBlockSize
+    // Mixed precision: digit planes of Phi (two scalings), exponents, FP32 gradients, P row exponents.
+    int *d_chunkOffset = nullptr;
+    signed char *d_D1 = nullptr, *d_E1 = nullptr, *d_D2 = nullptr, *d_E2 = nullptr;
+    float *d_G = nullptr;
+    signed char *d_DW = nullptr, *d_EW = nullptr;   // W digits per step (same layout as D1, E1)
+    int *d_eP = nullptr;
+    long long nChunks = 0;
     double bfValues = 0.0;                // sum over blocks of kept functions x points
     double bfSquares = 0.0;               // sum over blocks of kept functions^2 x points
 
@@ -123,6 +130,8 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
 int integrate(xq_grid *g, int nIDs, const int *ids, const double *P, double *eXC, double *nE, double *V, double *stats);
 void free_basis(xq_grid *g);
 void free_work(xq_grid *g);
+int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz);
+void free_mixed(xq_grid *g);
 
 }  // namespace xq
 
