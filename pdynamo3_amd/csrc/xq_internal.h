@@ -75,21 +75,22 @@ struct xq_grid {
     long long nValueRows = 0;             // sum over blocks of kept functions
     int *d_idx = nullptr;                 // kept-function indices, per block
     void *d_f = nullptr, *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;   // values and gradients, rows of kBlockSize
-    // Mixed precision: digit planes of Phi (two scalings), exponents, FP32 gradients, P row exponents.
-    int *d_chunkOffset = nullptr;
-    signed char *d_D1 = nullptr, *d_E1 = nullptr, *d_D2 = nullptr, *d_E2 = nullptr;
-    float *d_G = nullptr;
-    signed char *d_DW = nullptr, *d_EW = nullptr;   // W digits per step (same layout as D1, E1)
+    // Mixed precision (layouts in xq_mixed.hip): chunk tables, Fock tile list, digit planes and exponents of Phi in two
+    // orders, FP32 gradients, per-step W digits, P digits and W coefficients.
+    int *d_chunkOffset = nullptr, *d_chunkBlock = nullptr;
+    int4 *d_tiles = nullptr;
+    signed char *d_DA = nullptr, *d_EA = nullptr, *d_DB = nullptr, *d_EB = nullptr;
+    float *d_GB = nullptr;
+    signed char *d_DW = nullptr, *d_EW = nullptr;
+    unsigned long long *d_Pq = nullptr;
     int *d_eP = nullptr;
-    long long nChunks = 0;
+    double *d_wcoef = nullptr;
+    long long nChunks = 0, nTiles = 0;
     double bfValues = 0.0;                // sum over blocks of kept functions x points
     double bfSquares = 0.0;               // sum over blocks of kept functions^2 x points
 
     // Work arrays for a quadrature.
     double *d_P = nullptr, *d_V = nullptr, *d_packed = nullptr;   // dense n x n P and V, packed staging
-    float *d_P32 = nullptr;
-    double *d_rho = nullptr;              // per point: rho, grad x, grad y, grad z
-    double *d_vrs = nullptr;              // per point: w vrho, w vsigma
     double *d_blockEN = nullptr;          // per block: E_xc and N_e partial sums
     int *d_skip = nullptr;                // per block: 1 if skipped by the density tolerance
     std::vector<double> h_blockEN;

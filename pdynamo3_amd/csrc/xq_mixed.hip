@@ -1,25 +1,43 @@
 // Mixed-precision quadrature: both contractions on int8 matrix cores with exact integer accumulation (Ozaki-style
 // slicing), FP64 everywhere else.
 //
-// Representation.  A value x in a group that shares the power of two 2^e (|x| < 2^e) is cut into signed 7-bit digits:
-//     y = x 2^-e;  for k = 0..D-1: y *= 128, d_k = trunc(y), y -= d_k      (d_k in [-127, 127], |remainder| < 2^-7D)
-//     x = 2^e sum_k d_k 2^(-7(k+1)) + r,   |r| < 2^(e - 7D).
-// (The last digit is rounded to nearest instead, see slice().)  Every step is exact in FP64.  A product of two sliced groups is sum_{k,l} 2^(-7(k+l+2)) sum d_k d_l; the inner sums are
-// int8 x int8 GEMMs with int32 accumulation (v_mfma_i32_32x32x32_i8), exact for the chunk lengths used here, and terms with
-// the same k + l share one int32 accumulator.  Only the representation error r and the dropped products (k + l above the
-// kept order) differ from FP64, both bounded relative to the group's power of two.
+// Representation.  A value x of a group that shares the power of two 2^e (|x| < 2^e) is rounded once to the integer
+// Q = round(x 2^(7D - e)), and |Q| is written in D base-128 digits that carry x's sign (d_k in [-127, 127]):
+//     x = 2^e sum_k d_k 2^(-7(k+1)) + r,   |r| <= 2^(e - 7D - 1)   (one round-to-nearest, unbiased).
+// A product of two sliced groups is sum_{k,l} 2^(-7(k+l+2)) sum d_k d_l.  The inner sums are int8 GEMMs with int32
+// accumulation (v_mfma_i32_32x32x32_i8), exact for the sizes used here, and products of equal order k + l share one
+// accumulator.  The accumulators of one result combine exactly into a 64-bit integer, sum_s acc_s 128^(S-1-s), which is
+// converted to FP64 once and scaled by the two groups' powers of two.  Only r and the dropped products (order above the
+// kept one) differ from FP64, both bounded relative to the groups' powers of two.
 //
-//   X = P_red Phi (density):     P rows (one power per row of P, per SCF step), 5 digits; Phi per (point, chunk of 32
-//                                kept functions), 4 digits, stored; kept products k + l <= 4.  Error ~2^-28 of the largest
-//                                term per point, ~2^-35 relative on P (P's rounding would be systematic across the grid).
-//   V += Phi W^T + W Phi^T:      Phi per (function, chunk of 32 points), 4 digits, stored; W per (function, chunk of 32
-//                                points), 4 digits, formed per step; kept products k + l <= 3.
-// The chunk partial sums are promoted to FP64 with their powers of two and summed in FP64.  The density epilogue uses Phi
-// rebuilt from its digits (28 bits) and grad Phi stored in FP32; the functional, the weights and every reduction are FP64.
+//   X = P_red Phi           P: 5 digits per element, one power of two per row of P (sliced once per step).
+//                           Phi: 4 digits, one power of two per point of a block (over the block's kept functions).
+//                           Orders k + l <= 4.  rho = sum Phi X and grad rho = 2 sum grad Phi X in FP64, with Phi from
+//                           its digits and grad Phi stored in FP32.
+//   V += Phi W^T + W Phi^T  Phi and W: 4 digits, one power of two per function row of a block (over its 128 points).
+//                           Orders k + l <= 3.  W = (w vrho / 2) Phi + 2 w vsigma (grad rho . grad Phi) is formed and
+//                           sliced per step.
+// The functional, the weights and every reduction are FP64.
+//
+// Data layouts.  The kept functions of a block are cut into chunks of 32 rows; chunk C of the grid is chunk rc of block b
+// (C = chunkOffset[b] + rc).  Rows past the block's kept count are zero.
+//   DA, DW [C][q (4)][digit (4)][lane (64)][16 bytes]  Phi and W digits in the operand order of the Fock product: lane
+//                                  (lr, lh) = (l % 32, l / 32) holds row 32 rc + lr at points 32 q + 16 lh .. + 15, which
+//                                  is its A operand (rows of tile I) and its B operand (rows of tile J).  Exponents
+//                                  EA, EW [C][32].
+//   DB     [C][digit (4)][point (128)][32 bytes]   Phi digits in the B operand order of the density product.  Position
+//                                  16 h + v of a point holds row 32 rc + 4 h + 8 (v / 4) + v % 4 (chunk_fun): the row
+//                                  whose result the MFMA lane (., h) receives in register v, so that the epilogue finds
+//                                  Phi for its own results in the operand it loads.  P_red tiles use the same K order.
+//                                  Exponents EB [block][128].
+//   GB     [C][3][point (128)][32] FP32            grad Phi in DB's order.
+//   Pq     [n][n] 64-bit words                     P digits 0..4 in bytes 0..4; row exponents eP [n].
 //
 // MFMA lane layout (32 x 32 x 32, int8): lane l supplies A[l % 32][16 (l / 32) .. + 15] and B[16 (l / 32) .. + 15][l % 32]
 // (16 consecutive K bytes each) and receives D[(v % 4) + 4 (l / 32) + 8 (v / 4)][l % 32] for v = 0..15.
 #include <hip/hip_runtime.h>
+
+#include <vector>
 
 #include "xq_functional.h"
 #include "xq_internal.h"
@@ -31,140 +49,198 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 namespace {
 
-constexpr int kC = 32;          // chunk length (functions or points) and MFMA tile edge
-constexpr int kDPhi = 4;        // digits of Phi and W
-constexpr int kDP = 5;          // digits of P
+constexpr int kC = 32;                  // chunk length (functions or points) and MFMA tile edge
+constexpr int kDPhi = 4;                // digits of Phi and W
+constexpr int kDP = 5;                  // digits of P
+constexpr int kChunkBytes = 16384;      // one chunk of DA, DW or DB: 4 digits x 32 rows x 128 points
+constexpr int kChunkFloats = 3 * kBlockSize * kC;   // one chunk of GB
+
+// Stored exponents are int8.  A group whose largest magnitude is below 2^-100 (contributions below 1e-30 of unit-sized
+// terms) gets exponent 0, which rounds all its digits to zero.  Magnitudes here stay far below 2^127.
+constexpr int kMinStoredExponent = -100;
 
 __device__ __forceinline__ v16i mfma_i8(const v4i &a, const v4i &b, const v16i &c) {
     return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
 }
 
-// Power of two bounding max|x| (frexp exponent); 0 for an all-zero group.
-__device__ __forceinline__ int group_exponent(double amax) {
+// Power of two strictly above max|x| (frexp exponent), clamped as described above.
+__device__ __forceinline__ int stored_exponent(double amax) {
     int e = 0;
     if (amax > 0.0) (void)frexp(amax, &e);
-    return e;
+    return e < kMinStoredExponent ? 0 : e;
 }
 
-// Stored exponents are int8: a group whose largest magnitude is below 2^-100 (contributions below 1e-30 of unit-sized
-// terms) is stored as zero digits with exponent 0.
-constexpr int kMinStoredExponent = -100;
-
-// Digits 0..D-2 truncate (the remainder keeps x's sign and stays below one unit); the last digit rounds to nearest, so the
-// representation error is unbiased and at most half a unit of the last digit (clamped at +-127 in the rare case
-// |y| >= 127.5, which the first D-1 truncations make possible only for the last digit).
 template <int D>
 __device__ __forceinline__ void slice(double x, int e, signed char *d) {
-    double y = ldexp(x, -e);
+    const long long Q = static_cast<long long>(rint(ldexp(x, 7 * D - e)));
+    // |x| < 2^e, so |Q| <= 2^(7D); the one value that rounds up to 2^(7D) is clamped to the largest D-digit magnitude.
+    unsigned long long a = static_cast<unsigned long long>(Q < 0 ? -Q : Q);
+    a = a < (1ull << (7 * D)) ? a : (1ull << (7 * D)) - 1ull;
+    const int sign = Q < 0 ? -1 : 1;
 #pragma unroll
-    for (int k = 0; k < D - 1; k++) {
-        y *= 128.0;
-        const double t = trunc(y);
-        d[k] = static_cast<signed char>(t);
-        y -= t;
-    }
-    const double t = fmin(fmax(rint(y * 128.0), -127.0), 127.0);
-    d[D - 1] = static_cast<signed char>(t);
+    for (int k = 0; k < D; k++) d[k] = static_cast<signed char>(sign * static_cast<int>((a >> (7 * (D - 1 - k))) & 127ull));
 }
 
-template <int D>
-__device__ __forceinline__ double unslice(const signed char *d, int e) {
-    double s = 0.0, w = 1.0 / 128.0;
+// Exact combination of digit-order accumulators: sum_s acc_s 128^(S-1-s) as a 64-bit integer.
+template <int S>
+__device__ __forceinline__ long long combine(const int *acc) {
+    long long t = 0;
 #pragma unroll
-    for (int k = 0; k < D; k++) { s += w * d[k]; w *= 1.0 / 128.0; }
-    return ldexp(s, e);
+    for (int k = 0; k < S; k++) t = t * 128 + static_cast<long long>(acc[k]);
+    return t;
 }
 
+// Signed byte v (0..15) of a 16-byte operand, and 16 bytes packed into one.
+__device__ __forceinline__ int byte_of(const v4i &x, int v) {
+    return static_cast<int>(static_cast<unsigned>(x[v >> 2]) << (24 - 8 * (v & 3))) >> 24;
+}
+
+__device__ __forceinline__ v4i pack16(const signed char (&d)[16]) {
+    v4i r;
+#pragma unroll
+    for (int w = 0; w < 4; w++)
+        r[w] = static_cast<int>((static_cast<unsigned>(d[4 * w]) & 255u) | ((static_cast<unsigned>(d[4 * w + 1]) & 255u) << 8) |
+                                ((static_cast<unsigned>(d[4 * w + 2]) & 255u) << 16) |
+                                ((static_cast<unsigned>(d[4 * w + 3]) & 255u) << 24));
+    return r;
+}
+
+// 4-digit value of byte v of the digit operands d[0..3]: sum_j d_j 128^(3-j).
+__device__ __forceinline__ int digits4(const v4i (&d)[kDPhi], int v) {
+    return ((byte_of(d[0], v) * 128 + byte_of(d[1], v)) * 128 + byte_of(d[2], v)) * 128 + byte_of(d[3], v);
+}
+
+// Row of a chunk held at density position 16 h + v, and the position of row k.
+__host__ __device__ constexpr int chunk_fun(int h, int v) { return 4 * h + 8 * (v >> 2) + (v & 3); }
+__host__ __device__ constexpr int chunk_pos(int k) { return 16 * ((k >> 2) & 1) + 4 * (k >> 3) + (k & 3); }
+
 // ------------------------------------------------------------------------------------------------------------------------
-// Set-up, once per geometry: FP64 values of one block -> the two digit layouts and FP32 gradients.
-//   D1  [row][chunk q (4)][digit (4)][32 points]   exponents E1 [row][q]       (rows of the block; per function & chunk)
-//   D2  [chunk c][digit (4)][point (128)][32 k]     exponents E2 [c][point]     (per point & chunk of 32 kept functions)
-//   G   [3][row][128] FP32
-// Offsets: rows start at the block's value-row offset R (D1, E1 at 512 R bytes and 4 R; G at 128 R floats); chunks c of
-// block b start at chunk offset C_b = sum over earlier blocks of ceil(nKept / 32) (D2 at 16384 C_b bytes, E2 at 128 C_b).
+// Set-up, once per geometry.
 // ------------------------------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlockSize) mixed_prepare_kernel(
-    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const double *__restrict__ f,
-    const double *__restrict__ fx, const double *__restrict__ fy, const double *__restrict__ fz,
-    signed char *__restrict__ D1, signed char *__restrict__ E1, signed char *__restrict__ D2, signed char *__restrict__ E2,
-    float *__restrict__ G, size_t rowsTotal) {
-    const int b = blockIdx.x;
+// EB: per point of a block, the power of two above max |Phi| over the block's kept functions (128 threads = points).
+__global__ void __launch_bounds__(kBlockSize) point_exponent_kernel(const xq::DevBlock *__restrict__ blocks,
+                                                                    const double *__restrict__ f,
+                                                                    signed char *__restrict__ EB) {
+    const int b = blockIdx.x, t = threadIdx.x;
     const xq::DevBlock blk = blocks[b];
-    const int nb = blk.nKept, t = threadIdx.x;   // t = point
     const size_t R = static_cast<size_t>(blk.vOffset);
-    const size_t C = static_cast<size_t>(chunkOffset[b]);
-    // Gradients in FP32.
-    for (int r = 0; r < nb; r++) {
-        const size_t o = (R + r) * kBlockSize + t;
-        G[o] = static_cast<float>(fx[o]);
-        G[rowsTotal * kBlockSize + o] = static_cast<float>(fy[o]);
-        G[2 * rowsTotal * kBlockSize + o] = static_cast<float>(fz[o]);
-    }
-    // D1/E1: per (row, chunk of 32 points).  Thread t handles point t; the chunk max is a reduction over 32 lanes.
-    const int q = t >> 5, pk = t & 31;
-    for (int r = 0; r < nb; r++) {
-        const double x = f[(R + r) * kBlockSize + t];
-        double m = fabs(x);
-#pragma unroll
-        for (int o = 16; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 32));
-        int e = group_exponent(m);
-        signed char d[kDPhi] = {0, 0, 0, 0};
-        if (e < kMinStoredExponent) e = 0;
-        else slice<kDPhi>(x, e, d);
-        const size_t base = ((R + r) * 4 + q) * (kDPhi * kC);
-#pragma unroll
-        for (int k = 0; k < kDPhi; k++) D1[base + k * kC + pk] = d[k];
-        if (pk == 0) E1[(R + r) * 4 + q] = static_cast<signed char>(e);
-    }
-    // D2/E2: per (point, chunk of 32 kept functions).
-    const int nChunks = (nb + kC - 1) / kC;
-    for (int c = 0; c < nChunks; c++) {
+    double m = 0.0;
+    for (int r = 0; r < blk.nKept; r++) m = fmax(m, fabs(f[(R + r) * kBlockSize + t]));
+    EB[static_cast<size_t>(b) * kBlockSize + t] = static_cast<signed char>(stored_exponent(m));
+}
+
+// One chunk per work-group (256 threads): EA, DA, DB and GB from the FP64 values of the chunk's rows.
+__global__ void __launch_bounds__(256) mixed_prepare_kernel(
+    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ chunkBlock,
+    const double *__restrict__ f, const double *__restrict__ fx, const double *__restrict__ fy,
+    const double *__restrict__ fz, const signed char *__restrict__ EB, signed char *__restrict__ DA,
+    signed char *__restrict__ EA, signed char *__restrict__ DB, float *__restrict__ GB) {
+    __shared__ int sE[kC];
+    const size_t C = blockIdx.x;
+    const int b = chunkBlock[C];
+    const xq::DevBlock blk = blocks[b];
+    const int nb = blk.nKept, rc = static_cast<int>(C) - chunkOffset[b], t = threadIdx.x;
+    const size_t R = static_cast<size_t>(blk.vOffset);
+    auto value = [&](const double *a, int row, int p) -> double {
+        return row < nb ? a[(R + row) * kBlockSize + p] : 0.0;
+    };
+    // EA: row i = t / 8 of the chunk, eight threads per row with 16 points each.
+    {
+        const int i = t >> 3, part = t & 7, row = kC * rc + i;
         double m = 0.0;
-        for (int k = 0; k < kC; k++) {
-            const int r = c * kC + k;
-            if (r < nb) m = fmax(m, fabs(f[(R + r) * kBlockSize + t]));
-        }
-        int e = group_exponent(m);
-        const bool tiny = e < kMinStoredExponent;
-        if (tiny) e = 0;
-        E2[(C + c) * kBlockSize + t] = static_cast<signed char>(e);
-        for (int k = 0; k < kC; k++) {
-            const int r = c * kC + k;
-            signed char d[kDPhi] = {0, 0, 0, 0};
-            if (r < nb && !tiny) slice<kDPhi>(f[(R + r) * kBlockSize + t], e, d);
+#pragma unroll 4
+        for (int k = 0; k < 16; k++) m = fmax(m, fabs(value(f, row, 16 * part + k)));
 #pragma unroll
-            for (int j = 0; j < kDPhi; j++) D2[(((C + c) * kDPhi + j) * kBlockSize + t) * kC + k] = d[j];
+        for (int o = 4; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 8));
+        if (part == 0) {
+            const int e = stored_exponent(m);
+            sE[i] = e;
+            EA[C * kC + i] = static_cast<signed char>(e);
         }
+    }
+    __syncthreads();
+    // DA: thread = (q, lane).
+    {
+        const int q = t >> 6, lane = t & 63, lr = lane & 31, lh = lane >> 5, row = kC * rc + lr, e = sE[lr];
+        signed char d[kDPhi][16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            signed char dd[kDPhi];
+            slice<kDPhi>(value(f, row, kC * q + 16 * lh + k), e, dd);
+#pragma unroll
+            for (int j = 0; j < kDPhi; j++) d[j][k] = dd[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kDPhi; j++)
+            *reinterpret_cast<v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]) = pack16(d[j]);
+    }
+    // DB and GB: thread = (point p, half h) with the 16 rows chunk_fun(h, 0..15).
+    {
+        const int p = t & (kBlockSize - 1), h = t >> 7;
+        const int e = EB[static_cast<size_t>(b) * kBlockSize + p];
+        signed char d[kDPhi][16];
+        float g[3][16];
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int row = kC * rc + chunk_fun(h, v);
+            signed char dd[kDPhi];
+            slice<kDPhi>(value(f, row, p), e, dd);
+#pragma unroll
+            for (int j = 0; j < kDPhi; j++) d[j][v] = dd[j];
+            g[0][v] = static_cast<float>(value(fx, row, p));
+            g[1][v] = static_cast<float>(value(fy, row, p));
+            g[2][v] = static_cast<float>(value(fz, row, p));
+        }
+#pragma unroll
+        for (int j = 0; j < kDPhi; j++)
+            *reinterpret_cast<v4i *>(&DB[C * kChunkBytes + (j * kBlockSize + p) * kC + 16 * h]) = pack16(d[j]);
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int v = 0; v < 16; v += 4)
+                *reinterpret_cast<float4 *>(&GB[C * kChunkFloats + (c * kBlockSize + p) * kC + 16 * h + v]) =
+                    make_float4(g[c][v], g[c][v + 1], g[c][v + 2], g[c][v + 3]);
     }
 }
 
-// Row exponents of P (power of two above max_k |P_ik|), once per step.
-__global__ void p_row_exponent_kernel(int n, const double *__restrict__ P, int *__restrict__ eP) {
-    const int i = blockIdx.x;
+// ------------------------------------------------------------------------------------------------------------------------
+// Per step.
+// ------------------------------------------------------------------------------------------------------------------------
+// P row i (one work-group of 256 threads per row): row exponent and the 5 digits of every element.
+__global__ void __launch_bounds__(256) p_slice_kernel(int n, const double *__restrict__ P, int *__restrict__ eP,
+                                                      unsigned long long *__restrict__ Pq) {
+    __shared__ double s[4];
+    const size_t i = blockIdx.x;
+    const double *row = P + i * n;
     double m = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmax(m, fabs(P[static_cast<size_t>(i) * n + k]));
+    for (int k = threadIdx.x; k < n; k += 256) m = fmax(m, fabs(row[k]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    __shared__ double s[4];
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) eP[i] = group_exponent(fmax(fmax(s[0], s[1]), fmax(s[2], s[3])));
+    const int e = stored_exponent(fmax(fmax(s[0], s[1]), fmax(s[2], s[3])));
+    for (int k = threadIdx.x; k < n; k += 256) {
+        signed char d[kDP];
+        slice<kDP>(row[k], e, d);
+        unsigned long long w = 0;
+#pragma unroll
+        for (int j = 0; j < kDP; j++) w |= (static_cast<unsigned long long>(d[j]) & 255ull) << (8 * j);
+        Pq[i * n + k] = w;
+    }
+    if (threadIdx.x == 0) eP[i] = e;
 }
 
-// ------------------------------------------------------------------------------------------------------------------------
 // Density kernel (per block, 256 threads = 4 waves): X = P_red Phi on int8 MFMA, rho and grad rho, the FP64 functional,
-// E_xc / N_e block sums; writes rho, grad rho and (w vrho, w vsigma) per point for the Fock stage.
-// Wave wv owns the 32 points 32 wv .. 32 wv + 31; the 32 x 32 P_red tile of each (row chunk I, column chunk c) is shared
-// through LDS, and the next tile's P entries are fetched into registers while the current tile is multiplied.
-// ------------------------------------------------------------------------------------------------------------------------
+// E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
+// point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  The P_red tile (I, c) is staged in LDS (double
+// buffered, one barrier per tile) by all four waves, its words fetched one tile ahead.
 __global__ void __launch_bounds__(256) density_kernel_mixed(
     const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ idxAll,
-    const signed char *__restrict__ D2, const signed char *__restrict__ E2, const float *__restrict__ G, size_t rowsTotal,
-    int n, const double *__restrict__ P, const int *__restrict__ eP, const double *__restrict__ w, FunctionalSet fs,
-    double rhoTol, double *__restrict__ rhoOut, double *__restrict__ vrs, double *__restrict__ blockEN,
-    int *__restrict__ skip) {
-    __shared__ __attribute__((aligned(16))) signed char sA[kDP][kC][kC];   // P_red tile digits [digit][row][k]
-    __shared__ int sPe[kC];
+    const signed char *__restrict__ DB, const signed char *__restrict__ EB, const float *__restrict__ GB, int n,
+    const unsigned long long *__restrict__ Pq, const int *__restrict__ eP, const double *__restrict__ w, FunctionalSet fs,
+    double rhoTol, double *__restrict__ wcoef, double *__restrict__ blockEN, int *__restrict__ skip) {
+    __shared__ __attribute__((aligned(16))) int sA[2][kDP][kC][8];   // [buffer][digit][row][32 positions]
+    __shared__ int sPe[2][kC];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
     const int b = blockIdx.x;
@@ -177,85 +253,73 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
         return;
     }
     const int *idx = idxAll + blk.iOffset;
-    const size_t R = static_cast<size_t>(blk.vOffset);
     const size_t C = static_cast<size_t>(chunkOffset[b]);
     const int nChunks = (nb + kC - 1) / kC;
-    const size_t plane = rowsTotal * kBlockSize;
-    const int p = 32 * wv + lr;          // this lane's point (its D column)
-    // Tile entries this thread stages: rows r = e >> 5, columns k = e & 31 for e = t + 256 q.
-    auto fetch = [&](int I, int c, double *pv) {
+    const int p = kC * wv + lr;
+    const signed char *DBb = DB + C * kChunkBytes;
+    // Staging: thread (row sr, group skq) fetches P_red[sr][4 skq .. + 3], written as one word per digit at chunk_pos.
+    const int sr = t >> 3, skq = t & 7, sdw = 4 * (skq & 1) + (skq >> 1);
+    auto fetch = [&](int I, int c, unsigned long long (&wd)[4]) {
+        const int r = kC * I + sr;
+        const size_t rowOff = r < nb ? static_cast<size_t>(idx[r]) * n : 0;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int e = t + 256 * q, gr = I * kC + (e >> 5), gk = c * kC + (e & 31);
-            pv[q] = (gr < nb && gk < nb) ? P[static_cast<size_t>(idx[gr]) * n + idx[gk]] : 0.0;
+        for (int jj = 0; jj < 4; jj++) {
+            const int k = kC * c + 4 * skq + jj;
+            wd[jj] = (r < nb && k < nb) ? Pq[rowOff + idx[k]] : 0ull;
         }
     };
+    unsigned long long wcur[4];
+    fetch(0, 0, wcur);
+    int buf = 0;
+    const int eb = EB[static_cast<size_t>(b) * kBlockSize + p];
     double pr = 0.0, pgx = 0.0, pgy = 0.0, pgz = 0.0;
-    double pnext[4];
-    fetch(0, 0, pnext);
     for (int I = 0; I < nChunks; I++) {
-        double X[16];
+        if (t < kC) sPe[I & 1][t] = kC * I + t < nb ? eP[idx[kC * I + t]] : 0;
+        v16i acc[kDP];
 #pragma unroll
-        for (int v = 0; v < 16; v++) X[v] = 0.0;
-        __syncthreads();
-        if (t < kC) {
-            const int r = I * kC + t;
-            sPe[t] = r < nb ? eP[idx[r]] : 0;
-        }
+        for (int s = 0; s < kDP; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int c = 0; c < nChunks; c++) {
-            __syncthreads();
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int e = t + 256 * q, r = e >> 5, k = e & 31;
-                signed char d[kDP] = {0, 0, 0, 0, 0};
-                if (pnext[q] != 0.0) slice<kDP>(pnext[q], sPe[r], d);
+            for (int j = 0; j < kDP; j++) {
+                unsigned x = 0;
 #pragma unroll
-                for (int j = 0; j < kDP; j++) sA[j][r][k] = d[j];
+                for (int jj = 0; jj < 4; jj++) x |= (static_cast<unsigned>(wcur[jj] >> (8 * j)) & 255u) << (8 * jj);
+                sA[buf][j][sr][sdw] = static_cast<int>(x);
             }
             __syncthreads();
-            // Prefetch the next tile's P entries while this one is multiplied.
-            const int cn = c + 1 < nChunks ? c + 1 : 0, In = c + 1 < nChunks ? I : I + 1;
-            if (In < nChunks) fetch(In, cn, pnext);
+            {
+                int In = I, cn = c + 1;
+                if (cn == nChunks) { cn = 0; In = I + 1; }
+                if (In < nChunks) fetch(In, cn, wcur);
+            }
             v4i a[kDP], bq[kDPhi];
 #pragma unroll
-            for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sA[j][lr][16 * lh]);
-#pragma unroll
             for (int j = 0; j < kDPhi; j++)
-                bq[j] = *reinterpret_cast<const v4i *>(&D2[(((C + c) * kDPhi + j) * kBlockSize + p) * kC + 16 * lh]);
-            v16i acc[5];
+                bq[j] = *reinterpret_cast<const v4i *>(&DBb[(c * kDPhi + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
 #pragma unroll
-            for (int s = 0; s < 5; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sA[buf][j][lr][4 * lh]);
 #pragma unroll
             for (int i = 0; i < kDP; i++)
 #pragma unroll
                 for (int j = 0; j < kDPhi; j++)
                     if (i + j <= 4) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
-            const int ePhi = E2[(C + c) * kBlockSize + p];
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int rl = (v & 3) + 4 * lh + 8 * (v >> 2);
-                double s = 0.0;
-#pragma unroll
-                for (int k = 4; k >= 0; k--) s = s * (1.0 / 128.0) + static_cast<double>(acc[k][v]);
-                X[v] += ldexp(s, sPe[rl] + ePhi - 14);
-            }
+            buf ^= 1;
         }
-        // Epilogue: rho += Phi X, grad rho += grad Phi X (Phi rebuilt from the point-scaled digits).
-        const int ePhiI = E2[(C + I) * kBlockSize + p];
+        // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi FP32.
+        v4i dI[kDPhi];
+#pragma unroll
+        for (int j = 0; j < kDPhi; j++)
+            dI[j] = *reinterpret_cast<const v4i *>(&DBb[(I * kDPhi + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
+        const float *g = GB + (C + I) * kChunkFloats + p * kC + 16 * lh;
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-            const int rl = (v & 3) + 4 * lh + 8 * (v >> 2), r = I * kC + rl;
-            if (r < nb) {
-                signed char d[kDPhi];
-#pragma unroll
-                for (int j = 0; j < kDPhi; j++) d[j] = D2[(((C + I) * kDPhi + j) * kBlockSize + p) * kC + rl];
-                const double phi = unslice<kDPhi>(d, ePhiI);
-                const size_t o = (R + r) * kBlockSize + p;
-                pr += phi * X[v];
-                pgx += static_cast<double>(G[o]) * X[v];
-                pgy += static_cast<double>(G[plane + o]) * X[v];
-                pgz += static_cast<double>(G[2 * plane + o]) * X[v];
-            }
+            const int s[kDP] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
+            const double x = ldexp(static_cast<double>(combine<kDP>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 42);
+            const double phi = ldexp(static_cast<double>(digits4(dI, v)), eb - 28);
+            pr += phi * x;
+            pgx += static_cast<double>(g[v]) * x;
+            pgy += static_cast<double>(g[kBlockSize * kC + v]) * x;
+            pgz += static_cast<double>(g[2 * kBlockSize * kC + v]) * x;
         }
     }
     pr += __shfl_xor(pr, 32, 64);
@@ -264,7 +328,7 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
     pgz += __shfl_xor(pgz, 32, 64);
     if (lh == 0) { rsum[0][p] = pr; rsum[1][p] = pgx; rsum[2][p] = pgy; rsum[3][p] = pgz; }
     __syncthreads();
-    // Functional, E and N.
+    // Functional, E and N, W coefficients.
     double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
     if (t < np) { rho = rsum[0][t]; gx = 2.0 * rsum[1][t]; gy = 2.0 * rsum[2][t]; gz = 2.0 * rsum[3][t]; }
     double mx = t < np ? fabs(rho) : 0.0;
@@ -286,13 +350,11 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
             e = eps * rho * ww;
             ne = rho * ww;
         }
-        const size_t pp = static_cast<size_t>(blk.pStart) + t;
-        rhoOut[4 * pp] = rho;
-        rhoOut[4 * pp + 1] = gx;
-        rhoOut[4 * pp + 2] = gy;
-        rhoOut[4 * pp + 3] = gz;
-        vrs[2 * pp] = vr;
-        vrs[2 * pp + 1] = vs;
+        double *cw = wcoef + 4 * (static_cast<size_t>(blk.pStart) + t);
+        cw[0] = 0.5 * vr;
+        cw[1] = 2.0 * vs * gx;
+        cw[2] = 2.0 * vs * gy;
+        cw[3] = 2.0 * vs * gz;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) { e += __shfl_xor(e, o, 64); ne += __shfl_xor(ne, o, 64); }
@@ -305,193 +367,213 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
     }
 }
 
-// ------------------------------------------------------------------------------------------------------------------------
-// W digits (per block, 128 threads = points): W_rp = (w vrho / 2) Phi_rp + 2 w vsigma (grad rho . grad Phi_rp), cut into
-// 4 digits per (row, chunk of 32 points), in D1's layout (DW, EW).  Phi comes from D1 (28 bits), grad Phi from G.
-// ------------------------------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlockSize) w_digits_kernel(
-    const xq::DevBlock *__restrict__ blocks, const signed char *__restrict__ D1, const signed char *__restrict__ E1,
-    const float *__restrict__ G, size_t rowsTotal, const double *__restrict__ rhoIn, const double *__restrict__ vrs,
+// W digits (one chunk per work-group, 256 threads = 4 waves = the 4 point chunks q): W_rp = c0_p Phi_rp + c_p . grad Phi_rp
+// for the chunk's 32 rows, one power of two per row over the block's 128 points, written in DA's order (DW, EW).
+__global__ void __launch_bounds__(256) w_digits_kernel(
+    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkBlock, const signed char *__restrict__ DA,
+    const signed char *__restrict__ EA, const float *__restrict__ GB, const double *__restrict__ wcoef,
     const int *__restrict__ skip, signed char *__restrict__ DW, signed char *__restrict__ EW) {
-    const int b = blockIdx.x;
+    __shared__ double sc[4][kBlockSize];
+    __shared__ double sm[4][kC];
+    const size_t C = blockIdx.x;
+    const int b = chunkBlock[C];
     if (skip[b]) return;
     const xq::DevBlock blk = blocks[b];
-    const int nb = blk.nKept, np = blk.nPoints, t = threadIdx.x;
-    const size_t R = static_cast<size_t>(blk.vOffset), plane = rowsTotal * kBlockSize;
-    const int q = t >> 5, pk = t & 31;
-    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
-    if (t < np) {
-        const size_t pp = static_cast<size_t>(blk.pStart) + t;
-        const double s2 = 2.0 * vrs[2 * pp + 1];
-        c0 = 0.5 * vrs[2 * pp];
-        c1 = s2 * rhoIn[4 * pp + 1];
-        c2 = s2 * rhoIn[4 * pp + 2];
-        c3 = s2 * rhoIn[4 * pp + 3];
+    const int t = threadIdx.x, lane = t & 63, q = t >> 6, lr = lane & 31, lh = lane >> 5;
+    if (t < kBlockSize) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) sc[k][t] = t < blk.nPoints ? wcoef[4 * (static_cast<size_t>(blk.pStart) + t) + k] : 0.0;
     }
-    for (int r = 0; r < nb; r++) {
-        const size_t base = ((R + r) * 4 + q) * (kDPhi * kC);
-        signed char d[kDPhi];
+    __syncthreads();
+    v4i d[kDPhi];
 #pragma unroll
-        for (int k = 0; k < kDPhi; k++) d[k] = D1[base + k * kC + pk];
-        const double phi = unslice<kDPhi>(d, E1[(R + r) * 4 + q]);
-        const size_t o = (R + r) * kBlockSize + t;
-        const double x = c0 * phi + c1 * static_cast<double>(G[o]) + c2 * static_cast<double>(G[plane + o]) +
-                         c3 * static_cast<double>(G[2 * plane + o]);
-        double m = fabs(x);
+    for (int j = 0; j < kDPhi; j++)
+        d[j] = *reinterpret_cast<const v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]);
+    const int ea = EA[C * kC + lr];
+    const float *g = GB + C * kChunkFloats + chunk_pos(lr);
+    double W[16];
+    double m = 0.0;
 #pragma unroll
-        for (int o2 = 16; o2 > 0; o2 >>= 1) m = fmax(m, __shfl_xor(m, o2, 32));
-        int e = group_exponent(m);
-        signed char dw[kDPhi] = {0, 0, 0, 0};
-        if (e < kMinStoredExponent) e = 0;
-        else slice<kDPhi>(x, e, dw);
-#pragma unroll
-        for (int k = 0; k < kDPhi; k++) DW[base + k * kC + pk] = dw[k];
-        if (pk == 0) EW[(R + r) * 4 + q] = static_cast<signed char>(e);
+    for (int k = 0; k < 16; k++) {
+        const int p = kC * q + 16 * lh + k;
+        const double phi = ldexp(static_cast<double>(digits4(d, k)), ea - 28);
+        W[k] = sc[0][p] * phi + sc[1][p] * static_cast<double>(g[p * kC]) +
+               sc[2][p] * static_cast<double>(g[(kBlockSize + p) * kC]) +
+               sc[3][p] * static_cast<double>(g[(2 * kBlockSize + p) * kC]);
+        m = fmax(m, fabs(W[k]));
     }
+    m = fmax(m, __shfl_xor(m, 32, 64));
+    if (lh == 0) sm[q][lr] = m;
+    __syncthreads();
+    const int e = stored_exponent(fmax(fmax(sm[0][lr], sm[1][lr]), fmax(sm[2][lr], sm[3][lr])));
+    signed char dw[kDPhi][16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        signed char dd[kDPhi];
+        slice<kDPhi>(W[k], e, dd);
+#pragma unroll
+        for (int j = 0; j < kDPhi; j++) dw[j][k] = dd[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kDPhi; j++)
+        *reinterpret_cast<v4i *>(&DW[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]) = pack16(dw[j]);
+    if (q == 0 && lh == 0) EW[C * kC + lr] = static_cast<signed char>(e);
 }
 
-// ------------------------------------------------------------------------------------------------------------------------
-// Fock kernel (per block, 256 threads = 4 waves): V += Phi W^T + W Phi^T on lower-triangle 32 x 32 tiles (I, J), J <= I.
-// The four waves share a tile and take one 32-point chunk each (q = wv).  Lane (lr, lh) loads, for rows I*32 + lr and
-// J*32 + lr, the 16 points 32 q + 16 lh .. + 15 of the Phi and W digits: its A operands (rows I) and B operands (rows J).
-// Partial tiles are summed over the waves in LDS and added to V with FP64 atomics.
-// ------------------------------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) fock_kernel_mixed(
-    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ idxAll, const signed char *__restrict__ D1,
-    const signed char *__restrict__ E1, const signed char *__restrict__ DW, const signed char *__restrict__ EW, int n,
+// Fock kernel: one wave per lower-triangle tile (I, J <= I) of a block's reduced V, four tiles per work-group, no LDS
+// and no barriers.  K runs over the block's 128 points in four chunks of 32; each product accumulates in int32 over all
+// of them (one power of two per row), then V += Phi_I W_J^T + W_I Phi_J^T is added with FP64 atomics.
+// Work-group g runs on XCD g % 8; tiles are dealt so that each XCD takes a contiguous range of the tile list (tiles of
+// one block share their chunks' digits in that XCD's L2).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) fock_kernel_mixed(
+    const int4 *__restrict__ tiles, int nTiles, int groupsPerXcd, const xq::DevBlock *__restrict__ blocks,
+    const int *__restrict__ chunkOffset, const int *__restrict__ idxAll, const signed char *__restrict__ DA,
+    const signed char *__restrict__ EA, const signed char *__restrict__ DW, const signed char *__restrict__ EW, int n,
     const int *__restrict__ skip, double *__restrict__ V) {
-    __shared__ double red[4][kC][kC + 1];
-    const int b = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 31, lh = lane >> 5;
+    const int group = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
+    const int tileIndex = 4 * group + wv;
+    if (tileIndex >= nTiles) return;
+    const int4 tl = tiles[tileIndex];
+    const int b = tl.x, I = tl.y, J = tl.z;
     if (skip[b]) return;
     const xq::DevBlock blk = blocks[b];
     const int nb = blk.nKept;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int lr = lane & 31, lh = lane >> 5;
     const int *idx = idxAll + blk.iOffset;
-    const size_t R = static_cast<size_t>(blk.vOffset);
-    const int nChunks = (nb + kC - 1) / kC;
-    const int q = wv;
-    for (int I = 0; I < nChunks; I++) {
-        for (int J = 0; J <= I; J++) {
-            v4i fd[2][kDPhi], wd[2][kDPhi];
-            int eF[2], eW[2];
+    const size_t CI = static_cast<size_t>(chunkOffset[b] + I), CJ = static_cast<size_t>(chunkOffset[b] + J);
+    const int gj = kC * J + lr;
+    // Two passes, Phi_I W_J^T then W_I Phi_J^T, each promoted into the FP64 results before the next starts.
+    double x[16];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int rr = (h == 0 ? I : J) * kC + lr;
-                if (rr < nb) {
-                    const size_t base = ((R + rr) * 4 + q) * (kDPhi * kC) + 16 * lh;
+    for (int v = 0; v < 16; v++) x[v] = 0.0;
+#pragma unroll 1
+    for (int pass = 0; pass < 2; pass++) {
+        const signed char *A = (pass == 0 ? DA : DW) + CI * kChunkBytes + lane * 16;
+        const signed char *B = (pass == 0 ? DW : DA) + CJ * kChunkBytes + lane * 16;
+        const signed char *eA = (pass == 0 ? EA : EW) + CI * kC, *eB = (pass == 0 ? EW : EA) + CJ * kC;
+        v16i acc[4];
 #pragma unroll
-                    for (int k = 0; k < kDPhi; k++) {
-                        fd[h][k] = *reinterpret_cast<const v4i *>(&D1[base + k * kC]);
-                        wd[h][k] = *reinterpret_cast<const v4i *>(&DW[base + k * kC]);
-                    }
-                    eF[h] = E1[(R + rr) * 4 + q];
-                    eW[h] = EW[(R + rr) * 4 + q];
-                } else {
+        for (int s = 0; s < 4; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 2
+        for (int q = 0; q < 4; q++) {
+            v4i xa[kDPhi], xb[kDPhi];
 #pragma unroll
-                    for (int k = 0; k < kDPhi; k++) { fd[h][k] = v4i{0, 0, 0, 0}; wd[h][k] = v4i{0, 0, 0, 0}; }
-                    eF[h] = 0;
-                    eW[h] = 0;
-                }
-            }
-            v16i a1[4], a2[4];
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                a1[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-                a2[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int j = 0; j < kDPhi; j++) {
+                xa[j] = *reinterpret_cast<const v4i *>(A + (q * kDPhi + j) * 1024);
+                xb[j] = *reinterpret_cast<const v4i *>(B + (q * kDPhi + j) * 1024);
             }
 #pragma unroll
             for (int i = 0; i < kDPhi; i++)
 #pragma unroll
                 for (int j = 0; j < kDPhi; j++)
-                    if (i + j <= 3) {
-                        a1[i + j] = mfma_i8(fd[0][i], wd[1][j], a1[i + j]);   // Phi_I W_J^T
-                        a2[i + j] = mfma_i8(wd[0][i], fd[1][j], a2[i + j]);   // W_I Phi_J^T
-                    }
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int rl = (v & 3) + 4 * lh + 8 * (v >> 2);
-                const int eFI = __shfl(eF[0], rl, 64), eWI = __shfl(eW[0], rl, 64);
-                double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-                for (int k = 3; k >= 0; k--) {
-                    s1 = s1 * (1.0 / 128.0) + static_cast<double>(a1[k][v]);
-                    s2 = s2 * (1.0 / 128.0) + static_cast<double>(a2[k][v]);
-                }
-                red[wv][rl][lr] = ldexp(s1, eFI + eW[1] - 14) + ldexp(s2, eWI + eF[1] - 14);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int el = t + 256 * u, i = el >> 5, j = el & 31;
-                const int gi = I * kC + i, gj = J * kC + j;
-                if (gi < nb && gj < nb && gj <= gi) {
-                    const double x = ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
-                    atomicAdd(&V[static_cast<size_t>(idx[gi]) * n + idx[gj]], x);
-                }
-            }
-            __syncthreads();
+                    if (i + j <= 3) acc[i + j] = mfma_i8(xa[i], xb[j], acc[i + j]);
         }
+        // Lane (lr, lh) holds rows 32 I + rl(v) and column 32 J + lr.
+        const int eCol = eB[lr] - 35;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int rl = (v & 3) + 4 * lh + 8 * (v >> 2);
+            const int tv[4] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v]};
+            x[v] += ldexp(static_cast<double>(combine<4>(tv)), eA[rl] + eCol);
+        }
+    }
+    const size_t col = gj < nb ? static_cast<size_t>(idx[gj]) : 0;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int gi = kC * I + (v & 3) + 4 * lh + 8 * (v >> 2);
+        if (gi < nb && gj < nb && gj <= gi) atomicAdd(&V[static_cast<size_t>(idx[gi]) * n + col], x[v]);
     }
 }
 
 }  // namespace
 
-// Host side: digit storage at basis set-up and the per-step launch.
+// Host side: digit storage at basis set-up and the per-step launches.
 int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz) {
-    std::vector<int> chunkOffset(g->nBlocks + 1, 0);
+    std::vector<int> chunkOffset(g->nBlocks + 1, 0), chunkBlock;
+    std::vector<int4> tiles;
     long long chunks = 0;
     for (int b = 0; b < g->nBlocks; b++) {
+        const int nb = g->blocks[b].nKept;
+        // The density accumulators hold up to 4 x 127^2 x nb per order in int32.
+        if (nb > 32767) { set_error("mixed precision supports at most 32767 kept functions per block"); return kInvalidArgument; }
+        const int T = (nb + kC - 1) / kC;
         chunkOffset[b] = static_cast<int>(chunks);
-        chunks += (g->blocks[b].nKept + kC - 1) / kC;
+        for (int I = 0; I < T; I++) {
+            chunkBlock.push_back(b);
+            for (int J = 0; J <= I; J++) tiles.push_back(make_int4(b, I, J, 0));
+        }
+        chunks += T;
     }
     chunkOffset[g->nBlocks] = static_cast<int>(chunks);
     g->nChunks = chunks;
-    const size_t rows = static_cast<size_t>(g->nValueRows > 0 ? g->nValueRows : 1);
+    g->nTiles = static_cast<long long>(tiles.size());
+    const size_t nc = static_cast<size_t>(chunks > 0 ? chunks : 1);
     XQ_TRY(dev_alloc(g, &g->d_chunkOffset, chunkOffset.size()));
-    XQ_TRY(dev_alloc(g, &g->d_D1, rows * 512));
-    XQ_TRY(dev_alloc(g, &g->d_E1, rows * 4));
-    XQ_TRY(dev_alloc(g, &g->d_D2, static_cast<size_t>(chunks > 0 ? chunks : 1) * 16384));
-    XQ_TRY(dev_alloc(g, &g->d_E2, static_cast<size_t>(chunks > 0 ? chunks : 1) * 128));
-    XQ_TRY(dev_alloc(g, &g->d_G, 3 * rows * kBlockSize));
-    XQ_TRY(dev_alloc(g, &g->d_DW, rows * 512));
-    XQ_TRY(dev_alloc(g, &g->d_EW, rows * 4));
-    XQ_CHECK(hipMemcpyAsync(g->d_chunkOffset, chunkOffset.data(), chunkOffset.size() * sizeof(int), hipMemcpyHostToDevice,
-                            g->stream));
+    XQ_TRY(dev_alloc(g, &g->d_chunkBlock, nc));
+    XQ_TRY(dev_alloc(g, &g->d_tiles, tiles.empty() ? 1 : tiles.size()));
+    XQ_TRY(dev_alloc(g, &g->d_DA, nc * kChunkBytes));
+    XQ_TRY(dev_alloc(g, &g->d_EA, nc * kC));
+    XQ_TRY(dev_alloc(g, &g->d_DB, nc * kChunkBytes));
+    XQ_TRY(dev_alloc(g, &g->d_EB, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * kBlockSize));
+    XQ_TRY(dev_alloc(g, &g->d_GB, nc * kChunkFloats));
+    XQ_TRY(dev_alloc(g, &g->d_DW, nc * kChunkBytes));
+    XQ_TRY(dev_alloc(g, &g->d_EW, nc * kC));
+    XQ_TRY(dev_alloc(g, &g->d_Pq, static_cast<size_t>(g->nFunctions) * g->nFunctions));
+    XQ_TRY(dev_alloc(g, &g->d_eP, static_cast<size_t>(g->nFunctions)));
+    XQ_TRY(dev_alloc(g, &g->d_wcoef, 4 * static_cast<size_t>(g->nPoints > 0 ? g->nPoints : 1)));
+    hipStream_t s = g->stream;
+    XQ_CHECK(hipMemcpyAsync(g->d_chunkOffset, chunkOffset.data(), chunkOffset.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    if (chunks > 0)
+        XQ_CHECK(hipMemcpyAsync(g->d_chunkBlock, chunkBlock.data(), chunkBlock.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    if (!tiles.empty())
+        XQ_CHECK(hipMemcpyAsync(g->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s));
     if (g->nBlocks > 0)
-        hipLaunchKernelGGL(mixed_prepare_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, g->stream, g->d_blocks,
-                           g->d_chunkOffset, f, fx, fy, fz, g->d_D1, g->d_E1, g->d_D2, g->d_E2, g->d_G,
-                           static_cast<size_t>(g->nValueRows));
+        hipLaunchKernelGGL(point_exponent_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, f, g->d_EB);
+    if (chunks > 0)
+        hipLaunchKernelGGL(mixed_prepare_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, g->d_blocks,
+                           g->d_chunkOffset, g->d_chunkBlock, f, fx, fy, fz, g->d_EB, g->d_DA, g->d_EA, g->d_DB, g->d_GB);
     XQ_CHECK(hipGetLastError());
+    // The host vectors above must outlive the copies.
+    XQ_CHECK(hipStreamSynchronize(s));
     return kOK;
 }
 
 void free_mixed(xq_grid *g) {
     dev_free(&g->d_chunkOffset);
-    dev_free(&g->d_D1);
-    dev_free(&g->d_E1);
-    dev_free(&g->d_D2);
-    dev_free(&g->d_E2);
-    dev_free(&g->d_G);
+    dev_free(&g->d_chunkBlock);
+    dev_free(&g->d_tiles);
+    dev_free(&g->d_DA);
+    dev_free(&g->d_EA);
+    dev_free(&g->d_DB);
+    dev_free(&g->d_EB);
+    dev_free(&g->d_GB);
     dev_free(&g->d_DW);
     dev_free(&g->d_EW);
+    dev_free(&g->d_Pq);
     dev_free(&g->d_eP);
+    dev_free(&g->d_wcoef);
+    g->nChunks = 0;
+    g->nTiles = 0;
 }
 
 int launch_fused_mixed(xq_grid *g, const FunctionalSet &fs) {
-    if (g->d_eP == nullptr) XQ_TRY(dev_alloc(g, &g->d_eP, static_cast<size_t>(g->nFunctions)));
     hipStream_t s = g->stream;
-    const size_t rows = static_cast<size_t>(g->nValueRows);
-    hipLaunchKernelGGL(p_row_exponent_kernel, dim3(g->nFunctions), dim3(256), 0, s, g->nFunctions, g->d_P, g->d_eP);
+    const int n = g->nFunctions;
+    if (n > 0) hipLaunchKernelGGL(p_slice_kernel, dim3(n), dim3(256), 0, s, n, g->d_P, g->d_eP, g->d_Pq);
     (void)hipEventRecord(g->evRho0, s);
     hipLaunchKernelGGL(density_kernel_mixed, dim3(g->nBlocks), dim3(256), 0, s, g->d_blocks, g->d_chunkOffset, g->d_idx,
-                       g->d_D2, g->d_E2, g->d_G, rows, g->nFunctions, g->d_P, g->d_eP, g->d_w, fs, g->rhoTol, g->d_rho,
-                       g->d_vrs, g->d_blockEN, g->d_skip);
+                       g->d_DB, g->d_EB, g->d_GB, n, g->d_Pq, g->d_eP, g->d_w, fs, g->rhoTol, g->d_wcoef, g->d_blockEN,
+                       g->d_skip);
     (void)hipEventRecord(g->evRho1, s);
-    hipLaunchKernelGGL(w_digits_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, g->d_D1, g->d_E1, g->d_G,
-                       rows, g->d_rho, g->d_vrs, g->d_skip, g->d_DW, g->d_EW);
-    hipLaunchKernelGGL(fock_kernel_mixed, dim3(g->nBlocks), dim3(256), 0, s, g->d_blocks, g->d_idx, g->d_D1, g->d_E1,
-                       g->d_DW, g->d_EW, g->nFunctions, g->d_skip, g->d_V);
+    if (g->nChunks > 0) {
+        hipLaunchKernelGGL(w_digits_kernel, dim3(static_cast<unsigned>(g->nChunks)), dim3(256), 0, s, g->d_blocks,
+                           g->d_chunkBlock, g->d_DA, g->d_EA, g->d_GB, g->d_wcoef, g->d_skip, g->d_DW, g->d_EW);
+        const long long groups = (g->nTiles + 3) / 4;
+        const int perXcd = static_cast<int>((groups + 7) / 8);
+        hipLaunchKernelGGL(fock_kernel_mixed, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, g->d_tiles,
+                           static_cast<int>(g->nTiles), perXcd, g->d_blocks, g->d_chunkOffset, g->d_idx, g->d_DA, g->d_EA,
+                           g->d_DW, g->d_EW, n, g->d_skip, g->d_V);
+    }
     (void)hipEventRecord(g->evFock1, s);
     return kOK;
 }
