@@ -29,9 +29,8 @@ WORKLOADS = {
     "h2o128": (128, 18.0, 4, "pbe"),
 }
 METRIC = "XC grid pts/s mixed vs FP64 at |dExc|<=1e-6 Eh; p50/p99 SCF-request latency"
-PEAK_FP64_TFLOPS = 78.6
-PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+PEAK_INT8_TOPS = 5000.0
 
 
 def parse():
@@ -44,7 +43,33 @@ def parse():
     p.add_argument("--cpu-baseline", default="reference", choices=["reference", "port", "none"])
     p.add_argument("--cpu-sample-waters", type=int, default=8)
     p.add_argument("--skip-fp64", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="after the measurement, copy 1 GiB device to device once (a plain copy kernel of known traffic "
+                        "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
+
+
+def copy_calibration():
+    """One device-to-device copy of exactly 1 GiB (read 1 GiB, write 1 GiB) by torch's copy kernel."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "{:s}_{:s}_pmc.txt".format(kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def log(*a):
@@ -80,6 +105,8 @@ def cpu_baseline(kind, n_waters, functional, spherical):
     import oracle
     from pdynamo3_amd import basis, densities, molecules
     from pdynamo3_amd.definitions import functional_ids
+    if kind == "reference" and not oracle.reference_available():
+        kind = "port"
     lib = oracle.reference() if kind == "reference" else oracle.oracle()
     Z, xa, rots, _ = molecules.water_cluster(n_waters, 9.0 * (n_waters / 16.0) ** (1.0 / 3.0), 1)
     xyz = molecules.to_bohr(xa)
@@ -87,9 +114,8 @@ def cpu_baseline(kind, n_waters, functional, spherical):
     P = densities.cluster_density(rots, spherical=spherical)
     h = lib.setup(2, Z, xyz, spec, spherical, functional_ids(functional))
     # The reference's first call evaluates and caches the basis values (inCore); time the second, as the GPU does.
-    arg = 1 if kind == "reference" else 1
-    h.integrate(P, arg)
-    _, _, _, sec = h.integrate(P, arg)
+    h.integrate(P, 1)
+    _, _, _, sec = h.integrate(P, 1)
     return {"value": h.npoints / sec, "unit": "grid points/s", "cores": 1, "kind": kind,
             "sample": "(H2O){:d} cluster, same generator, {:s}, Medium grid ({:d} points), one in-core quadrature call, "
                       "{:.2f} s".format(n_waters, functional.upper(), h.npoints, sec)}
@@ -129,18 +155,23 @@ def main():
     st = mixed["stats"]
     ms = float(np.mean(mixed["device_ms"]))
     value = G / (ms * 1e-3)
-    # Roofline of the dominant kernel (largest mean device time).
-    flops = 2.0 * st["bf_squares"]
-    kernels = {"density_kernel": float(np.mean(mixed["rho_ms"])), "fock_kernel": float(np.mean(mixed["fock_ms"]))}
-    dom = max(kernels, key=kernels.get)
-    achieved = flops / (kernels[dom] * 1e-3) / 1e12
-    roofline = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "algorithmic": "2 * sum_b nb_b^2 G_b = {:.4g} flop per launch".format(flops),
-                "kernel_ms": kernels}
+    # Roofline of the dominant kernel: density_kernel_mixed (one kernel between the rho events; the Fock stage is two
+    # kernels).  Both its HBM time and its int8 MFMA time at peak are computed; the larger names the bound.
+    rho_ms = float(np.mean(mixed["rho_ms"]))
+    rho_bytes = st["rho_bytes"]
+    int8_ops = 2.0 * 14.0 * st["bf_squares"]          # 14 digit products per element of P_red Phi
+    t_hbm, t_mfma = rho_bytes / (PEAK_HBM_GBS * 1e9), int8_ops / (PEAK_INT8_TOPS * 1e12)
+    if t_hbm >= t_mfma:
+        achieved, peak, unit, bound = rho_bytes / (rho_ms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s", "hbm"
+    else:
+        achieved, peak, unit, bound = int8_ops / (rho_ms * 1e-3) / 1e12, PEAK_INT8_TOPS, "TOP/s", "mfma"
+    roofline = {"kernel": "density_kernel_mixed", "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+                "frac": achieved / peak, "traffic": measured_traffic("density_kernel_mixed", args.workload),
+                "algorithmic_bytes": rho_bytes, "algorithmic_int8_ops": int8_ops, "kernel_ms": rho_ms,
+                "fock_stage_ms": float(np.mean(mixed["fock_ms"])), "fock_stage_bytes": st["fock_bytes"]}
     out = {"metric": METRIC, "value": value, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "fp32-storage/fp64-accumulate", "data": "synthetic",
+           "dtype": "int8 digits x int8 -> int32 (exact) with f64 functional and sums", "data": "synthetic",
            "config": {"workload": "(H2O){:d} cluster, {:.0f} A cube, seed {:d}, 6-31G* {:s} ({:d} functions), {:s}, "
                                   "Medium grid ({:d} points)".format(n_w, cube, seed,
                                                                      "spherical" if spherical else "Cartesian", n,
@@ -165,6 +196,8 @@ def main():
             out["cpu_baseline"] = {"value": None, "error": repr(exc)}
     print(json.dumps(out), flush=True)
     grid.close()
+    if args.copy_calibration:
+        copy_calibration()
     return 0
 
 

@@ -57,8 +57,9 @@ int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, 
 !  densityP: packed total density (n(n+1)/2).  fockV: packed, incremented by V_xc.  Statistics (optional, may be NULL, else
 !  XQ_NSTATS doubles): [0] sum over blocks of kept functions x points ("Quadrature BF Values"), [1] blocks skipped by the
 !  density tolerance, [2] device ms from P expansion to packed V, [3] device ms of the density/functional kernel, [4] device
-!  ms of the Fock kernel, [5] sum over blocks of kept functions^2 x points (the contractions' algorithmic flops are
-!  2 [5] each). */
+!  ms of the Fock stage, [5] sum over blocks of kept functions^2 x points (the contractions' algorithmic flops are
+!  2 [5] each), [6] and [7] algorithmic HBM bytes of the density and Fock stages (each stored operand read once, each
+!  result written once; for FP64 both stages are one kernel and [7] is 0). */
 # define XQ_NSTATS 8
 int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
                                   double *nElectrons, double *fockV, double *stats ) ;

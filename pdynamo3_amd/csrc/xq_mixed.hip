@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256) p_slice_kernel(int n, const double *__res
 // Density kernel (per block, 256 threads = 4 waves): X = P_red Phi on int8 MFMA, rho and grad rho, the FP64 functional,
 // E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
 // point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  The P_red tile (I, c) is staged in LDS (double
-// buffered, one barrier per tile) by all four waves, its words fetched one tile ahead.
+// buffered, one barrier per tile) by all four waves; its words and the Phi operands are fetched one tile ahead.
 __global__ void __launch_bounds__(256) density_kernel_mixed(
     const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ idxAll,
     const signed char *__restrict__ DB, const signed char *__restrict__ EB, const float *__restrict__ GB, int n,
@@ -270,6 +270,13 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
     };
     unsigned long long wcur[4];
     fetch(0, 0, wcur);
+    auto fetchB = [&](int c, v4i (&bq)[kDPhi]) {
+#pragma unroll
+        for (int j = 0; j < kDPhi; j++)
+            bq[j] = *reinterpret_cast<const v4i *>(&DBb[(c * kDPhi + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
+    };
+    v4i bnext[kDPhi];
+    fetchB(0, bnext);
     int buf = 0;
     const int eb = EB[static_cast<size_t>(b) * kBlockSize + p];
     double pr = 0.0, pgx = 0.0, pgy = 0.0, pgz = 0.0;
@@ -287,15 +294,14 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
                 sA[buf][j][sr][sdw] = static_cast<int>(x);
             }
             __syncthreads();
+            v4i a[kDP], bq[kDPhi];
+#pragma unroll
+            for (int j = 0; j < kDPhi; j++) bq[j] = bnext[j];
             {
                 int In = I, cn = c + 1;
                 if (cn == nChunks) { cn = 0; In = I + 1; }
-                if (In < nChunks) fetch(In, cn, wcur);
+                if (In < nChunks) { fetch(In, cn, wcur); fetchB(cn, bnext); }
             }
-            v4i a[kDP], bq[kDPhi];
-#pragma unroll
-            for (int j = 0; j < kDPhi; j++)
-                bq[j] = *reinterpret_cast<const v4i *>(&DBb[(c * kDPhi + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
 #pragma unroll
             for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sA[buf][j][lr][4 * lh]);
 #pragma unroll
@@ -307,9 +313,7 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
         }
         // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi FP32.
         v4i dI[kDPhi];
-#pragma unroll
-        for (int j = 0; j < kDPhi; j++)
-            dI[j] = *reinterpret_cast<const v4i *>(&DBb[(I * kDPhi + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
+        fetchB(I, dI);
         const float *g = GB + (C + I) * kChunkFloats + p * kC + 16 * lh;
 #pragma unroll
         for (int v = 0; v < 16; v++) {

@@ -74,7 +74,8 @@ class QuadratureGrid:
         _check(_lib.lib.xq_integrate(self._h, ids.size, _lib.iptr(ids), _lib.dptr(P), ctypes.byref(e), ctypes.byref(ne),
                                      _lib.dptr(V), _lib.dptr(stats)), "integrate")
         return e.value, ne.value, {"bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2],
-                                   "rho_ms": stats[3], "fock_ms": stats[4], "bf_squares": stats[5]}
+                                   "rho_ms": stats[3], "fock_ms": stats[4], "bf_squares": stats[5],
+                                   "rho_bytes": stats[6], "fock_bytes": stats[7]}
 
     def device_bytes(self):
         b = ctypes.c_longlong()
