@@ -157,22 +157,26 @@ __device__ __forceinline__ double becke_poly(double nu) {
     return sum;
 }
 
-// Factor of P_k from the pair (k, m), k != m: the pair data is stored for the ordered pair (max, min).
-__device__ __forceinline__ double becke_factor(int k, int m, double rk, double rm, const double *invR, const double *aij) {
-    int hi = k > m ? k : m, lo = k > m ? m : k;
-    int ij = (hi * (hi - 1)) / 2 + lo;
-    double rhi = k > m ? rk : rm, rlo = k > m ? rm : rk;
-    double mu = (rhi - rlo) * invR[ij];
-    double nu = mu + aij[ij] * (1.0 - mu * mu);
-    double s = kBeckeA * becke_poly(nu);
-    return k > m ? (0.5 - s) : (0.5 + s);
+// Factor of P_k from the pair (k, m), k != m, given the pair's 1/R and size adjustment (stored once for the ordered pair
+// (max, min)): s(nu) for k > m, 1 - s(nu) for k < m, with mu = (r_max - r_min) / R.
+__device__ __forceinline__ double becke_factor_of(bool kHigh, double rk, double rm, double invR, double a) {
+    const double rhi = kHigh ? rk : rm, rlo = kHigh ? rm : rk;
+    const double mu = (rhi - rlo) * invR;
+    const double nu = mu + a * (1.0 - mu * mu);
+    const double s = kBeckeA * becke_poly(nu);
+    return kHigh ? (0.5 - s) : (0.5 + s);
 }
 
-__device__ __forceinline__ double dist(const double *xyz, int i, double px, double py, double pz) {
-    double dx = xyz[3 * i] - px, dy = xyz[3 * i + 1] - py, dz = xyz[3 * i + 2] - pz;
-    return sqrt(dx * dx + dy * dy + dz * dz);
+__device__ __forceinline__ int pair_index(int k, int m) {
+    const int hi = k > m ? k : m, lo = k > m ? m : k;
+    return (hi * (hi - 1)) / 2 + lo;
 }
 
+// Candidate points of one radial shell per work-group (thread = angular point) and their Becke weights
+// w = 4 pi w_r w_ang P_A / sum_i P_i (DFTGrid.c:158-211, DFTGridWeights.c:220-258).  The atom coordinates and the pair
+// data of the shell's atom A (row A of 1/R and a) are staged in LDS; P_A is computed in full, and each other P_i stops
+// once it falls below 2^-64 P_A (it can no longer change the sum).  The arithmetic of every factor is that of the host
+// restatement, so the weights, and the points kept by |w| > 1e-30, are the same bit for bit.
 __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell *__restrict__ shells,
                                                                const double *__restrict__ rules,
                                                                const double *__restrict__ xyz, int nAtoms,
@@ -181,28 +185,44 @@ __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell
                                                                double *__restrict__ cx, double *__restrict__ cy,
                                                                double *__restrict__ cz, double *__restrict__ cw,
                                                                int *__restrict__ keep) {
+    extern __shared__ double lds[];
+    double *sx = lds, *sy = lds + nAtoms, *sz = lds + 2 * nAtoms, *sInv = lds + 3 * nAtoms, *sA = lds + 4 * nAtoms;
     const RadialShell sh = shells[blockIdx.x];
     const int A = sh.atom;
-    const double ax = xyz[3 * A], ay = xyz[3 * A + 1], az = xyz[3 * A + 2];
+    for (int j = threadIdx.x; j < nAtoms; j += blockDim.x) {
+        sx[j] = xyz[3 * j];
+        sy[j] = xyz[3 * j + 1];
+        sz[j] = xyz[3 * j + 2];
+        const int ij = j == A ? 0 : pair_index(A, j);
+        sInv[j] = j == A ? 0.0 : invR[ij];
+        sA[j] = j == A ? 0.0 : aij[ij];
+    }
+    __syncthreads();
+    auto dist = [&](int i, double px, double py, double pz) {
+        const double dx = sx[i] - px, dy = sy[i] - py, dz = sz[i] - pz;
+        return sqrt(dx * dx + dy * dy + dz * dz);
+    };
+    const double ax = sx[A], ay = sy[A], az = sz[A];
     for (int k = threadIdx.x; k < sh.nAng; k += blockDim.x) {
         const double *u = rules + 4 * (sh.rule + k);
         const double px = u[0] * sh.r + ax, py = u[1] * sh.r + ay, pz = u[2] * sh.r + az;
-        const double rA = dist(xyz, A, px, py, pz);
+        const double rA = dist(A, px, py, pz);
         // P_A in full.
         double PA = 1.0;
         for (int j = 0; j < nAtoms; j++) {
             if (j == A) continue;
-            PA *= becke_factor(A, j, rA, dist(xyz, j, px, py, pz), invR, aij);
+            PA *= becke_factor_of(A > j, rA, dist(j, px, py, pz), sInv[j], sA[j]);
         }
         const double floor = PA * 0x1p-64;
         double sum = PA;
         for (int i = 0; i < nAtoms; i++) {
             if (i == A) continue;
-            const double ri = dist(xyz, i, px, py, pz);
-            double Pi = becke_factor(i, A, ri, rA, invR, aij);
+            const double ri = dist(i, px, py, pz);
+            double Pi = becke_factor_of(i > A, ri, rA, sInv[i], sA[i]);
             for (int j = 0; j < nAtoms && fabs(Pi) >= floor; j++) {
                 if (j == i || j == A) continue;
-                Pi *= becke_factor(i, j, ri, dist(xyz, j, px, py, pz), invR, aij);
+                const int ij = pair_index(i, j);
+                Pi *= becke_factor_of(i > j, ri, dist(j, px, py, pz), invR[ij], aij[ij]);
             }
             if (fabs(Pi) >= floor) sum += Pi;
         }
@@ -232,30 +252,66 @@ __global__ void compact_points_kernel(int n, const int *__restrict__ keep, const
     }
 }
 
-// Single work-group exclusive scan (n up to a few million: each thread scans a contiguous chunk).
-__global__ void __launch_bounds__(1024) exclusive_scan_kernel(int n, const int *__restrict__ in, int *__restrict__ out,
-                                                              int *__restrict__ total) {
-    __shared__ int partial[1024];
-    const int t = threadIdx.x, T = blockDim.x;
-    const int chunk = (n + T - 1) / T;
-    const int lo = t * chunk, hi = min(n, lo + chunk);
-    int s = 0;
-    for (int i = lo; i < hi; i++) s += in[i];
-    partial[t] = s;
+// Exclusive prefix sum of keep in three passes over tiles of kScanTile entries: tile sums, a scan of the tile sums in
+// one work-group, then each tile's own scan offset by its tile's prefix.
+constexpr int kScanTile = 1024;
+
+__device__ __forceinline__ int block_exclusive_scan_256(int v, int *sh, int *total) {
+    // Work-group of 256: inclusive wave scans, then the four wave totals.
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
     __syncthreads();
-    // Hillis-Steele inclusive scan over the per-thread sums.
-    for (int off = 1; off < T; off <<= 1) {
-        int v = t >= off ? partial[t - off] : 0;
-        __syncthreads();
-        partial[t] += v;
-        __syncthreads();
+    int base = 0;
+    for (int k = 0; k < wv; k++) base += sh[k];
+    if (total) *total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return base + x - v;
+}
+
+__global__ void __launch_bounds__(256) scan_tile_sums_kernel(int n, const int *__restrict__ in, int *__restrict__ sums) {
+    __shared__ int sh[4];
+    const int base = blockIdx.x * kScanTile + 4 * threadIdx.x;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v += base + k < n ? in[base + k] : 0;
+    int total = 0;
+    (void)block_exclusive_scan_256(v, sh, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(256) scan_sums_kernel(int nTiles, int *__restrict__ sums, int *__restrict__ total) {
+    __shared__ int sh[4];
+    int run = 0;
+    for (int base = 0; base < nTiles; base += 256) {
+        const int i = base + threadIdx.x;
+        const int v = i < nTiles ? sums[i] : 0;
+        int chunk = 0;
+        const int ex = block_exclusive_scan_256(v, sh, &chunk);
+        if (i < nTiles) sums[i] = run + ex;
+        run += chunk;
     }
-    int run = t > 0 ? partial[t - 1] : 0;
-    for (int i = lo; i < hi; i++) {
-        out[i] = run;
-        run += in[i];
+    if (threadIdx.x == 0) *total = run;
+}
+
+__global__ void __launch_bounds__(256) scan_apply_kernel(int n, const int *__restrict__ in, const int *__restrict__ sums,
+                                                         int *__restrict__ out) {
+    __shared__ int sh[4];
+    const int base = blockIdx.x * kScanTile + 4 * threadIdx.x;
+    int e[4], v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) { e[k] = base + k < n ? in[base + k] : 0; v += e[k]; }
+    int run = sums[blockIdx.x] + block_exclusive_scan_256(v, sh, nullptr);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += e[k];
     }
-    if (t == T - 1) *total = partial[T - 1];
 }
 
 }  // namespace
@@ -330,11 +386,18 @@ int build_grid(xq_grid *g) {
     RadialShell *d_shells = nullptr;
     double *d_rules = nullptr, *d_xyz = nullptr, *d_invR = nullptr, *d_aij = nullptr;
     double *cx = nullptr, *cy = nullptr, *cz = nullptr, *cw = nullptr;
-    int *keep = nullptr, *dst = nullptr, *total = nullptr;
+    int *keep = nullptr, *dst = nullptr, *total = nullptr, *tileSums = nullptr;
+    // LDS of the candidate kernel: atom coordinates and the shell atom's pair row (5 doubles per atom).
+    const size_t lds = 5 * sizeof(double) * static_cast<size_t>(nA);
+    if (lds > 64 * 1024) {
+        set_error("grid construction supports at most 1638 atoms");
+        return kInvalidArgument;
+    }
     int status = kOK;
     auto cleanup = [&]() {
         dev_free(&d_shells); dev_free(&d_rules); dev_free(&d_xyz); dev_free(&d_invR); dev_free(&d_aij);
         dev_free(&cx); dev_free(&cy); dev_free(&cz); dev_free(&cw); dev_free(&keep); dev_free(&dst); dev_free(&total);
+        dev_free(&tileSums);
     };
     const long long before = g->deviceBytes;
     do {
@@ -350,6 +413,7 @@ int build_grid(xq_grid *g) {
         if ((status = dev_alloc(g, &keep, nCand)) != kOK) break;
         if ((status = dev_alloc(g, &dst, nCand)) != kOK) break;
         if ((status = dev_alloc(g, &total, 1)) != kOK) break;
+        if ((status = dev_alloc(g, &tileSums, static_cast<size_t>((nCand + kScanTile - 1) / kScanTile))) != kOK) break;
         hipStream_t s = g->stream;
         if ((status = check(hipMemcpyAsync(d_shells, shells.data(), shells.size() * sizeof(RadialShell),
                                            hipMemcpyHostToDevice, s), "copy shells")) != kOK) break;
@@ -362,11 +426,14 @@ int build_grid(xq_grid *g) {
         if ((status = check(hipMemcpyAsync(d_aij, aij.data(), nPairs * sizeof(double), hipMemcpyHostToDevice, s),
                             "copy aij")) != kOK) break;
         if (!shells.empty()) {
-            hipLaunchKernelGGL(candidate_points_kernel, dim3(shells.size()), dim3(256), 0, s, d_shells, d_rules, d_xyz, nA,
+            hipLaunchKernelGGL(candidate_points_kernel, dim3(shells.size()), dim3(256), lds, s, d_shells, d_rules, d_xyz, nA,
                                d_invR, d_aij, cx, cy, cz, cw, keep);
             if ((status = check(hipGetLastError(), "candidate_points_kernel")) != kOK) break;
-            hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, s, nCand, keep, dst, total);
-            if ((status = check(hipGetLastError(), "exclusive_scan_kernel")) != kOK) break;
+            const int nTiles = (nCand + kScanTile - 1) / kScanTile;
+            hipLaunchKernelGGL(scan_tile_sums_kernel, dim3(nTiles), dim3(256), 0, s, nCand, keep, tileSums);
+            hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, s, nTiles, tileSums, total);
+            hipLaunchKernelGGL(scan_apply_kernel, dim3(nTiles), dim3(256), 0, s, nCand, keep, tileSums, dst);
+            if ((status = check(hipGetLastError(), "scan kernels")) != kOK) break;
         }
         // Per-atom kept counts from the scan at the atom boundaries.
         std::vector<int> scanned(nCand > 0 ? nCand : 1, 0), kept(nCand > 0 ? nCand : 1, 0);

@@ -153,12 +153,12 @@ int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
             stats[6] = 32.0 * static_cast<double>(g->nValueRows) * kBlockSize + 8.0 * nn + 8.0 * G + 20.0 * B + 16.0 * nn;
             stats[7] = 0.0;
         } else {
-            // Density: Phi digits (16 KB per chunk), FP32 gradients (48 KB per chunk), P digits, point exponents,
-            // weights in, W coefficients out, block sums.  Fock stage: W formation (Phi digits, gradients,
+            // Density: Phi digits (16 KB per chunk), fixed-point gradients (36 KB per chunk), P digits, point
+            // exponents, weights in, W coefficients out, block sums.  Fock stage: W formation (Phi digits, gradients,
             // coefficients in, W digits out) and the Fock product (Phi and W digits in, V read and written).
             const double ch = static_cast<double>(g->nChunks);
-            stats[6] = ch * (16384.0 + 49152.0) + 8.0 * nn + 128.0 * B + 40.0 * G + 20.0 * B;
-            stats[7] = ch * (16384.0 + 49152.0 + 16384.0 + 32.0) + 32.0 * G + ch * 2.0 * 16384.0 + 16.0 * nn;
+            stats[6] = ch * (16384.0 + 36864.0) + 8.0 * nn + 512.0 * B + 40.0 * G + 20.0 * B;
+            stats[7] = ch * (16384.0 + 36864.0 + 16384.0 + 32.0) + 32.0 * G + 384.0 * B + ch * 2.0 * 16384.0 + 16.0 * nn;
         }
     }
     return kOK;

@@ -76,11 +76,13 @@ struct xq_grid {
     int *d_idx = nullptr;                 // kept-function indices, per block
     void *d_f = nullptr, *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;   // values and gradients, rows of kBlockSize
     // Mixed precision (layouts in xq_mixed.hip): chunk tables, Fock tile list, digit planes and exponents of Phi in two
-    // orders, FP32 gradients, per-step W digits, P digits and W coefficients.
+    // orders, fixed-point gradients, per-step W digits, P digits and W coefficients.
     int *d_chunkOffset = nullptr, *d_chunkBlock = nullptr;
     int4 *d_tiles = nullptr;
     signed char *d_DA = nullptr, *d_EA = nullptr, *d_DB = nullptr, *d_EB = nullptr;
-    float *d_GB = nullptr;
+    short *d_GH = nullptr;
+    unsigned char *d_GL = nullptr;
+    signed char *d_EG = nullptr;
     signed char *d_DW = nullptr, *d_EW = nullptr;
     unsigned long long *d_Pq = nullptr;
     int *d_eP = nullptr;

@@ -17,7 +17,8 @@
 //   V += Phi W^T + W Phi^T  Phi and W: 4 digits, one power of two per function row of a block (over its 128 points).
 //                           Orders k + l <= 3.  W = (w vrho / 2) Phi + 2 w vsigma (grad rho . grad Phi) is formed and
 //                           sliced per step.
-// The functional, the weights and every reduction are FP64.
+// The functional, the weights and every reduction are FP64.  grad Phi is stored as 24-bit fixed point relative to
+// each point's largest component (a per-value FP32 rounding biases sigma = |grad rho|^2 measurably; this does not).
 //
 // Data layouts.  The kept functions of a block are cut into chunks of 32 rows; chunk C of the grid is chunk rc of block b
 // (C = chunkOffset[b] + rc).  Rows past the block's kept count are zero.
@@ -30,7 +31,9 @@
 //                                  whose result the MFMA lane (., h) receives in register v, so that the epilogue finds
 //                                  Phi for its own results in the operand it loads.  P_red tiles use the same K order.
 //                                  Exponents EB [block][128].
-//   GB     [C][3][point (128)][32] FP32            grad Phi in DB's order.
+//   GH, GL [C][3][point (128)][32]                 grad Phi in DB's order as 24-bit fixed point: value =
+//                                  (256 GH + GL) 2^(EG - 23), GH int16, GL uint8, one power of two per point and
+//                                  component over the block's kept functions (EG [block][3][128]).
 //   Pq     [n][n] 64-bit words                     P digits 0..4 in bytes 0..4; row exponents eP [n].
 //
 // MFMA lane layout (32 x 32 x 32, int8): lane l supplies A[l % 32][16 (l / 32) .. + 15] and B[16 (l / 32) .. + 15][l % 32]
@@ -53,7 +56,10 @@ constexpr int kC = 32;                  // chunk length (functions or points) an
 constexpr int kDPhi = 4;                // digits of Phi and W
 constexpr int kDP = 5;                  // digits of P
 constexpr int kChunkBytes = 16384;      // one chunk of DA, DW or DB: 4 digits x 32 rows x 128 points
-constexpr int kChunkFloats = 3 * kBlockSize * kC;   // one chunk of GB
+constexpr int kChunkGrad = 3 * kBlockSize * kC;     // one chunk of GH or GL (elements)
+constexpr int kGradBits = 23;           // grad Phi fixed point: |value| < 2^EG is stored as a multiple of 2^(EG - 23)
+constexpr int kPanel = 8;               // chunks of P_red columns staged at once by the density kernel
+constexpr int kPanelDwords = 8 * kPanel + 4;        // one panel row in LDS (padded against bank conflicts)
 
 // Stored exponents are int8.  A group whose largest magnitude is below 2^-100 (contributions below 1e-30 of unit-sized
 // terms) gets exponent 0, which rounds all its digits to zero.  Magnitudes here stay far below 2^127.
@@ -110,6 +116,24 @@ __device__ __forceinline__ int digits4(const v4i (&d)[kDPhi], int v) {
     return ((byte_of(d[0], v) * 128 + byte_of(d[1], v)) * 128 + byte_of(d[2], v)) * 128 + byte_of(d[3], v);
 }
 
+// Signed 16-bit element v (0..15) of 32 bytes, and unsigned byte v of 16 bytes.
+__device__ __forceinline__ int short_of(const v4i (&h)[2], int v) {
+    const unsigned w = static_cast<unsigned>(h[v >> 3][(v >> 1) & 3]);
+    return static_cast<int>(w << (16 - 16 * (v & 1))) >> 16;
+}
+__device__ __forceinline__ int ubyte_of(const v4i &x, int v) {
+    return static_cast<int>((static_cast<unsigned>(x[v >> 2]) >> (8 * (v & 3))) & 255u);
+}
+
+// 24-bit fixed point of g with |g| < 2^e: Q = round(g 2^(23 - e)) in [-(2^23 - 1), 2^23 - 1], split as 256 hi + lo.
+__device__ __forceinline__ void grad_encode(double g, int e, short &hi, unsigned char &lo) {
+    long long Q = static_cast<long long>(rint(ldexp(g, kGradBits - e)));
+    const long long lim = (1ll << kGradBits) - 1;
+    Q = Q > lim ? lim : (Q < -lim ? -lim : Q);
+    hi = static_cast<short>(Q >> 8);
+    lo = static_cast<unsigned char>(Q & 255);
+}
+
 // Row of a chunk held at density position 16 h + v, and the position of row k.
 __host__ __device__ constexpr int chunk_fun(int h, int v) { return 4 * h + 8 * (v >> 2) + (v & 3); }
 __host__ __device__ constexpr int chunk_pos(int k) { return 16 * ((k >> 2) & 1) + 4 * (k >> 3) + (k & 3); }
@@ -117,24 +141,37 @@ __host__ __device__ constexpr int chunk_pos(int k) { return 16 * ((k >> 2) & 1) 
 // ------------------------------------------------------------------------------------------------------------------------
 // Set-up, once per geometry.
 // ------------------------------------------------------------------------------------------------------------------------
-// EB: per point of a block, the power of two above max |Phi| over the block's kept functions (128 threads = points).
-__global__ void __launch_bounds__(kBlockSize) point_exponent_kernel(const xq::DevBlock *__restrict__ blocks,
-                                                                    const double *__restrict__ f,
-                                                                    signed char *__restrict__ EB) {
+// EB and EG: per point of a block, the powers of two above max |Phi| and above max |d Phi / dx_c| over the block's kept
+// functions (128 threads = points).
+__global__ void __launch_bounds__(kBlockSize) point_exponent_kernel(
+    const xq::DevBlock *__restrict__ blocks, const double *__restrict__ f, const double *__restrict__ fx,
+    const double *__restrict__ fy, const double *__restrict__ fz, signed char *__restrict__ EB,
+    signed char *__restrict__ EG) {
     const int b = blockIdx.x, t = threadIdx.x;
     const xq::DevBlock blk = blocks[b];
     const size_t R = static_cast<size_t>(blk.vOffset);
-    double m = 0.0;
-    for (int r = 0; r < blk.nKept; r++) m = fmax(m, fabs(f[(R + r) * kBlockSize + t]));
+    double m = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    for (int r = 0; r < blk.nKept; r++) {
+        const size_t o = (R + r) * kBlockSize + t;
+        m = fmax(m, fabs(f[o]));
+        mx = fmax(mx, fabs(fx[o]));
+        my = fmax(my, fabs(fy[o]));
+        mz = fmax(mz, fabs(fz[o]));
+    }
     EB[static_cast<size_t>(b) * kBlockSize + t] = static_cast<signed char>(stored_exponent(m));
+    signed char *eg = EG + static_cast<size_t>(b) * 3 * kBlockSize + t;
+    eg[0] = static_cast<signed char>(stored_exponent(mx));
+    eg[kBlockSize] = static_cast<signed char>(stored_exponent(my));
+    eg[2 * kBlockSize] = static_cast<signed char>(stored_exponent(mz));
 }
 
-// One chunk per work-group (256 threads): EA, DA, DB and GB from the FP64 values of the chunk's rows.
+// One chunk per work-group (256 threads): EA, DA, DB, GH and GL from the FP64 values of the chunk's rows.
 __global__ void __launch_bounds__(256) mixed_prepare_kernel(
     const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ chunkBlock,
     const double *__restrict__ f, const double *__restrict__ fx, const double *__restrict__ fy,
-    const double *__restrict__ fz, const signed char *__restrict__ EB, signed char *__restrict__ DA,
-    signed char *__restrict__ EA, signed char *__restrict__ DB, float *__restrict__ GB) {
+    const double *__restrict__ fz, const signed char *__restrict__ EB, const signed char *__restrict__ EG,
+    signed char *__restrict__ DA, signed char *__restrict__ EA, signed char *__restrict__ DB, short *__restrict__ GH,
+    unsigned char *__restrict__ GL) {
     __shared__ int sE[kC];
     const size_t C = blockIdx.x;
     const int b = chunkBlock[C];
@@ -174,32 +211,33 @@ __global__ void __launch_bounds__(256) mixed_prepare_kernel(
         for (int j = 0; j < kDPhi; j++)
             *reinterpret_cast<v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]) = pack16(d[j]);
     }
-    // DB and GB: thread = (point p, half h) with the 16 rows chunk_fun(h, 0..15).
+    // DB, GH and GL: thread = (point p, half h) with the 16 rows chunk_fun(h, 0..15).
     {
         const int p = t & (kBlockSize - 1), h = t >> 7;
         const int e = EB[static_cast<size_t>(b) * kBlockSize + p];
         signed char d[kDPhi][16];
-        float g[3][16];
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-            const int row = kC * rc + chunk_fun(h, v);
             signed char dd[kDPhi];
-            slice<kDPhi>(value(f, row, p), e, dd);
+            slice<kDPhi>(value(f, kC * rc + chunk_fun(h, v), p), e, dd);
 #pragma unroll
             for (int j = 0; j < kDPhi; j++) d[j][v] = dd[j];
-            g[0][v] = static_cast<float>(value(fx, row, p));
-            g[1][v] = static_cast<float>(value(fy, row, p));
-            g[2][v] = static_cast<float>(value(fz, row, p));
         }
 #pragma unroll
         for (int j = 0; j < kDPhi; j++)
             *reinterpret_cast<v4i *>(&DB[C * kChunkBytes + (j * kBlockSize + p) * kC + 16 * h]) = pack16(d[j]);
+        const double *grads[3] = {fx, fy, fz};
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+        for (int c = 0; c < 3; c++) {
+            const int eg = EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p];
+            short hi[16];
+            unsigned char lo[16];
 #pragma unroll
-            for (int v = 0; v < 16; v += 4)
-                *reinterpret_cast<float4 *>(&GB[C * kChunkFloats + (c * kBlockSize + p) * kC + 16 * h + v]) =
-                    make_float4(g[c][v], g[c][v + 1], g[c][v + 2], g[c][v + 3]);
+            for (int v = 0; v < 16; v++) grad_encode(value(grads[c], kC * rc + chunk_fun(h, v), p), eg, hi[v], lo[v]);
+            const size_t o = C * kChunkGrad + (c * kBlockSize + p) * kC + 16 * h;
+#pragma unroll
+            for (int v = 0; v < 16; v++) { GH[o + v] = hi[v]; GL[o + v] = lo[v]; }
+        }
     }
 }
 
@@ -232,14 +270,14 @@ __global__ void __launch_bounds__(256) p_slice_kernel(int n, const double *__res
 
 // Density kernel (per block, 256 threads = 4 waves): X = P_red Phi on int8 MFMA, rho and grad rho, the FP64 functional,
 // E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
-// point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  The P_red tile (I, c) is staged in LDS (double
-// buffered, one barrier per tile) by all four waves; its words and the Phi operands are fetched one tile ahead.
-__global__ void __launch_bounds__(256) density_kernel_mixed(
+// point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  P_red is staged in LDS by row panels shared by
+// the four waves; the Phi operands are fetched one step ahead.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) density_kernel_mixed(
     const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ idxAll,
-    const signed char *__restrict__ DB, const signed char *__restrict__ EB, const float *__restrict__ GB, int n,
-    const unsigned long long *__restrict__ Pq, const int *__restrict__ eP, const double *__restrict__ w, FunctionalSet fs,
+    const signed char *__restrict__ DB, const signed char *__restrict__ EB, const short *__restrict__ GH,
+    const unsigned char *__restrict__ GL, const signed char *__restrict__ EG, int n, const unsigned long long *__restrict__ Pq, const int *__restrict__ eP, const double *__restrict__ w, FunctionalSet fs,
     double rhoTol, double *__restrict__ wcoef, double *__restrict__ blockEN, int *__restrict__ skip) {
-    __shared__ __attribute__((aligned(16))) int sA[2][kDP][kC][8];   // [buffer][digit][row][32 positions]
+    __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
     __shared__ int sPe[2][kC];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
@@ -257,19 +295,10 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
     const int nChunks = (nb + kC - 1) / kC;
     const int p = kC * wv + lr;
     const signed char *DBb = DB + C * kChunkBytes;
-    // Staging: thread (row sr, group skq) fetches P_red[sr][4 skq .. + 3], written as one word per digit at chunk_pos.
-    const int sr = t >> 3, skq = t & 7, sdw = 4 * (skq & 1) + (skq >> 1);
-    auto fetch = [&](int I, int c, unsigned long long (&wd)[4]) {
-        const int r = kC * I + sr;
-        const size_t rowOff = r < nb ? static_cast<size_t>(idx[r]) * n : 0;
-#pragma unroll
-        for (int jj = 0; jj < 4; jj++) {
-            const int k = kC * c + 4 * skq + jj;
-            wd[jj] = (r < nb && k < nb) ? Pq[rowOff + idx[k]] : 0ull;
-        }
-    };
-    unsigned long long wcur[4];
-    fetch(0, 0, wcur);
+    // P_red is staged one row panel at a time (rows 32 I .. + 31, up to kPanel chunks of columns), all of the panel's
+    // words fetched before the barrier; the panel's MFMA steps then run without barriers.  Thread (row sr, slot s8)
+    // fetches columns 4 s8 .. + 3 of every chunk of the panel and writes them as one word per digit at chunk_pos.
+    const int sr = t >> 3, s8 = t & 7, sdw = 4 * (s8 & 1) + (s8 >> 1);
     auto fetchB = [&](int c, v4i (&bq)[kDPhi]) {
 #pragma unroll
         for (int j = 0; j < kDPhi; j++)
@@ -277,54 +306,86 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
     };
     v4i bnext[kDPhi];
     fetchB(0, bnext);
-    int buf = 0;
     const int eb = EB[static_cast<size_t>(b) * kBlockSize + p];
+    double gscale[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) gscale[c] = ldexp(1.0, EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p] - kGradBits);
     double pr = 0.0, pgx = 0.0, pgy = 0.0, pgz = 0.0;
     for (int I = 0; I < nChunks; I++) {
-        if (t < kC) sPe[I & 1][t] = kC * I + t < nb ? eP[idx[kC * I + t]] : 0;
         v16i acc[kDP];
+        const int r = kC * I + sr;
+        const size_t rowOff = r < nb ? static_cast<size_t>(idx[r]) * n : 0;
+        for (int c0 = 0; c0 < nChunks; c0 += kPanel) {
+            const int nc = nChunks - c0 < kPanel ? nChunks - c0 : kPanel;
+            unsigned long long wd[kPanel][4];
 #pragma unroll
-        for (int s = 0; s < kDP; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int c = 0; c < nChunks; c++) {
+            for (int m = 0; m < kPanel; m++)
 #pragma unroll
-            for (int j = 0; j < kDP; j++) {
-                unsigned x = 0;
+                for (int jj = 0; jj < 4; jj++) {
+                    const int k = kC * (c0 + m) + 4 * s8 + jj;
+                    wd[m][jj] = (m < nc && r < nb && k < nb) ? Pq[rowOff + idx[k]] : 0ull;
+                }
+            __syncthreads();   // every wave is done with the previous panel
 #pragma unroll
-                for (int jj = 0; jj < 4; jj++) x |= (static_cast<unsigned>(wcur[jj] >> (8 * j)) & 255u) << (8 * jj);
-                sA[buf][j][sr][sdw] = static_cast<int>(x);
+            for (int m = 0; m < kPanel; m++) {
+                if (m < nc) {
+#pragma unroll
+                    for (int j = 0; j < kDP; j++) {
+                        unsigned x = 0;
+#pragma unroll
+                        for (int jj = 0; jj < 4; jj++) x |= (static_cast<unsigned>(wd[m][jj] >> (8 * j)) & 255u) << (8 * jj);
+                        sP[j][sr][8 * m + sdw] = static_cast<int>(x);
+                    }
+                }
+            }
+            if (c0 == 0) {
+                if (t < kC) sPe[I & 1][t] = kC * I + t < nb ? eP[idx[kC * I + t]] : 0;
+#pragma unroll
+                for (int s = 0; s < kDP; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
             }
             __syncthreads();
-            v4i a[kDP], bq[kDPhi];
+            for (int cl = 0; cl < nc; cl++) {
+                const int c = c0 + cl;
+                v4i a[kDP], bq[kDPhi];
 #pragma unroll
-            for (int j = 0; j < kDPhi; j++) bq[j] = bnext[j];
-            {
-                int In = I, cn = c + 1;
-                if (cn == nChunks) { cn = 0; In = I + 1; }
-                if (In < nChunks) { fetch(In, cn, wcur); fetchB(cn, bnext); }
+                for (int j = 0; j < kDPhi; j++) bq[j] = bnext[j];
+                if (c + 1 < nChunks) fetchB(c + 1, bnext);
+                else if (I + 1 < nChunks) fetchB(0, bnext);
+#pragma unroll
+                for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sP[j][lr][8 * cl + 4 * lh]);
+#pragma unroll
+                for (int i = 0; i < kDP; i++)
+#pragma unroll
+                    for (int j = 0; j < kDPhi; j++)
+                        if (i + j <= 4) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
             }
-#pragma unroll
-            for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sA[buf][j][lr][4 * lh]);
-#pragma unroll
-            for (int i = 0; i < kDP; i++)
-#pragma unroll
-                for (int j = 0; j < kDPhi; j++)
-                    if (i + j <= 4) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
-            buf ^= 1;
         }
-        // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi FP32.
+        // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi from
+        // its fixed point (the sums over v are scaled once).
         v4i dI[kDPhi];
         fetchB(I, dI);
-        const float *g = GB + (C + I) * kChunkFloats + p * kC + 16 * lh;
+        v4i gh[3][2], gl[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const size_t o = (C + I) * kChunkGrad + (c * kBlockSize + p) * kC + 16 * lh;
+            gh[c][0] = *reinterpret_cast<const v4i *>(GH + o);
+            gh[c][1] = *reinterpret_cast<const v4i *>(GH + o + 8);
+            gl[c] = *reinterpret_cast<const v4i *>(GL + o);
+        }
+        double sr = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int s[kDP] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
             const double x = ldexp(static_cast<double>(combine<kDP>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 42);
-            const double phi = ldexp(static_cast<double>(digits4(dI, v)), eb - 28);
-            pr += phi * x;
-            pgx += static_cast<double>(g[v]) * x;
-            pgy += static_cast<double>(g[kBlockSize * kC + v]) * x;
-            pgz += static_cast<double>(g[2 * kBlockSize * kC + v]) * x;
+            sr += static_cast<double>(digits4(dI, v)) * x;
+            sx += static_cast<double>(short_of(gh[0], v) * 256 + ubyte_of(gl[0], v)) * x;
+            sy += static_cast<double>(short_of(gh[1], v) * 256 + ubyte_of(gl[1], v)) * x;
+            sz += static_cast<double>(short_of(gh[2], v) * 256 + ubyte_of(gl[2], v)) * x;
         }
+        pr += ldexp(sr, eb - 28);
+        pgx += sx * gscale[0];
+        pgy += sy * gscale[1];
+        pgz += sz * gscale[2];
     }
     pr += __shfl_xor(pr, 32, 64);
     pgx += __shfl_xor(pgx, 32, 64);
@@ -375,8 +436,9 @@ __global__ void __launch_bounds__(256) density_kernel_mixed(
 // for the chunk's 32 rows, one power of two per row over the block's 128 points, written in DA's order (DW, EW).
 __global__ void __launch_bounds__(256) w_digits_kernel(
     const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkBlock, const signed char *__restrict__ DA,
-    const signed char *__restrict__ EA, const float *__restrict__ GB, const double *__restrict__ wcoef,
-    const int *__restrict__ skip, signed char *__restrict__ DW, signed char *__restrict__ EW) {
+    const signed char *__restrict__ EA, const short *__restrict__ GH, const unsigned char *__restrict__ GL,
+    const signed char *__restrict__ EG, const double *__restrict__ wcoef, const int *__restrict__ skip,
+    signed char *__restrict__ DW, signed char *__restrict__ EW) {
     __shared__ double sc[4][kBlockSize];
     __shared__ double sm[4][kC];
     const size_t C = blockIdx.x;
@@ -384,9 +446,13 @@ __global__ void __launch_bounds__(256) w_digits_kernel(
     if (skip[b]) return;
     const xq::DevBlock blk = blocks[b];
     const int t = threadIdx.x, lane = t & 63, q = t >> 6, lr = lane & 31, lh = lane >> 5;
+    // Per point: w vrho / 2, and 2 w vsigma grad rho scaled by the point's gradient powers of two.
     if (t < kBlockSize) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) sc[k][t] = t < blk.nPoints ? wcoef[4 * (static_cast<size_t>(blk.pStart) + t) + k] : 0.0;
+        for (int k = 0; k < 4; k++) {
+            const double c = t < blk.nPoints ? wcoef[4 * (static_cast<size_t>(blk.pStart) + t) + k] : 0.0;
+            sc[k][t] = k == 0 ? c : ldexp(c, EG[(static_cast<size_t>(b) * 3 + (k - 1)) * kBlockSize + t] - kGradBits);
+        }
     }
     __syncthreads();
     v4i d[kDPhi];
@@ -394,16 +460,18 @@ __global__ void __launch_bounds__(256) w_digits_kernel(
     for (int j = 0; j < kDPhi; j++)
         d[j] = *reinterpret_cast<const v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]);
     const int ea = EA[C * kC + lr];
-    const float *g = GB + C * kChunkFloats + chunk_pos(lr);
+    const size_t go = C * kChunkGrad + chunk_pos(lr);
+    auto grad = [&](int c, int p) {
+        const size_t o = go + (c * kBlockSize + p) * kC;
+        return static_cast<double>(static_cast<int>(GH[o]) * 256 + static_cast<int>(GL[o]));
+    };
     double W[16];
     double m = 0.0;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         const int p = kC * q + 16 * lh + k;
         const double phi = ldexp(static_cast<double>(digits4(d, k)), ea - 28);
-        W[k] = sc[0][p] * phi + sc[1][p] * static_cast<double>(g[p * kC]) +
-               sc[2][p] * static_cast<double>(g[(kBlockSize + p) * kC]) +
-               sc[3][p] * static_cast<double>(g[(2 * kBlockSize + p) * kC]);
+        W[k] = sc[0][p] * phi + sc[1][p] * grad(0, p) + sc[2][p] * grad(1, p) + sc[3][p] * grad(2, p);
         m = fmax(m, fabs(W[k]));
     }
     m = fmax(m, __shfl_xor(m, 32, 64));
@@ -519,7 +587,9 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     XQ_TRY(dev_alloc(g, &g->d_EA, nc * kC));
     XQ_TRY(dev_alloc(g, &g->d_DB, nc * kChunkBytes));
     XQ_TRY(dev_alloc(g, &g->d_EB, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * kBlockSize));
-    XQ_TRY(dev_alloc(g, &g->d_GB, nc * kChunkFloats));
+    XQ_TRY(dev_alloc(g, &g->d_GH, nc * kChunkGrad));
+    XQ_TRY(dev_alloc(g, &g->d_GL, nc * kChunkGrad));
+    XQ_TRY(dev_alloc(g, &g->d_EG, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * 3 * kBlockSize));
     XQ_TRY(dev_alloc(g, &g->d_DW, nc * kChunkBytes));
     XQ_TRY(dev_alloc(g, &g->d_EW, nc * kC));
     XQ_TRY(dev_alloc(g, &g->d_Pq, static_cast<size_t>(g->nFunctions) * g->nFunctions));
@@ -532,10 +602,12 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     if (!tiles.empty())
         XQ_CHECK(hipMemcpyAsync(g->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s));
     if (g->nBlocks > 0)
-        hipLaunchKernelGGL(point_exponent_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, f, g->d_EB);
+        hipLaunchKernelGGL(point_exponent_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, f, fx, fy, fz,
+                           g->d_EB, g->d_EG);
     if (chunks > 0)
         hipLaunchKernelGGL(mixed_prepare_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, g->d_blocks,
-                           g->d_chunkOffset, g->d_chunkBlock, f, fx, fy, fz, g->d_EB, g->d_DA, g->d_EA, g->d_DB, g->d_GB);
+                           g->d_chunkOffset, g->d_chunkBlock, f, fx, fy, fz, g->d_EB, g->d_EG, g->d_DA, g->d_EA,
+                           g->d_DB, g->d_GH, g->d_GL);
     XQ_CHECK(hipGetLastError());
     // The host vectors above must outlive the copies.
     XQ_CHECK(hipStreamSynchronize(s));
@@ -550,7 +622,9 @@ void free_mixed(xq_grid *g) {
     dev_free(&g->d_EA);
     dev_free(&g->d_DB);
     dev_free(&g->d_EB);
-    dev_free(&g->d_GB);
+    dev_free(&g->d_GH);
+    dev_free(&g->d_GL);
+    dev_free(&g->d_EG);
     dev_free(&g->d_DW);
     dev_free(&g->d_EW);
     dev_free(&g->d_Pq);
@@ -566,12 +640,13 @@ int launch_fused_mixed(xq_grid *g, const FunctionalSet &fs) {
     if (n > 0) hipLaunchKernelGGL(p_slice_kernel, dim3(n), dim3(256), 0, s, n, g->d_P, g->d_eP, g->d_Pq);
     (void)hipEventRecord(g->evRho0, s);
     hipLaunchKernelGGL(density_kernel_mixed, dim3(g->nBlocks), dim3(256), 0, s, g->d_blocks, g->d_chunkOffset, g->d_idx,
-                       g->d_DB, g->d_EB, g->d_GB, n, g->d_Pq, g->d_eP, g->d_w, fs, g->rhoTol, g->d_wcoef, g->d_blockEN,
+                       g->d_DB, g->d_EB, g->d_GH, g->d_GL, g->d_EG, n, g->d_Pq, g->d_eP, g->d_w, fs, g->rhoTol, g->d_wcoef, g->d_blockEN,
                        g->d_skip);
     (void)hipEventRecord(g->evRho1, s);
     if (g->nChunks > 0) {
         hipLaunchKernelGGL(w_digits_kernel, dim3(static_cast<unsigned>(g->nChunks)), dim3(256), 0, s, g->d_blocks,
-                           g->d_chunkBlock, g->d_DA, g->d_EA, g->d_GB, g->d_wcoef, g->d_skip, g->d_DW, g->d_EW);
+                           g->d_chunkBlock, g->d_DA, g->d_EA, g->d_GH, g->d_GL, g->d_EG, g->d_wcoef, g->d_skip,
+                           g->d_DW, g->d_EW);
         const long long groups = (g->nTiles + 3) / 4;
         const int perXcd = static_cast<int>((groups + 7) / 8);
         hipLaunchKernelGGL(fock_kernel_mixed, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, g->d_tiles,
