@@ -43,6 +43,11 @@ def parse():
     p.add_argument("--cpu-baseline", default="reference", choices=["reference", "port", "none"])
     p.add_argument("--cpu-sample-waters", type=int, default=8)
     p.add_argument("--skip-fp64", action="store_true")
+    p.add_argument("--serve-requests", type=int, default=256,
+                   help="requests in the serving stream (0 skips it)")
+    p.add_argument("--serve-slots", type=int, default=16, help="jobs set up or running at once in the serving loop")
+    p.add_argument("--serve-load", type=float, default=1.2,
+                   help="arrival rate as a multiple of the estimated capacity of the mixed path")
     p.add_argument("--copy-calibration", action="store_true",
                    help="after the measurement, copy 1 GiB device to device once (a plain copy kernel of known traffic "
                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -121,6 +126,29 @@ def cpu_baseline(kind, n_waters, functional, spherical):
                       "{:.2f} s".format(n_waters, functional.upper(), h.npoints, sec)}
 
 
+def serving_leg(args, points_per_s, log_fn):
+    """The serving stream: SCF requests on (H2O)N, N log-uniform in [4, 64] (seed 3), BLYP/PBE by coin flip, Poisson
+    arrivals at serve-load x the capacity estimated from the measured points/s (mean culled points ~ 0.8 x the
+    pre-cull estimate, ~12 quadratures per request)."""
+    from pdynamo3_amd import integrator, serving
+    reqs = serving.request_stream(n_requests=args.serve_requests, seed=3, rate=1.0)
+    mean_points = float(np.mean([0.8 * integrator.estimated_points(2, r.Z) for r in reqs]))
+    rate = args.serve_load * points_per_s / (mean_points * 12.0)
+    for r in reqs:
+        r.arrival /= rate
+    srv = serving.Server(precision="mixed", max_jobs=args.serve_slots)
+    finished, loop = srv.run(reqs)
+    srv.close()
+    out = serving.summarize(finished, loop)
+    out.update({"arrival_rate_per_s": rate, "load": args.serve_load, "slots": args.serve_slots,
+                "hbm_budget_bytes": srv.hbm_budget,
+                "stream": "{:d} requests, (H2O)N with N log-uniform in [4, 64] (seed 3), 6-31G* Cartesian, BLYP/PBE by "
+                          "coin flip, Poisson arrivals; each request is an SCF density sequence converging geometrically "
+                          "(ratio 0.4) until |dE_xc| < 1e-6 Eh".format(len(reqs))})
+    log_fn("serving:", json.dumps(out))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -179,8 +207,8 @@ def main():
                       "parallelism": "single GPU"},
            "grid_points": G, "grid_build_s": grid_s, "basis_eval_s": mixed["basis_s"],
            "host_wall_ms_per_call_pcie_inclusive": float(np.mean(mixed["wall_ms"])),
-           "latency_ms_p50": float(np.percentile(mixed["wall_ms"], 50)),
-           "latency_ms_p99": float(np.percentile(mixed["wall_ms"], 99)),
+           "call_ms_p50": float(np.percentile(mixed["wall_ms"], 50)),
+           "call_ms_p99": float(np.percentile(mixed["wall_ms"], 99)),
            "E_xc": mixed["E"], "N_e": mixed["N"], "device_bytes": mixed["device_bytes"], "roofline": roofline}
     if fp64 is not None:
         ms64 = float(np.mean(fp64["device_ms"]))
@@ -189,13 +217,18 @@ def main():
         out["speedup_mixed_vs_fp64"] = ms64 / ms
         out["dExc_mixed_vs_fp64"] = mixed["E"] - fp64["E"]
         out["max_dVxc_mixed_vs_fp64"] = float(np.max(np.abs(mixed["V"] - fp64["V"])))
+    grid.close()
+    if args.serve_requests > 0:
+        serve = serving_leg(args, value, log)
+        out["latency_ms_p50"] = 1e3 * serve["latency_s_p50"]
+        out["latency_ms_p99"] = 1e3 * serve["latency_s_p99"]
+        out["serving"] = serve
     if args.cpu_baseline != "none":
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_baseline, args.cpu_sample_waters, functional, spherical)
         except Exception as exc:  # the bench line is still printed; the failure is named
             out["cpu_baseline"] = {"value": None, "error": repr(exc)}
     print(json.dumps(out), flush=True)
-    grid.close()
     if args.copy_calibration:
         copy_calibration()
     return 0

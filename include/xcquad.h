@@ -64,13 +64,29 @@ int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, 
 int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
                                   double *nElectrons, double *fockV, double *stats ) ;
 
-/* Device bytes this library holds for a molecule with the given grid size, function count and mean kept functions per
-!  block (admission control for the request loop). */
+/* Peak device bytes of one handle for a molecule with the given grid size, function count and mean kept functions per
+!  block, set-up included (admission control for the request loop); xq_device_bytes is what a handle holds now, and
+!  xq_device_memory the device's free and total bytes (hipMemGetInfo). */
 int         xq_estimate_bytes   ( long long nPoints, int nFunctions, double meanKeptFunctions, int precision, long long *bytes ) ;
 int         xq_device_bytes     ( const xq_grid *grid, long long *bytes ) ;
+int         xq_device_memory    ( long long *freeBytes, long long *totalBytes ) ;
 
 void        xq_free             ( xq_grid *grid ) ;
 const char *xq_last_error       ( void ) ;
+
+/* Batched quadratures for a request loop.  No counterpart in the reference (its integrator takes one molecule per call):
+!  the quadratures of several handles (one per job, each with its own P, V and functional) run on the batch's stream, and
+!  each mixed-precision stage is one launch over all the jobs' blocks.  Every job's results are those of xq_integrate
+!  on its handle alone: E_xc and N_e bit for bit (per-block FP64 sums in block order), V_xc up to the order of the FP64
+!  atomic additions.  ids holds 4 entries per job (the first nIDs[j] are used); stats (optional) is XQ_NSTATS doubles for
+!  the whole batch.  All jobs must share one precision; a handle may appear once per batch, and must not be in use by
+!  another call meanwhile.  Calls on one batch must not overlap. */
+typedef struct xq_batch xq_batch ;
+int         xq_batch_create     ( int maxJobs, xq_batch **batch ) ;
+int         xq_integrate_batch  ( xq_batch *batch, int nJobs, xq_grid *const *grids, const int *nIDs, const int *ids,
+                                  const double *const *densityP, double *eXC, double *nElectrons, double *const *fockV,
+                                  double *stats ) ;
+void        xq_batch_free       ( xq_batch *batch ) ;
 
 # ifdef __cplusplus
 }

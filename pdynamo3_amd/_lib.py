@@ -27,8 +27,13 @@ SIGNATURES = {
     "xq_integrate": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp]),
     "xq_estimate_bytes": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, ctypes.c_double, ctypes.c_int, _llp]),
     "xq_device_bytes": (ctypes.c_int, [_vp, _llp]),
+    "xq_device_memory": (ctypes.c_int, [_llp, _llp]),
     "xq_free": (None, [_vp]),
     "xq_last_error": (ctypes.c_char_p, []),
+    "xq_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "xq_integrate_batch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), _ip, _ip, ctypes.POINTER(_dp), _dp, _dp,
+                                          ctypes.POINTER(_dp), _dp]),
+    "xq_batch_free": (None, [_vp]),
 }
 
 

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "xq_internal.h"
 
@@ -111,21 +112,95 @@ int xq_integrate(xq_grid *grid, int nIDs, const int *ids, const double *densityP
 }
 
 int xq_estimate_bytes(long long nPoints, int nFunctions, double meanKeptFunctions, int precision, long long *bytes) {
-    if (bytes == nullptr || nPoints < 0 || nFunctions < 0 || meanKeptFunctions < 0.0) {
+    if (bytes == nullptr || nPoints < 0 || nFunctions < 0 || meanKeptFunctions < 0.0 ||
+        (precision != XQ_PRECISION_FP64 && precision != XQ_PRECISION_MIXED)) {
         set_error("invalid arguments");
         return kInvalidArgument;
     }
+    // Peak device bytes of one handle: during the basis set-up the FP64 values and gradients (32 bytes per stored
+    // value) and the per-block kept-function scratch coexist with the mixed-precision planes, which then replace the
+    // FP64 values.
+    const double G = static_cast<double>(nPoints);
     const double blocks = static_cast<double>((nPoints + kBlockSize - 1) / kBlockSize);
-    const double elt = precision == XQ_PRECISION_FP64 ? 8.0 : 4.0;
     const double n = static_cast<double>(nFunctions);
-    double b = 32.0 * nPoints                                   // points and weights
-             + blocks * sizeof(DevBlock)                       // block table
-             + blocks * meanKeptFunctions * (4.0 + 4.0 * elt * kBlockSize)   // kept indices + values and gradients
-             + 8.0 * (2.0 * n * n + n * (n + 1) / 2.0)         // dense P, V and the packed staging
-             + (precision == XQ_PRECISION_FP64 ? 0.0 : 4.0 * n * n)
-             + 48.0 * nPoints + 20.0 * blocks;                 // per-point and per-block work arrays
+    const double values = blocks * meanKeptFunctions * kBlockSize;
+    const double grid = 32.0 * G + blocks * sizeof(DevBlock);
+    const double kept = 4.0 * blocks * meanKeptFunctions + 4.0 * blocks * n;           // indices + set-up scratch
+    const double work = 8.0 * (2.0 * n * n + n * (n + 1) / 2.0) + 20.0 * blocks;     // P, V, packed, block sums
+    double b = grid + kept + work + 32.0 * values;
+    if (precision == XQ_PRECISION_MIXED)
+        // Phi digits in two orders, W digits, 24-bit gradients (21 bytes per stored value, rows padded to 32),
+        // per-point exponents, P digits, W coefficients.
+        b += 1.1 * 21.0 * values + 512.0 * blocks + 8.0 * n * n + 4.0 * n + 32.0 * G;
     *bytes = static_cast<long long>(b);
     return kOK;
+}
+
+int xq_device_memory(long long *freeBytes, long long *totalBytes) {
+    if (freeBytes == nullptr || totalBytes == nullptr) { set_error("invalid arguments"); return kInvalidArgument; }
+    size_t f = 0, t = 0;
+    const int status = check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    *freeBytes = static_cast<long long>(f);
+    *totalBytes = static_cast<long long>(t);
+    return status;
+}
+
+int xq_batch_create(int maxJobs, xq_batch **batch) {
+    if (batch == nullptr || maxJobs < 1 || maxJobs > 4096) { set_error("invalid batch arguments"); return kInvalidArgument; }
+    *batch = nullptr;
+    xq_batch *b = new (std::nothrow) xq_batch();
+    if (b == nullptr) { set_error("host allocation failed"); return kOutOfMemory; }
+    b->maxJobs = maxJobs;
+    int status = check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hipEvent_t *evs[5] = {&b->ev0, &b->ev1, &b->evRho0, &b->evRho1, &b->evFock1};
+    for (hipEvent_t *e : evs)
+        if (status == kOK) status = check(hipEventCreate(e), "hipEventCreate");
+    if (status == kOK)
+        status = check(hipHostMalloc(reinterpret_cast<void **>(&b->h_jobs), maxJobs * sizeof(MixedJob)), "hipHostMalloc");
+    if (status == kOK)
+        status = check(hipMalloc(reinterpret_cast<void **>(&b->d_jobs), maxJobs * sizeof(MixedJob)), "hipMalloc");
+    if (status != kOK) { xq_batch_free(b); return status; }
+    *batch = b;
+    return kOK;
+}
+
+int xq_integrate_batch(xq_batch *batch, int nJobs, xq_grid *const *grids, const int *nIDs, const int *ids,
+                       const double *const *densityP, double *eXC, double *nElectrons, double *const *fockV,
+                       double *stats) {
+    if (batch == nullptr || grids == nullptr || nIDs == nullptr || ids == nullptr || densityP == nullptr ||
+        eXC == nullptr || nElectrons == nullptr || fockV == nullptr || nJobs < 1 || nJobs > batch->maxJobs) {
+        set_error("invalid batch integrate arguments");
+        return kInvalidArgument;
+    }
+    std::vector<FunctionalSet> fs(nJobs);
+    for (int j = 0; j < nJobs; j++) {
+        eXC[j] = 0.0;
+        nElectrons[j] = 0.0;
+        if (grids[j] == nullptr || densityP[j] == nullptr || fockV[j] == nullptr) {
+            set_error("invalid batch integrate arguments");
+            return kInvalidArgument;
+        }
+        if (!grids[j]->hasBasis) { set_error("no basis set on a grid of the batch"); return kInvalidArgument; }
+        for (int k = 0; k < j; k++)
+            if (grids[k] == grids[j]) { set_error("a handle appears twice in one batch"); return kInvalidArgument; }
+        const int status = functional_set(nIDs[j], ids + 4 * j, &fs[j]);
+        if (status != kOK) return status;
+    }
+    const StageEvents ev{batch->ev0, batch->ev1, batch->evRho0, batch->evRho1, batch->evFock1};
+    return integrate_jobs(batch->stream, ev, nJobs, grids, fs.data(), densityP, eXC, nElectrons, fockV, stats,
+                          batch->h_jobs, batch->d_jobs);
+}
+
+void xq_batch_free(xq_batch *batch) {
+    if (batch == nullptr) return;
+    if (batch->stream) (void)hipStreamSynchronize(batch->stream);
+    if (batch->h_jobs) (void)hipHostFree(batch->h_jobs);
+    if (batch->d_jobs) (void)hipFree(batch->d_jobs);
+    hipEvent_t evs[5] = {batch->ev0, batch->ev1, batch->evRho0, batch->evRho1, batch->evFock1};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (batch->stream) (void)hipStreamDestroy(batch->stream);
+    delete batch;
 }
 
 int xq_device_bytes(const xq_grid *grid, long long *bytes) {

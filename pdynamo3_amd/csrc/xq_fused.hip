@@ -211,8 +211,8 @@ __global__ void __launch_bounds__(256) fused_block_kernel_f64(
 
 }  // namespace
 
-void launch_fused_f64(xq_grid *g, const FunctionalSet &fs) {
-    hipLaunchKernelGGL(fused_block_kernel_f64, dim3(g->nBlocks), dim3(256), 0, g->stream, g->d_blocks, g->d_idx,
+void launch_fused_f64(xq_grid *g, const FunctionalSet &fs, hipStream_t s) {
+    hipLaunchKernelGGL(fused_block_kernel_f64, dim3(g->nBlocks), dim3(256), 0, s, g->d_blocks, g->d_idx,
                        static_cast<const double *>(g->d_f), static_cast<const double *>(g->d_fx),
                        static_cast<const double *>(g->d_fy), static_cast<const double *>(g->d_fz), g->nFunctions, g->d_P,
                        g->d_w, fs, g->rhoTol, g->d_blockEN, g->d_skip, g->d_V);

@@ -14,6 +14,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "xq_functional.h"
 #include "xq_internal.h"
@@ -54,10 +55,9 @@ __global__ void zero_kernel(size_t n, double *__restrict__ a) {
 
 }  // namespace
 
-void launch_fused_f64(xq_grid *g, const FunctionalSet &fs);
-int launch_fused_mixed(xq_grid *g, const FunctionalSet &fs);
-
 void free_work(xq_grid *g) {
+    if (g->h_job != nullptr) { (void)hipHostFree(g->h_job); g->h_job = nullptr; }
+    dev_free(&g->d_job);
     dev_free(&g->d_P);
     dev_free(&g->d_V);
     dev_free(&g->d_packed);
@@ -78,90 +78,149 @@ static int ensure_work(xq_grid *g) {
     return kOK;
 }
 
-int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, double *eXC, double *nE, double *V,
-              double *stats) {
+int functional_set(int nIDs, const int *ids, FunctionalSet *fs) {
     if (nIDs < 1 || nIDs > 4) { set_error("between one and four functionals are supported"); return kInvalidArgument; }
-    FunctionalSet fs;
-    fs.n = nIDs;
+    fs->n = nIDs;
     for (int k = 0; k < nIDs; k++) {
         if (ids[k] != 101 && ids[k] != 106 && ids[k] != 130 && ids[k] != 131) {
             set_error("unsupported functional ID (supported: 101, 106, 130, 131)");
             return kInvalidArgument;   // DFTFunctionalModel.c:196-202
         }
-        fs.id[k] = ids[k];
+        fs->id[k] = ids[k];
     }
-    XQ_TRY(ensure_work(g));
-    const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
-    hipStream_t s = g->stream;
-    XQ_CHECK(hipMemcpyAsync(g->d_packed, Ppacked, nPacked * sizeof(double), hipMemcpyHostToDevice, s));
-    XQ_CHECK(hipEventRecord(g->ev0, s));
-    const int grid = static_cast<int>(std::min<size_t>((n * n + 255) / 256, 16384));
-    hipLaunchKernelGGL(expand_packed_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, g->nFunctions, g->d_packed, g->d_P);
-    hipLaunchKernelGGL(zero_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, n * n, g->d_V);
-    XQ_CHECK(hipGetLastError());
-    if (g->nBlocks > 0) {
-        if (g->precision == XQ_PRECISION_FP64) {
-            (void)hipEventRecord(g->evRho0, s);
-            launch_fused_f64(g, fs);
-            (void)hipEventRecord(g->evRho1, s);
-            (void)hipEventRecord(g->evFock1, s);
-        } else {
-            XQ_TRY(launch_fused_mixed(g, fs));
+    for (int k = nIDs; k < 4; k++) fs->id[k] = 0;
+    return kOK;
+}
+
+// Algorithmic HBM bytes of the two stages of one handle's quadrature: every stored operand read once, every result
+// written once.
+static void stage_bytes(const xq_grid *g, double *rho, double *fock) {
+    const double n = static_cast<double>(g->nFunctions), nn = n * n, G = static_cast<double>(g->nPoints),
+                 B = static_cast<double>(g->nBlocks);
+    if (g->precision == XQ_PRECISION_FP64) {
+        // Fused kernel: values and gradients (4 x 8 bytes per stored value), P, weights, block sums, V.
+        *rho = 32.0 * static_cast<double>(g->nValueRows) * kBlockSize + 8.0 * nn + 8.0 * G + 20.0 * B + 16.0 * nn;
+        *fock = 0.0;
+    } else {
+        // Density: Phi digits (16 KB per chunk), fixed-point gradients (36 KB per chunk), P digits, point exponents,
+        // weights in, W coefficients out, block sums.  Fock stage: W formation (Phi digits, gradients, coefficients in,
+        // W digits out) and the Fock product (Phi and W digits in, V read and written).
+        const double ch = static_cast<double>(g->nChunks);
+        *rho = ch * (16384.0 + 36864.0) + 8.0 * nn + 512.0 * B + 40.0 * G + 20.0 * B;
+        *fock = ch * (16384.0 + 36864.0 + 16384.0 + 32.0) + 32.0 * G + 384.0 * B + ch * 2.0 * 16384.0 + 16.0 * nn;
+    }
+}
+
+int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
+                   const double *const *Ppacked, double *eXC, double *nE, double *const *V, double *stats,
+                   MixedJob *h_table, MixedJob *d_table) {
+    const int precision = grids[0]->precision;
+    for (int j = 0; j < nJobs; j++) {
+        if (grids[j]->precision != precision) {
+            set_error("the jobs of one batch must share their precision");
+            return kInvalidArgument;
         }
-        XQ_CHECK(hipGetLastError());
+        XQ_TRY(ensure_work(grids[j]));
     }
-    const int pgrid = static_cast<int>(std::min<size_t>((nPacked + 255) / 256, 16384));
-    hipLaunchKernelGGL(pack_lower_kernel, dim3(pgrid > 0 ? pgrid : 1), dim3(256), 0, s, g->nFunctions, g->d_V, g->d_packed);
+    for (int j = 0; j < nJobs; j++) {
+        const xq_grid *g = grids[j];
+        const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
+        XQ_CHECK(hipMemcpyAsync(g->d_packed, Ppacked[j], nPacked * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    XQ_CHECK(hipEventRecord(ev.ev0, s));
+    for (int j = 0; j < nJobs; j++) {
+        const xq_grid *g = grids[j];
+        const size_t n = static_cast<size_t>(g->nFunctions);
+        const int grid = static_cast<int>(std::min<size_t>((n * n + 255) / 256, 16384));
+        hipLaunchKernelGGL(expand_packed_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, g->nFunctions, g->d_packed,
+                           g->d_P);
+        hipLaunchKernelGGL(zero_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, n * n, g->d_V);
+    }
     XQ_CHECK(hipGetLastError());
-    XQ_CHECK(hipEventRecord(g->ev1, s));
-    std::vector<double> vpacked(nPacked);
-    XQ_CHECK(hipMemcpyAsync(vpacked.data(), g->d_packed, nPacked * sizeof(double), hipMemcpyDeviceToHost, s));
-    if (g->nBlocks > 0) {
-        XQ_CHECK(hipMemcpyAsync(g->h_blockEN.data(), g->d_blockEN, 2 * g->nBlocks * sizeof(double), hipMemcpyDeviceToHost, s));
-        XQ_CHECK(hipMemcpyAsync(g->h_skip.data(), g->d_skip, g->nBlocks * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (precision == XQ_PRECISION_FP64) {
+        XQ_CHECK(hipEventRecord(ev.evRho0, s));
+        for (int j = 0; j < nJobs; j++)
+            if (grids[j]->nBlocks > 0) launch_fused_f64(grids[j], fs[j], s);
+        XQ_CHECK(hipEventRecord(ev.evRho1, s));
+        XQ_CHECK(hipEventRecord(ev.evFock1, s));
+    } else {
+        XQ_TRY(launch_mixed(s, ev, nJobs, grids, fs, h_table, d_table));
+    }
+    XQ_CHECK(hipGetLastError());
+    for (int j = 0; j < nJobs; j++) {
+        const xq_grid *g = grids[j];
+        const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
+        const int pgrid = static_cast<int>(std::min<size_t>((nPacked + 255) / 256, 16384));
+        hipLaunchKernelGGL(pack_lower_kernel, dim3(pgrid > 0 ? pgrid : 1), dim3(256), 0, s, g->nFunctions, g->d_V,
+                           g->d_packed);
+    }
+    XQ_CHECK(hipGetLastError());
+    XQ_CHECK(hipEventRecord(ev.ev1, s));
+    std::vector<std::vector<double>> vpacked(nJobs);
+    for (int j = 0; j < nJobs; j++) {
+        xq_grid *g = grids[j];
+        const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
+        vpacked[j].resize(nPacked);
+        XQ_CHECK(hipMemcpyAsync(vpacked[j].data(), g->d_packed, nPacked * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (g->nBlocks > 0) {
+            XQ_CHECK(hipMemcpyAsync(g->h_blockEN.data(), g->d_blockEN, 2 * g->nBlocks * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+            XQ_CHECK(hipMemcpyAsync(g->h_skip.data(), g->d_skip, g->nBlocks * sizeof(int), hipMemcpyDeviceToHost, s));
+        }
     }
     XQ_CHECK(hipStreamSynchronize(s));
-    // FP64 reductions in block order (DFTIntegrator.c:350-352).
-    double e = 0.0, ne = 0.0;
+    double bfv = 0.0, bfs = 0.0, rhoBytes = 0.0, fockBytes = 0.0;
     long long skipped = 0;
-    for (int b = 0; b < g->nBlocks; b++) {
-        e += g->h_blockEN[2 * b];
-        ne += g->h_blockEN[2 * b + 1];
-        if (g->h_skip[b] && g->blocks[b].nKept > 0) skipped++;
+    for (int j = 0; j < nJobs; j++) {
+        const xq_grid *g = grids[j];
+        // FP64 reductions in block order (DFTIntegrator.c:350-352).
+        double e = 0.0, ne = 0.0;
+        for (int b = 0; b < g->nBlocks; b++) {
+            e += g->h_blockEN[2 * b];
+            ne += g->h_blockEN[2 * b + 1];
+            if (g->h_skip[b] && g->blocks[b].nKept > 0) skipped++;
+        }
+        const size_t nPacked = vpacked[j].size();
+        for (size_t k = 0; k < nPacked; k++) V[j][k] += vpacked[j][k];
+        eXC[j] = e;
+        nE[j] = ne;
+        bfv += g->bfValues;
+        bfs += g->bfSquares;
+        double rb, fb;
+        stage_bytes(g, &rb, &fb);
+        rhoBytes += rb;
+        fockBytes += fb;
     }
-    for (size_t k = 0; k < nPacked; k++) V[k] += vpacked[k];
-    *eXC = e;
-    *nE = ne;
     if (stats != nullptr) {
         float ms = 0.0f, msRho = 0.0f, msFock = 0.0f;
-        (void)hipEventElapsedTime(&ms, g->ev0, g->ev1);
-        if (g->nBlocks > 0) {
-            (void)hipEventElapsedTime(&msRho, g->evRho0, g->evRho1);
-            (void)hipEventElapsedTime(&msFock, g->evRho1, g->evFock1);
-        }
-        stats[0] = g->bfValues;
+        (void)hipEventElapsedTime(&ms, ev.ev0, ev.ev1);
+        (void)hipEventElapsedTime(&msRho, ev.evRho0, ev.evRho1);
+        (void)hipEventElapsedTime(&msFock, ev.evRho1, ev.evFock1);
+        stats[0] = bfv;
         stats[1] = static_cast<double>(skipped);
         stats[2] = static_cast<double>(ms);
         stats[3] = static_cast<double>(msRho);
         stats[4] = static_cast<double>(msFock);
-        stats[5] = g->bfSquares;
-        // Algorithmic HBM bytes of the two stages: every stored operand read once, every result written once.
-        const double nn = static_cast<double>(n) * static_cast<double>(n), G = static_cast<double>(g->nPoints),
-                     B = static_cast<double>(g->nBlocks);
-        if (g->precision == XQ_PRECISION_FP64) {
-            // Fused kernel: values and gradients (4 x 8 bytes per stored value), P, weights, block sums, V.
-            stats[6] = 32.0 * static_cast<double>(g->nValueRows) * kBlockSize + 8.0 * nn + 8.0 * G + 20.0 * B + 16.0 * nn;
-            stats[7] = 0.0;
-        } else {
-            // Density: Phi digits (16 KB per chunk), fixed-point gradients (36 KB per chunk), P digits, point
-            // exponents, weights in, W coefficients out, block sums.  Fock stage: W formation (Phi digits, gradients,
-            // coefficients in, W digits out) and the Fock product (Phi and W digits in, V read and written).
-            const double ch = static_cast<double>(g->nChunks);
-            stats[6] = ch * (16384.0 + 36864.0) + 8.0 * nn + 512.0 * B + 40.0 * G + 20.0 * B;
-            stats[7] = ch * (16384.0 + 36864.0 + 16384.0 + 32.0) + 32.0 * G + 384.0 * B + ch * 2.0 * 16384.0 + 16.0 * nn;
-        }
+        stats[5] = bfs;
+        stats[6] = rhoBytes;
+        stats[7] = fockBytes;
     }
     return kOK;
+}
+
+int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, double *eXC, double *nE, double *V,
+              double *stats) {
+    FunctionalSet fs;
+    XQ_TRY(functional_set(nIDs, ids, &fs));
+    if (g->h_job == nullptr) {
+        XQ_CHECK(hipHostMalloc(reinterpret_cast<void **>(&g->h_job), sizeof(MixedJob)));
+        XQ_TRY(dev_alloc(g, &g->d_job, 1));
+    }
+    const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1};
+    xq_grid *grids[1] = {g};
+    const double *P[1] = {Ppacked};
+    double *Vs[1] = {V};
+    return integrate_jobs(g->stream, ev, 1, grids, &fs, P, eXC, nE, Vs, stats, g->h_job, g->d_job);
 }
 
 }  // namespace xq

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/xcquad.h"
+#include "xq_functional.h"
 
 namespace xq {
 
@@ -46,6 +47,27 @@ struct DevBlock {
     long long vOffset;         // offset of the block's first value row (rows of kBlockSize entries)
     int iOffset;               // offset of the block's kept-function indices
     int pad;
+};
+
+// One job of a mixed-precision quadrature launch: a handle's device arrays, its functional, and the first work-group of
+// the job in each stage's launch (0 for a single job; running offsets when several jobs share one launch).
+struct MixedJob {
+    const DevBlock *blocks;
+    const int *chunkOffset, *chunkBlock, *idx;
+    const int4 *tiles;
+    const signed char *DA, *EA, *DB, *EB, *EG;
+    const short *GH;
+    const unsigned char *GL;
+    const double *P, *w;
+    int *eP;
+    unsigned long long *Pq;
+    double *wcoef, *blockEN, *V;
+    int *skip;
+    signed char *DW, *EW;
+    FunctionalSet fs;
+    double rhoTol;
+    int n, nBlocks, nChunks, nTiles;
+    int rowBase, blockBase, chunkBase, groupBase;
 };
 
 }  // namespace xq
@@ -97,7 +119,16 @@ struct xq_grid {
     int *d_skip = nullptr;                // per block: 1 if skipped by the density tolerance
     std::vector<double> h_blockEN;
     std::vector<int> h_skip;
+    xq::MixedJob *h_job = nullptr, *d_job = nullptr;   // one-entry job table of single calls (pinned host, device)
     long long deviceBytes = 0;
+};
+
+// A batch context: its own stream and events, and a job table of up to maxJobs entries (pinned host, device).
+struct xq_batch {
+    int maxJobs = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evRho0 = nullptr, evRho1 = nullptr, evFock1 = nullptr;
+    xq::MixedJob *h_jobs = nullptr, *d_jobs = nullptr;
 };
 
 namespace xq {
@@ -131,6 +162,17 @@ int check(hipError_t e, const char *what);
 int build_grid(xq_grid *g);
 int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int precision);
 int integrate(xq_grid *g, int nIDs, const int *ids, const double *P, double *eXC, double *nE, double *V, double *stats);
+// Quadratures of several handles on one stream; the mixed-precision stages run as one launch each over all the jobs.
+struct StageEvents {
+    hipEvent_t ev0, ev1, evRho0, evRho1, evFock1;
+};
+int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
+                   const double *const *P, double *eXC, double *nE, double *const *V, double *stats, MixedJob *h_table,
+                   MixedJob *d_table);
+int functional_set(int nIDs, const int *ids, FunctionalSet *fs);
+int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
+                 MixedJob *h_table, MixedJob *d_table);
+void launch_fused_f64(xq_grid *g, const FunctionalSet &fs, hipStream_t s);
 void free_basis(xq_grid *g);
 void free_work(xq_grid *g);
 int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz);

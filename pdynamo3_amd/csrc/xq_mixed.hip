@@ -40,6 +40,8 @@
 // (16 consecutive K bytes each) and receives D[(v % 4) + 4 (l / 32) + 8 (v / 4)][l % 32] for v = 0..15.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+#include <cstdint>
 #include <vector>
 
 #include "xq_functional.h"
@@ -137,6 +139,19 @@ __device__ __forceinline__ void grad_encode(double g, int e, short &hi, unsigned
 // Row of a chunk held at density position 16 h + v, and the position of row k.
 __host__ __device__ constexpr int chunk_fun(int h, int v) { return 4 * h + 8 * (v >> 2) + (v & 3); }
 __host__ __device__ constexpr int chunk_pos(int k) { return 16 * ((k >> 2) & 1) + 4 * (k >> 3) + (k & 3); }
+
+// Job of work-group g in a launch over several jobs: the last job whose first work-group in this stage is <= g (jobs
+// without work in the stage share the next job's first work-group and are never returned for it).
+template <typename F>
+__device__ __forceinline__ int job_of(const xq::MixedJob *jobs, int nJobs, int g, F base) {
+    int lo = 0, hi = nJobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base(jobs[mid]) <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // ------------------------------------------------------------------------------------------------------------------------
 // Set-up, once per geometry.
@@ -244,11 +259,15 @@ __global__ void __launch_bounds__(256) mixed_prepare_kernel(
 // ------------------------------------------------------------------------------------------------------------------------
 // Per step.
 // ------------------------------------------------------------------------------------------------------------------------
-// P row i (one work-group of 256 threads per row): row exponent and the 5 digits of every element.
-__global__ void __launch_bounds__(256) p_slice_kernel(int n, const double *__restrict__ P, int *__restrict__ eP,
-                                                      unsigned long long *__restrict__ Pq) {
+// P row i (one work-group of 256 threads per row of every job): row exponent and the 5 digits of every element.
+__global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     __shared__ double s[4];
-    const size_t i = blockIdx.x;
+    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, blockIdx.x, [](const xq::MixedJob &j) { return j.rowBase; })];
+    const int n = J.n;
+    const double *__restrict__ P = J.P;
+    int *__restrict__ eP = J.eP;
+    unsigned long long *__restrict__ Pq = J.Pq;
+    const size_t i = blockIdx.x - J.rowBase;
     const double *row = P + i * n;
     double m = 0.0;
     for (int k = threadIdx.x; k < n; k += 256) m = fmax(m, fabs(row[k]));
@@ -272,16 +291,31 @@ __global__ void __launch_bounds__(256) p_slice_kernel(int n, const double *__res
 // E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
 // point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  P_red is staged in LDS by row panels shared by
 // the four waves; the Phi operands are fetched one step ahead.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) density_kernel_mixed(
-    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ idxAll,
-    const signed char *__restrict__ DB, const signed char *__restrict__ EB, const short *__restrict__ GH,
-    const unsigned char *__restrict__ GL, const signed char *__restrict__ EG, int n, const unsigned long long *__restrict__ Pq, const int *__restrict__ eP, const double *__restrict__ w, FunctionalSet fs,
-    double rhoTol, double *__restrict__ wcoef, double *__restrict__ blockEN, int *__restrict__ skip) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
     __shared__ int sPe[2][kC];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
-    const int b = blockIdx.x;
+    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, blockIdx.x, [](const xq::MixedJob &j) { return j.blockBase; })];
+    const xq::DevBlock *__restrict__ blocks = J.blocks;
+    const int *__restrict__ chunkOffset = J.chunkOffset;
+    const int *__restrict__ idxAll = J.idx;
+    const signed char *__restrict__ DB = J.DB;
+    const signed char *__restrict__ EB = J.EB;
+    const short *__restrict__ GH = J.GH;
+    const unsigned char *__restrict__ GL = J.GL;
+    const signed char *__restrict__ EG = J.EG;
+    const int n = J.n;
+    const unsigned long long *__restrict__ Pq = J.Pq;
+    const int *__restrict__ eP = J.eP;
+    const double *__restrict__ w = J.w;
+    const FunctionalSet fs = J.fs;
+    const double rhoTol = J.rhoTol;
+    double *__restrict__ wcoef = J.wcoef;
+    double *__restrict__ blockEN = J.blockEN;
+    int *__restrict__ skip = J.skip;
+    const int b = blockIdx.x - J.blockBase;
     const xq::DevBlock blk = blocks[b];
     const int nb = blk.nKept, np = blk.nPoints;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -434,14 +468,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) d
 
 // W digits (one chunk per work-group, 256 threads = 4 waves = the 4 point chunks q): W_rp = c0_p Phi_rp + c_p . grad Phi_rp
 // for the chunk's 32 rows, one power of two per row over the block's 128 points, written in DA's order (DW, EW).
-__global__ void __launch_bounds__(256) w_digits_kernel(
-    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkBlock, const signed char *__restrict__ DA,
-    const signed char *__restrict__ EA, const short *__restrict__ GH, const unsigned char *__restrict__ GL,
-    const signed char *__restrict__ EG, const double *__restrict__ wcoef, const int *__restrict__ skip,
-    signed char *__restrict__ DW, signed char *__restrict__ EW) {
+__global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     __shared__ double sc[4][kBlockSize];
     __shared__ double sm[4][kC];
-    const size_t C = blockIdx.x;
+    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, blockIdx.x, [](const xq::MixedJob &j) { return j.chunkBase; })];
+    const xq::DevBlock *__restrict__ blocks = J.blocks;
+    const int *__restrict__ chunkBlock = J.chunkBlock;
+    const signed char *__restrict__ DA = J.DA;
+    const signed char *__restrict__ EA = J.EA;
+    const short *__restrict__ GH = J.GH;
+    const unsigned char *__restrict__ GL = J.GL;
+    const signed char *__restrict__ EG = J.EG;
+    const double *__restrict__ wcoef = J.wcoef;
+    const int *__restrict__ skip = J.skip;
+    signed char *__restrict__ DW = J.DW;
+    signed char *__restrict__ EW = J.EW;
+    const size_t C = blockIdx.x - J.chunkBase;
     const int b = chunkBlock[C];
     if (skip[b]) return;
     const xq::DevBlock blk = blocks[b];
@@ -497,15 +539,25 @@ __global__ void __launch_bounds__(256) w_digits_kernel(
 // of them (one power of two per row), then V += Phi_I W_J^T + W_I Phi_J^T is added with FP64 atomics.
 // Work-group g runs on XCD g % 8; tiles are dealt so that each XCD takes a contiguous range of the tile list (tiles of
 // one block share their chunks' digits in that XCD's L2).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) fock_kernel_mixed(
-    const int4 *__restrict__ tiles, int nTiles, int groupsPerXcd, const xq::DevBlock *__restrict__ blocks,
-    const int *__restrict__ chunkOffset, const int *__restrict__ idxAll, const signed char *__restrict__ DA,
-    const signed char *__restrict__ EA, const signed char *__restrict__ DW, const signed char *__restrict__ EW, int n,
-    const int *__restrict__ skip, double *__restrict__ V) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+fock_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 31, lh = lane >> 5;
     const int group = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
-    const int tileIndex = 4 * group + wv;
-    if (tileIndex >= nTiles) return;
+    if (group >= nGroups) return;
+    const xq::MixedJob &job = jobs[job_of(jobs, nJobs, group, [](const xq::MixedJob &j) { return j.groupBase; })];
+    const int tileIndex = 4 * (group - job.groupBase) + wv;
+    if (tileIndex >= job.nTiles) return;
+    const int4 *__restrict__ tiles = job.tiles;
+    const xq::DevBlock *__restrict__ blocks = job.blocks;
+    const int *__restrict__ chunkOffset = job.chunkOffset;
+    const int *__restrict__ idxAll = job.idx;
+    const signed char *__restrict__ DA = job.DA;
+    const signed char *__restrict__ EA = job.EA;
+    const signed char *__restrict__ DW = job.DW;
+    const signed char *__restrict__ EW = job.EW;
+    const int n = job.n;
+    const int *__restrict__ skip = job.skip;
+    double *__restrict__ V = job.V;
     const int4 tl = tiles[tileIndex];
     const int b = tl.x, I = tl.y, J = tl.z;
     if (skip[b]) return;
@@ -634,26 +686,70 @@ void free_mixed(xq_grid *g) {
     g->nTiles = 0;
 }
 
-int launch_fused_mixed(xq_grid *g, const FunctionalSet &fs) {
-    hipStream_t s = g->stream;
-    const int n = g->nFunctions;
-    if (n > 0) hipLaunchKernelGGL(p_slice_kernel, dim3(n), dim3(256), 0, s, n, g->d_P, g->d_eP, g->d_Pq);
-    (void)hipEventRecord(g->evRho0, s);
-    hipLaunchKernelGGL(density_kernel_mixed, dim3(g->nBlocks), dim3(256), 0, s, g->d_blocks, g->d_chunkOffset, g->d_idx,
-                       g->d_DB, g->d_EB, g->d_GH, g->d_GL, g->d_EG, n, g->d_Pq, g->d_eP, g->d_w, fs, g->rhoTol, g->d_wcoef, g->d_blockEN,
-                       g->d_skip);
-    (void)hipEventRecord(g->evRho1, s);
-    if (g->nChunks > 0) {
-        hipLaunchKernelGGL(w_digits_kernel, dim3(static_cast<unsigned>(g->nChunks)), dim3(256), 0, s, g->d_blocks,
-                           g->d_chunkBlock, g->d_DA, g->d_EA, g->d_GH, g->d_GL, g->d_EG, g->d_wcoef, g->d_skip,
-                           g->d_DW, g->d_EW);
-        const long long groups = (g->nTiles + 3) / 4;
-        const int perXcd = static_cast<int>((groups + 7) / 8);
-        hipLaunchKernelGGL(fock_kernel_mixed, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, g->d_tiles,
-                           static_cast<int>(g->nTiles), perXcd, g->d_blocks, g->d_chunkOffset, g->d_idx, g->d_DA, g->d_EA,
-                           g->d_DW, g->d_EW, n, g->d_skip, g->d_V);
+// The job table of a launch over the given handles (each with its functional) and the four stage launches, each one
+// launch over all the jobs.  h_table (pinned) and d_table hold at least nJobs entries; the caller synchronises the stream
+// before h_table is written again.
+int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
+                 MixedJob *h_table, MixedJob *d_table) {
+    long long rows = 0, blocks = 0, chunks = 0, groups = 0;
+    for (int j = 0; j < nJobs; j++) {
+        const xq_grid *g = grids[j];
+        MixedJob &J = h_table[j];
+        J.blocks = g->d_blocks;
+        J.chunkOffset = g->d_chunkOffset;
+        J.chunkBlock = g->d_chunkBlock;
+        J.idx = g->d_idx;
+        J.tiles = g->d_tiles;
+        J.DA = g->d_DA;
+        J.EA = g->d_EA;
+        J.DB = g->d_DB;
+        J.EB = g->d_EB;
+        J.EG = g->d_EG;
+        J.GH = g->d_GH;
+        J.GL = g->d_GL;
+        J.P = g->d_P;
+        J.w = g->d_w;
+        J.eP = g->d_eP;
+        J.Pq = g->d_Pq;
+        J.wcoef = g->d_wcoef;
+        J.blockEN = g->d_blockEN;
+        J.V = g->d_V;
+        J.skip = g->d_skip;
+        J.DW = g->d_DW;
+        J.EW = g->d_EW;
+        J.fs = fs[j];
+        J.rhoTol = g->rhoTol;
+        J.n = g->nFunctions;
+        J.nBlocks = g->nBlocks;
+        J.nChunks = static_cast<int>(g->nChunks);
+        J.nTiles = static_cast<int>(g->nTiles);
+        J.rowBase = static_cast<int>(rows);
+        J.blockBase = static_cast<int>(blocks);
+        J.chunkBase = static_cast<int>(chunks);
+        J.groupBase = static_cast<int>(groups);
+        rows += J.n;
+        blocks += J.nBlocks;
+        chunks += J.nChunks;
+        groups += (J.nTiles + 3) / 4;
     }
-    (void)hipEventRecord(g->evFock1, s);
+    if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
+        set_error("batch too large for one launch");
+        return kInvalidArgument;
+    }
+    XQ_CHECK(hipMemcpyAsync(d_table, h_table, nJobs * sizeof(MixedJob), hipMemcpyHostToDevice, s));
+    if (rows > 0) hipLaunchKernelGGL(p_slice_kernel, dim3(static_cast<unsigned>(rows)), dim3(256), 0, s, d_table, nJobs);
+    XQ_CHECK(hipEventRecord(ev.evRho0, s));
+    if (blocks > 0)
+        hipLaunchKernelGGL(density_kernel_mixed, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, d_table, nJobs);
+    XQ_CHECK(hipEventRecord(ev.evRho1, s));
+    if (chunks > 0) {
+        hipLaunchKernelGGL(w_digits_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, d_table, nJobs);
+        const int perXcd = static_cast<int>((groups + 7) / 8);
+        hipLaunchKernelGGL(fock_kernel_mixed, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,
+                           perXcd, static_cast<int>(groups));
+    }
+    XQ_CHECK(hipEventRecord(ev.evFock1, s));
+    XQ_CHECK(hipGetLastError());
     return kOK;
 }
 

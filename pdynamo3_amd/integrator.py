@@ -94,6 +94,70 @@ class QuadratureGrid:
             pass
 
 
+class QuadratureBatch:
+    """Quadratures of several grids (one per job) in one set of launches: xq_integrate_batch.
+
+    Each job's E_xc and N_e equal those of QuadratureGrid.integrate on the same grid alone; its V_xc equals it up to the
+    order of the FP64 atomic additions.
+    """
+
+    def __init__(self, max_jobs):
+        self.max_jobs = int(max_jobs)
+        self._h = ctypes.c_void_p()
+        _check(_lib.lib.xq_batch_create(self.max_jobs, ctypes.byref(self._h)), "batch create")
+
+    def integrate(self, jobs):
+        """jobs: sequence of (grid, ids, density, fock); each fock (packed float64, or None) is incremented in place.
+        Returns ([(E_xc, N_e) per job], stats of the whole batch)."""
+        n = len(jobs)
+        if n < 1 or n > self.max_jobs:
+            raise QCModelError("Batch of {:d} jobs; 1 to {:d} allowed.".format(n, self.max_jobs))
+        grids = (ctypes.c_void_p * n)()
+        n_ids = np.zeros(n, dtype=np.int32)
+        ids = np.zeros((n, 4), dtype=np.int32)
+        keep = []
+        P_ptr, V_ptr = (_lib._dp * n)(), (_lib._dp * n)()
+        for j, (grid, job_ids, density, fock) in enumerate(jobs):
+            grids[j] = grid._h.value
+            job_ids = _lib.i32(job_ids)
+            n_ids[j] = job_ids.size
+            ids[j, :job_ids.size] = job_ids[:4]
+            P = _lib.f64(density)
+            m = grid.number_of_functions
+            if P.size != m * (m + 1) // 2:
+                raise QCModelError("Density of job {:d} has {:d} entries; {:d} expected.".format(j, P.size, m * (m + 1) // 2))
+            V = fock if fock is not None else np.zeros_like(P)
+            if V.dtype != np.float64 or not V.flags.c_contiguous or V.size != P.size:
+                raise QCModelError("Fock matrix of job {:d} must be a contiguous float64 array of the density's size.".format(j))
+            keep += [P, V]
+            P_ptr[j] = _lib.dptr(P)
+            V_ptr[j] = _lib.dptr(V)
+        e, ne, stats = np.zeros(n), np.zeros(n), np.zeros(8)
+        _check(_lib.lib.xq_integrate_batch(self._h, n, grids, _lib.iptr(n_ids), _lib.iptr(ids), P_ptr, _lib.dptr(e),
+                                           _lib.dptr(ne), V_ptr, _lib.dptr(stats)), "batch integrate")
+        return [(float(e[j]), float(ne[j])) for j in range(n)], {
+            "bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2], "rho_ms": stats[3],
+            "fock_ms": stats[4], "bf_squares": stats[5], "rho_bytes": stats[6], "fock_bytes": stats[7]}
+
+    def close(self):
+        if self._h:
+            _lib.lib.xq_batch_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_memory():
+    """(free, total) device bytes."""
+    f, t = ctypes.c_longlong(), ctypes.c_longlong()
+    _check(_lib.lib.xq_device_memory(ctypes.byref(f), ctypes.byref(t)), "device memory")
+    return f.value, t.value
+
+
 def estimated_points(accuracy, atomic_numbers):
     """DFTGrid_EstimatedPoints (DFTGrid.c:301-356): grid size before the weight cull."""
     z = _lib.i32(atomic_numbers)
@@ -103,7 +167,7 @@ def estimated_points(accuracy, atomic_numbers):
 
 
 def estimate_bytes(number_of_points, number_of_functions, mean_kept_functions, precision="fp64"):
-    """Device bytes for a job of the given size (admission control)."""
+    """Peak device bytes of a job of the given size, set-up included (admission control)."""
     b = ctypes.c_longlong()
     _check(_lib.lib.xq_estimate_bytes(int(number_of_points), int(number_of_functions), float(mean_kept_functions),
                                       PRECISIONS[precision], ctypes.byref(b)), "estimate bytes")

@@ -59,7 +59,23 @@ def test_memory_estimate_scales():
     big = integrator.estimate_bytes(16 * 42086, 304, 200.0, "fp64")
     mixed = integrator.estimate_bytes(16 * 42086, 304, 200.0, "mixed")
     assert 0 < small < big
-    assert mixed < big
+    # The estimate is the peak: a mixed-precision set-up holds the FP64 values while it cuts them into digits.
+    assert big < mixed < 2 * big
+    values = (16 * 42086 / 128) * 200.0 * 128
+    assert big > 32 * values
+
+
+def test_batch_argument_errors_return_status_codes():
+    from pdynamo3_amd import _lib
+    b = ctypes.c_void_p()
+    assert _lib.lib.xq_batch_create(0, ctypes.byref(b)) == 3
+    assert _lib.lib.xq_batch_create(1, None) == 3
+    one = np.zeros(1)
+    ids = np.zeros(4, dtype=np.int32)
+    n = np.ones(1, dtype=np.int32)
+    assert _lib.lib.xq_integrate_batch(None, 1, None, _lib.iptr(n), _lib.iptr(ids), None, _lib.dptr(one), _lib.dptr(one),
+                                       None, None) == 3
+    assert "invalid" in _lib.last_error()
 
 
 def test_bench_refuses_more_than_one_gpu():
