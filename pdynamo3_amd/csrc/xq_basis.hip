@@ -313,6 +313,7 @@ __global__ void __launch_bounds__(kBlockSize) basis_eval_kernel(
 
 void free_basis(xq_grid *g) {
     free_mixed(g);
+    free_f64(g);
     dev_free(&g->d_shells);
     dev_free(&g->d_coef);
     dev_free(&g->d_trans);
@@ -492,16 +493,19 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
                                static_cast<double *>(g->d_fy), static_cast<double *>(g->d_fz));
             if ((status = check(hipGetLastError(), "basis_eval_kernel")) != kOK) break;
         }
-        if (precision == XQ_PRECISION_MIXED) {
-            // Digit planes and FP32 gradients replace the FP64 values.
-            if ((status = prepare_mixed(g, static_cast<double *>(g->d_f), static_cast<double *>(g->d_fx),
-                                        static_cast<double *>(g->d_fy), static_cast<double *>(g->d_fz))) != kOK) break;
-            if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
-            for (void **a : arrays) {
-                (void)hipFree(*a);
-                *a = nullptr;
-                g->deviceBytes -= static_cast<long long>(nVal * elt);
-            }
+        // The operand-order planes of the chosen precision replace the FP64 rows.
+        if (precision == XQ_PRECISION_MIXED)
+            status = prepare_mixed(g, static_cast<double *>(g->d_f), static_cast<double *>(g->d_fx),
+                                   static_cast<double *>(g->d_fy), static_cast<double *>(g->d_fz));
+        else
+            status = prepare_f64(g, static_cast<double *>(g->d_f), static_cast<double *>(g->d_fx),
+                                 static_cast<double *>(g->d_fy), static_cast<double *>(g->d_fz));
+        if (status != kOK) break;
+        if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
+        for (void **a : arrays) {
+            (void)hipFree(*a);
+            *a = nullptr;
+            g->deviceBytes -= static_cast<long long>(nVal * elt);
         }
         status = check(hipStreamSynchronize(s), "basis sync");
     } while (false);

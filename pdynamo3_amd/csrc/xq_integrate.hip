@@ -8,8 +8,8 @@
 //   W = vrho Phi / 2 + 2 vsigma (grad rho . grad Phi), which is the sum of DFTIntegrator_Fock's three terms (:517-618).
 //
 // This file holds the host side of one quadrature (P upload and expansion, the precision's kernels, V packing and the
-// FP64 reductions of E_xc and N_e in block order).  The kernels are in xq_fused.hip (FP64) and xq_mixed.hip (int8
-// matrix cores).
+// FP64 reductions of E_xc and N_e in block order).  The kernels are in xq_f64.hip (FP64 matrix cores) and xq_mixed.hip
+// (int8 matrix cores).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -97,15 +97,17 @@ int functional_set(int nIDs, const int *ids, FunctionalSet *fs) {
 static void stage_bytes(const xq_grid *g, double *rho, double *fock) {
     const double n = static_cast<double>(g->nFunctions), nn = n * n, G = static_cast<double>(g->nPoints),
                  B = static_cast<double>(g->nBlocks);
+    const double ch = static_cast<double>(g->nChunks);
     if (g->precision == XQ_PRECISION_FP64) {
-        // Fused kernel: values and gradients (4 x 8 bytes per stored value), P, weights, block sums, V.
-        *rho = 32.0 * static_cast<double>(g->nValueRows) * kBlockSize + 8.0 * nn + 8.0 * G + 20.0 * B + 16.0 * nn;
-        *fock = 0.0;
+        // Density: Phi and grad Phi planes (4 x 32 KB per chunk), P, weights in, W coefficients out, block sums.
+        // Fock stage: W formation (the same planes and the coefficients in, W out) and the Fock product (Phi and W in,
+        // V read and written).
+        *rho = ch * 4.0 * 32768.0 + 8.0 * nn + 40.0 * G + 20.0 * B;
+        *fock = ch * (4.0 * 32768.0 + 32768.0) + 32.0 * G + ch * 2.0 * 32768.0 + 16.0 * nn;
     } else {
         // Density: Phi digits (16 KB per chunk), fixed-point gradients (36 KB per chunk), P digits, point exponents,
         // weights in, W coefficients out, block sums.  Fock stage: W formation (Phi digits, gradients, coefficients in,
         // W digits out) and the Fock product (Phi and W digits in, V read and written).
-        const double ch = static_cast<double>(g->nChunks);
         *rho = ch * (16384.0 + 36864.0) + 8.0 * nn + 512.0 * B + 40.0 * G + 20.0 * B;
         *fock = ch * (16384.0 + 36864.0 + 16384.0 + 32.0) + 32.0 * G + 384.0 * B + ch * 2.0 * 16384.0 + 16.0 * nn;
     }
@@ -138,11 +140,7 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
     }
     XQ_CHECK(hipGetLastError());
     if (precision == XQ_PRECISION_FP64) {
-        XQ_CHECK(hipEventRecord(ev.evRho0, s));
-        for (int j = 0; j < nJobs; j++)
-            if (grids[j]->nBlocks > 0) launch_fused_f64(grids[j], fs[j], s);
-        XQ_CHECK(hipEventRecord(ev.evRho1, s));
-        XQ_CHECK(hipEventRecord(ev.evFock1, s));
+        XQ_TRY(launch_f64(s, ev, nJobs, grids, fs, h_table, d_table));
     } else {
         XQ_TRY(launch_mixed(s, ev, nJobs, grids, fs, h_table, d_table));
     }

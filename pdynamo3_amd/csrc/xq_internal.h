@@ -64,6 +64,8 @@ struct MixedJob {
     double *wcoef, *blockEN, *V;
     int *skip;
     signed char *DW, *EW;
+    const double *FPB, *GPB, *FPA;     // FP64 path: Phi and grad Phi in the density's operand order, Phi in the Fock's
+    double *WPA;                       // FP64 path: W in the Fock's operand order (per step)
     FunctionalSet fs;
     double rhoTol;
     int n, nBlocks, nChunks, nTiles;
@@ -110,6 +112,8 @@ struct xq_grid {
     int *d_eP = nullptr;
     double *d_wcoef = nullptr;
     long long nChunks = 0, nTiles = 0;
+    // FP64 path (layouts in xq_f64.hip): Phi and grad Phi planes in operand order, W per step.
+    double *d_FPB = nullptr, *d_GPB = nullptr, *d_FPA = nullptr, *d_WPA = nullptr;
     double bfValues = 0.0;                // sum over blocks of kept functions x points
     double bfSquares = 0.0;               // sum over blocks of kept functions^2 x points
 
@@ -172,7 +176,14 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
 int functional_set(int nIDs, const int *ids, FunctionalSet *fs);
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
                  MixedJob *h_table, MixedJob *d_table);
-void launch_fused_f64(xq_grid *g, const FunctionalSet &fs, hipStream_t s);
+int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
+               MixedJob *h_table, MixedJob *d_table);
+// Shared by both precisions: the job table entries of the given handles, the per-block chunk tables and Fock tile list.
+void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rows,
+               long long *blocks, long long *chunks, long long *groups);
+int chunk_tables(xq_grid *g);
+int prepare_f64(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz);
+void free_f64(xq_grid *g);
 void free_basis(xq_grid *g);
 void free_work(xq_grid *g);
 int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz);

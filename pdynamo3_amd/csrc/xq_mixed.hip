@@ -612,14 +612,17 @@ fock_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPe
 }  // namespace
 
 // Host side: digit storage at basis set-up and the per-step launches.
-int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz) {
+// Chunks of 32 kept functions per block (chunkOffset: first chunk of each block; chunkBlock: block of each chunk) and the
+// lower-triangle Fock tiles (block, I, J <= I) of every block, for both precisions.
+int chunk_tables(xq_grid *g) {
+    if (g->d_chunkOffset != nullptr) return kOK;
     std::vector<int> chunkOffset(g->nBlocks + 1, 0), chunkBlock;
     std::vector<int4> tiles;
     long long chunks = 0;
     for (int b = 0; b < g->nBlocks; b++) {
         const int nb = g->blocks[b].nKept;
-        // The density accumulators hold up to 4 x 127^2 x nb per order in int32.
-        if (nb > 32767) { set_error("mixed precision supports at most 32767 kept functions per block"); return kInvalidArgument; }
+        // The density accumulators of the mixed path hold up to 4 x 127^2 x nb per order in int32.
+        if (nb > 32767) { set_error("at most 32767 kept functions per block are supported"); return kInvalidArgument; }
         const int T = (nb + kC - 1) / kC;
         chunkOffset[b] = static_cast<int>(chunks);
         for (int I = 0; I < T; I++) {
@@ -629,12 +632,31 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
         chunks += T;
     }
     chunkOffset[g->nBlocks] = static_cast<int>(chunks);
+    if (chunks > INT32_MAX / 4 || tiles.size() > static_cast<size_t>(INT32_MAX / 4)) {
+        set_error("too many chunks for one grid");
+        return kInvalidArgument;
+    }
     g->nChunks = chunks;
     g->nTiles = static_cast<long long>(tiles.size());
     const size_t nc = static_cast<size_t>(chunks > 0 ? chunks : 1);
     XQ_TRY(dev_alloc(g, &g->d_chunkOffset, chunkOffset.size()));
     XQ_TRY(dev_alloc(g, &g->d_chunkBlock, nc));
     XQ_TRY(dev_alloc(g, &g->d_tiles, tiles.empty() ? 1 : tiles.size()));
+    hipStream_t s = g->stream;
+    XQ_CHECK(hipMemcpyAsync(g->d_chunkOffset, chunkOffset.data(), chunkOffset.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    if (chunks > 0)
+        XQ_CHECK(hipMemcpyAsync(g->d_chunkBlock, chunkBlock.data(), chunkBlock.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    if (!tiles.empty())
+        XQ_CHECK(hipMemcpyAsync(g->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    // The host vectors must outlive the copies.
+    XQ_CHECK(hipStreamSynchronize(s));
+    return kOK;
+}
+
+int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz) {
+    XQ_TRY(chunk_tables(g));
+    const long long chunks = g->nChunks;
+    const size_t nc = static_cast<size_t>(chunks > 0 ? chunks : 1);
     XQ_TRY(dev_alloc(g, &g->d_DA, nc * kChunkBytes));
     XQ_TRY(dev_alloc(g, &g->d_EA, nc * kC));
     XQ_TRY(dev_alloc(g, &g->d_DB, nc * kChunkBytes));
@@ -648,11 +670,6 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     XQ_TRY(dev_alloc(g, &g->d_eP, static_cast<size_t>(g->nFunctions)));
     XQ_TRY(dev_alloc(g, &g->d_wcoef, 4 * static_cast<size_t>(g->nPoints > 0 ? g->nPoints : 1)));
     hipStream_t s = g->stream;
-    XQ_CHECK(hipMemcpyAsync(g->d_chunkOffset, chunkOffset.data(), chunkOffset.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    if (chunks > 0)
-        XQ_CHECK(hipMemcpyAsync(g->d_chunkBlock, chunkBlock.data(), chunkBlock.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    if (!tiles.empty())
-        XQ_CHECK(hipMemcpyAsync(g->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s));
     if (g->nBlocks > 0)
         hipLaunchKernelGGL(point_exponent_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, f, fx, fy, fz,
                            g->d_EB, g->d_EG);
@@ -661,7 +678,6 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
                            g->d_chunkOffset, g->d_chunkBlock, f, fx, fy, fz, g->d_EB, g->d_EG, g->d_DA, g->d_EA,
                            g->d_DB, g->d_GH, g->d_GL);
     XQ_CHECK(hipGetLastError());
-    // The host vectors above must outlive the copies.
     XQ_CHECK(hipStreamSynchronize(s));
     return kOK;
 }
@@ -686,11 +702,9 @@ void free_mixed(xq_grid *g) {
     g->nTiles = 0;
 }
 
-// The job table of a launch over the given handles (each with its functional) and the four stage launches, each one
-// launch over all the jobs.  h_table (pinned) and d_table hold at least nJobs entries; the caller synchronises the stream
-// before h_table is written again.
-int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-                 MixedJob *h_table, MixedJob *d_table) {
+// The job table entries of the given handles (each with its functional) and each stage's total work-groups.
+void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rowsOut,
+               long long *blocksOut, long long *chunksOut, long long *groupsOut) {
     long long rows = 0, blocks = 0, chunks = 0, groups = 0;
     for (int j = 0; j < nJobs; j++) {
         const xq_grid *g = grids[j];
@@ -717,6 +731,10 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
         J.skip = g->d_skip;
         J.DW = g->d_DW;
         J.EW = g->d_EW;
+        J.FPB = g->d_FPB;
+        J.GPB = g->d_GPB;
+        J.FPA = g->d_FPA;
+        J.WPA = g->d_WPA;
         J.fs = fs[j];
         J.rhoTol = g->rhoTol;
         J.n = g->nFunctions;
@@ -732,6 +750,18 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
         chunks += J.nChunks;
         groups += (J.nTiles + 3) / 4;
     }
+    *rowsOut = rows;
+    *blocksOut = blocks;
+    *chunksOut = chunks;
+    *groupsOut = groups;
+}
+
+// The four stage launches of the mixed path, each one launch over all the jobs.  h_table (pinned) and d_table hold at
+// least nJobs entries; the caller synchronises the stream before h_table is written again.
+int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
+                 MixedJob *h_table, MixedJob *d_table) {
+    long long rows, blocks, chunks, groups;
+    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups);
     if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
         set_error("batch too large for one launch");
         return kInvalidArgument;
