@@ -104,8 +104,9 @@ def run_leg(grid, ids, P, precision, steps, warmup, spec, spherical):
             "device_bytes": grid.device_bytes()}
 
 
-def cpu_baseline(kind, n_waters, functional, spherical):
-    """Single-threaded CPU quadrature of a (H2O)n sample cut from the same generator; points/s."""
+def cpu_baseline(kind, n_waters, functional, spherical, threads=1):
+    """CPU quadrature of a (H2O)n sample cut from the same generator; points/s.  The compiled reference is
+    single-threaded; the port (the oracle's C restatement) runs its blocks on `threads` OpenMP threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from pdynamo3_amd import basis, densities, molecules
@@ -119,9 +120,10 @@ def cpu_baseline(kind, n_waters, functional, spherical):
     P = densities.cluster_density(rots, spherical=spherical)
     h = lib.setup(2, Z, xyz, spec, spherical, functional_ids(functional))
     # The reference's first call evaluates and caches the basis values (inCore); time the second, as the GPU does.
-    h.integrate(P, 1)
-    _, _, _, sec = h.integrate(P, 1)
-    return {"value": h.npoints / sec, "unit": "grid points/s", "cores": 1, "kind": kind,
+    threads = 1 if kind == "reference" else threads
+    h.integrate(P, threads)
+    _, _, _, sec = h.integrate(P, threads)
+    return {"value": h.npoints / sec, "unit": "grid points/s", "cores": threads, "kind": kind,
             "sample": "(H2O){:d} cluster, same generator, {:s}, Medium grid ({:d} points), one in-core quadrature call, "
                       "{:.2f} s".format(n_waters, functional.upper(), h.npoints, sec)}
 
@@ -228,6 +230,12 @@ def main():
             out["cpu_baseline"] = cpu_baseline(args.cpu_baseline, args.cpu_sample_waters, functional, spherical)
         except Exception as exc:  # the bench line is still printed; the failure is named
             out["cpu_baseline"] = {"value": None, "error": repr(exc)}
+        try:
+            # The restatement on the host cores this job may use (16 per GPU on the box).
+            out["cpu_baseline_all_cores"] = cpu_baseline("port", args.cpu_sample_waters, functional, spherical,
+                                                         threads=min(16, os.cpu_count() or 1))
+        except Exception as exc:
+            out["cpu_baseline_all_cores"] = {"value": None, "error": repr(exc)}
     print(json.dumps(out), flush=True)
     if args.copy_calibration:
         copy_calibration()

@@ -127,11 +127,15 @@ int xq_estimate_bytes(long long nPoints, int nFunctions, double meanKeptFunction
     const double grid = 32.0 * G + blocks * sizeof(DevBlock);
     const double kept = 4.0 * blocks * meanKeptFunctions + 4.0 * blocks * n;           // indices + set-up scratch
     const double work = 8.0 * (2.0 * n * n + n * (n + 1) / 2.0) + 20.0 * blocks;     // P, V, packed, block sums
-    double b = grid + kept + work + 32.0 * values;
+    // Fock tile list and chunk tables (rows padded to chunks of 32: about 10% more stored values than kept).
+    double b = grid + kept + work + 32.0 * values + 16.0 * blocks * 16.0 + 32.0 * G;
     if (precision == XQ_PRECISION_MIXED)
-        // Phi digits in two orders, W digits, 24-bit gradients (21 bytes per stored value, rows padded to 32),
-        // per-point exponents, P digits, W coefficients.
-        b += 1.1 * 21.0 * values + 512.0 * blocks + 8.0 * n * n + 4.0 * n + 32.0 * G;
+        // Phi digits in two orders, W digits, 24-bit gradients (21 bytes per stored value), per-point exponents,
+        // P digits.
+        b += 1.1 * 21.0 * values + 512.0 * blocks + 8.0 * n * n + 4.0 * n;
+    else
+        // Phi in two operand orders, grad Phi, W (48 bytes per stored value).
+        b += 1.1 * 48.0 * values;
     *bytes = static_cast<long long>(b);
     return kOK;
 }

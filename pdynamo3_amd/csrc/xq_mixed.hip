@@ -406,20 +406,20 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
             gh[c][1] = *reinterpret_cast<const v4i *>(GH + o + 8);
             gl[c] = *reinterpret_cast<const v4i *>(GL + o);
         }
-        double sr = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+        double sumR = 0.0, sumX = 0.0, sumY = 0.0, sumZ = 0.0;
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int s[kDP] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
             const double x = ldexp(static_cast<double>(combine<kDP>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 42);
-            sr += static_cast<double>(digits4(dI, v)) * x;
-            sx += static_cast<double>(short_of(gh[0], v) * 256 + ubyte_of(gl[0], v)) * x;
-            sy += static_cast<double>(short_of(gh[1], v) * 256 + ubyte_of(gl[1], v)) * x;
-            sz += static_cast<double>(short_of(gh[2], v) * 256 + ubyte_of(gl[2], v)) * x;
+            sumR += static_cast<double>(digits4(dI, v)) * x;
+            sumX += static_cast<double>(short_of(gh[0], v) * 256 + ubyte_of(gl[0], v)) * x;
+            sumY += static_cast<double>(short_of(gh[1], v) * 256 + ubyte_of(gl[1], v)) * x;
+            sumZ += static_cast<double>(short_of(gh[2], v) * 256 + ubyte_of(gl[2], v)) * x;
         }
-        pr += ldexp(sr, eb - 28);
-        pgx += sx * gscale[0];
-        pgy += sy * gscale[1];
-        pgz += sz * gscale[2];
+        pr += ldexp(sumR, eb - 28);
+        pgx += sumX * gscale[0];
+        pgy += sumY * gscale[1];
+        pgz += sumZ * gscale[2];
     }
     pr += __shfl_xor(pr, 32, 64);
     pgx += __shfl_xor(pgx, 32, 64);

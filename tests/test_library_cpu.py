@@ -59,10 +59,11 @@ def test_memory_estimate_scales():
     big = integrator.estimate_bytes(16 * 42086, 304, 200.0, "fp64")
     mixed = integrator.estimate_bytes(16 * 42086, 304, 200.0, "mixed")
     assert 0 < small < big
-    # The estimate is the peak: a mixed-precision set-up holds the FP64 values while it cuts them into digits.
-    assert big < mixed < 2 * big
+    # The estimate is the peak: a set-up holds the FP64 rows (32 B per stored value) while it writes the operand planes
+    # of its precision (48 B per value for FP64, 21 B for the digit planes).
     values = (16 * 42086 / 128) * 200.0 * 128
-    assert big > 32 * values
+    assert 32 * values + 48 * values < big < 2 * (32 + 48) * values
+    assert 32 * values + 21 * values < mixed < big
 
 
 def test_batch_argument_errors_return_status_codes():
