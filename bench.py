@@ -55,14 +55,19 @@ def parse():
 
 
 def copy_calibration():
-    """One device-to-device copy of exactly 1 GiB (read 1 GiB, write 1 GiB) by torch's copy kernel."""
-    import torch
-    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    del a, b
+    """One device-to-device copy of exactly 1 GiB (read 1 GiB, write 1 GiB) by the HIP runtime's copy kernel."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    size = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(size)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(size)):
+        raise RuntimeError("hipMalloc failed for the copy calibration")
+    hip.hipMemset(a, 1, ctypes.c_size_t(size))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(size), 3)    # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def measured_traffic(kernel, workload):
