@@ -82,12 +82,16 @@ __global__ void __launch_bounds__(256) f64_prepare_kernel(
 // coefficients per point.  P_red is staged in LDS by row panels of kPanelF chunks; the Phi operands of the next chunk
 // are loaded while the current one is multiplied.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-density_kernel_f64(const MixedJob *__restrict__ jobs, int nJobs) {
+density_kernel_f64(const MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
     __shared__ double sPf[kC][kPanelPitch];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
-    const MixedJob &J = jobs[job_of(jobs, nJobs, blockIdx.x, [](const MixedJob &j) { return j.blockBase; })];
-    const int b = blockIdx.x - J.blockBase;
+    // Work-group g runs on XCD g % 8; each XCD takes a contiguous range of blocks (one region of the molecule), so that
+    // its L2 holds the P rows those blocks gather.
+    const int wg = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
+    if (wg >= nGroups) return;
+    const MixedJob &J = jobs[job_of(jobs, nJobs, wg, [](const MixedJob &j) { return j.blockBase; })];
+    const int b = wg - J.blockBase;
     const DevBlock blk = J.blocks[b];
     const int nb = blk.nKept, np = blk.nPoints, n = J.n;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, l16 = lane & 15, l4 = lane >> 4;
@@ -353,8 +357,11 @@ int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *
     }
     XQ_CHECK(hipMemcpyAsync(d_table, h_table, nJobs * sizeof(MixedJob), hipMemcpyHostToDevice, s));
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
-    if (blocks > 0)
-        hipLaunchKernelGGL(density_kernel_f64, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, d_table, nJobs);
+    if (blocks > 0) {
+        const int perXcd = static_cast<int>((blocks + 7) / 8);
+        hipLaunchKernelGGL(density_kernel_f64, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,
+                           perXcd, static_cast<int>(blocks));
+    }
     XQ_CHECK(hipEventRecord(ev.evRho1, s));
     if (chunks > 0) {
         hipLaunchKernelGGL(w_kernel_f64, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, d_table, nJobs);

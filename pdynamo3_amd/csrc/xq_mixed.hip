@@ -292,12 +292,16 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
 // point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  P_red is staged in LDS by row panels shared by
 // the four waves; the Phi operands are fetched one step ahead.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
+density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
     __shared__ int sPe[2][kC];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
-    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, blockIdx.x, [](const xq::MixedJob &j) { return j.blockBase; })];
+    // Work-group g runs on XCD g % 8; each XCD takes a contiguous range of blocks (one region of the molecule), so that
+    // its L2 holds the P rows those blocks gather.
+    const int wg = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
+    if (wg >= nGroups) return;
+    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, wg, [](const xq::MixedJob &j) { return j.blockBase; })];
     const xq::DevBlock *__restrict__ blocks = J.blocks;
     const int *__restrict__ chunkOffset = J.chunkOffset;
     const int *__restrict__ idxAll = J.idx;
@@ -315,7 +319,7 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     double *__restrict__ wcoef = J.wcoef;
     double *__restrict__ blockEN = J.blockEN;
     int *__restrict__ skip = J.skip;
-    const int b = blockIdx.x - J.blockBase;
+    const int b = wg - J.blockBase;
     const xq::DevBlock blk = blocks[b];
     const int nb = blk.nKept, np = blk.nPoints;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -769,8 +773,11 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
     XQ_CHECK(hipMemcpyAsync(d_table, h_table, nJobs * sizeof(MixedJob), hipMemcpyHostToDevice, s));
     if (rows > 0) hipLaunchKernelGGL(p_slice_kernel, dim3(static_cast<unsigned>(rows)), dim3(256), 0, s, d_table, nJobs);
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
-    if (blocks > 0)
-        hipLaunchKernelGGL(density_kernel_mixed, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, d_table, nJobs);
+    if (blocks > 0) {
+        const int perXcd = static_cast<int>((blocks + 7) / 8);
+        hipLaunchKernelGGL(density_kernel_mixed, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,
+                           perXcd, static_cast<int>(blocks));
+    }
     XQ_CHECK(hipEventRecord(ev.evRho1, s));
     if (chunks > 0) {
         hipLaunchKernelGGL(w_digits_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, d_table, nJobs);
