@@ -35,6 +35,7 @@ constexpr int kC = 32;                   // chunk rows
 constexpr int kChunk = 4096;             // doubles per chunk plane
 constexpr int kPanelF = 4;               // chunks of P_red columns staged at once
 constexpr int kPanelPitch = kPanelF * kC + 2;   // LDS row pitch in doubles (padded against bank conflicts)
+constexpr int kIdxCache = 1024;          // kept-function indices of a block cached in LDS by the density kernel
 
 __device__ __forceinline__ v4d mfma_f64(double a, double b, const v4d &c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -84,6 +85,7 @@ __global__ void __launch_bounds__(256) f64_prepare_kernel(
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 density_kernel_f64(const MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
     __shared__ double sPf[kC][kPanelPitch];
+    __shared__ int sIdx[kIdxCache];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
     // Work-group g runs on XCD g % 8; each XCD takes a contiguous range of blocks (one region of the molecule), so that
@@ -99,7 +101,12 @@ density_kernel_f64(const MixedJob *__restrict__ jobs, int nJobs, int groupsPerXc
         if (t == 0) { J.blockEN[2 * b] = 0.0; J.blockEN[2 * b + 1] = 0.0; J.skip[b] = 1; }
         return;
     }
-    const int *idx = J.idx + blk.iOffset;
+    // The block's kept-function indices, cached in LDS for the P_red gathers (blocks with more kept functions than the
+    // cache read them from memory).
+    const int *idxG = J.idx + blk.iOffset;
+    for (int k = t; k < nb && k < kIdxCache; k += 256) sIdx[k] = idxG[k];
+    __syncthreads();
+    const int *idx = nb <= kIdxCache ? sIdx : idxG;
     const size_t C = static_cast<size_t>(J.chunkOffset[b]);
     const int nChunks = (nb + kC - 1) / kC;
     const double *__restrict__ P = J.P;

@@ -62,6 +62,7 @@ constexpr int kChunkGrad = 3 * kBlockSize * kC;     // one chunk of GH or GL (el
 constexpr int kGradBits = 23;           // grad Phi fixed point: |value| < 2^EG is stored as a multiple of 2^(EG - 23)
 constexpr int kPanel = 8;               // chunks of P_red columns staged at once by the density kernel
 constexpr int kPanelDwords = 8 * kPanel + 4;        // one panel row in LDS (padded against bank conflicts)
+constexpr int kIdxCache = 1024;         // kept-function indices of a block cached in LDS by the density kernel
 
 // Stored exponents are int8.  A group whose largest magnitude is below 2^-100 (contributions below 1e-30 of unit-sized
 // terms) gets exponent 0, which rounds all its digits to zero.  Magnitudes here stay far below 2^127.
@@ -295,6 +296,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
     __shared__ int sPe[2][kC];
+    __shared__ int sIdx[kIdxCache];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
     // Work-group g runs on XCD g % 8; each XCD takes a contiguous range of blocks (one region of the molecule), so that
@@ -328,7 +330,12 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
         if (t == 0) { blockEN[2 * b] = 0.0; blockEN[2 * b + 1] = 0.0; skip[b] = 1; }
         return;
     }
-    const int *idx = idxAll + blk.iOffset;
+    // The block's kept-function indices, cached in LDS for the P_red gathers (blocks with more kept functions than the
+    // cache read them from memory).
+    const int *idxG = idxAll + blk.iOffset;
+    for (int k = t; k < nb && k < kIdxCache; k += 256) sIdx[k] = idxG[k];
+    __syncthreads();
+    const int *idx = nb <= kIdxCache ? sIdx : idxG;
     const size_t C = static_cast<size_t>(chunkOffset[b]);
     const int nChunks = (nb + kC - 1) / kC;
     const int p = kC * wv + lr;
