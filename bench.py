@@ -46,8 +46,8 @@ def parse():
     p.add_argument("--serve-requests", type=int, default=256,
                    help="requests in the serving stream (0 skips it)")
     p.add_argument("--serve-slots", type=int, default=16, help="jobs set up or running at once in the serving loop")
-    p.add_argument("--serve-load", type=float, default=1.2,
-                   help="arrival rate as a multiple of the estimated capacity of the mixed path")
+    p.add_argument("--serve-load", type=float, default=1.0,
+                   help="arrival rate as a multiple of the serving loop's measured capacity")
     p.add_argument("--copy-calibration", action="store_true",
                    help="after the measurement, copy 1 GiB device to device once (a plain copy kernel of known traffic "
                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -133,21 +133,24 @@ def cpu_baseline(kind, n_waters, functional, spherical, threads=1):
                       "{:.2f} s".format(n_waters, functional.upper(), h.npoints, sec)}
 
 
-def serving_leg(args, points_per_s, log_fn):
+def serving_leg(args, log_fn):
     """The serving stream: SCF requests on (H2O)N, N log-uniform in [4, 64] (seed 3), BLYP/PBE by coin flip, Poisson
-    arrivals at serve-load x the capacity estimated from the measured points/s (mean culled points ~ 0.8 x the
-    pre-cull estimate, ~12 quadratures per request)."""
-    from pdynamo3_amd import integrator, serving
+    arrivals at serve-load x the loop's capacity.  The capacity is measured first, on a calibration stream of 32 other
+    requests (seed 7) that all arrive at once."""
+    from pdynamo3_amd import serving
+    srv = serving.Server(precision="mixed", max_jobs=args.serve_slots)
+    calib = serving.request_stream(n_requests=32, seed=7, rate=1.0e6)
+    done, cl = srv.run(calib)
+    capacity = len(done) / cl["wall_s"]
     reqs = serving.request_stream(n_requests=args.serve_requests, seed=3, rate=1.0)
-    mean_points = float(np.mean([0.8 * integrator.estimated_points(2, r.Z) for r in reqs]))
-    rate = args.serve_load * points_per_s / (mean_points * 12.0)
+    rate = args.serve_load * capacity
     for r in reqs:
         r.arrival /= rate
-    srv = serving.Server(precision="mixed", max_jobs=args.serve_slots)
     finished, loop = srv.run(reqs)
     srv.close()
     out = serving.summarize(finished, loop)
-    out.update({"arrival_rate_per_s": rate, "load": args.serve_load, "slots": args.serve_slots,
+    out.update({"arrival_rate_per_s": rate, "capacity_requests_per_s": capacity, "load": args.serve_load,
+                "slots": args.serve_slots,
                 "hbm_budget_bytes": srv.hbm_budget,
                 "stream": "{:d} requests, (H2O)N with N log-uniform in [4, 64] (seed 3), 6-31G* Cartesian, BLYP/PBE by "
                           "coin flip, Poisson arrivals; each request is an SCF density sequence converging geometrically "
@@ -226,7 +229,7 @@ def main():
         out["max_dVxc_mixed_vs_fp64"] = float(np.max(np.abs(mixed["V"] - fp64["V"])))
     grid.close()
     if args.serve_requests > 0:
-        serve = serving_leg(args, value, log)
+        serve = serving_leg(args, log)
         out["latency_ms_p50"] = 1e3 * serve["latency_s_p50"]
         out["latency_ms_p99"] = 1e3 * serve["latency_s_p99"]
         out["serving"] = serve

@@ -21,11 +21,15 @@
 // each point's largest component (a per-value FP32 rounding biases sigma = |grad rho|^2 measurably; this does not).
 //
 // Data layouts.  The kept functions of a block are cut into chunks of 32 rows; chunk C of the grid is chunk rc of block b
-// (C = chunkOffset[b] + rc).  Rows past the block's kept count are zero.
-//   DA, DW [C][q (4)][digit (4)][lane (64)][16 bytes]  Phi and W digits in the operand order of the Fock product: lane
-//                                  (lr, lh) = (l % 32, l / 32) holds row 32 rc + lr at points 32 q + 16 lh .. + 15, which
-//                                  is its A operand (rows of tile I) and its B operand (rows of tile J).  Exponents
-//                                  EA, EW [C][32].
+// (C = chunkOffset[b] + rc).  Rows past the block's kept count are zero.  The Fock side works on superblocks: up to
+// kSuper consecutive blocks of one atom, whose rows are the union of their kept functions (uIdx).  Each block of a
+// superblock holds its Phi and W digits over the union rows, zero for a row that the block does not keep, so that
+// the superblock's V tiles sum all its blocks before one FP64 update of V; union chunk CU of the grid is union chunk rc
+// of block b (CU = uChunkOffset[b] + rc), and uRow maps each union chunk row to its row in the block's kept list.
+//   DA, DW [CU][q (4)][digit (4)][lane (64)][16 bytes]  Phi and W digits in the operand order of the Fock product: lane
+//                                  (lr, lh) = (l % 32, l / 32) holds union row 32 rc + lr at points 32 q + 16 lh .. + 15,
+//                                  which is its A operand (rows of tile I) and its B operand (rows of tile J).  Exponents
+//                                  EA, EW [CU][32], one per row over the block's own 128 points.
 //   DB     [C][digit (4)][point (128)][32 bytes]   Phi digits in the B operand order of the density product.  Position
 //                                  16 h + v of a point holds row 32 rc + 4 h + 8 (v / 4) + v % 4 (chunk_fun): the row
 //                                  whose result the MFMA lane (., h) receives in register v, so that the epilogue finds
@@ -40,6 +44,7 @@
 // (16 consecutive K bytes each) and receives D[(v % 4) + 4 (l / 32) + 8 (v / 4)][l % 32] for v = 0..15.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 #include <vector>
@@ -63,6 +68,7 @@ constexpr int kGradBits = 23;           // grad Phi fixed point: |value| < 2^EG 
 constexpr int kPanel = 8;               // chunks of P_red columns staged at once by the density kernel
 constexpr int kPanelDwords = 8 * kPanel + 4;        // one panel row in LDS (padded against bank conflicts)
 constexpr int kIdxCache = 1024;         // kept-function indices of a block cached in LDS by the density kernel
+constexpr int kSuper = 4;               // blocks per superblock of the Fock product
 
 // Stored exponents are int8.  A group whose largest magnitude is below 2^-100 (contributions below 1e-30 of unit-sized
 // terms) gets exponent 0, which rounds all its digits to zero.  Magnitudes here stay far below 2^127.
@@ -181,28 +187,30 @@ __global__ void __launch_bounds__(kBlockSize) point_exponent_kernel(
     eg[2 * kBlockSize] = static_cast<signed char>(stored_exponent(mz));
 }
 
-// One chunk per work-group (256 threads): EA, DA, DB, GH and GL from the FP64 values of the chunk's rows.
-__global__ void __launch_bounds__(256) mixed_prepare_kernel(
-    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ chunkBlock,
-    const double *__restrict__ f, const double *__restrict__ fx, const double *__restrict__ fy,
-    const double *__restrict__ fz, const signed char *__restrict__ EB, const signed char *__restrict__ EG,
-    signed char *__restrict__ DA, signed char *__restrict__ EA, signed char *__restrict__ DB, short *__restrict__ GH,
-    unsigned char *__restrict__ GL) {
+// One union chunk per work-group (256 threads): EA and DA of the chunk's 32 union rows from the FP64 values of the
+// block's kept rows (zero for the rows the block does not keep).
+__global__ void __launch_bounds__(256) fock_operand_prepare_kernel(
+    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ uChunkBlock, const int *__restrict__ uRow,
+    const double *__restrict__ f, signed char *__restrict__ DA, signed char *__restrict__ EA) {
     __shared__ int sE[kC];
+    __shared__ int sRow[kC];
     const size_t C = blockIdx.x;
-    const int b = chunkBlock[C];
+    const int b = uChunkBlock[C];
     const xq::DevBlock blk = blocks[b];
-    const int nb = blk.nKept, rc = static_cast<int>(C) - chunkOffset[b], t = threadIdx.x;
+    const int t = threadIdx.x;
     const size_t R = static_cast<size_t>(blk.vOffset);
-    auto value = [&](const double *a, int row, int p) -> double {
-        return row < nb ? a[(R + row) * kBlockSize + p] : 0.0;
+    if (t < kC) sRow[t] = uRow[C * kC + t];
+    __syncthreads();
+    auto value = [&](int i, int p) -> double {
+        const int k = sRow[i];
+        return k >= 0 ? f[(R + k) * kBlockSize + p] : 0.0;
     };
     // EA: row i = t / 8 of the chunk, eight threads per row with 16 points each.
     {
-        const int i = t >> 3, part = t & 7, row = kC * rc + i;
+        const int i = t >> 3, part = t & 7;
         double m = 0.0;
 #pragma unroll 4
-        for (int k = 0; k < 16; k++) m = fmax(m, fabs(value(f, row, 16 * part + k)));
+        for (int k = 0; k < 16; k++) m = fmax(m, fabs(value(i, 16 * part + k)));
 #pragma unroll
         for (int o = 4; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 8));
         if (part == 0) {
@@ -213,20 +221,34 @@ __global__ void __launch_bounds__(256) mixed_prepare_kernel(
     }
     __syncthreads();
     // DA: thread = (q, lane).
-    {
-        const int q = t >> 6, lane = t & 63, lr = lane & 31, lh = lane >> 5, row = kC * rc + lr, e = sE[lr];
-        signed char d[kDPhi][16];
+    const int q = t >> 6, lane = t & 63, lr = lane & 31, lh = lane >> 5, e = sE[lr];
+    signed char d[kDPhi][16];
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            signed char dd[kDPhi];
-            slice<kDPhi>(value(f, row, kC * q + 16 * lh + k), e, dd);
+    for (int k = 0; k < 16; k++) {
+        signed char dd[kDPhi];
+        slice<kDPhi>(value(lr, kC * q + 16 * lh + k), e, dd);
 #pragma unroll
-            for (int j = 0; j < kDPhi; j++) d[j][k] = dd[j];
-        }
-#pragma unroll
-        for (int j = 0; j < kDPhi; j++)
-            *reinterpret_cast<v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]) = pack16(d[j]);
+        for (int j = 0; j < kDPhi; j++) d[j][k] = dd[j];
     }
+#pragma unroll
+    for (int j = 0; j < kDPhi; j++)
+        *reinterpret_cast<v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]) = pack16(d[j]);
+}
+
+// One chunk per work-group (256 threads): DB, GH and GL from the FP64 values of the chunk's rows.
+__global__ void __launch_bounds__(256) density_operand_prepare_kernel(
+    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ chunkBlock,
+    const double *__restrict__ f, const double *__restrict__ fx, const double *__restrict__ fy,
+    const double *__restrict__ fz, const signed char *__restrict__ EB, const signed char *__restrict__ EG,
+    signed char *__restrict__ DB, short *__restrict__ GH, unsigned char *__restrict__ GL) {
+    const size_t C = blockIdx.x;
+    const int b = chunkBlock[C];
+    const xq::DevBlock blk = blocks[b];
+    const int nb = blk.nKept, rc = static_cast<int>(C) - chunkOffset[b], t = threadIdx.x;
+    const size_t R = static_cast<size_t>(blk.vOffset);
+    auto value = [&](const double *a, int row, int p) -> double {
+        return row < nb ? a[(R + row) * kBlockSize + p] : 0.0;
+    };
     // DB, GH and GL: thread = (point p, half h) with the 16 rows chunk_fun(h, 0..15).
     {
         const int p = t & (kBlockSize - 1), h = t >> 7;
@@ -477,14 +499,16 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     }
 }
 
-// W digits (one chunk per work-group, 256 threads = 4 waves = the 4 point chunks q): W_rp = c0_p Phi_rp + c_p . grad Phi_rp
-// for the chunk's 32 rows, one power of two per row over the block's 128 points, written in DA's order (DW, EW).
+// W digits (one union chunk per work-group, 256 threads = 4 waves = the 4 point chunks q):
+// W_rp = c0_p Phi_rp + c_p . grad Phi_rp for the chunk's 32 union rows (zero for the rows the block does not keep), one
+// power of two per row over the block's 128 points, written in DA's order (DW, EW).
 __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     __shared__ double sc[4][kBlockSize];
     __shared__ double sm[4][kC];
     const xq::MixedJob &J = jobs[job_of(jobs, nJobs, blockIdx.x, [](const xq::MixedJob &j) { return j.chunkBase; })];
     const xq::DevBlock *__restrict__ blocks = J.blocks;
-    const int *__restrict__ chunkBlock = J.chunkBlock;
+    const int *__restrict__ uChunkBlock = J.uChunkBlock;
+    const int *__restrict__ chunkOffset = J.chunkOffset;
     const signed char *__restrict__ DA = J.DA;
     const signed char *__restrict__ EA = J.EA;
     const short *__restrict__ GH = J.GH;
@@ -495,10 +519,12 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     signed char *__restrict__ DW = J.DW;
     signed char *__restrict__ EW = J.EW;
     const size_t C = blockIdx.x - J.chunkBase;
-    const int b = chunkBlock[C];
+    const int b = uChunkBlock[C];
     if (skip[b]) return;
     const xq::DevBlock blk = blocks[b];
     const int t = threadIdx.x, lane = t & 63, q = t >> 6, lr = lane & 31, lh = lane >> 5;
+    // Row lr of the union chunk is row k of the block's kept list (its gradients are in the block's own chunks).
+    const int k = J.uRow[C * kC + lr];
     // Per point: w vrho / 2, and 2 w vsigma grad rho scaled by the point's gradient powers of two.
     if (t < kBlockSize) {
 #pragma unroll
@@ -513,7 +539,10 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     for (int j = 0; j < kDPhi; j++)
         d[j] = *reinterpret_cast<const v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]);
     const int ea = EA[C * kC + lr];
-    const size_t go = C * kChunkGrad + chunk_pos(lr);
+    // A row that the block does not keep reads the block's first chunk and is masked to zero.
+    const size_t go = k >= 0 ? (static_cast<size_t>(chunkOffset[b]) + (k >> 5)) * kChunkGrad + chunk_pos(k & 31)
+                             : static_cast<size_t>(chunkOffset[b]) * kChunkGrad;
+    const double gmask = k >= 0 ? 1.0 : 0.0;
     auto grad = [&](int c, int p) {
         const size_t o = go + (c * kBlockSize + p) * kC;
         return static_cast<double>(static_cast<int>(GH[o]) * 256 + static_cast<int>(GL[o]));
@@ -521,11 +550,11 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     double W[16];
     double m = 0.0;
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const int p = kC * q + 16 * lh + k;
-        const double phi = ldexp(static_cast<double>(digits4(d, k)), ea - 28);
-        W[k] = sc[0][p] * phi + sc[1][p] * grad(0, p) + sc[2][p] * grad(1, p) + sc[3][p] * grad(2, p);
-        m = fmax(m, fabs(W[k]));
+    for (int v = 0; v < 16; v++) {
+        const int p = kC * q + 16 * lh + v;
+        const double phi = ldexp(static_cast<double>(digits4(d, v)), ea - 28);
+        W[v] = sc[0][p] * phi + gmask * (sc[1][p] * grad(0, p) + sc[2][p] * grad(1, p) + sc[3][p] * grad(2, p));
+        m = fmax(m, fabs(W[v]));
     }
     m = fmax(m, __shfl_xor(m, 32, 64));
     if (lh == 0) sm[q][lr] = m;
@@ -533,11 +562,11 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     const int e = stored_exponent(fmax(fmax(sm[0][lr], sm[1][lr]), fmax(sm[2][lr], sm[3][lr])));
     signed char dw[kDPhi][16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
+    for (int v = 0; v < 16; v++) {
         signed char dd[kDPhi];
-        slice<kDPhi>(W[k], e, dd);
+        slice<kDPhi>(W[v], e, dd);
 #pragma unroll
-        for (int j = 0; j < kDPhi; j++) dw[j][k] = dd[j];
+        for (int j = 0; j < kDPhi; j++) dw[j][v] = dd[j];
     }
 #pragma unroll
     for (int j = 0; j < kDPhi; j++)
@@ -545,11 +574,12 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     if (q == 0 && lh == 0) EW[C * kC + lr] = static_cast<signed char>(e);
 }
 
-// Fock kernel: one wave per lower-triangle tile (I, J <= I) of a block's reduced V, four tiles per work-group, no LDS
-// and no barriers.  K runs over the block's 128 points in four chunks of 32; each product accumulates in int32 over all
-// of them (one power of two per row), then V += Phi_I W_J^T + W_I Phi_J^T is added with FP64 atomics.
+// Fock kernel: one wave per lower-triangle tile (I, J <= I) of a superblock's reduced V, four tiles per work-group, no
+// LDS and no barriers.  For each block of the superblock that is not skipped, K runs over its 128 points in four chunks
+// of 32; each product accumulates in int32 over all of them (one power of two per row and block) and is promoted into
+// the tile's FP64 sums; V += sum_b (Phi_I W_J^T + W_I Phi_J^T) is then added with one set of FP64 atomics.
 // Work-group g runs on XCD g % 8; tiles are dealt so that each XCD takes a contiguous range of the tile list (tiles of
-// one block share their chunks' digits in that XCD's L2).
+// one superblock share their chunks' digits in that XCD's L2).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 fock_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 31, lh = lane >> 5;
@@ -558,10 +588,7 @@ fock_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPe
     const xq::MixedJob &job = jobs[job_of(jobs, nJobs, group, [](const xq::MixedJob &j) { return j.groupBase; })];
     const int tileIndex = 4 * (group - job.groupBase) + wv;
     if (tileIndex >= job.nTiles) return;
-    const int4 *__restrict__ tiles = job.tiles;
-    const xq::DevBlock *__restrict__ blocks = job.blocks;
-    const int *__restrict__ chunkOffset = job.chunkOffset;
-    const int *__restrict__ idxAll = job.idx;
+    const int *__restrict__ uChunkOffset = job.uChunkOffset;
     const signed char *__restrict__ DA = job.DA;
     const signed char *__restrict__ EA = job.EA;
     const signed char *__restrict__ DW = job.DW;
@@ -569,54 +596,60 @@ fock_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPe
     const int n = job.n;
     const int *__restrict__ skip = job.skip;
     double *__restrict__ V = job.V;
-    const int4 tl = tiles[tileIndex];
-    const int b = tl.x, I = tl.y, J = tl.z;
-    if (skip[b]) return;
-    const xq::DevBlock blk = blocks[b];
-    const int nb = blk.nKept;
-    const int *idx = idxAll + blk.iOffset;
-    const size_t CI = static_cast<size_t>(chunkOffset[b] + I), CJ = static_cast<size_t>(chunkOffset[b] + J);
+    const int4 tl = job.uTiles[tileIndex];
+    const int I = tl.y, J = tl.z;
+    const int4 sb = job.supers[tl.x];      // first block, blocks, union offset, union size
+    const int nu = sb.w;
+    const int *idx = job.uIdx + sb.z;
     const int gj = kC * J + lr;
-    // Two passes, Phi_I W_J^T then W_I Phi_J^T, each promoted into the FP64 results before the next starts.
     double x[16];
 #pragma unroll
     for (int v = 0; v < 16; v++) x[v] = 0.0;
+    bool any = false;
 #pragma unroll 1
-    for (int pass = 0; pass < 2; pass++) {
-        const signed char *A = (pass == 0 ? DA : DW) + CI * kChunkBytes + lane * 16;
-        const signed char *B = (pass == 0 ? DW : DA) + CJ * kChunkBytes + lane * 16;
-        const signed char *eA = (pass == 0 ? EA : EW) + CI * kC, *eB = (pass == 0 ? EW : EA) + CJ * kC;
-        v16i acc[4];
+    for (int b = sb.x; b < sb.x + sb.y; b++) {
+        if (skip[b]) continue;
+        any = true;
+        const size_t CI = static_cast<size_t>(uChunkOffset[b] + I), CJ = static_cast<size_t>(uChunkOffset[b] + J);
+        // Two passes, Phi_I W_J^T then W_I Phi_J^T, each promoted into the FP64 sums before the next starts.
+#pragma unroll 1
+        for (int pass = 0; pass < 2; pass++) {
+            const signed char *A = (pass == 0 ? DA : DW) + CI * kChunkBytes + lane * 16;
+            const signed char *B = (pass == 0 ? DW : DA) + CJ * kChunkBytes + lane * 16;
+            const signed char *eA = (pass == 0 ? EA : EW) + CI * kC, *eB = (pass == 0 ? EW : EA) + CJ * kC;
+            v16i acc[4];
 #pragma unroll
-        for (int s = 0; s < 4; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int s = 0; s < 4; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll 2
-        for (int q = 0; q < 4; q++) {
-            v4i xa[kDPhi], xb[kDPhi];
+            for (int q = 0; q < 4; q++) {
+                v4i xa[kDPhi], xb[kDPhi];
 #pragma unroll
-            for (int j = 0; j < kDPhi; j++) {
-                xa[j] = *reinterpret_cast<const v4i *>(A + (q * kDPhi + j) * 1024);
-                xb[j] = *reinterpret_cast<const v4i *>(B + (q * kDPhi + j) * 1024);
+                for (int j = 0; j < kDPhi; j++) {
+                    xa[j] = *reinterpret_cast<const v4i *>(A + (q * kDPhi + j) * 1024);
+                    xb[j] = *reinterpret_cast<const v4i *>(B + (q * kDPhi + j) * 1024);
+                }
+#pragma unroll
+                for (int i = 0; i < kDPhi; i++)
+#pragma unroll
+                    for (int j = 0; j < kDPhi; j++)
+                        if (i + j <= 3) acc[i + j] = mfma_i8(xa[i], xb[j], acc[i + j]);
             }
+            // Lane (lr, lh) holds rows 32 I + rl(v) and column 32 J + lr.
+            const int eCol = eB[lr] - 35;
 #pragma unroll
-            for (int i = 0; i < kDPhi; i++)
-#pragma unroll
-                for (int j = 0; j < kDPhi; j++)
-                    if (i + j <= 3) acc[i + j] = mfma_i8(xa[i], xb[j], acc[i + j]);
-        }
-        // Lane (lr, lh) holds rows 32 I + rl(v) and column 32 J + lr.
-        const int eCol = eB[lr] - 35;
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int rl = (v & 3) + 4 * lh + 8 * (v >> 2);
-            const int tv[4] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v]};
-            x[v] += ldexp(static_cast<double>(combine<4>(tv)), eA[rl] + eCol);
+            for (int v = 0; v < 16; v++) {
+                const int rl = (v & 3) + 4 * lh + 8 * (v >> 2);
+                const int tv[4] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v]};
+                x[v] += ldexp(static_cast<double>(combine<4>(tv)), eA[rl] + eCol);
+            }
         }
     }
-    const size_t col = gj < nb ? static_cast<size_t>(idx[gj]) : 0;
+    if (!any) return;
+    const size_t col = gj < nu ? static_cast<size_t>(idx[gj]) : 0;
 #pragma unroll
     for (int v = 0; v < 16; v++) {
         const int gi = kC * I + (v & 3) + 4 * lh + 8 * (v >> 2);
-        if (gi < nb && gj < nb && gj <= gi) atomicAdd(&V[static_cast<size_t>(idx[gi]) * n + col], x[v]);
+        if (gi < nu && gj < nu && gj <= gi) atomicAdd(&V[static_cast<size_t>(idx[gi]) * n + col], x[v]);
     }
 }
 
@@ -664,19 +697,101 @@ int chunk_tables(xq_grid *g) {
     return kOK;
 }
 
+// Superblocks of the Fock product: runs of up to kSuper consecutive blocks of one atom, the union of their kept
+// functions (sorted), each block's union chunks and the map from union rows to the block's kept rows, and the
+// lower-triangle tiles (superblock, I, J <= I) over the union chunks.
+static int union_tables(xq_grid *g) {
+    const int nB = g->nBlocks;
+    std::vector<int> idx(static_cast<size_t>(g->nValueRows > 0 ? g->nValueRows : 1));
+    if (g->nValueRows > 0)
+        XQ_CHECK(hipMemcpyAsync(idx.data(), g->d_idx, g->nValueRows * sizeof(int), hipMemcpyDeviceToHost, g->stream));
+    XQ_CHECK(hipStreamSynchronize(g->stream));
+    std::vector<int4> supers, uTiles;
+    std::vector<int> uIdx, uChunkOffset(static_cast<size_t>(nB) + 1, 0), uChunkBlock, uRow;
+    std::vector<int> stamp(static_cast<size_t>(g->nFunctions > 0 ? g->nFunctions : 1), -1);
+    long long chunks = 0;
+    for (int b0 = 0; b0 < nB;) {
+        int b1 = b0 + 1;
+        while (b1 < nB && b1 - b0 < kSuper && g->blocks[b1].atom == g->blocks[b0].atom) b1++;
+        const int s = static_cast<int>(supers.size());
+        const size_t u0 = uIdx.size();
+        for (int b = b0; b < b1; b++)
+            for (int k = 0; k < g->blocks[b].nKept; k++) {
+                const int fn = idx[static_cast<size_t>(g->blocks[b].iOffset) + k];
+                if (stamp[fn] != s) { stamp[fn] = s; uIdx.push_back(fn); }
+            }
+        std::sort(uIdx.begin() + u0, uIdx.end());
+        const int nu = static_cast<int>(uIdx.size() - u0);
+        if (nu > 32767) { set_error("at most 32767 kept functions per superblock are supported"); return kInvalidArgument; }
+        supers.push_back(make_int4(b0, b1 - b0, static_cast<int>(u0), nu));
+        const int T = (nu + kC - 1) / kC;
+        for (int b = b0; b < b1; b++) {
+            // Both lists are sorted: one merge gives each union row's place in the block's kept list.
+            const int *kept = idx.data() + g->blocks[b].iOffset;
+            const int nk = g->blocks[b].nKept;
+            uChunkOffset[b] = static_cast<int>(chunks);
+            int k = 0;
+            for (int r = 0; r < T * kC; r++) {
+                int m = -1;
+                if (r < nu) {
+                    const int fn = uIdx[u0 + r];
+                    while (k < nk && kept[k] < fn) k++;
+                    if (k < nk && kept[k] == fn) m = k;
+                }
+                uRow.push_back(m);
+            }
+            for (int I = 0; I < T; I++) uChunkBlock.push_back(b);
+            chunks += T;
+        }
+        for (int I = 0; I < T; I++)
+            for (int J = 0; J <= I; J++) uTiles.push_back(make_int4(s, I, J, 0));
+        b0 = b1;
+    }
+    uChunkOffset[nB] = static_cast<int>(chunks);
+    if (chunks > INT32_MAX / 4 || uTiles.size() > static_cast<size_t>(INT32_MAX / 4) ||
+        uIdx.size() > static_cast<size_t>(INT32_MAX)) {
+        set_error("too many union chunks for one grid");
+        return kInvalidArgument;
+    }
+    g->nSupers = static_cast<long long>(supers.size());
+    g->nChunksU = chunks;
+    g->nTilesU = static_cast<long long>(uTiles.size());
+    g->nUnionRows = static_cast<long long>(uIdx.size());
+    XQ_TRY(dev_alloc(g, &g->d_supers, supers.size()));
+    XQ_TRY(dev_alloc(g, &g->d_uTiles, uTiles.size()));
+    XQ_TRY(dev_alloc(g, &g->d_uIdx, uIdx.size()));
+    XQ_TRY(dev_alloc(g, &g->d_uChunkOffset, uChunkOffset.size()));
+    XQ_TRY(dev_alloc(g, &g->d_uChunkBlock, uChunkBlock.size()));
+    XQ_TRY(dev_alloc(g, &g->d_uRow, uRow.size()));
+    hipStream_t st = g->stream;
+    auto up = [&](auto *dst, const auto &v) {
+        return v.empty() ? hipSuccess : hipMemcpyAsync(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, st);
+    };
+    XQ_CHECK(up(g->d_supers, supers));
+    XQ_CHECK(up(g->d_uTiles, uTiles));
+    XQ_CHECK(up(g->d_uIdx, uIdx));
+    XQ_CHECK(up(g->d_uChunkOffset, uChunkOffset));
+    XQ_CHECK(up(g->d_uChunkBlock, uChunkBlock));
+    XQ_CHECK(up(g->d_uRow, uRow));
+    // The host vectors must outlive the copies.
+    XQ_CHECK(hipStreamSynchronize(st));
+    return kOK;
+}
+
 int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz) {
     XQ_TRY(chunk_tables(g));
-    const long long chunks = g->nChunks;
-    const size_t nc = static_cast<size_t>(chunks > 0 ? chunks : 1);
-    XQ_TRY(dev_alloc(g, &g->d_DA, nc * kChunkBytes));
-    XQ_TRY(dev_alloc(g, &g->d_EA, nc * kC));
+    XQ_TRY(union_tables(g));
+    const long long chunks = g->nChunks, chunksU = g->nChunksU;
+    const size_t nc = static_cast<size_t>(chunks > 0 ? chunks : 1), ncU = static_cast<size_t>(chunksU > 0 ? chunksU : 1);
+    XQ_TRY(dev_alloc(g, &g->d_DA, ncU * kChunkBytes));
+    XQ_TRY(dev_alloc(g, &g->d_EA, ncU * kC));
     XQ_TRY(dev_alloc(g, &g->d_DB, nc * kChunkBytes));
     XQ_TRY(dev_alloc(g, &g->d_EB, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * kBlockSize));
     XQ_TRY(dev_alloc(g, &g->d_GH, nc * kChunkGrad));
     XQ_TRY(dev_alloc(g, &g->d_GL, nc * kChunkGrad));
     XQ_TRY(dev_alloc(g, &g->d_EG, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * 3 * kBlockSize));
-    XQ_TRY(dev_alloc(g, &g->d_DW, nc * kChunkBytes));
-    XQ_TRY(dev_alloc(g, &g->d_EW, nc * kC));
+    XQ_TRY(dev_alloc(g, &g->d_DW, ncU * kChunkBytes));
+    XQ_TRY(dev_alloc(g, &g->d_EW, ncU * kC));
     XQ_TRY(dev_alloc(g, &g->d_Pq, static_cast<size_t>(g->nFunctions) * g->nFunctions));
     XQ_TRY(dev_alloc(g, &g->d_eP, static_cast<size_t>(g->nFunctions)));
     XQ_TRY(dev_alloc(g, &g->d_wcoef, 4 * static_cast<size_t>(g->nPoints > 0 ? g->nPoints : 1)));
@@ -685,9 +800,12 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
         hipLaunchKernelGGL(point_exponent_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, f, fx, fy, fz,
                            g->d_EB, g->d_EG);
     if (chunks > 0)
-        hipLaunchKernelGGL(mixed_prepare_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, g->d_blocks,
-                           g->d_chunkOffset, g->d_chunkBlock, f, fx, fy, fz, g->d_EB, g->d_EG, g->d_DA, g->d_EA,
-                           g->d_DB, g->d_GH, g->d_GL);
+        hipLaunchKernelGGL(density_operand_prepare_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s,
+                           g->d_blocks, g->d_chunkOffset, g->d_chunkBlock, f, fx, fy, fz, g->d_EB, g->d_EG, g->d_DB,
+                           g->d_GH, g->d_GL);
+    if (chunksU > 0)
+        hipLaunchKernelGGL(fock_operand_prepare_kernel, dim3(static_cast<unsigned>(chunksU)), dim3(256), 0, s,
+                           g->d_blocks, g->d_uChunkBlock, g->d_uRow, f, g->d_DA, g->d_EA);
     XQ_CHECK(hipGetLastError());
     XQ_CHECK(hipStreamSynchronize(s));
     return kOK;
@@ -697,6 +815,12 @@ void free_mixed(xq_grid *g) {
     dev_free(&g->d_chunkOffset);
     dev_free(&g->d_chunkBlock);
     dev_free(&g->d_tiles);
+    dev_free(&g->d_supers);
+    dev_free(&g->d_uTiles);
+    dev_free(&g->d_uIdx);
+    dev_free(&g->d_uChunkOffset);
+    dev_free(&g->d_uChunkBlock);
+    dev_free(&g->d_uRow);
     dev_free(&g->d_DA);
     dev_free(&g->d_EA);
     dev_free(&g->d_DB);
@@ -711,6 +835,10 @@ void free_mixed(xq_grid *g) {
     dev_free(&g->d_wcoef);
     g->nChunks = 0;
     g->nTiles = 0;
+    g->nSupers = 0;
+    g->nChunksU = 0;
+    g->nTilesU = 0;
+    g->nUnionRows = 0;
 }
 
 // The job table entries of the given handles (each with its functional) and each stage's total work-groups.
@@ -742,6 +870,12 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
         J.skip = g->d_skip;
         J.DW = g->d_DW;
         J.EW = g->d_EW;
+        J.supers = g->d_supers;
+        J.uTiles = g->d_uTiles;
+        J.uIdx = g->d_uIdx;
+        J.uChunkOffset = g->d_uChunkOffset;
+        J.uChunkBlock = g->d_uChunkBlock;
+        J.uRow = g->d_uRow;
         J.FPB = g->d_FPB;
         J.GPB = g->d_GPB;
         J.FPA = g->d_FPA;
@@ -750,8 +884,10 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
         J.rhoTol = g->rhoTol;
         J.n = g->nFunctions;
         J.nBlocks = g->nBlocks;
-        J.nChunks = static_cast<int>(g->nChunks);
-        J.nTiles = static_cast<int>(g->nTiles);
+        // The per-chunk and per-tile stages run over union chunks and superblock tiles on the mixed path.
+        const bool mixed = g->precision == XQ_PRECISION_MIXED;
+        J.nChunks = static_cast<int>(mixed ? g->nChunksU : g->nChunks);
+        J.nTiles = static_cast<int>(mixed ? g->nTilesU : g->nTiles);
         J.rowBase = static_cast<int>(rows);
         J.blockBase = static_cast<int>(blocks);
         J.chunkBase = static_cast<int>(chunks);

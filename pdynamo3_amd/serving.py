@@ -143,6 +143,7 @@ class Server:
         reserved = 0
         deferred_for_memory = 0
         batch_sizes = []
+        t_batch = t_device = t_idle = 0.0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         while pending or queue or setting_up or active:
@@ -171,16 +172,21 @@ class Server:
                 active.append(job)
             if not active:
                 # Nothing to run: wait for a set-up to finish or the next arrival.
+                t_wait = now()
                 if setting_up:
                     concurrent.futures.wait(list(setting_up), timeout=0.05,
                                             return_when=concurrent.futures.FIRST_COMPLETED)
                 elif pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
+                t_idle += now() - t_wait
                 continue
             # One batched quadrature over every job that is set up.
             batch_sizes.append(len(active))
             work = [(j.grid, functional_ids(j.request.functional), j.scf.density(), None) for j in active]
-            results, _ = self.batch.integrate(work)
+            t_call = now()
+            results, stats = self.batch.integrate(work)
+            t_batch += now() - t_call
+            t_device += 1e-3 * stats["device_ms"]
             still = []
             for job, (E, N) in zip(active, results):
                 job.quadratures += 1
@@ -202,7 +208,8 @@ class Server:
         return finished, {"wall_s": wall, "batches": len(batch_sizes),
                           "mean_batch": float(np.mean(batch_sizes)) if batch_sizes else 0.0,
                           "max_batch": int(max(batch_sizes)) if batch_sizes else 0,
-                          "deferred_for_memory": deferred_for_memory}
+                          "deferred_for_memory": deferred_for_memory, "batch_call_s": t_batch,
+                          "batch_device_s": t_device, "idle_s": t_idle}
 
     def close(self):
         self.setup_pool.shutdown(wait=True)
