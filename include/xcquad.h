@@ -66,7 +66,8 @@ int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const
 
 /* Peak device bytes of one handle for a molecule with the given grid size, function count and mean kept functions per
 !  block, set-up included (admission control for the request loop); xq_device_bytes is what a handle holds now, and
-!  xq_device_memory the device's free and total bytes (hipMemGetInfo). */
+!  xq_device_memory the device's free and total bytes (hipMemGetInfo; free includes the device blocks the library holds
+!  idle for reuse by its handles). */
 int         xq_estimate_bytes   ( long long nPoints, int nFunctions, double meanKeptFunctions, int precision, long long *bytes ) ;
 int         xq_device_bytes     ( const xq_grid *grid, long long *bytes ) ;
 int         xq_device_memory    ( long long *freeBytes, long long *totalBytes ) ;

@@ -1,0 +1,86 @@
+// Device memory of libxcquad: a process-wide caching allocator over hipMalloc.
+//
+// hipFree synchronises the whole device, so a request loop that sets up and releases handles while other handles'
+// quadratures run would stall them at every release.  Freed blocks are kept instead and handed out again: a request
+// takes the smallest cached block of at least its size and at most a quarter larger (sizes are rounded up to 512 bytes,
+// and to 2 MiB above 1 MiB), so a loop whose jobs come and go reuses device memory without runtime calls.  When
+// hipMalloc fails, every cached block is released (hipFree) and the allocation is tried once more.  A block is cached
+// only after the work that used it has completed: every caller synchronises the owning stream before it frees.
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "xq_internal.h"
+
+namespace xq {
+
+namespace {
+
+std::mutex g_mutex;
+std::multimap<size_t, void *> g_cached;            // size -> idle block
+std::unordered_map<void *, size_t> g_live;         // block -> size
+size_t g_cachedBytes = 0;
+
+size_t rounded(size_t bytes) {
+    if (bytes == 0) bytes = 1;
+    const size_t q = bytes > (size_t(1) << 20) ? (size_t(2) << 20) : size_t(512);
+    return (bytes + q - 1) / q * q;
+}
+
+void release_cached_locked() {
+    for (auto &kv : g_cached) (void)hipFree(kv.second);
+    g_cached.clear();
+    g_cachedBytes = 0;
+}
+
+}  // namespace
+
+hipError_t cache_alloc(void **ptr, size_t bytes) {
+    const size_t size = rounded(bytes);
+    std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_cached.lower_bound(size);
+    if (it != g_cached.end() && it->first <= size + size / 4) {
+        *ptr = it->second;
+        g_live[it->second] = it->first;
+        g_cachedBytes -= it->first;
+        g_cached.erase(it);
+        return hipSuccess;
+    }
+    hipError_t e = hipMalloc(ptr, size);
+    if (e != hipSuccess && !g_cached.empty()) {
+        (void)hipGetLastError();
+        release_cached_locked();
+        e = hipMalloc(ptr, size);
+    }
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return e;
+    }
+    g_live[*ptr] = size;
+    return hipSuccess;
+}
+
+void cache_free(void *ptr) {
+    if (ptr == nullptr) return;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_live.find(ptr);
+    if (it == g_live.end()) return;
+    g_cached.emplace(it->second, ptr);
+    g_cachedBytes += it->second;
+    g_live.erase(it);
+}
+
+size_t cache_idle_bytes() {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    return g_cachedBytes;
+}
+
+void cache_release() {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    release_cached_locked();
+}
+
+}  // namespace xq

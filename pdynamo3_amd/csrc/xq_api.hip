@@ -144,7 +144,8 @@ int xq_device_memory(long long *freeBytes, long long *totalBytes) {
     if (freeBytes == nullptr || totalBytes == nullptr) { set_error("invalid arguments"); return kInvalidArgument; }
     size_t f = 0, t = 0;
     const int status = check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
-    *freeBytes = static_cast<long long>(f);
+    // Blocks the library holds idle for reuse are free for its handles.
+    *freeBytes = static_cast<long long>(f + cache_idle_bytes());
     *totalBytes = static_cast<long long>(t);
     return status;
 }
@@ -218,11 +219,11 @@ void xq_free(xq_grid *grid) {
     if (grid->stream) (void)hipStreamSynchronize(grid->stream);
     free_work(grid);
     free_basis(grid);
-    dev_free(&grid->d_x);
-    dev_free(&grid->d_y);
-    dev_free(&grid->d_z);
-    dev_free(&grid->d_w);
-    dev_free(&grid->d_blocks);
+    dev_free(grid, &grid->d_x);
+    dev_free(grid, &grid->d_y);
+    dev_free(grid, &grid->d_z);
+    dev_free(grid, &grid->d_w);
+    dev_free(grid, &grid->d_blocks);
     if (grid->ev0) (void)hipEventDestroy(grid->ev0);
     if (grid->ev1) (void)hipEventDestroy(grid->ev1);
     if (grid->evRho0) (void)hipEventDestroy(grid->evRho0);

@@ -314,15 +314,15 @@ __global__ void __launch_bounds__(kBlockSize) basis_eval_kernel(
 void free_basis(xq_grid *g) {
     free_mixed(g);
     free_f64(g);
-    dev_free(&g->d_shells);
-    dev_free(&g->d_coef);
-    dev_free(&g->d_trans);
-    dev_free(&g->d_pow);
-    dev_free(&g->d_idx);
-    if (g->d_f) { (void)hipFree(g->d_f); g->d_f = nullptr; }
-    if (g->d_fx) { (void)hipFree(g->d_fx); g->d_fx = nullptr; }
-    if (g->d_fy) { (void)hipFree(g->d_fy); g->d_fy = nullptr; }
-    if (g->d_fz) { (void)hipFree(g->d_fz); g->d_fz = nullptr; }
+    dev_free(g, &g->d_shells);
+    dev_free(g, &g->d_coef);
+    dev_free(g, &g->d_trans);
+    dev_free(g, &g->d_pow);
+    dev_free(g, &g->d_idx);
+    if (g->d_f) { cache_free(g->d_f); g->d_f = nullptr; }
+    if (g->d_fx) { cache_free(g->d_fx); g->d_fx = nullptr; }
+    if (g->d_fy) { cache_free(g->d_fy); g->d_fy = nullptr; }
+    if (g->d_fz) { cache_free(g->d_fz); g->d_fz = nullptr; }
     g->hasBasis = false;
 }
 
@@ -457,6 +457,8 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
             if ((status = check(hipGetLastError(), "basis_count_kernel")) != kOK) break;
         }
         std::vector<int> h_counts(g->nBlocks > 0 ? g->nBlocks : 1, 0);
+        // Wait for the kernels first: a pageable copy that waits for its stream holds the runtime's staging path.
+        if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
         if (g->nBlocks > 0 &&
             (status = check(hipMemcpyAsync(h_counts.data(), counts, g->nBlocks * sizeof(int), hipMemcpyDeviceToHost, s),
                             "copy counts")) != kOK) break;
@@ -481,7 +483,7 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
         if ((status = dev_alloc(g, &g->d_idx, static_cast<size_t>(rows > 0 ? rows : 1))) != kOK) break;
         void **arrays[4] = {&g->d_f, &g->d_fx, &g->d_fy, &g->d_fz};
         for (void **a : arrays) {
-            hipError_t e = hipMalloc(a, nVal * elt);
+            hipError_t e = cache_alloc(a, nVal * elt);
             if (e != hipSuccess) { set_error("hipMalloc failed for basis values"); status = kOutOfMemory; break; }
             g->deviceBytes += static_cast<long long>(nVal * elt);
         }
@@ -503,7 +505,7 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
         if (status != kOK) break;
         if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
         for (void **a : arrays) {
-            (void)hipFree(*a);
+            cache_free(*a);
             *a = nullptr;
             g->deviceBytes -= static_cast<long long>(nVal * elt);
         }
@@ -511,9 +513,10 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
     } while (false);
     const long long temp = static_cast<long long>(sizeof(int)) * (static_cast<long long>(g->nBlocks) * f + g->nBlocks +
                                                                  static_cast<long long>(shellOf.size()));
-    dev_free(&tmpIdx);
-    dev_free(&counts);
-    dev_free(&d_shellOf);
+    (void)hipStreamSynchronize(s);   // the kernels that use the scratch arrays are done before they are reused
+    dev_free(g, &tmpIdx);
+    dev_free(g, &counts);
+    dev_free(g, &d_shellOf);
     g->deviceBytes -= temp > 0 ? temp : 0;
     (void)base;
     if (status != kOK) { free_basis(g); return status; }

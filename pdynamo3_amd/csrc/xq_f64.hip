@@ -348,10 +348,10 @@ int prepare_f64(xq_grid *g, const double *f, const double *fx, const double *fy,
 }
 
 void free_f64(xq_grid *g) {
-    dev_free(&g->d_FPB);
-    dev_free(&g->d_GPB);
-    dev_free(&g->d_FPA);
-    dev_free(&g->d_WPA);
+    dev_free(g, &g->d_FPB);
+    dev_free(g, &g->d_GPB);
+    dev_free(g, &g->d_FPA);
+    dev_free(g, &g->d_WPA);
 }
 
 int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,

@@ -395,9 +395,11 @@ int build_grid(xq_grid *g) {
     }
     int status = kOK;
     auto cleanup = [&]() {
-        dev_free(&d_shells); dev_free(&d_rules); dev_free(&d_xyz); dev_free(&d_invR); dev_free(&d_aij);
-        dev_free(&cx); dev_free(&cy); dev_free(&cz); dev_free(&cw); dev_free(&keep); dev_free(&dst); dev_free(&total);
-        dev_free(&tileSums);
+        // The kernels that use these arrays must be done before the arrays are reused.
+        (void)hipStreamSynchronize(g->stream);
+        dev_free(g, &d_shells); dev_free(g, &d_rules); dev_free(g, &d_xyz); dev_free(g, &d_invR); dev_free(g, &d_aij);
+        dev_free(g, &cx); dev_free(g, &cy); dev_free(g, &cz); dev_free(g, &cw); dev_free(g, &keep); dev_free(g, &dst); dev_free(g, &total);
+        dev_free(g, &tileSums);
     };
     const long long before = g->deviceBytes;
     do {
@@ -439,6 +441,8 @@ int build_grid(xq_grid *g) {
         std::vector<int> scanned(nCand > 0 ? nCand : 1, 0), kept(nCand > 0 ? nCand : 1, 0);
         int nKept = 0;
         if (nCand > 0) {
+            // Wait for the kernels first: a pageable copy that waits for its stream holds the runtime's staging path.
+            if ((status = check(hipStreamSynchronize(s), "grid sync")) != kOK) break;
             if ((status = check(hipMemcpyAsync(scanned.data(), dst, nCand * sizeof(int), hipMemcpyDeviceToHost, s),
                                 "copy scan")) != kOK) break;
             if ((status = check(hipMemcpyAsync(kept.data(), keep, nCand * sizeof(int), hipMemcpyDeviceToHost, s),

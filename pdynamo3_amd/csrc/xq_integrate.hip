@@ -57,12 +57,12 @@ __global__ void zero_kernel(size_t n, double *__restrict__ a) {
 
 void free_work(xq_grid *g) {
     if (g->h_job != nullptr) { (void)hipHostFree(g->h_job); g->h_job = nullptr; }
-    dev_free(&g->d_job);
-    dev_free(&g->d_P);
-    dev_free(&g->d_V);
-    dev_free(&g->d_packed);
-    dev_free(&g->d_blockEN);
-    dev_free(&g->d_skip);
+    dev_free(g, &g->d_job);
+    dev_free(g, &g->d_P);
+    dev_free(g, &g->d_V);
+    dev_free(g, &g->d_packed);
+    dev_free(g, &g->d_blockEN);
+    dev_free(g, &g->d_skip);
 }
 
 static int ensure_work(xq_grid *g) {
@@ -157,6 +157,9 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
     }
     XQ_CHECK(hipGetLastError());
     XQ_CHECK(hipEventRecord(ev.ev1, s));
+    // Wait for the kernels first: a pageable copy that waits for its stream holds the runtime's staging path, which
+    // would stall other threads' copies (set-up of other handles) for the whole quadrature.
+    XQ_CHECK(hipStreamSynchronize(s));
     std::vector<std::vector<double>> vpacked(nJobs);
     for (int j = 0; j < nJobs; j++) {
         xq_grid *g = grids[j];

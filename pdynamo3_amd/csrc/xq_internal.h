@@ -146,12 +146,21 @@ struct xq_batch {
 
 namespace xq {
 
-// Allocation helpers that account device bytes on the handle.
+// Process-wide caching allocator (xq_alloc.hip): blocks freed by the library are kept for reuse instead of being
+// returned with hipFree, which synchronises the device.  The caller must have completed all work on a block before it
+// frees it.  cache_idle_bytes: bytes held idle; cache_release: hipFree every idle block.
+hipError_t cache_alloc(void **ptr, size_t bytes);
+void cache_free(void *ptr);
+size_t cache_idle_bytes();
+void cache_release();
+
+// Allocation helpers that account device bytes on the handle.  Every stage synchronises the handle's stream before it
+// frees what it allocated.
 template <typename T>
 int dev_alloc(xq_grid *g, T **ptr, size_t count) {
     *ptr = nullptr;
     if (count == 0) count = 1;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(ptr), count * sizeof(T));
+    hipError_t e = cache_alloc(reinterpret_cast<void **>(ptr), count * sizeof(T));
     if (e != hipSuccess) {
         set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
         *ptr = nullptr;
@@ -162,9 +171,9 @@ int dev_alloc(xq_grid *g, T **ptr, size_t count) {
 }
 
 template <typename T>
-void dev_free(T **ptr) {
+void dev_free(xq_grid *, T **ptr) {
     if (*ptr != nullptr) {
-        (void)hipFree(reinterpret_cast<void *>(*ptr));
+        cache_free(reinterpret_cast<void *>(*ptr));
         *ptr = nullptr;
     }
 }

@@ -703,6 +703,7 @@ int chunk_tables(xq_grid *g) {
 static int union_tables(xq_grid *g) {
     const int nB = g->nBlocks;
     std::vector<int> idx(static_cast<size_t>(g->nValueRows > 0 ? g->nValueRows : 1));
+    XQ_CHECK(hipStreamSynchronize(g->stream));
     if (g->nValueRows > 0)
         XQ_CHECK(hipMemcpyAsync(idx.data(), g->d_idx, g->nValueRows * sizeof(int), hipMemcpyDeviceToHost, g->stream));
     XQ_CHECK(hipStreamSynchronize(g->stream));
@@ -812,27 +813,27 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
 }
 
 void free_mixed(xq_grid *g) {
-    dev_free(&g->d_chunkOffset);
-    dev_free(&g->d_chunkBlock);
-    dev_free(&g->d_tiles);
-    dev_free(&g->d_supers);
-    dev_free(&g->d_uTiles);
-    dev_free(&g->d_uIdx);
-    dev_free(&g->d_uChunkOffset);
-    dev_free(&g->d_uChunkBlock);
-    dev_free(&g->d_uRow);
-    dev_free(&g->d_DA);
-    dev_free(&g->d_EA);
-    dev_free(&g->d_DB);
-    dev_free(&g->d_EB);
-    dev_free(&g->d_GH);
-    dev_free(&g->d_GL);
-    dev_free(&g->d_EG);
-    dev_free(&g->d_DW);
-    dev_free(&g->d_EW);
-    dev_free(&g->d_Pq);
-    dev_free(&g->d_eP);
-    dev_free(&g->d_wcoef);
+    dev_free(g, &g->d_chunkOffset);
+    dev_free(g, &g->d_chunkBlock);
+    dev_free(g, &g->d_tiles);
+    dev_free(g, &g->d_supers);
+    dev_free(g, &g->d_uTiles);
+    dev_free(g, &g->d_uIdx);
+    dev_free(g, &g->d_uChunkOffset);
+    dev_free(g, &g->d_uChunkBlock);
+    dev_free(g, &g->d_uRow);
+    dev_free(g, &g->d_DA);
+    dev_free(g, &g->d_EA);
+    dev_free(g, &g->d_DB);
+    dev_free(g, &g->d_EB);
+    dev_free(g, &g->d_GH);
+    dev_free(g, &g->d_GL);
+    dev_free(g, &g->d_EG);
+    dev_free(g, &g->d_DW);
+    dev_free(g, &g->d_EW);
+    dev_free(g, &g->d_Pq);
+    dev_free(g, &g->d_eP);
+    dev_free(g, &g->d_wcoef);
     g->nChunks = 0;
     g->nTiles = 0;
     g->nSupers = 0;
