@@ -11,9 +11,11 @@
 // reference orders them, so the order-preserving cull reproduces the reference's point and block order.
 //
 // Becke weight: P_A / sum_i P_i with P_i = prod_{j != i} s(nu_ij).  P_A (the point's own atom) is formed in full.  For
-// i != A the product is formed factor by factor, starting with the factor from atom A, and stops once it falls below
-// 2^-64 P_A: all later factors lie in [0, 1], so the dropped part of sum_i P_i is below nAtoms * 2^-64 of P_A <= the sum.
-// This turns the reference's O(nAtoms^2) pair loop per point into O(nAtoms) for points that sit in their own atom's cell.
+// i != A the product is formed factor by factor, starting with the factor from atom A and then nearest atom to A first
+// (the factors that can make it vanish come early), and stops once it falls below 2^-64 P_A: all later factors lie in
+// [0, 1], so the dropped part of sum_i P_i is below nAtoms * 2^-64 of P_A <= the sum.  This turns the reference's
+// O(nAtoms^2) pair loop per point into O(nAtoms) for points that sit in their own atom's cell.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <hip/hip_runtime.h>
@@ -173,23 +175,28 @@ __device__ __forceinline__ int pair_index(int k, int m) {
 }
 
 // Candidate points of one radial shell per work-group (thread = angular point) and their Becke weights
-// w = 4 pi w_r w_ang P_A / sum_i P_i (DFTGrid.c:158-211, DFTGridWeights.c:220-258).  The atom coordinates and the pair
-// data of the shell's atom A (row A of 1/R and a) are staged in LDS; P_A is computed in full, and each other P_i stops
-// once it falls below 2^-64 P_A (it can no longer change the sum).  The arithmetic of every factor is that of the host
-// restatement, so the weights, and the points kept by |w| > 1e-30, are the same bit for bit.
+// w = 4 pi w_r w_ang P_A / sum_i P_i (DFTGrid.c:158-211, DFTGridWeights.c:220-258).  The atom coordinates, the pair
+// data of the shell's atom A (row A of 1/R and a) and the atoms in order of distance from A are staged in LDS.  P_A is
+// computed in full; each other P_i takes its factors nearest atom first and stops once it falls below 2^-64 P_A, where
+// it no longer changes the sum beyond rounding.  Each factor is the reference's cell function of the same mu and nu,
+// rounded as the reference rounds it (so factors that vanish there vanish here); the products are taken in another
+// order, so weights agree with the reference's to rounding and the same points pass the cull.
 __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell *__restrict__ shells,
                                                                const double *__restrict__ rules,
                                                                const double *__restrict__ xyz, int nAtoms,
                                                                const double *__restrict__ invR,
                                                                const double *__restrict__ aij,
+                                                               const int *__restrict__ order,
                                                                double *__restrict__ cx, double *__restrict__ cy,
                                                                double *__restrict__ cz, double *__restrict__ cw,
                                                                int *__restrict__ keep) {
     extern __shared__ double lds[];
     double *sx = lds, *sy = lds + nAtoms, *sz = lds + 2 * nAtoms, *sInv = lds + 3 * nAtoms, *sA = lds + 4 * nAtoms;
+    int *sOrd = reinterpret_cast<int *>(lds + 5 * nAtoms);
     const RadialShell sh = shells[blockIdx.x];
     const int A = sh.atom;
     for (int j = threadIdx.x; j < nAtoms; j += blockDim.x) {
+        sOrd[j] = order[static_cast<size_t>(A) * nAtoms + j];
         sx[j] = xyz[3 * j];
         sy[j] = xyz[3 * j + 1];
         sz[j] = xyz[3 * j + 2];
@@ -219,7 +226,8 @@ __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell
             if (i == A) continue;
             const double ri = dist(i, px, py, pz);
             double Pi = becke_factor_of(i > A, ri, rA, sInv[i], sA[i]);
-            for (int j = 0; j < nAtoms && fabs(Pi) >= floor; j++) {
+            for (int jj = 0; jj < nAtoms && fabs(Pi) >= floor; jj++) {
+                const int j = sOrd[jj];
                 if (j == i || j == A) continue;
                 const int ij = pair_index(i, j);
                 Pi *= becke_factor_of(i > j, ri, dist(j, px, py, pz), invR[ij], aij[ij]);
@@ -385,19 +393,37 @@ int build_grid(xq_grid *g) {
     // Device work.
     RadialShell *d_shells = nullptr;
     double *d_rules = nullptr, *d_xyz = nullptr, *d_invR = nullptr, *d_aij = nullptr;
+    int *d_order = nullptr;
     double *cx = nullptr, *cy = nullptr, *cz = nullptr, *cw = nullptr;
     int *keep = nullptr, *dst = nullptr, *total = nullptr, *tileSums = nullptr;
-    // LDS of the candidate kernel: atom coordinates and the shell atom's pair row (5 doubles per atom).
-    const size_t lds = 5 * sizeof(double) * static_cast<size_t>(nA);
+    // LDS of the candidate kernel: atom coordinates, the shell atom's pair row and its distance order (44 bytes per
+    // atom).
+    const size_t lds = (5 * sizeof(double) + sizeof(int)) * static_cast<size_t>(nA);
     if (lds > 64 * 1024) {
-        set_error("grid construction supports at most 1638 atoms");
+        set_error("grid construction supports at most 1489 atoms");
         return kInvalidArgument;
+    }
+    // Per atom A, all atoms in order of distance from A (A first).
+    std::vector<int> order(static_cast<size_t>(nA) * nA);
+    {
+        std::vector<double> d2(nA);
+        for (int a = 0; a < nA; a++) {
+            int *row = order.data() + static_cast<size_t>(a) * nA;
+            for (int j = 0; j < nA; j++) {
+                const double dx = g->xyz[3 * a] - g->xyz[3 * j], dy = g->xyz[3 * a + 1] - g->xyz[3 * j + 1],
+                             dz = g->xyz[3 * a + 2] - g->xyz[3 * j + 2];
+                d2[j] = dx * dx + dy * dy + dz * dz;
+                row[j] = j;
+            }
+            std::stable_sort(row, row + nA, [&](int p, int q) { return d2[p] < d2[q]; });
+        }
     }
     int status = kOK;
     auto cleanup = [&]() {
         // The kernels that use these arrays must be done before the arrays are reused.
         (void)hipStreamSynchronize(g->stream);
         dev_free(g, &d_shells); dev_free(g, &d_rules); dev_free(g, &d_xyz); dev_free(g, &d_invR); dev_free(g, &d_aij);
+        dev_free(g, &d_order);
         dev_free(g, &cx); dev_free(g, &cy); dev_free(g, &cz); dev_free(g, &cw); dev_free(g, &keep); dev_free(g, &dst); dev_free(g, &total);
         dev_free(g, &tileSums);
     };
@@ -408,6 +434,7 @@ int build_grid(xq_grid *g) {
         if ((status = dev_alloc(g, &d_xyz, g->xyz.size())) != kOK) break;
         if ((status = dev_alloc(g, &d_invR, nPairs)) != kOK) break;
         if ((status = dev_alloc(g, &d_aij, nPairs)) != kOK) break;
+        if ((status = dev_alloc(g, &d_order, order.size())) != kOK) break;
         if ((status = dev_alloc(g, &cx, nCand)) != kOK) break;
         if ((status = dev_alloc(g, &cy, nCand)) != kOK) break;
         if ((status = dev_alloc(g, &cz, nCand)) != kOK) break;
@@ -427,9 +454,11 @@ int build_grid(xq_grid *g) {
                             "copy invR")) != kOK) break;
         if ((status = check(hipMemcpyAsync(d_aij, aij.data(), nPairs * sizeof(double), hipMemcpyHostToDevice, s),
                             "copy aij")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                            "copy order")) != kOK) break;
         if (!shells.empty()) {
             hipLaunchKernelGGL(candidate_points_kernel, dim3(shells.size()), dim3(256), lds, s, d_shells, d_rules, d_xyz, nA,
-                               d_invR, d_aij, cx, cy, cz, cw, keep);
+                               d_invR, d_aij, d_order, cx, cy, cz, cw, keep);
             if ((status = check(hipGetLastError(), "candidate_points_kernel")) != kOK) break;
             const int nTiles = (nCand + kScanTile - 1) / kScanTile;
             hipLaunchKernelGGL(scan_tile_sums_kernel, dim3(nTiles), dim3(256), 0, s, nCand, keep, tileSums);
