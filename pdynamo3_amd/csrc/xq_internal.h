@@ -68,9 +68,11 @@ struct MixedJob {
     double *WPA;                       // FP64 path: W in the Fock's operand order (per step)
     // Mixed path, Fock side: superblocks (up to four consecutive blocks of one atom) over the union of their kept
     // functions.  supers[s] = {first block, blocks, offset into uIdx, union size}; per block, its first union chunk;
-    // per union chunk, its block; per union chunk row, the row's place in the block's own kept list (-1: not kept).
+    // per union chunk, its block; per union chunk, grad Phi as fixed point in the W kernel's order (GHU, GLU).
     const int4 *supers, *uTiles;
-    const int *uIdx, *uChunkOffset, *uChunkBlock, *uRow;
+    const int *uIdx, *uChunkOffset, *uChunkBlock;
+    const short *GHU;
+    const unsigned char *GLU;
     FunctionalSet fs;
     double rhoTol;
     int n, nBlocks, nChunks, nTiles;
@@ -117,9 +119,12 @@ struct xq_grid {
     int *d_eP = nullptr;
     double *d_wcoef = nullptr;
     long long nChunks = 0, nTiles = 0;
-    // Mixed path, Fock side (xq_mixed.hip): superblock table, union indices, union chunk tables and row map, tiles.
+    // Mixed path, Fock side (xq_mixed.hip): superblock table, union indices, union chunk tables, tiles, grad Phi in
+    // the union layout; the union row map is kept only during the set-up.
     int4 *d_supers = nullptr, *d_uTiles = nullptr;
     int *d_uIdx = nullptr, *d_uChunkOffset = nullptr, *d_uChunkBlock = nullptr, *d_uRow = nullptr;
+    short *d_GHU = nullptr;
+    unsigned char *d_GLU = nullptr;
     long long nSupers = 0, nChunksU = 0, nTilesU = 0, nUnionRows = 0;
     // FP64 path (layouts in xq_f64.hip): Phi and grad Phi planes in operand order, W per step.
     double *d_FPB = nullptr, *d_GPB = nullptr, *d_FPA = nullptr, *d_WPA = nullptr;

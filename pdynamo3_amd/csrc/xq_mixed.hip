@@ -25,11 +25,13 @@
 // kSuper consecutive blocks of one atom, whose rows are the union of their kept functions (uIdx).  Each block of a
 // superblock holds its Phi and W digits over the union rows, zero for a row that the block does not keep, so that
 // the superblock's V tiles sum all its blocks before one FP64 update of V; union chunk CU of the grid is union chunk rc
-// of block b (CU = uChunkOffset[b] + rc), and uRow maps each union chunk row to its row in the block's kept list.
+// of block b (CU = uChunkOffset[b] + rc); at set-up, uRow maps each union chunk row to its row in the block's kept list.
 //   DA, DW [CU][q (4)][digit (4)][lane (64)][16 bytes]  Phi and W digits in the operand order of the Fock product: lane
 //                                  (lr, lh) = (l % 32, l / 32) holds union row 32 rc + lr at points 32 q + 16 lh .. + 15,
 //                                  which is its A operand (rows of tile I) and its B operand (rows of tile J).  Exponents
 //                                  EA, EW [CU][32], one per row over the block's own 128 points.
+//   GHU, GLU [CU][3][q (4)][lane (64)][16]        grad Phi of the union rows as 24-bit fixed point (as GH, GL below) in
+//                                  DA's order, read by the W kernel with one 16-byte load per plane.
 //   DB     [C][digit (4)][point (128)][32 bytes]   Phi digits in the B operand order of the density product.  Position
 //                                  16 h + v of a point holds row 32 rc + 4 h + 8 (v / 4) + v % 4 (chunk_fun): the row
 //                                  whose result the MFMA lane (., h) receives in register v, so that the epilogue finds
@@ -187,11 +189,13 @@ __global__ void __launch_bounds__(kBlockSize) point_exponent_kernel(
     eg[2 * kBlockSize] = static_cast<signed char>(stored_exponent(mz));
 }
 
-// One union chunk per work-group (256 threads): EA and DA of the chunk's 32 union rows from the FP64 values of the
-// block's kept rows (zero for the rows the block does not keep).
+// One union chunk per work-group (256 threads): EA, DA, GHU and GLU of the chunk's 32 union rows from the FP64 values of
+// the block's kept rows (zero for the rows the block does not keep).
 __global__ void __launch_bounds__(256) fock_operand_prepare_kernel(
     const xq::DevBlock *__restrict__ blocks, const int *__restrict__ uChunkBlock, const int *__restrict__ uRow,
-    const double *__restrict__ f, signed char *__restrict__ DA, signed char *__restrict__ EA) {
+    const double *__restrict__ f, const double *__restrict__ fx, const double *__restrict__ fy,
+    const double *__restrict__ fz, const signed char *__restrict__ EG, signed char *__restrict__ DA,
+    signed char *__restrict__ EA, short *__restrict__ GHU, unsigned char *__restrict__ GLU) {
     __shared__ int sE[kC];
     __shared__ int sRow[kC];
     const size_t C = blockIdx.x;
@@ -201,16 +205,16 @@ __global__ void __launch_bounds__(256) fock_operand_prepare_kernel(
     const size_t R = static_cast<size_t>(blk.vOffset);
     if (t < kC) sRow[t] = uRow[C * kC + t];
     __syncthreads();
-    auto value = [&](int i, int p) -> double {
+    auto value = [&](const double *a, int i, int p) -> double {
         const int k = sRow[i];
-        return k >= 0 ? f[(R + k) * kBlockSize + p] : 0.0;
+        return k >= 0 ? a[(R + k) * kBlockSize + p] : 0.0;
     };
     // EA: row i = t / 8 of the chunk, eight threads per row with 16 points each.
     {
         const int i = t >> 3, part = t & 7;
         double m = 0.0;
 #pragma unroll 4
-        for (int k = 0; k < 16; k++) m = fmax(m, fabs(value(i, 16 * part + k)));
+        for (int k = 0; k < 16; k++) m = fmax(m, fabs(value(f, i, 16 * part + k)));
 #pragma unroll
         for (int o = 4; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 8));
         if (part == 0) {
@@ -226,13 +230,34 @@ __global__ void __launch_bounds__(256) fock_operand_prepare_kernel(
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         signed char dd[kDPhi];
-        slice<kDPhi>(value(lr, kC * q + 16 * lh + k), e, dd);
+        slice<kDPhi>(value(f, lr, kC * q + 16 * lh + k), e, dd);
 #pragma unroll
         for (int j = 0; j < kDPhi; j++) d[j][k] = dd[j];
     }
 #pragma unroll
     for (int j = 0; j < kDPhi; j++)
         *reinterpret_cast<v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]) = pack16(d[j]);
+    // GHU, GLU: the same thread's 16 values of each gradient component, scaled by each point's component exponent.
+    const double *grads[3] = {fx, fy, fz};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        short hi[16];
+        unsigned char lo[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int p = kC * q + 16 * lh + k;
+            grad_encode(value(grads[c], lr, p), EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p], hi[k], lo[k]);
+        }
+        const size_t o = C * kChunkGrad + ((c * 4 + q) * 64 + lane) * 16;
+        v4i h2[2];
+#pragma unroll
+        for (int w = 0; w < 8; w++)
+            h2[w >> 2][w & 3] = static_cast<int>((static_cast<unsigned>(hi[2 * w]) & 0xffffu) |
+                                                 (static_cast<unsigned>(hi[2 * w + 1]) << 16));
+        *reinterpret_cast<v4i *>(GHU + o) = h2[0];
+        *reinterpret_cast<v4i *>(GHU + o + 8) = h2[1];
+        *reinterpret_cast<v4i *>(GLU + o) = pack16(reinterpret_cast<const signed char (&)[16]>(lo));
+    }
 }
 
 // One chunk per work-group (256 threads): DB, GH and GL from the FP64 values of the chunk's rows.
@@ -508,11 +533,10 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     const xq::MixedJob &J = jobs[job_of(jobs, nJobs, blockIdx.x, [](const xq::MixedJob &j) { return j.chunkBase; })];
     const xq::DevBlock *__restrict__ blocks = J.blocks;
     const int *__restrict__ uChunkBlock = J.uChunkBlock;
-    const int *__restrict__ chunkOffset = J.chunkOffset;
     const signed char *__restrict__ DA = J.DA;
     const signed char *__restrict__ EA = J.EA;
-    const short *__restrict__ GH = J.GH;
-    const unsigned char *__restrict__ GL = J.GL;
+    const short *__restrict__ GHU = J.GHU;
+    const unsigned char *__restrict__ GLU = J.GLU;
     const signed char *__restrict__ EG = J.EG;
     const double *__restrict__ wcoef = J.wcoef;
     const int *__restrict__ skip = J.skip;
@@ -523,8 +547,6 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     if (skip[b]) return;
     const xq::DevBlock blk = blocks[b];
     const int t = threadIdx.x, lane = t & 63, q = t >> 6, lr = lane & 31, lh = lane >> 5;
-    // Row lr of the union chunk is row k of the block's kept list (its gradients are in the block's own chunks).
-    const int k = J.uRow[C * kC + lr];
     // Per point: w vrho / 2, and 2 w vsigma grad rho scaled by the point's gradient powers of two.
     if (t < kBlockSize) {
 #pragma unroll
@@ -539,13 +561,16 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     for (int j = 0; j < kDPhi; j++)
         d[j] = *reinterpret_cast<const v4i *>(&DA[C * kChunkBytes + (q * kDPhi + j) * 1024 + lane * 16]);
     const int ea = EA[C * kC + lr];
-    // A row that the block does not keep reads the block's first chunk and is masked to zero.
-    const size_t go = k >= 0 ? (static_cast<size_t>(chunkOffset[b]) + (k >> 5)) * kChunkGrad + chunk_pos(k & 31)
-                             : static_cast<size_t>(chunkOffset[b]) * kChunkGrad;
-    const double gmask = k >= 0 ? 1.0 : 0.0;
-    auto grad = [&](int c, int p) {
-        const size_t o = go + (c * kBlockSize + p) * kC;
-        return static_cast<double>(static_cast<int>(GH[o]) * 256 + static_cast<int>(GL[o]));
+    v4i gh[3][2], gl[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const size_t o = C * kChunkGrad + ((c * 4 + q) * 64 + lane) * 16;
+        gh[c][0] = *reinterpret_cast<const v4i *>(GHU + o);
+        gh[c][1] = *reinterpret_cast<const v4i *>(GHU + o + 8);
+        gl[c] = *reinterpret_cast<const v4i *>(GLU + o);
+    }
+    auto grad = [&](int c, int v) {
+        return static_cast<double>(short_of(gh[c], v) * 256 + ubyte_of(gl[c], v));
     };
     double W[16];
     double m = 0.0;
@@ -553,7 +578,7 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
     for (int v = 0; v < 16; v++) {
         const int p = kC * q + 16 * lh + v;
         const double phi = ldexp(static_cast<double>(digits4(d, v)), ea - 28);
-        W[v] = sc[0][p] * phi + gmask * (sc[1][p] * grad(0, p) + sc[2][p] * grad(1, p) + sc[3][p] * grad(2, p));
+        W[v] = sc[0][p] * phi + sc[1][p] * grad(0, v) + sc[2][p] * grad(1, v) + sc[3][p] * grad(2, v);
         m = fmax(m, fabs(W[v]));
     }
     m = fmax(m, __shfl_xor(m, 32, 64));
@@ -792,6 +817,8 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     XQ_TRY(dev_alloc(g, &g->d_GL, nc * kChunkGrad));
     XQ_TRY(dev_alloc(g, &g->d_EG, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * 3 * kBlockSize));
     XQ_TRY(dev_alloc(g, &g->d_DW, ncU * kChunkBytes));
+    XQ_TRY(dev_alloc(g, &g->d_GHU, ncU * kChunkGrad));
+    XQ_TRY(dev_alloc(g, &g->d_GLU, ncU * kChunkGrad));
     XQ_TRY(dev_alloc(g, &g->d_EW, ncU * kC));
     XQ_TRY(dev_alloc(g, &g->d_Pq, static_cast<size_t>(g->nFunctions) * g->nFunctions));
     XQ_TRY(dev_alloc(g, &g->d_eP, static_cast<size_t>(g->nFunctions)));
@@ -806,9 +833,11 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
                            g->d_GH, g->d_GL);
     if (chunksU > 0)
         hipLaunchKernelGGL(fock_operand_prepare_kernel, dim3(static_cast<unsigned>(chunksU)), dim3(256), 0, s,
-                           g->d_blocks, g->d_uChunkBlock, g->d_uRow, f, g->d_DA, g->d_EA);
+                           g->d_blocks, g->d_uChunkBlock, g->d_uRow, f, fx, fy, fz, g->d_EG, g->d_DA, g->d_EA,
+                           g->d_GHU, g->d_GLU);
     XQ_CHECK(hipGetLastError());
     XQ_CHECK(hipStreamSynchronize(s));
+    dev_free(g, &g->d_uRow);   // used by the set-up only
     return kOK;
 }
 
@@ -822,6 +851,8 @@ void free_mixed(xq_grid *g) {
     dev_free(g, &g->d_uChunkOffset);
     dev_free(g, &g->d_uChunkBlock);
     dev_free(g, &g->d_uRow);
+    dev_free(g, &g->d_GHU);
+    dev_free(g, &g->d_GLU);
     dev_free(g, &g->d_DA);
     dev_free(g, &g->d_EA);
     dev_free(g, &g->d_DB);
@@ -876,7 +907,8 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
         J.uIdx = g->d_uIdx;
         J.uChunkOffset = g->d_uChunkOffset;
         J.uChunkBlock = g->d_uChunkBlock;
-        J.uRow = g->d_uRow;
+        J.GHU = g->d_GHU;
+        J.GLU = g->d_GLU;
         J.FPB = g->d_FPB;
         J.GPB = g->d_GPB;
         J.FPA = g->d_FPA;
