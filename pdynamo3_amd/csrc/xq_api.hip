@@ -130,10 +130,10 @@ int xq_estimate_bytes(long long nPoints, int nFunctions, double meanKeptFunction
     // Fock tile list and chunk tables (rows padded to chunks of 32: about 10% more stored values than kept).
     double b = grid + kept + work + 32.0 * values + 16.0 * blocks * 16.0 + 32.0 * G;
     if (precision == XQ_PRECISION_MIXED)
-        // Phi digits in the density's order and 24-bit gradients (13 bytes per stored value), Phi and W digits and the
-        // gradients again over the superblocks' union rows (17 bytes per stored value, about 15% more rows), per-point
+        // Phi digits in the density's order and 32-bit gradients (17 bytes per stored value), Phi and W digits and
+        // 24-bit gradients over the superblocks' union rows (17 bytes per stored value, about 15% more rows), per-point
         // exponents, P digits.
-        b += 1.1 * (13.0 + 1.15 * 17.0) * values + 512.0 * blocks + 8.0 * n * n + 4.0 * n;
+        b += 1.1 * (17.0 + 1.15 * 17.0) * values + 512.0 * blocks + 8.0 * n * n + 4.0 * n;
     else
         // Phi in two operand orders, grad Phi, W (48 bytes per stored value).
         b += 1.1 * 48.0 * values;

@@ -105,12 +105,12 @@ static void stage_bytes(const xq_grid *g, double *rho, double *fock) {
         *rho = ch * 4.0 * 32768.0 + 8.0 * nn + 40.0 * G + 20.0 * B;
         *fock = ch * (4.0 * 32768.0 + 32768.0) + 32.0 * G + ch * 2.0 * 32768.0 + 16.0 * nn;
     } else {
-        // Density: Phi digits (16 KB per chunk), fixed-point gradients (36 KB per chunk), P digits, point exponents,
+        // Density: Phi digits (20 KB per chunk), fixed-point gradients (48 KB per chunk), P digits, point exponents,
         // weights in, W coefficients out, block sums.  Fock stage: W formation (Phi digits and fixed-point gradients
         // per union chunk, coefficients in, W digits out) and the Fock product (Phi and W digits per union chunk in, V
         // read and written).
         const double chU = static_cast<double>(g->nChunksU);
-        *rho = ch * (16384.0 + 36864.0) + 8.0 * nn + 512.0 * B + 40.0 * G + 20.0 * B;
+        *rho = ch * (20480.0 + 49152.0) + 8.0 * nn + 512.0 * B + 40.0 * G + 20.0 * B;
         *fock = chU * (16384.0 + 36864.0 + 16384.0 + 32.0) + 32.0 * G + 384.0 * B + chU * 2.0 * 16384.0 + 16.0 * nn;
     }
 }

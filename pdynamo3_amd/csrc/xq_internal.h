@@ -56,8 +56,7 @@ struct MixedJob {
     const int *chunkOffset, *chunkBlock, *idx;
     const int4 *tiles;
     const signed char *DA, *EA, *DB, *EB, *EG;
-    const short *GH;
-    const unsigned char *GL;
+    const int *GF;
     const double *P, *w;
     int *eP;
     unsigned long long *Pq;
@@ -68,7 +67,7 @@ struct MixedJob {
     double *WPA;                       // FP64 path: W in the Fock's operand order (per step)
     // Mixed path, Fock side: superblocks (up to four consecutive blocks of one atom) over the union of their kept
     // functions.  supers[s] = {first block, blocks, offset into uIdx, union size}; per block, its first union chunk;
-    // per union chunk, its block; per union chunk, grad Phi as fixed point in the W kernel's order (GHU, GLU).
+    // per union chunk, its block; per union chunk, grad Phi as 24-bit fixed point in the W kernel's order (GHU, GLU).
     const int4 *supers, *uTiles;
     const int *uIdx, *uChunkOffset, *uChunkBlock;
     const short *GHU;
@@ -111,8 +110,7 @@ struct xq_grid {
     int *d_chunkOffset = nullptr, *d_chunkBlock = nullptr;
     int4 *d_tiles = nullptr;
     signed char *d_DA = nullptr, *d_EA = nullptr, *d_DB = nullptr, *d_EB = nullptr;
-    short *d_GH = nullptr;
-    unsigned char *d_GL = nullptr;
+    int *d_GF = nullptr;
     signed char *d_EG = nullptr;
     signed char *d_DW = nullptr, *d_EW = nullptr;
     unsigned long long *d_Pq = nullptr;

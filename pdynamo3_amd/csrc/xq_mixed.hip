@@ -11,14 +11,20 @@
 // kept one) differ from FP64, both bounded relative to the groups' powers of two.
 //
 //   X = P_red Phi           P: 5 digits per element, one power of two per row of P (sliced once per step).
-//                           Phi: 4 digits, one power of two per point of a block (over the block's kept functions).
-//                           Orders k + l <= 4.  rho = sum Phi X and grad rho = 2 sum grad Phi X in FP64, with Phi from
-//                           its digits and grad Phi stored in FP32.
+//                           Phi: 5 digits, one power of two per point of a block (over the block's kept functions).
+//                           With 4 digits, the values more than 2^-28 below a point's largest round to zero; they lie
+//                           in the functions' tails and all of the same sign, so rho came out low (E_xc about +1.2e-8
+//                           Eh per water).  Orders k + l <= 4: a dropped product carries its pair's sign, so dropping
+//                           products also shrinks the terms of rho toward zero, by about 2e-10 Eh per water in E_xc
+//                           (order 5 would add three products per step for that).  rho = sum Phi X and
+//                           grad rho = 2 sum grad Phi X in FP64, with Phi from its digits and grad Phi from its fixed
+//                           point.
 //   V += Phi W^T + W Phi^T  Phi and W: 4 digits, one power of two per function row of a block (over its 128 points).
 //                           Orders k + l <= 3.  W = (w vrho / 2) Phi + 2 w vsigma (grad rho . grad Phi) is formed and
 //                           sliced per step.
-// The functional, the weights and every reduction are FP64.  grad Phi is stored as 24-bit fixed point relative to
-// each point's largest component (a per-value FP32 rounding biases sigma = |grad rho|^2 measurably; this does not).
+// The functional, the weights and every reduction are FP64.  grad Phi is stored as 32-bit fixed point relative to
+// each point's largest component.  Rounding it shifts E_xc through sigma = |grad rho|^2 roughly with the square of the
+// rounding unit: at 24 bits (or as FP32 values) by about +6e-9 Eh per water, at 32 bits by below 1e-11.
 //
 // Data layouts.  The kept functions of a block are cut into chunks of 32 rows; chunk C of the grid is chunk rc of block b
 // (C = chunkOffset[b] + rc).  Rows past the block's kept count are zero.  The Fock side works on superblocks: up to
@@ -30,16 +36,17 @@
 //                                  (lr, lh) = (l % 32, l / 32) holds union row 32 rc + lr at points 32 q + 16 lh .. + 15,
 //                                  which is its A operand (rows of tile I) and its B operand (rows of tile J).  Exponents
 //                                  EA, EW [CU][32], one per row over the block's own 128 points.
-//   GHU, GLU [CU][3][q (4)][lane (64)][16]        grad Phi of the union rows as 24-bit fixed point (as GH, GL below) in
-//                                  DA's order, read by the W kernel with one 16-byte load per plane.
-//   DB     [C][digit (4)][point (128)][32 bytes]   Phi digits in the B operand order of the density product.  Position
+//   GHU, GLU [CU][3][q (4)][lane (64)][16]        grad Phi of the union rows in DA's order as 24-bit fixed point:
+//                                  value = (256 GHU + GLU) 2^(EG - 23), GHU int16, GLU uint8 (W enters V only, whose
+//                                  error its 4-digit rows bound); the W kernel reads them with 16-byte loads.
+//   DB     [C][digit (5)][point (128)][32 bytes]   Phi digits in the B operand order of the density product.  Position
 //                                  16 h + v of a point holds row 32 rc + 4 h + 8 (v / 4) + v % 4 (chunk_fun): the row
 //                                  whose result the MFMA lane (., h) receives in register v, so that the epilogue finds
 //                                  Phi for its own results in the operand it loads.  P_red tiles use the same K order.
 //                                  Exponents EB [block][128].
-//   GH, GL [C][3][point (128)][32]                 grad Phi in DB's order as 24-bit fixed point: value =
-//                                  (256 GH + GL) 2^(EG - 23), GH int16, GL uint8, one power of two per point and
-//                                  component over the block's kept functions (EG [block][3][128]).
+//   GF     [C][3][point (128)][32 int32]           grad Phi in DB's order as 32-bit fixed point: value = GF 2^(EG - 31),
+//                                  one power of two per point and component over the block's kept functions
+//                                  (EG [block][3][128]).
 //   Pq     [n][n] 64-bit words                     P digits 0..4 in bytes 0..4; row exponents eP [n].
 //
 // MFMA lane layout (32 x 32 x 32, int8): lane l supplies A[l % 32][16 (l / 32) .. + 15] and B[16 (l / 32) .. + 15][l % 32]
@@ -62,11 +69,15 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 namespace {
 
 constexpr int kC = 32;                  // chunk length (functions or points) and MFMA tile edge
-constexpr int kDPhi = 4;                // digits of Phi and W
+constexpr int kDPhi = 4;                // digits of Phi and W in the Fock product
+constexpr int kDB = 5;                  // digits of Phi in the density product
 constexpr int kDP = 5;                  // digits of P
-constexpr int kChunkBytes = 16384;      // one chunk of DA, DW or DB: 4 digits x 32 rows x 128 points
-constexpr int kChunkGrad = 3 * kBlockSize * kC;     // one chunk of GH or GL (elements)
-constexpr int kGradBits = 23;           // grad Phi fixed point: |value| < 2^EG is stored as a multiple of 2^(EG - 23)
+constexpr int kXOrders = 5;             // digit orders 0..4 kept in X = P_red Phi
+constexpr int kChunkBytes = 16384;      // one chunk of DA or DW: 4 digits x 32 rows x 128 points
+constexpr int kChunkBytesB = kDB * kBlockSize * kC;   // one chunk of DB: 5 digits x 128 points x 32 rows
+constexpr int kChunkGrad = 3 * kBlockSize * kC;     // one chunk of GF, GHU or GLU (elements)
+constexpr int kGradBits = 31;           // GF: |value| < 2^EG is stored as a multiple of 2^(EG - 31)
+constexpr int kGradBitsW = 23;          // GHU, GLU: |value| < 2^EG is stored as a multiple of 2^(EG - 23)
 constexpr int kPanel = 8;               // chunks of P_red columns staged at once by the density kernel
 constexpr int kPanelDwords = 8 * kPanel + 4;        // one panel row in LDS (padded against bank conflicts)
 constexpr int kIdxCache = 1024;         // kept-function indices of a block cached in LDS by the density kernel
@@ -127,6 +138,12 @@ __device__ __forceinline__ int digits4(const v4i (&d)[kDPhi], int v) {
     return ((byte_of(d[0], v) * 128 + byte_of(d[1], v)) * 128 + byte_of(d[2], v)) * 128 + byte_of(d[3], v);
 }
 
+// 5-digit value of byte v of the digit operands d[0..4]: sum_j d_j 128^(4-j).
+__device__ __forceinline__ long long digits5(const v4i (&d)[kDB], int v) {
+    const int hi = ((byte_of(d[0], v) * 128 + byte_of(d[1], v)) * 128 + byte_of(d[2], v)) * 128 + byte_of(d[3], v);
+    return static_cast<long long>(hi) * 128 + byte_of(d[4], v);
+}
+
 // Signed 16-bit element v (0..15) of 32 bytes, and unsigned byte v of 16 bytes.
 __device__ __forceinline__ int short_of(const v4i (&h)[2], int v) {
     const unsigned w = static_cast<unsigned>(h[v >> 3][(v >> 1) & 3]);
@@ -136,13 +153,16 @@ __device__ __forceinline__ int ubyte_of(const v4i &x, int v) {
     return static_cast<int>((static_cast<unsigned>(x[v >> 2]) >> (8 * (v & 3))) & 255u);
 }
 
-// 24-bit fixed point of g with |g| < 2^e: Q = round(g 2^(23 - e)) in [-(2^23 - 1), 2^23 - 1], split as 256 hi + lo.
-__device__ __forceinline__ void grad_encode(double g, int e, short &hi, unsigned char &lo) {
-    long long Q = static_cast<long long>(rint(ldexp(g, kGradBits - e)));
-    const long long lim = (1ll << kGradBits) - 1;
+// Element v (0..15) of sixteen int32 held in four 16-byte operands.
+__device__ __forceinline__ double int_of(const v4i (&g)[4], int v) { return static_cast<double>(g[v >> 2][v & 3]); }
+
+// B-bit fixed point of g with |g| < 2^e: round(g 2^(B - e)) in [-(2^B - 1), 2^B - 1].
+template <int B>
+__device__ __forceinline__ int grad_encode(double g, int e) {
+    long long Q = static_cast<long long>(rint(ldexp(g, B - e)));
+    const long long lim = (1ll << B) - 1;
     Q = Q > lim ? lim : (Q < -lim ? -lim : Q);
-    hi = static_cast<short>(Q >> 8);
-    lo = static_cast<unsigned char>(Q & 255);
+    return static_cast<int>(Q);
 }
 
 // Row of a chunk held at density position 16 h + v, and the position of row k.
@@ -241,31 +261,36 @@ __global__ void __launch_bounds__(256) fock_operand_prepare_kernel(
     const double *grads[3] = {fx, fy, fz};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        short hi[16];
-        unsigned char lo[16];
+        int Q[16];
 #pragma unroll
         for (int k = 0; k < 16; k++) {
             const int p = kC * q + 16 * lh + k;
-            grad_encode(value(grads[c], lr, p), EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p], hi[k], lo[k]);
+            Q[k] = grad_encode<kGradBitsW>(value(grads[c], lr, p), EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p]);
         }
-        const size_t o = C * kChunkGrad + ((c * 4 + q) * 64 + lane) * 16;
-        v4i h2[2];
+        // Q = 256 hi + lo with lo in [0, 255]: hi = Q >> 8 (arithmetic), lo = Q & 255.
+        v4i hq[2], lq;
 #pragma unroll
         for (int w = 0; w < 8; w++)
-            h2[w >> 2][w & 3] = static_cast<int>((static_cast<unsigned>(hi[2 * w]) & 0xffffu) |
-                                                 (static_cast<unsigned>(hi[2 * w + 1]) << 16));
-        *reinterpret_cast<v4i *>(GHU + o) = h2[0];
-        *reinterpret_cast<v4i *>(GHU + o + 8) = h2[1];
-        *reinterpret_cast<v4i *>(GLU + o) = pack16(reinterpret_cast<const signed char (&)[16]>(lo));
+            hq[w >> 2][w & 3] = static_cast<int>((static_cast<unsigned>(Q[2 * w] >> 8) & 0xffffu) |
+                                                 (static_cast<unsigned>(Q[2 * w + 1] >> 8) << 16));
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+            lq[w] = static_cast<int>((static_cast<unsigned>(Q[4 * w]) & 255u) | ((static_cast<unsigned>(Q[4 * w + 1]) & 255u) << 8) |
+                                     ((static_cast<unsigned>(Q[4 * w + 2]) & 255u) << 16) |
+                                     ((static_cast<unsigned>(Q[4 * w + 3]) & 255u) << 24));
+        const size_t o = C * kChunkGrad + ((c * 4 + q) * 64 + lane) * 16;
+        *reinterpret_cast<v4i *>(GHU + o) = hq[0];
+        *reinterpret_cast<v4i *>(GHU + o + 8) = hq[1];
+        *reinterpret_cast<v4i *>(GLU + o) = lq;
     }
 }
 
-// One chunk per work-group (256 threads): DB, GH and GL from the FP64 values of the chunk's rows.
+// One chunk per work-group (256 threads): DB and GF from the FP64 values of the chunk's rows.
 __global__ void __launch_bounds__(256) density_operand_prepare_kernel(
     const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ chunkBlock,
     const double *__restrict__ f, const double *__restrict__ fx, const double *__restrict__ fy,
     const double *__restrict__ fz, const signed char *__restrict__ EB, const signed char *__restrict__ EG,
-    signed char *__restrict__ DB, short *__restrict__ GH, unsigned char *__restrict__ GL) {
+    signed char *__restrict__ DB, int *__restrict__ GF) {
     const size_t C = blockIdx.x;
     const int b = chunkBlock[C];
     const xq::DevBlock blk = blocks[b];
@@ -274,32 +299,32 @@ __global__ void __launch_bounds__(256) density_operand_prepare_kernel(
     auto value = [&](const double *a, int row, int p) -> double {
         return row < nb ? a[(R + row) * kBlockSize + p] : 0.0;
     };
-    // DB, GH and GL: thread = (point p, half h) with the 16 rows chunk_fun(h, 0..15).
+    // DB and GF: thread = (point p, half h) with the 16 rows chunk_fun(h, 0..15).
     {
         const int p = t & (kBlockSize - 1), h = t >> 7;
         const int e = EB[static_cast<size_t>(b) * kBlockSize + p];
-        signed char d[kDPhi][16];
+        signed char d[kDB][16];
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-            signed char dd[kDPhi];
-            slice<kDPhi>(value(f, kC * rc + chunk_fun(h, v), p), e, dd);
+            signed char dd[kDB];
+            slice<kDB>(value(f, kC * rc + chunk_fun(h, v), p), e, dd);
 #pragma unroll
-            for (int j = 0; j < kDPhi; j++) d[j][v] = dd[j];
+            for (int j = 0; j < kDB; j++) d[j][v] = dd[j];
         }
 #pragma unroll
-        for (int j = 0; j < kDPhi; j++)
-            *reinterpret_cast<v4i *>(&DB[C * kChunkBytes + (j * kBlockSize + p) * kC + 16 * h]) = pack16(d[j]);
+        for (int j = 0; j < kDB; j++)
+            *reinterpret_cast<v4i *>(&DB[C * kChunkBytesB + (j * kBlockSize + p) * kC + 16 * h]) = pack16(d[j]);
         const double *grads[3] = {fx, fy, fz};
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int eg = EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p];
-            short hi[16];
-            unsigned char lo[16];
+            v4i gq[4];
 #pragma unroll
-            for (int v = 0; v < 16; v++) grad_encode(value(grads[c], kC * rc + chunk_fun(h, v), p), eg, hi[v], lo[v]);
+            for (int v = 0; v < 16; v++)
+                gq[v >> 2][v & 3] = grad_encode<kGradBits>(value(grads[c], kC * rc + chunk_fun(h, v), p), eg);
             const size_t o = C * kChunkGrad + (c * kBlockSize + p) * kC + 16 * h;
 #pragma unroll
-            for (int v = 0; v < 16; v++) { GH[o + v] = hi[v]; GL[o + v] = lo[v]; }
+            for (int w = 0; w < 4; w++) *reinterpret_cast<v4i *>(GF + o + 4 * w) = gq[w];
         }
     }
 }
@@ -356,8 +381,7 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     const int *__restrict__ idxAll = J.idx;
     const signed char *__restrict__ DB = J.DB;
     const signed char *__restrict__ EB = J.EB;
-    const short *__restrict__ GH = J.GH;
-    const unsigned char *__restrict__ GL = J.GL;
+    const int *__restrict__ GF = J.GF;
     const signed char *__restrict__ EG = J.EG;
     const int n = J.n;
     const unsigned long long *__restrict__ Pq = J.Pq;
@@ -386,17 +410,17 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     const size_t C = static_cast<size_t>(chunkOffset[b]);
     const int nChunks = (nb + kC - 1) / kC;
     const int p = kC * wv + lr;
-    const signed char *DBb = DB + C * kChunkBytes;
+    const signed char *DBb = DB + C * kChunkBytesB;
     // P_red is staged one row panel at a time (rows 32 I .. + 31, up to kPanel chunks of columns), all of the panel's
     // words fetched before the barrier; the panel's MFMA steps then run without barriers.  Thread (row sr, slot s8)
     // fetches columns 4 s8 .. + 3 of every chunk of the panel and writes them as one word per digit at chunk_pos.
     const int sr = t >> 3, s8 = t & 7, sdw = 4 * (s8 & 1) + (s8 >> 1);
-    auto fetchB = [&](int c, v4i (&bq)[kDPhi]) {
+    auto fetchB = [&](int c, v4i (&bq)[kDB]) {
 #pragma unroll
-        for (int j = 0; j < kDPhi; j++)
-            bq[j] = *reinterpret_cast<const v4i *>(&DBb[(c * kDPhi + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
+        for (int j = 0; j < kDB; j++)
+            bq[j] = *reinterpret_cast<const v4i *>(&DBb[(c * kDB + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
     };
-    v4i bnext[kDPhi];
+    v4i bnext[kDB];
     fetchB(0, bnext);
     const int eb = EB[static_cast<size_t>(b) * kBlockSize + p];
     double gscale[3];
@@ -404,7 +428,7 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     for (int c = 0; c < 3; c++) gscale[c] = ldexp(1.0, EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p] - kGradBits);
     double pr = 0.0, pgx = 0.0, pgy = 0.0, pgz = 0.0;
     for (int I = 0; I < nChunks; I++) {
-        v16i acc[kDP];
+        v16i acc[kXOrders];
         const int r = kC * I + sr;
         const size_t rowOff = r < nb ? static_cast<size_t>(idx[r]) * n : 0;
         for (int c0 = 0; c0 < nChunks; c0 += kPanel) {
@@ -433,14 +457,14 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
             if (c0 == 0) {
                 if (t < kC) sPe[I & 1][t] = kC * I + t < nb ? eP[idx[kC * I + t]] : 0;
 #pragma unroll
-                for (int s = 0; s < kDP; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int s = 0; s < kXOrders; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
             }
             __syncthreads();
             for (int cl = 0; cl < nc; cl++) {
                 const int c = c0 + cl;
-                v4i a[kDP], bq[kDPhi];
+                v4i a[kDP], bq[kDB];
 #pragma unroll
-                for (int j = 0; j < kDPhi; j++) bq[j] = bnext[j];
+                for (int j = 0; j < kDB; j++) bq[j] = bnext[j];
                 if (c + 1 < nChunks) fetchB(c + 1, bnext);
                 else if (I + 1 < nChunks) fetchB(0, bnext);
 #pragma unroll
@@ -448,33 +472,35 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
 #pragma unroll
                 for (int i = 0; i < kDP; i++)
 #pragma unroll
-                    for (int j = 0; j < kDPhi; j++)
-                        if (i + j <= 4) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
+                    for (int j = 0; j < kDB; j++)
+                        if (i + j < kXOrders) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
             }
         }
         // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi from
         // its fixed point (the sums over v are scaled once).
-        v4i dI[kDPhi];
+        // X in FP64 first, so that the accumulators are free before the epilogue's operands arrive.
+        double x[16];
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int s[kXOrders] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
+            x[v] = ldexp(static_cast<double>(combine<kXOrders>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 7 * (kXOrders + 1));
+        }
+        v4i dI[kDB];
         fetchB(I, dI);
-        v4i gh[3][2], gl[3];
+        double sumR = 0.0, sumG[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int v = 0; v < 16; v++) sumR += static_cast<double>(digits5(dI, v)) * x[v];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const size_t o = (C + I) * kChunkGrad + (c * kBlockSize + p) * kC + 16 * lh;
-            gh[c][0] = *reinterpret_cast<const v4i *>(GH + o);
-            gh[c][1] = *reinterpret_cast<const v4i *>(GH + o + 8);
-            gl[c] = *reinterpret_cast<const v4i *>(GL + o);
-        }
-        double sumR = 0.0, sumX = 0.0, sumY = 0.0, sumZ = 0.0;
+            v4i gq[4];
 #pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int s[kDP] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
-            const double x = ldexp(static_cast<double>(combine<kDP>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 42);
-            sumR += static_cast<double>(digits4(dI, v)) * x;
-            sumX += static_cast<double>(short_of(gh[0], v) * 256 + ubyte_of(gl[0], v)) * x;
-            sumY += static_cast<double>(short_of(gh[1], v) * 256 + ubyte_of(gl[1], v)) * x;
-            sumZ += static_cast<double>(short_of(gh[2], v) * 256 + ubyte_of(gl[2], v)) * x;
+            for (int w = 0; w < 4; w++) gq[w] = *reinterpret_cast<const v4i *>(GF + o + 4 * w);
+#pragma unroll
+            for (int v = 0; v < 16; v++) sumG[c] += int_of(gq, v) * x[v];
         }
-        pr += ldexp(sumR, eb - 28);
+        const double sumX = sumG[0], sumY = sumG[1], sumZ = sumG[2];
+        pr += ldexp(sumR, eb - 7 * kDB);
         pgx += sumX * gscale[0];
         pgy += sumY * gscale[1];
         pgz += sumZ * gscale[2];
@@ -552,7 +578,7 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const double c = t < blk.nPoints ? wcoef[4 * (static_cast<size_t>(blk.pStart) + t) + k] : 0.0;
-            sc[k][t] = k == 0 ? c : ldexp(c, EG[(static_cast<size_t>(b) * 3 + (k - 1)) * kBlockSize + t] - kGradBits);
+            sc[k][t] = k == 0 ? c : ldexp(c, EG[(static_cast<size_t>(b) * 3 + (k - 1)) * kBlockSize + t] - kGradBitsW);
         }
     }
     __syncthreads();
@@ -569,9 +595,7 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
         gh[c][1] = *reinterpret_cast<const v4i *>(GHU + o + 8);
         gl[c] = *reinterpret_cast<const v4i *>(GLU + o);
     }
-    auto grad = [&](int c, int v) {
-        return static_cast<double>(short_of(gh[c], v) * 256 + ubyte_of(gl[c], v));
-    };
+    auto grad = [&](int c, int v) { return static_cast<double>(short_of(gh[c], v) * 256 + ubyte_of(gl[c], v)); };
     double W[16];
     double m = 0.0;
 #pragma unroll
@@ -690,8 +714,9 @@ int chunk_tables(xq_grid *g) {
     long long chunks = 0;
     for (int b = 0; b < g->nBlocks; b++) {
         const int nb = g->blocks[b].nKept;
-        // The density accumulators of the mixed path hold up to 4 x 127^2 x nb per order in int32.
-        if (nb > 32767) { set_error("at most 32767 kept functions per block are supported"); return kInvalidArgument; }
+        // The density accumulators of the mixed path hold up to 5 x 127^2 x nb per order in int32 (nb <= 26630), and
+        // their exact combination sum_s acc_s 128^(4-s) stays far inside int64.
+        if (nb > 26000) { set_error("at most 26000 kept functions per block are supported"); return kInvalidArgument; }
         const int T = (nb + kC - 1) / kC;
         chunkOffset[b] = static_cast<int>(chunks);
         for (int I = 0; I < T; I++) {
@@ -811,10 +836,9 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     const size_t nc = static_cast<size_t>(chunks > 0 ? chunks : 1), ncU = static_cast<size_t>(chunksU > 0 ? chunksU : 1);
     XQ_TRY(dev_alloc(g, &g->d_DA, ncU * kChunkBytes));
     XQ_TRY(dev_alloc(g, &g->d_EA, ncU * kC));
-    XQ_TRY(dev_alloc(g, &g->d_DB, nc * kChunkBytes));
+    XQ_TRY(dev_alloc(g, &g->d_DB, nc * kChunkBytesB));
     XQ_TRY(dev_alloc(g, &g->d_EB, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * kBlockSize));
-    XQ_TRY(dev_alloc(g, &g->d_GH, nc * kChunkGrad));
-    XQ_TRY(dev_alloc(g, &g->d_GL, nc * kChunkGrad));
+    XQ_TRY(dev_alloc(g, &g->d_GF, nc * kChunkGrad));
     XQ_TRY(dev_alloc(g, &g->d_EG, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * 3 * kBlockSize));
     XQ_TRY(dev_alloc(g, &g->d_DW, ncU * kChunkBytes));
     XQ_TRY(dev_alloc(g, &g->d_GHU, ncU * kChunkGrad));
@@ -830,7 +854,7 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     if (chunks > 0)
         hipLaunchKernelGGL(density_operand_prepare_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s,
                            g->d_blocks, g->d_chunkOffset, g->d_chunkBlock, f, fx, fy, fz, g->d_EB, g->d_EG, g->d_DB,
-                           g->d_GH, g->d_GL);
+                           g->d_GF);
     if (chunksU > 0)
         hipLaunchKernelGGL(fock_operand_prepare_kernel, dim3(static_cast<unsigned>(chunksU)), dim3(256), 0, s,
                            g->d_blocks, g->d_uChunkBlock, g->d_uRow, f, fx, fy, fz, g->d_EG, g->d_DA, g->d_EA,
@@ -857,8 +881,7 @@ void free_mixed(xq_grid *g) {
     dev_free(g, &g->d_EA);
     dev_free(g, &g->d_DB);
     dev_free(g, &g->d_EB);
-    dev_free(g, &g->d_GH);
-    dev_free(g, &g->d_GL);
+    dev_free(g, &g->d_GF);
     dev_free(g, &g->d_EG);
     dev_free(g, &g->d_DW);
     dev_free(g, &g->d_EW);
@@ -890,8 +913,7 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
         J.DB = g->d_DB;
         J.EB = g->d_EB;
         J.EG = g->d_EG;
-        J.GH = g->d_GH;
-        J.GL = g->d_GL;
+        J.GF = g->d_GF;
         J.P = g->d_P;
         J.w = g->d_w;
         J.eP = g->d_eP;
