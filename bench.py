@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 WORKLOADS = {
-    # name: (waters, cube in Angstrom, seed, functional) -- BASELINE configurations 1, 2 and 4.
+    # name: (waters, cube in Angstrom, seed, functional): the (H2O)16 PBE, (H2O)64 BLYP and (H2O)128 PBE clusters.
     "h2o16": (16, 9.0, 1, "pbe"),
     "h2o64": (64, 14.0, 2, "blyp"),
     "h2o128": (128, 18.0, 4, "pbe"),
@@ -39,7 +39,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="h2o64", choices=sorted(WORKLOADS))
-    p.add_argument("--spherical", action="store_true", help="spherical d (reference default); default Cartesian as in BASELINE")
+    p.add_argument("--spherical", action="store_true", help="spherical d (reference default); default Cartesian d")
     p.add_argument("--cpu-baseline", default="reference", choices=["reference", "port", "none"])
     p.add_argument("--cpu-sample-waters", type=int, default=8)
     p.add_argument("--skip-fp64", action="store_true")
@@ -141,9 +141,11 @@ def cpu_baseline(kind, n_waters, functional, spherical, threads=1):
 def serving_leg(args, log_fn):
     """The serving stream: SCF requests on (H2O)N, N log-uniform in [4, 64] (seed 3), BLYP/PBE by coin flip, Poisson
     arrivals at serve-load x the loop's capacity.  The capacity is measured first, on a calibration stream of 32 other
-    requests (seed 7) that all arrive at once."""
+    requests (seed 7) that all arrive at once, after a warm-up pass of 16 more (seed 8) that fills the device memory
+    cache and loads the kernels."""
     from pdynamo3_amd import serving
     srv = serving.Server(precision="mixed", max_jobs=args.serve_slots)
+    srv.run(serving.request_stream(n_requests=16, seed=8, rate=1.0e6))
     calib = serving.request_stream(n_requests=32, seed=7, rate=1.0e6)
     done, cl = srv.run(calib)
     capacity = len(done) / cl["wall_s"]
@@ -168,7 +170,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, and the quadrature runs on one "
-              "GPU (the sharding design is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "GPU; across GPUs it would shard by grid block. Run with --gpus 1.", file=sys.stderr)
         return 2
     from pdynamo3_amd import basis, densities, integrator, molecules
     from pdynamo3_amd.definitions import functional_ids

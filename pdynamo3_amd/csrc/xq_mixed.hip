@@ -669,7 +669,8 @@ fock_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPe
             v16i acc[4];
 #pragma unroll
             for (int s = 0; s < 4; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll 2
+            // All four point chunks' operands are loaded before their products (32 loads in flight per lane).
+#pragma unroll
             for (int q = 0; q < 4; q++) {
                 v4i xa[kDPhi], xb[kDPhi];
 #pragma unroll

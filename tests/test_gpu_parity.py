@@ -119,18 +119,23 @@ def test_h2o16_matches_compiled_reference(precision):
     assert abs(np.linalg.norm(V) - c["V_norm"]) <= (1e-9 if precision == "fp64" else 1e-5)
 
 
-def test_h2o64_mixed_against_fp64():
-    """Mixed-precision quadrature of (H2O)64 (BLYP, Cartesian): |dE_xc| <= 1e-6 Eh and max|dV| <= 1e-7 against FP64."""
+@pytest.mark.parametrize("waters,cube,seed,functional", [(64, 14.0, 2, "blyp"), (128, 18.0, 4, "pbe")])
+def test_large_cluster_mixed_against_fp64(waters, cube, seed, functional):
+    """Mixed-precision quadrature of (H2O)64 (BLYP) and (H2O)128 (PBE, 2432 functions), Cartesian: |dE_xc| <= 1e-6 Eh
+    and max|dV| <= 1e-7 against FP64.  Rounding biases grow with the molecule (the error of E_xc is extensive), so the
+    largest cluster is the one that tests them; N_e, which depends on rho alone, is held to the same bar."""
     I = _integrator()
-    Z, xyz, spec, P = _cluster(64, 14.0, 2, False)
+    Z, xyz, spec, P = _cluster(waters, cube, seed, False)
     g = I.QuadratureGrid(Z, xyz, 2)
     g.set_basis(spec, False, "fp64")
     V0 = np.zeros_like(P)
-    e0, n0, _ = g.integrate(ids("blyp"), P, V0)
+    e0, n0, _ = g.integrate(ids(functional), P, V0)
     g.set_basis(spec, False, "mixed")
     V = np.zeros_like(P)
-    e, ne, _ = g.integrate(ids("blyp"), P, V)
+    e, ne, _ = g.integrate(ids(functional), P, V)
     dv = float(np.max(np.abs(V - V0)))
     print("dE", e - e0, "dN", ne - n0, "max dV", dv)
     assert abs(e - e0) <= 1e-6
+    assert abs(ne - n0) <= 1e-6
     assert dv <= 1e-7
+    g.close()
