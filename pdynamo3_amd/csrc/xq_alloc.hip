@@ -1,4 +1,4 @@
-// Device memory of libxcquad: a process-wide caching allocator over hipMalloc.
+// Device and pinned host memory of libxcquad: process-wide caching allocators over hipMalloc and hipHostMalloc.
 //
 // hipFree synchronises the whole device, so a request loop that sets up and releases handles while other handles'
 // quadratures run would stall them at every release.  Freed blocks are kept instead and handed out again: a request
@@ -6,6 +6,7 @@
 // and to 2 MiB above 1 MiB), so a loop whose jobs come and go reuses device memory without runtime calls.  When
 // hipMalloc fails, every cached block is released (hipFree) and the allocation is tried once more.  A block is cached
 // only after the work that used it has completed: every caller synchronises the owning stream before it frees.
+// Pinned host blocks (the staging of P and V) are cached the same way, since hipHostFree synchronises the device too.
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -23,6 +24,8 @@ std::mutex g_mutex;
 std::multimap<size_t, void *> g_cached;            // size -> idle block
 std::unordered_map<void *, size_t> g_live;         // block -> size
 size_t g_cachedBytes = 0;
+std::multimap<size_t, void *> g_hostCached;        // pinned host blocks, as above
+std::unordered_map<void *, size_t> g_hostLive;
 
 size_t rounded(size_t bytes) {
     if (bytes == 0) bytes = 1;
@@ -71,6 +74,40 @@ void cache_free(void *ptr) {
     g_cached.emplace(it->second, ptr);
     g_cachedBytes += it->second;
     g_live.erase(it);
+}
+
+hipError_t cache_host_alloc(void **ptr, size_t bytes) {
+    const size_t size = rounded(bytes);
+    std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_hostCached.lower_bound(size);
+    if (it != g_hostCached.end() && it->first <= size + size / 4) {
+        *ptr = it->second;
+        g_hostLive[it->second] = it->first;
+        g_hostCached.erase(it);
+        return hipSuccess;
+    }
+    hipError_t e = hipHostMalloc(ptr, size, hipHostMallocDefault);
+    if (e != hipSuccess && !g_hostCached.empty()) {
+        (void)hipGetLastError();
+        for (auto &kv : g_hostCached) (void)hipHostFree(kv.second);
+        g_hostCached.clear();
+        e = hipHostMalloc(ptr, size, hipHostMallocDefault);
+    }
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return e;
+    }
+    g_hostLive[*ptr] = size;
+    return hipSuccess;
+}
+
+void cache_host_free(void *ptr) {
+    if (ptr == nullptr) return;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_hostLive.find(ptr);
+    if (it == g_hostLive.end()) return;
+    g_hostCached.emplace(it->second, ptr);
+    g_hostLive.erase(it);
 }
 
 size_t cache_idle_bytes() {

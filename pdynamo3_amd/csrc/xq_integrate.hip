@@ -56,7 +56,10 @@ __global__ void zero_kernel(size_t n, double *__restrict__ a) {
 }  // namespace
 
 void free_work(xq_grid *g) {
-    if (g->h_job != nullptr) { (void)hipHostFree(g->h_job); g->h_job = nullptr; }
+    if (g->h_job != nullptr) { cache_host_free(g->h_job); g->h_job = nullptr; }
+    if (g->h_stage != nullptr) { cache_host_free(g->h_stage); g->h_stage = nullptr; }
+    if (g->h_blockEN != nullptr) { cache_host_free(g->h_blockEN); g->h_blockEN = nullptr; }
+    if (g->h_skip != nullptr) { cache_host_free(g->h_skip); g->h_skip = nullptr; }
     dev_free(g, &g->d_job);
     dev_free(g, &g->d_P);
     dev_free(g, &g->d_V);
@@ -73,8 +76,10 @@ static int ensure_work(xq_grid *g) {
     XQ_TRY(dev_alloc(g, &g->d_packed, n * (n + 1) / 2));
     XQ_TRY(dev_alloc(g, &g->d_blockEN, 2 * static_cast<size_t>(g->nBlocks)));
     XQ_TRY(dev_alloc(g, &g->d_skip, static_cast<size_t>(g->nBlocks)));
-    g->h_blockEN.assign(2 * static_cast<size_t>(g->nBlocks), 0.0);
-    g->h_skip.assign(static_cast<size_t>(g->nBlocks), 0);
+    const size_t nb = static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1);
+    XQ_CHECK(cache_host_alloc(reinterpret_cast<void **>(&g->h_stage), n * (n + 1) / 2 * sizeof(double) + 8));
+    XQ_CHECK(cache_host_alloc(reinterpret_cast<void **>(&g->h_blockEN), 2 * nb * sizeof(double)));
+    XQ_CHECK(cache_host_alloc(reinterpret_cast<void **>(&g->h_skip), nb * sizeof(int)));
     return kOK;
 }
 
@@ -126,10 +131,12 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
         }
         XQ_TRY(ensure_work(grids[j]));
     }
+    // P crosses PCIe from each handle's pinned staging buffer (an asynchronous DMA; the buffer then receives V).
     for (int j = 0; j < nJobs; j++) {
         const xq_grid *g = grids[j];
         const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
-        XQ_CHECK(hipMemcpyAsync(g->d_packed, Ppacked[j], nPacked * sizeof(double), hipMemcpyHostToDevice, s));
+        std::memcpy(g->h_stage, Ppacked[j], nPacked * sizeof(double));
+        XQ_CHECK(hipMemcpyAsync(g->d_packed, g->h_stage, nPacked * sizeof(double), hipMemcpyHostToDevice, s));
     }
     XQ_CHECK(hipEventRecord(ev.ev0, s));
     for (int j = 0; j < nJobs; j++) {
@@ -156,19 +163,15 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
     }
     XQ_CHECK(hipGetLastError());
     XQ_CHECK(hipEventRecord(ev.ev1, s));
-    // Wait for the kernels first: a pageable copy that waits for its stream holds the runtime's staging path, which
-    // would stall other threads' copies (set-up of other handles) for the whole quadrature.
-    XQ_CHECK(hipStreamSynchronize(s));
-    std::vector<std::vector<double>> vpacked(nJobs);
+    // Results into the pinned buffers (asynchronous DMAs ordered after the kernels), then one wait.
     for (int j = 0; j < nJobs; j++) {
         xq_grid *g = grids[j];
         const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
-        vpacked[j].resize(nPacked);
-        XQ_CHECK(hipMemcpyAsync(vpacked[j].data(), g->d_packed, nPacked * sizeof(double), hipMemcpyDeviceToHost, s));
+        XQ_CHECK(hipMemcpyAsync(g->h_stage, g->d_packed, nPacked * sizeof(double), hipMemcpyDeviceToHost, s));
         if (g->nBlocks > 0) {
-            XQ_CHECK(hipMemcpyAsync(g->h_blockEN.data(), g->d_blockEN, 2 * g->nBlocks * sizeof(double),
-                                    hipMemcpyDeviceToHost, s));
-            XQ_CHECK(hipMemcpyAsync(g->h_skip.data(), g->d_skip, g->nBlocks * sizeof(int), hipMemcpyDeviceToHost, s));
+            XQ_CHECK(hipMemcpyAsync(g->h_blockEN, g->d_blockEN, 2 * g->nBlocks * sizeof(double), hipMemcpyDeviceToHost,
+                                    s));
+            XQ_CHECK(hipMemcpyAsync(g->h_skip, g->d_skip, g->nBlocks * sizeof(int), hipMemcpyDeviceToHost, s));
         }
     }
     XQ_CHECK(hipStreamSynchronize(s));
@@ -183,8 +186,8 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
             ne += g->h_blockEN[2 * b + 1];
             if (g->h_skip[b] && g->blocks[b].nKept > 0) skipped++;
         }
-        const size_t nPacked = vpacked[j].size();
-        for (size_t k = 0; k < nPacked; k++) V[j][k] += vpacked[j][k];
+        const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
+        for (size_t k = 0; k < nPacked; k++) V[j][k] += g->h_stage[k];
         eXC[j] = e;
         nE[j] = ne;
         bfv += g->bfValues;
@@ -216,7 +219,7 @@ int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
     FunctionalSet fs;
     XQ_TRY(functional_set(nIDs, ids, &fs));
     if (g->h_job == nullptr) {
-        XQ_CHECK(hipHostMalloc(reinterpret_cast<void **>(&g->h_job), sizeof(MixedJob)));
+        XQ_CHECK(cache_host_alloc(reinterpret_cast<void **>(&g->h_job), sizeof(MixedJob)));
         XQ_TRY(dev_alloc(g, &g->d_job, 1));
     }
     const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1};

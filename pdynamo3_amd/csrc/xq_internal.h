@@ -133,8 +133,9 @@ struct xq_grid {
     double *d_P = nullptr, *d_V = nullptr, *d_packed = nullptr;   // dense n x n P and V, packed staging
     double *d_blockEN = nullptr;          // per block: E_xc and N_e partial sums
     int *d_skip = nullptr;                // per block: 1 if skipped by the density tolerance
-    std::vector<double> h_blockEN;
-    std::vector<int> h_skip;
+    double *h_stage = nullptr;            // pinned: packed P in, packed V out
+    double *h_blockEN = nullptr;          // pinned: per block E_xc and N_e partial sums
+    int *h_skip = nullptr;                // pinned: per block skip flags
     xq::MixedJob *h_job = nullptr, *d_job = nullptr;   // one-entry job table of single calls (pinned host, device)
     long long deviceBytes = 0;
 };
@@ -154,6 +155,8 @@ namespace xq {
 // frees it.  cache_idle_bytes: bytes held idle; cache_release: hipFree every idle block.
 hipError_t cache_alloc(void **ptr, size_t bytes);
 void cache_free(void *ptr);
+hipError_t cache_host_alloc(void **ptr, size_t bytes);   // pinned host memory, cached the same way
+void cache_host_free(void *ptr);
 size_t cache_idle_bytes();
 void cache_release();
 
