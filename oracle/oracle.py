@@ -7,6 +7,7 @@ package.  Both libraries expose the same operations:
     npoints / nblocks / nfunctions
     grid()            -> (xyzw [G,4], block_atom [B], block_count [B])
     integrate(P)      -> (E_xc, N_e, V_packed, seconds)          (V starts at zero here)
+    gradients(P)      -> XC nuclear gradients [nAtoms, 3] (from zero)
     block_values(b)   -> (indices, f, fx, fy, fz)
     functional(ids, rho, sigma) -> (eps, vrho, vsigma)
     c2s(lLow, lHigh)  -> matrix
@@ -51,6 +52,8 @@ class _Library:
         f("grid").argtypes = [ctypes.c_void_p, _dp, _ip, _ip]
         f("integrate").restype = ctypes.c_int
         f("integrate").argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        f("gradients").restype = ctypes.c_int
+        f("gradients").argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
         f("block_values").restype = ctypes.c_int
         f("block_values").argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
         f("functional").restype = ctypes.c_int
@@ -100,6 +103,7 @@ class _Handle:
         self.npoints = lib._fn("npoints")(self.h)
         self.nblocks = lib._fn("nblocks")(self.h)
         self.nfunctions = lib._fn("nfunctions")(self.h)
+        self.natoms = len(self._Z)
 
     def grid(self):
         xyzw = np.zeros((self.npoints, 4))
@@ -120,6 +124,19 @@ class _Handle:
         if st != 0:
             raise ValueError("integrate status {:d}".format(st))
         return float(e[0]), float(ne[0]), V, float(sec[0])
+
+    def gradients(self, P_packed, threads=1):
+        """XC nuclear gradients (nAtoms x 3, hartree/bohr) of E_xc at the density P (the oracle's threads, or the reference's
+        in-core flag, as the third argument)."""
+        P = np.ascontiguousarray(P_packed, dtype=np.float64)
+        n = self.nfunctions
+        if P.size != n * (n + 1) // 2:
+            raise ValueError("P has {:d} entries, expected {:d}".format(P.size, n * (n + 1) // 2))
+        g = np.zeros((self.natoms, 3))
+        st = self.lib._fn("gradients")(self.h, _d(P), int(threads), _d(g))
+        if st != 0:
+            raise ValueError("gradients status {:d}".format(st))
+        return g
 
     def block_values(self, block):
         n = self.nfunctions

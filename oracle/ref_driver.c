@@ -181,6 +181,31 @@ int xcref_integrate ( void *vh, const double *P, int inCore, double *eXC, double
     return ( int ) st ;
 }
 
+/* . XC nuclear gradients: DFTIntegrator_Integrate with gradients3 and no Fock matrix, as DFTGridIntegrator.Gradients
+!    (DFTGridIntegrator.py:72-88) calls it.  grad (nAtoms x 3) is incremented. */
+int xcref_gradients ( void *vh, const double *P, int inCore, double *grad )
+{
+    RefHandle *h = ( RefHandle * ) vh ;
+    Status st = Status_OK ;
+    int n = h->nFunctions, k, i, size = ( n * ( n + 1 ) ) / 2 ;
+    Real eXC = 0.0, rho = 0.0 ;
+    SymmetricMatrix *dP = SymmetricMatrix_AllocateWithExtent ( n, &st ) ;
+    Coordinates3 *g3 = Coordinates3_Allocate ( h->nAtoms, &st ) ;
+    for ( k = 0 ; k < size ; k++ ) dP->data[k] = P[k] ;
+    for ( i = 0 ; i < h->nAtoms ; i++ ) Coordinates3_SetRow ( g3, i, grad[3*i], grad[3*i+1], grad[3*i+2] ) ;
+    DFTIntegrator_Integrate ( h->functional, h->grid, h->bases, h->coordinates, dP, NULL, ( inCore ? True : False ), False,
+                              &eXC, &rho, NULL, NULL, g3, &st ) ;
+    for ( i = 0 ; i < h->nAtoms ; i++ )
+    {
+        grad[3*i  ] = Coordinates3_Item ( g3, i, 0 ) ;
+        grad[3*i+1] = Coordinates3_Item ( g3, i, 1 ) ;
+        grad[3*i+2] = Coordinates3_Item ( g3, i, 2 ) ;
+    }
+    SymmetricMatrix_Deallocate ( &dP ) ;
+    Coordinates3_Deallocate ( &g3 ) ;
+    return ( int ) st ;
+}
+
 /* . Basis values and first derivatives of one block after the reference's |phi| > bfTol filter.
 !    Outputs: nb, indices[nb], f/fx/fy/fz [nb x nPoints] row-major.  Returns nb. */
 int xcref_block_values ( void *vh, int record, int *indices, double *f, double *fx, double *fy, double *fz )

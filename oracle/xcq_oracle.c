@@ -409,12 +409,14 @@ static int BuildBasis ( OHandle *h, int nSpec, const double *spec )
     return 0 ;
 }
 
-/* Values and gradients of all functions at nP points (GaussianBasisIntegrals_f1Op1.c:100-197 + GaussianBasisTransform.c:85-125):
-!  e0 = sum_p c e^{-a r^2}, e1 = -2 sum_p c a e^{-a r^2}; phi = x^i y^j z^k e0; d/dx phi = (i x^{i-1} e0 + x^{i+1} e1) y^j z^k;
-!  spherical = c2s^T Cartesian.  Output rows = functions, columns = points (row-major, ld = nP). */
-static void EvaluateBasis ( const OHandle *h, int nP, const double *pts, double *f, double *fx, double *fy, double *fz )
+/* Values, gradients and (nOut = 10) second derivatives of all functions at nP points (GaussianBasisIntegrals_f1Op1.c:100-197
+!  and :203-331 + GaussianBasisTransform.c:85-125): e0 = sum_p c e^{-a r^2}, e1 = -2 sum_p c a e^{-a r^2}, e2 = 4 sum_p c a^2 e^{-a r^2};
+!  phi = x^i y^j z^k e0; d/dx phi = (i x^{i-1} e0 + x^{i+1} e1) y^j z^k; d2/dx2 phi = (i(i-1) x^{i-2} e0 + (2i+1) x^i e1 + x^{i+2} e2) y^j z^k;
+!  d2/dxdy phi = (i j x^{i-1} y^{j-1} e0 + (i x^{i-1} y^{j+1} + j x^{i+1} y^{j-1}) e1 + x^{i+1} y^{j+1} e2) z^k; spherical = c2s^T Cartesian.
+!  out[0..nOut-1] = f, fx, fy, fz, fxx, fxy, fxz, fyy, fyz, fzz; rows = functions, columns = points (row-major, ld = nP). */
+static void EvaluateBasisN ( const OHandle *h, int nP, const double *pts, double **out, int nOut )
 {
-    double g[4][64], e0[64], e1[64] ;
+    double g[10][64], e0[64], e1[64], e2[64] ;
     for ( int p = 0 ; p < nP ; p++ )
     {
         for ( int s = 0 ; s < h->nShells ; s++ )
@@ -422,34 +424,44 @@ static void EvaluateBasis ( const OHandle *h, int nP, const double *pts, double 
             const OShell *sh = &h->shells[s] ;
             double dx = pts[4*p] - h->xyz[3*sh->atom], dy = pts[4*p+1] - h->xyz[3*sh->atom+1], dz = pts[4*p+2] - h->xyz[3*sh->atom+2] ;
             double r2 = dx * dx + dy * dy + dz * dz ;
-            double X0[8], Y0[8], Z0[8], X1[8], Y1[8], Z1[8] ;
-            X0[0] = Y0[0] = Z0[0] = 1.0 ; X1[0] = Y1[0] = Z1[0] = 0.0 ;
-            for ( int i = 1 ; i <= sh->lHigh + 1 ; i++ )
+            double X0[10], Y0[10], Z0[10], X1[10], Y1[10], Z1[10], X2[10], Y2[10], Z2[10] ;
+            X0[0] = Y0[0] = Z0[0] = 1.0 ; X1[0] = Y1[0] = Z1[0] = 0.0 ; X2[0] = Y2[0] = Z2[0] = 0.0 ;
+            for ( int i = 1 ; i <= sh->lHigh + 2 ; i++ )
             {
                 X0[i] = dx * X0[i-1] ; Y0[i] = dy * Y0[i-1] ; Z0[i] = dz * Z0[i-1] ;
                 X1[i] = i * X0[i-1] ; Y1[i] = i * Y0[i-1] ; Z1[i] = i * Z0[i-1] ;
+                X2[i] = i * X1[i-1] ; Y2[i] = i * Y1[i-1] ; Z2[i] = i * Z1[i-1] ;
             }
-            for ( int c = 0 ; c < sh->nCBF ; c++ ) { e0[c] = 0.0 ; e1[c] = 0.0 ; }
+            for ( int c = 0 ; c < sh->nCBF ; c++ ) { e0[c] = 0.0 ; e1[c] = 0.0 ; e2[c] = 0.0 ; }
             for ( int q = 0 ; q < sh->nPrim ; q++ )
             {
-                double ex = exp ( - sh->exps[q] * r2 ) ;
+                double a = sh->exps[q], ex = exp ( - a * r2 ) ;
                 for ( int c = 0 ; c < sh->nCBF ; c++ )
                 {
                     e0[c] += sh->ccbf[q * sh->nCBF + c] * ex ;
-                    e1[c] += sh->ccbf[q * sh->nCBF + c] * sh->exps[q] * ex ;
+                    e1[c] += sh->ccbf[q * sh->nCBF + c] * a * ex ;
+                    e2[c] += sh->ccbf[q * sh->nCBF + c] * ( a * a ) * ex ;
                 }
             }
             for ( int c = 0 ; c < sh->nCBF ; c++ )
             {
                 int i = sh->px[c], j = sh->py[c], k = sh->pz[c] ;
-                double E1 = -2.0 * e1[c] ;
-                g[0][c] = X0[i] * Y0[j] * Z0[k] * e0[c] ;
-                g[1][c] = ( X1[i] * e0[c] + X0[i+1] * E1 ) * Y0[j] * Z0[k] ;
-                g[2][c] = ( Y1[j] * e0[c] + Y0[j+1] * E1 ) * X0[i] * Z0[k] ;
-                g[3][c] = ( Z1[k] * e0[c] + Z0[k+1] * E1 ) * X0[i] * Y0[j] ;
+                double E0 = e0[c], E1 = -2.0 * e1[c], E2 = 4.0 * e2[c] ;
+                g[0][c] = X0[i] * Y0[j] * Z0[k] * E0 ;
+                g[1][c] = ( X1[i] * E0 + X0[i+1] * E1 ) * Y0[j] * Z0[k] ;
+                g[2][c] = ( Y1[j] * E0 + Y0[j+1] * E1 ) * X0[i] * Z0[k] ;
+                g[3][c] = ( Z1[k] * E0 + Z0[k+1] * E1 ) * X0[i] * Y0[j] ;
+                if ( nOut > 4 )
+                {
+                    g[4][c] = ( X2[i] * E0 + ( dx * X1[i] + X1[i+1] ) * E1 + X0[i+2] * E2 ) * Y0[j] * Z0[k] ;
+                    g[5][c] = ( X1[i] * Y1[j] * E0 + ( X1[i] * Y0[j+1] + X0[i+1] * Y1[j] ) * E1 + X0[i+1] * Y0[j+1] * E2 ) * Z0[k] ;
+                    g[6][c] = ( X1[i] * Z1[k] * E0 + ( X1[i] * Z0[k+1] + X0[i+1] * Z1[k] ) * E1 + X0[i+1] * Z0[k+1] * E2 ) * Y0[j] ;
+                    g[7][c] = ( Y2[j] * E0 + ( dy * Y1[j] + Y1[j+1] ) * E1 + Y0[j+2] * E2 ) * X0[i] * Z0[k] ;
+                    g[8][c] = ( Y1[j] * Z1[k] * E0 + ( Y1[j] * Z0[k+1] + Y0[j+1] * Z1[k] ) * E1 + Y0[j+1] * Z0[k+1] * E2 ) * X0[i] ;
+                    g[9][c] = ( Z2[k] * E0 + ( dz * Z1[k] + Z1[k+1] ) * E1 + Z0[k+2] * E2 ) * X0[i] * Y0[j] ;
+                }
             }
-            double *out[4] = { f, fx, fy, fz } ;
-            for ( int d = 0 ; d < 4 ; d++ )
+            for ( int d = 0 ; d < nOut ; d++ )
             {
                 for ( int b = 0 ; b < sh->nBasis ; b++ )
                 {
@@ -461,6 +473,13 @@ static void EvaluateBasis ( const OHandle *h, int nP, const double *pts, double 
             }
         }
     }
+}
+
+/* Values and gradients only (the quadrature's data). */
+static void EvaluateBasis ( const OHandle *h, int nP, const double *pts, double *f, double *fx, double *fy, double *fz )
+{
+    double *out[4] = { f, fx, fy, fz } ;
+    EvaluateBasisN ( h, nP, pts, out, 4 ) ;
 }
 
 /* Keep rows with max_p |f| > tol, compacting in place; returns the kept count (GridFunctionDataBlock.c:130-185). */
@@ -760,6 +779,23 @@ int xqo_block_values ( void *vh, int block, int *indices, double *f, double *fx,
     return nb ;
 }
 
+/* As FilterRows, for n arrays (the first decides). */
+static int FilterRowsN ( int nF, int nP, double tol, int *idx, double **a, int nArr )
+{
+    int n = 0 ;
+    for ( int m = 0 ; m < nF ; m++ )
+    {
+        double mx = 0.0 ;
+        for ( int p = 0 ; p < nP ; p++ ) { double v = fabs ( a[0][m * nP + p] ) ; if ( v > mx ) mx = v ; }
+        if ( mx > tol )
+        {
+            if ( m != n ) for ( int k = 0 ; k < nArr ; k++ ) memcpy ( a[k] + n * nP, a[k] + m * nP, nP * sizeof ( double ) ) ;
+            idx[n++] = m ;
+        }
+    }
+    return n ;
+}
+
 /*----------------------------------------------------------------------------------------------------------------------------------
 ! Quadrature (DFTIntegrator.c:159-509, restricted closed shell, Fock only).
 !   rho_p = sum_mn phi_mp P_mn phi_np (X = P_red Phi, :838-839); grad rho = 2 sum_m grad phi_mp X_mp; sigma = |grad rho|^2 (:843-850);
@@ -880,6 +916,193 @@ int xqo_integrate ( void *vh, const double *Ppacked, int nThreads, double *eXC, 
     *eXC = eTot ; *nE = nTot ;
     h->bfValues = bfv ; h->blocksSkipped = skipped ;
     if ( seconds != NULL ) *seconds = ( t1.tv_sec - t0.tv_sec ) + 1.0e-9 * ( t1.tv_nsec - t0.tv_nsec ) ;
+    return 0 ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! XC nuclear gradients (DFTIntegrator.c:159-509 with doGradients, restricted closed shell, GGA).  Per block, with the basis values,
+! gradients and second derivatives (order + 1, :203-207) after the |phi| > bfTol filter:
+!   - rho and its gradient as in the quadrature; skip the block if max rho <= rhoTol; vrho, vsigma *= w; eXC = eps rho (:332-350);
+!   - direct terms (DFTIntegrator_Gradients :643-747, DFTIntegrator_GradientsSigma :752-808, factor 4 for the restricted case):
+!       work = 2 vrho (P_red Phi)                         with (d/dx, d/dy, d/dz) Phi,
+!       work = 4 vsigma (P_red (grad Phi . grad rho))     with (d/dx, d/dy, d/dz) Phi,
+!       work = 4 vsigma (P_red Phi) d rho / dc            with (d/dx, d/dy, d/dz) d/dc Phi for c = x, y, z;
+!     each work row i adds g = sum_p work_ip (d Phi_ip) as -g to the atom of function i and +g to the block's atom (UGradientContributions,
+!     :909-941, the dE/dr_g term of a grid that moves with its atom);
+!   - weight terms (DFTGridWeights_Derivatives, DFTGridWeights.c:98-215): per point, A_i = prod_j s(nu_ij) and dA_i/dmu_ij, then the
+!     derivatives of mu_ij with respect to the point and to atoms i and j, weighted by eXC w / (p sum_i A_i).
+! grad (nAtoms x 3) is incremented.
+!---------------------------------------------------------------------------------------------------------------------------------*/
+int xqo_gradients ( void *vh, const double *Ppacked, int nThreads, double *grad )
+{
+    OHandle *h = ( OHandle * ) vh ;
+    int nF = h->nFunctions, nA = h->nAtoms ;
+    if ( nThreads < 1 ) nThreads = 1 ;
+# ifdef _OPENMP
+    omp_set_num_threads ( nThreads ) ;
+# endif
+    int *atomOf = ( int * ) malloc ( ( nF > 0 ? nF : 1 ) * sizeof ( int ) ) ;
+    for ( int s = 0 ; s < h->nShells ; s++ ) for ( int b = 0 ; b < h->shells[s].nBasis ; b++ ) atomOf[h->shells[s].start + b] = h->shells[s].atom ;
+    double *radii = ( double * ) malloc ( nA * sizeof ( double ) ) ;
+    for ( int a = 0 ; a < nA ; a++ ) radii[a] = BraggRadius ( h->Z[a] ) ;
+    BeckePairs bp ;
+    BeckePairs_Make ( &bp, nA, h->xyz, radii ) ;
+    double *Gt = ( double * ) calloc ( ( size_t ) nThreads * 3 * nA, sizeof ( double ) ) ;
+    # pragma omp parallel num_threads(nThreads)
+    {
+        int tid = 0 ;
+# ifdef _OPENMP
+        tid = omp_get_thread_num ( ) ;
+# endif
+        double *G = Gt + ( size_t ) tid * 3 * nA ;
+        double *buf = ( double * ) malloc ( 10 * ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *X   = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *T   = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *Y   = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *Pr  = ( double * ) malloc ( ( size_t ) nF * nF * sizeof ( double ) ) ;
+        int    *idx = ( int * ) malloc ( nF * sizeof ( int ) ) ;
+        double *A = ( double * ) malloc ( nA * sizeof ( double ) ), *R = ( double * ) malloc ( nA * sizeof ( double ) ) ;
+        double *dAdm = ( double * ) malloc ( ( size_t ) nA * nA * sizeof ( double ) ) ;
+        double rho[BLOCK_SIZE], gx[BLOCK_SIZE], gy[BLOCK_SIZE], gz[BLOCK_SIZE], sig[BLOCK_SIZE], eps[BLOCK_SIZE], vr[BLOCK_SIZE], vs[BLOCK_SIZE] ;
+        # pragma omp for schedule(dynamic,1)
+        for ( int b = 0 ; b < h->nBlocks ; b++ )
+        {
+            int nP = h->blockCount[b], gAtom = h->blockAtom[b] ;
+            const double *pts = h->points + 4 * ( size_t ) h->blockStart[b] ;
+            double *a[10] ;
+            for ( int k = 0 ; k < 10 ; k++ ) a[k] = buf + ( size_t ) k * nF * nP ;
+            EvaluateBasisN ( h, nP, pts, a, 10 ) ;
+            int nb = FilterRowsN ( nF, nP, h->bfTol, idx, a, 10 ) ;
+            if ( nb <= 0 ) continue ;
+            double *f = a[0], *fx = a[1], *fy = a[2], *fz = a[3] ;
+            for ( int i = 0 ; i < nb ; i++ )
+                for ( int j = 0 ; j <= i ; j++ )
+                {
+                    int m = idx[i], n = idx[j] ;
+                    double v = Ppacked[( ( size_t ) m * ( m + 1 ) ) / 2 + n] ;
+                    Pr[i * nb + j] = v ; Pr[j * nb + i] = v ;
+                }
+            for ( int i = 0 ; i < nb ; i++ )
+                for ( int p = 0 ; p < nP ; p++ )
+                {
+                    double s = 0.0 ;
+                    for ( int k = 0 ; k < nb ; k++ ) s += Pr[i * nb + k] * f[k * nP + p] ;
+                    X[i * nP + p] = s ;
+                }
+            double mx = 0.0 ;
+            for ( int p = 0 ; p < nP ; p++ )
+            {
+                double r = 0.0, ax = 0.0, ay = 0.0, az = 0.0 ;
+                for ( int i = 0 ; i < nb ; i++ )
+                {
+                    double xv = X[i * nP + p] ;
+                    r  += f [i * nP + p] * xv ;
+                    ax += fx[i * nP + p] * xv ;
+                    ay += fy[i * nP + p] * xv ;
+                    az += fz[i * nP + p] * xv ;
+                }
+                rho[p] = r ;
+                sig[p] = 4.0 * ( ax * ax + ay * ay + az * az ) ;
+                gx[p] = 2.0 * ax ; gy[p] = 2.0 * ay ; gz[p] = 2.0 * az ;
+                if ( fabs ( r ) > mx ) mx = fabs ( r ) ;
+            }
+            if ( mx <= h->rhoTol ) continue ;
+            xqo_functional ( h->nIDs, h->ids, nP, rho, sig, eps, vr, vs ) ;
+            for ( int p = 0 ; p < nP ; p++ ) { double w = pts[4 * p + 3] ; vr[p] *= w ; vs[p] *= w ; }
+            /* Direct terms: rows of the three work arrays against the derivative rows they pair with. */
+            for ( int i = 0 ; i < nb ; i++ )
+                for ( int p = 0 ; p < nP ; p++ )
+                    T[i * nP + p] = fx[i * nP + p] * gx[p] + fy[i * nP + p] * gy[p] + fz[i * nP + p] * gz[p] ;
+            for ( int i = 0 ; i < nb ; i++ )
+                for ( int p = 0 ; p < nP ; p++ )
+                {
+                    double s = 0.0 ;
+                    for ( int k = 0 ; k < nb ; k++ ) s += Pr[i * nb + k] * T[k * nP + p] ;
+                    Y[i * nP + p] = s ;
+                }
+            for ( int i = 0 ; i < nb ; i++ )
+            {
+                double g3[3] = { 0.0, 0.0, 0.0 } ;
+                for ( int p = 0 ; p < nP ; p++ )
+                {
+                    int o = i * nP + p ;
+                    double w1 = 2.0 * vr[p] * X[o] + 4.0 * vs[p] * Y[o], w2 = 4.0 * vs[p] * X[o] ;
+                    double hx = a[4][o] * gx[p] + a[5][o] * gy[p] + a[6][o] * gz[p] ;
+                    double hy = a[5][o] * gx[p] + a[7][o] * gy[p] + a[8][o] * gz[p] ;
+                    double hz = a[6][o] * gx[p] + a[8][o] * gy[p] + a[9][o] * gz[p] ;
+                    g3[0] += w1 * fx[o] + w2 * hx ;
+                    g3[1] += w1 * fy[o] + w2 * hy ;
+                    g3[2] += w1 * fz[o] + w2 * hz ;
+                }
+                int c = atomOf[idx[i]] ;
+                for ( int d = 0 ; d < 3 ; d++ ) { G[3 * c + d] -= g3[d] ; G[3 * gAtom + d] += g3[d] ; }
+            }
+            /* Weight terms. */
+            for ( int p = 0 ; p < nP ; p++ )
+            {
+                double rgX = pts[4 * p], rgY = pts[4 * p + 1], rgZ = pts[4 * p + 2], w = pts[4 * p + 3] ;
+                for ( int i = 0 ; i < nA ; i++ )
+                {
+                    double x = h->xyz[3 * i] - rgX, y = h->xyz[3 * i + 1] - rgY, z = h->xyz[3 * i + 2] - rgZ ;
+                    A[i] = 1.0 ;
+                    R[i] = sqrt ( x * x + y * y + z * z ) ;
+                }
+                for ( size_t k = 0 ; k < ( size_t ) nA * nA ; k++ ) dAdm[k] = 1.0 ;
+                for ( int i = 0, ij = 0 ; i < nA ; i++ )
+                    for ( int j = 0 ; j < i ; j++, ij++ )
+                    {
+                        double mu = ( R[i] - R[j] ) * bp.invR[ij] ;
+                        double nu = mu + bp.aij[ij] * ( 1.0 - mu * mu ), nu2 = nu * nu, dnu = 1.0, snu = nu, dsum = 0.0, sum = 0.0 ;
+                        for ( int t = 0 ; t <= 10 ; t++ )
+                        {
+                            dsum += BECKE_C[t] * dnu * ( double ) ( 2 * t + 1 ) ;
+                            sum  += BECKE_C[t] * snu ;
+                            dnu  *= nu2 ;
+                            snu  *= nu2 ;
+                        }
+                        double sij = 0.5 - BECKE_A * sum, sji = 0.5 + BECKE_A * sum ;
+                        A[i] *= sij ;
+                        A[j] *= sji ;
+                        dsum *= - BECKE_A * ( 1.0 - 2.0 * bp.aij[ij] * mu ) ;
+                        for ( int k = 0 ; k < nA ; k++ )
+                        {
+                            dAdm[( size_t ) i * nA + k] *= ( k == j ? dsum : sij ) ;
+                            dAdm[( size_t ) j * nA + k] *= ( k == i ? dsum : sji ) ;
+                        }
+                    }
+                double sum = 0.0 ;
+                for ( int i = 0 ; i < nA ; i++ ) sum += A[i] ;
+                double pw = A[gAtom] / sum ;
+                double ew = eps[p] * rho[p] * w / ( pw * sum ) ;
+                for ( int i = 0 ; i < nA ; i++ )
+                {
+                    double xi = h->xyz[3 * i], yi = h->xyz[3 * i + 1], zi = h->xyz[3 * i + 2] ;
+                    double fac = ( i == gAtom ) ? ew * ( 1.0 - pw ) : - ew * pw ;
+                    for ( int j = 0 ; j < nA ; j++ )
+                    {
+                        if ( i == j ) continue ;
+                        double xj = h->xyz[3 * j], yj = h->xyz[3 * j + 1], zj = h->xyz[3 * j + 2] ;
+                        int ij = i > j ? ( i * ( i - 1 ) ) / 2 + j : ( j * ( j - 1 ) ) / 2 + i ;
+                        double rij = bp.invR[ij], mu = ( R[i] - R[j] ) * rij ;
+                        double dxi = ( xi - rgX ) * rij / R[i], dyi = ( yi - rgY ) * rij / R[i], dzi = ( zi - rgZ ) * rij / R[i] ;
+                        double dxj = - ( xj - rgX ) * rij / R[j], dyj = - ( yj - rgY ) * rij / R[j], dzj = - ( zj - rgZ ) * rij / R[j] ;
+                        double dxg = - ( dxi + dxj ), dyg = - ( dyi + dyj ), dzg = - ( dzi + dzj ) ;
+                        double tx = ( xi - xj ) * mu * rij * rij, ty = ( yi - yj ) * mu * rij * rij, tz = ( zi - zj ) * mu * rij * rij ;
+                        dxi -= tx ; dyi -= ty ; dzi -= tz ;
+                        dxj += tx ; dyj += ty ; dzj += tz ;
+                        double ifac = dAdm[( size_t ) i * nA + j] * fac ;
+                        G[3 * i] += ifac * dxi ; G[3 * i + 1] += ifac * dyi ; G[3 * i + 2] += ifac * dzi ;
+                        G[3 * j] += ifac * dxj ; G[3 * j + 1] += ifac * dyj ; G[3 * j + 2] += ifac * dzj ;
+                        G[3 * gAtom] += ifac * dxg ; G[3 * gAtom + 1] += ifac * dyg ; G[3 * gAtom + 2] += ifac * dzg ;
+                    }
+                }
+            }
+        }
+        free ( buf ) ; free ( X ) ; free ( T ) ; free ( Y ) ; free ( Pr ) ; free ( idx ) ; free ( A ) ; free ( R ) ; free ( dAdm ) ;
+    }
+    for ( int t = 0 ; t < nThreads ; t++ )
+        for ( int k = 0 ; k < 3 * nA ; k++ ) grad[k] += Gt[( size_t ) t * 3 * nA + k] ;
+    free ( Gt ) ; free ( atomOf ) ; free ( radii ) ; free ( bp.invR ) ; free ( bp.aij ) ;
     return 0 ;
 }
 
