@@ -64,6 +64,13 @@ int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, 
 int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
                                   double *nElectrons, double *fockV, double *stats ) ;
 
+/* XC nuclear gradients: DFTIntegrator_Integrate with a gradients3 argument and no Fock matrix (DFTIntegrator.c:159-509, the
+!  doGradients branch :405-487 with DFTGridWeights_Derivatives, DFTGridWeights.c:98-215), as DFTGridIntegrator.Gradients
+!  calls it (DFTGridIntegrator.py:72-88).  Restricted closed shell, GGA functionals, FP64 on either precision's handle (the
+!  basis values and second derivatives are evaluated afresh).  gradients3: nAtoms x 3 in hartree/bohr, INCREMENTED as the
+!  reference's Coordinates3 is.  At most 682 atoms. */
+int         xq_gradients        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *gradients3 ) ;
+
 /* Peak device bytes of one handle for a molecule with the given grid size, function count and mean kept functions per
 !  block, set-up included (admission control for the request loop); xq_device_bytes is what a handle holds now, and
 !  xq_device_memory the device's free and total bytes (hipMemGetInfo; free includes the device blocks the library holds

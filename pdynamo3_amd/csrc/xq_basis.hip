@@ -309,6 +309,132 @@ __global__ void __launch_bounds__(kBlockSize) basis_eval_kernel(
     }
 }
 
+// Values, gradients and second derivatives of one shell (GaussianBasisIntegrals_f1Op1.c:203-331): with e2 = 4 sum c a^2 e^{-a r^2},
+// d2/dx2 phi = (i(i-1) x^{i-2} e0 + (2i+1) x^i e1 + x^{i+2} e2) y^j z^k and
+// d2/dxdy phi = (i j x^{i-1} y^{j-1} e0 + (i x^{i-1} y^{j+1} + j x^{i+1} y^{j-1}) e1 + x^{i+1} y^{j+1} e2) z^k; out[0..9] = phi, x, y,
+// z, xx, xy, xz, yy, yz, zz (nBasis values each, spherical = c2s^T Cartesian).
+template <int LL, int LH, bool SPH>
+__device__ __forceinline__ void shell_eval2(const xq::DevShell &sh, const double *__restrict__ coef,
+                                            const double *__restrict__ trans, double dx, double dy, double dz,
+                                            double (*out)[16]) {
+    constexpr int NC = ShellDims<LL, LH>::nCBF;
+    constexpr int NB = SPH ? ShellDims<LL, LH>::nSph : NC;
+    constexpr int C0 = cart_offset(LL);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double e0[NC], e1[NC], e2[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) { e0[c] = 0.0; e1[c] = 0.0; e2[c] = 0.0; }
+    const double *ex = coef + sh.coef, *cc = coef + sh.coef + sh.nPrim;
+    for (int q = 0; q < sh.nPrim; q++) {
+        const double a = ex[q], e = exp(-a * r2);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double k = cc[q * NC + c];
+            e0[c] += k * e;
+            e1[c] += k * a * e;
+            e2[c] += k * (a * a) * e;
+        }
+    }
+    double X0[LH + 3], Y0[LH + 3], Z0[LH + 3], X1[LH + 3], Y1[LH + 3], Z1[LH + 3], X2[LH + 3], Y2[LH + 3], Z2[LH + 3];
+    X0[0] = Y0[0] = Z0[0] = 1.0;
+    X1[0] = Y1[0] = Z1[0] = X2[0] = Y2[0] = Z2[0] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= LH + 2; i++) {
+        X0[i] = dx * X0[i - 1]; Y0[i] = dy * Y0[i - 1]; Z0[i] = dz * Z0[i - 1];
+        X1[i] = i * X0[i - 1]; Y1[i] = i * Y0[i - 1]; Z1[i] = i * Z0[i - 1];
+        X2[i] = i * X1[i - 1]; Y2[i] = i * Y1[i - 1]; Z2[i] = i * Z1[i - 1];
+    }
+    double g[10][NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int i = kCart.x[C0 + c], j = kCart.y[C0 + c], k = kCart.z[C0 + c];
+        const double E0 = e0[c], E1 = -2.0 * e1[c], E2 = 4.0 * e2[c];
+        g[0][c] = X0[i] * Y0[j] * Z0[k] * E0;
+        g[1][c] = (X1[i] * E0 + X0[i + 1] * E1) * Y0[j] * Z0[k];
+        g[2][c] = (Y1[j] * E0 + Y0[j + 1] * E1) * X0[i] * Z0[k];
+        g[3][c] = (Z1[k] * E0 + Z0[k + 1] * E1) * X0[i] * Y0[j];
+        g[4][c] = (X2[i] * E0 + (dx * X1[i] + X1[i + 1]) * E1 + X0[i + 2] * E2) * Y0[j] * Z0[k];
+        g[5][c] = (X1[i] * Y1[j] * E0 + (X1[i] * Y0[j + 1] + X0[i + 1] * Y1[j]) * E1 + X0[i + 1] * Y0[j + 1] * E2) * Z0[k];
+        g[6][c] = (X1[i] * Z1[k] * E0 + (X1[i] * Z0[k + 1] + X0[i + 1] * Z1[k]) * E1 + X0[i + 1] * Z0[k + 1] * E2) * Y0[j];
+        g[7][c] = (Y2[j] * E0 + (dy * Y1[j] + Y1[j + 1]) * E1 + Y0[j + 2] * E2) * X0[i] * Z0[k];
+        g[8][c] = (Y1[j] * Z1[k] * E0 + (Y1[j] * Z0[k + 1] + Y0[j + 1] * Z1[k]) * E1 + Y0[j + 1] * Z0[k + 1] * E2) * X0[i];
+        g[9][c] = (Z2[k] * E0 + (dz * Z1[k] + Z1[k + 1]) * E1 + Z0[k + 2] * E2) * X0[i] * Y0[j];
+    }
+#pragma unroll
+    for (int d = 0; d < 10; d++) {
+        if (SPH) {
+            const double *T = trans + sh.trans;
+#pragma unroll
+            for (int s = 0; s < NB; s++) {
+                double a = 0.0;
+#pragma unroll
+                for (int c = 0; c < NC; c++) a += T[c * NB + s] * g[d][c];
+                out[d][s] = a;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < NB; s++) out[d][s] = g[d][s];
+        }
+    }
+}
+
+// Dispatch on the shell type; calls body(nBasis, out).
+template <typename Body>
+__device__ __forceinline__ void with_shell2(const xq::DevShell &sh, bool sph, const double *coef, const double *trans,
+                                            double dx, double dy, double dz, Body body) {
+    double out[10][16];
+    const int key = sh.lLow * 4 + sh.lHigh;
+    if (sph) {
+        switch (key) {
+            case 0: shell_eval2<0, 0, true>(sh, coef, trans, dx, dy, dz, out); body(1, out); break;
+            case 5: shell_eval2<1, 1, true>(sh, coef, trans, dx, dy, dz, out); body(3, out); break;
+            case 1: shell_eval2<0, 1, true>(sh, coef, trans, dx, dy, dz, out); body(4, out); break;
+            case 10: shell_eval2<2, 2, true>(sh, coef, trans, dx, dy, dz, out); body(5, out); break;
+            case 15: shell_eval2<3, 3, true>(sh, coef, trans, dx, dy, dz, out); body(7, out); break;
+            default: break;
+        }
+    } else {
+        switch (key) {
+            case 0: shell_eval2<0, 0, false>(sh, coef, trans, dx, dy, dz, out); body(1, out); break;
+            case 5: shell_eval2<1, 1, false>(sh, coef, trans, dx, dy, dz, out); body(3, out); break;
+            case 1: shell_eval2<0, 1, false>(sh, coef, trans, dx, dy, dz, out); body(4, out); break;
+            case 10: shell_eval2<2, 2, false>(sh, coef, trans, dx, dy, dz, out); body(6, out); break;
+            case 15: shell_eval2<3, 3, false>(sh, coef, trans, dx, dy, dz, out); body(10, out); break;
+            default: break;
+        }
+    }
+}
+
+// Values and first and second derivatives of the kept functions of blocks b0 + blockIdx.x into rows
+// [vOffset - rowBase, + nKept) of the ten planes out[0..9] (row stride kBlockSize; zero past a block's points).
+__global__ void __launch_bounds__(kBlockSize) basis_eval2_kernel(
+    int b0, long long rowBase, const xq::DevBlock *__restrict__ blocks, const double *__restrict__ px,
+    const double *__restrict__ py, const double *__restrict__ pz, const xq::DevShell *__restrict__ shells,
+    const int *__restrict__ shellOf, const double *__restrict__ coef, const double *__restrict__ trans, int spherical,
+    const int *__restrict__ idx, xq::Planes10 out) {
+    const int b = b0 + static_cast<int>(blockIdx.x);
+    const xq::DevBlock blk = blocks[b];
+    const int t = threadIdx.x;
+    const bool valid = t < blk.nPoints;
+    const double x = valid ? px[blk.pStart + t] : 0.0, y = valid ? py[blk.pStart + t] : 0.0,
+                 z = valid ? pz[blk.pStart + t] : 0.0;
+    const int *kept = idx + blk.iOffset;
+    int k = 0;
+    while (k < blk.nKept) {
+        const xq::DevShell sh = shells[shellOf[kept[k]]];
+        with_shell2(sh, spherical != 0, coef, trans, x - sh.x, y - sh.y, z - sh.z, [&](int nB, const double (*v)[16]) {
+            for (int j = 0; j < nB; j++) {
+                if (k < blk.nKept && kept[k] == sh.fStart + j) {
+                    const size_t o = static_cast<size_t>(blk.vOffset - rowBase + k) * kBlockSize + t;
+#pragma unroll
+                    for (int d = 0; d < 10; d++) out.p[d][o] = valid ? v[d][j] : 0.0;
+                    k++;
+                }
+            }
+        });
+    }
+}
+
 }  // namespace
 
 void free_basis(xq_grid *g) {
@@ -319,6 +445,9 @@ void free_basis(xq_grid *g) {
     dev_free(g, &g->d_trans);
     dev_free(g, &g->d_pow);
     dev_free(g, &g->d_idx);
+    dev_free(g, &g->d_shellOf);
+    g->h_shellOf.clear();
+    g->h_funcAtom.clear();
     if (g->d_f) { cache_free(g->d_f); g->d_f = nullptr; }
     if (g->d_fx) { cache_free(g->d_fx); g->d_fx = nullptr; }
     if (g->d_fy) { cache_free(g->d_fy); g->d_fy = nullptr; }
@@ -520,7 +649,27 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
     g->deviceBytes -= temp > 0 ? temp : 0;
     (void)base;
     if (status != kOK) { free_basis(g); return status; }
+    g->h_shellOf = shellOf;     // kept for the second-derivative evaluation of the XC gradients
+    g->h_funcAtom.resize(shellOf.size());
+    for (size_t fn = 0; fn < shellOf.size(); fn++) g->h_funcAtom[fn] = shells[shellOf[fn]].atom;
     g->hasBasis = true;
+    return kOK;
+}
+
+
+// Basis values, gradients and second derivatives of blocks [b0, b1) into the planes (rows from the blocks' value-row offsets
+// less rowBase), for the XC gradients.
+int eval_basis2(xq_grid *g, int b0, int b1, long long rowBase, const Planes10 &out, hipStream_t s) {
+    if (b1 <= b0) return kOK;
+    if (g->d_shellOf == nullptr) {
+        XQ_TRY(dev_alloc(g, &g->d_shellOf, g->h_shellOf.size()));
+        XQ_CHECK(hipMemcpyAsync(g->d_shellOf, g->h_shellOf.data(), g->h_shellOf.size() * sizeof(int), hipMemcpyHostToDevice,
+                                g->stream));
+        XQ_CHECK(hipStreamSynchronize(g->stream));
+    }
+    hipLaunchKernelGGL(basis_eval2_kernel, dim3(b1 - b0), dim3(kBlockSize), 0, s, b0, rowBase, g->d_blocks, g->d_x, g->d_y,
+                       g->d_z, g->d_shells, g->d_shellOf, g->d_coef, g->d_trans, g->spherical, g->d_idx, out);
+    XQ_CHECK(hipGetLastError());
     return kOK;
 }
 

@@ -322,6 +322,160 @@ __global__ void __launch_bounds__(256) scan_apply_kernel(int n, const int *__res
     }
 }
 
+// Becke pair data (DFTGridWeights.c:43-80): 1/R_ij and the size adjustment a_ij for i > j, packed as pair_index.
+void becke_pairs(const xq_grid *g, std::vector<double> &invR, std::vector<double> &aij) {
+    const int nA = g->nAtoms;
+    const size_t nPairs = nA > 1 ? static_cast<size_t>(nA) * (nA - 1) / 2 : 1;
+    invR.assign(nPairs, 0.0);
+    aij.assign(nPairs, 0.0);
+    for (int i = 0, ij = 0; i < nA; i++) {
+        for (int j = 0; j < i; j++, ij++) {
+            double dx = g->xyz[3 * i] - g->xyz[3 * j], dy = g->xyz[3 * i + 1] - g->xyz[3 * j + 1],
+                   dz = g->xyz[3 * i + 2] - g->xyz[3 * j + 2];
+            double chi = bragg_radius(g->Z[i]) / bragg_radius(g->Z[j]);
+            double u = (chi - 1.0) / (chi + 1.0), a = u / (u * u - 1.0);
+            invR[ij] = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+            aij[ij] = a > 0.5 ? 0.5 : (a < -0.5 ? -0.5 : a);
+        }
+    }
+}
+
+// Atoms per lane of the weight-derivative kernel (at most 64 x 16 = 1024 atoms).
+constexpr int kLaneAtoms = 16;
+
+// Product of atom k's cell factors with the factor of partner j left out, from the product of its non-zero factors and
+// the count of its zero factors (no division by a zero factor).
+__device__ __forceinline__ double excluded_product(double prodNZ, int zeros, double s) {
+    if (zeros == 0) return prodNZ / s;
+    return (zeros == 1 && s == 0.0) ? prodNZ : 0.0;
+}
+
+// XC gradient weight terms (DFTGridWeights_Derivatives, DFTGridWeights.c:98-215).  One wave per point, four points per
+// round of a work-group; lane l owns atoms l, l + 64, ... and accumulates their gradient in registers.  Per point (grid
+// atom g, weight w, energy density eXC = eps rho): A_i = prod_j s_ij; p = A_g / sum_i A_i; ew = eXC w / (p sum_i A_i);
+// dA_i/dmu_ij = ds_ij prod_{k != j} s_ik; for every ordered pair (i, j) the reference adds ifac = dA_i/dmu_ij fac_i,
+// fac_i = ew (1 - p) for i = g and -ew p otherwise, times the derivatives of mu_ij with respect to atom i, atom j and the
+// point (which moves with atom g).  Each lane adds, for its atom k, the terms of the pairs (k, m) and (m, k) that move k,
+// and the point terms of the pairs (k, m); the reference's O(nAtoms^3) update of dA/dmu becomes O(nAtoms^2) per point.
+// LDS per wave: R, the non-zero products and the zero counts (3 nAtoms doubles).
+__global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int pointsPerGroup,
+                                                              const double *__restrict__ px, const double *__restrict__ py,
+                                                              const double *__restrict__ pz, const double *__restrict__ pw,
+                                                              const int *__restrict__ pointAtom,
+                                                              const double *__restrict__ eXC, const double *__restrict__ xyz,
+                                                              int nAtoms, const double *__restrict__ invR,
+                                                              const double *__restrict__ aij, double *__restrict__ grad) {
+    extern __shared__ double lds[];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double *R = lds + static_cast<size_t>(wv) * 3 * nAtoms, *prodNZ = R + nAtoms, *zeros = prodNZ + nAtoms;
+    double G[kLaneAtoms][3];
+#pragma unroll
+    for (int a = 0; a < kLaneAtoms; a++) G[a][0] = G[a][1] = G[a][2] = 0.0;
+    const int p0 = blockIdx.x * pointsPerGroup;
+    const int p1 = p0 + pointsPerGroup < nPoints ? p0 + pointsPerGroup : nPoints;
+    for (int base = p0; base < p1; base += 4) {
+        const int p = base + wv;
+        const bool active = p < p1 && eXC[p] != 0.0;
+        const double rx = active ? px[p] : 0.0, ry = active ? py[p] : 0.0, rz = active ? pz[p] : 0.0;
+        if (active)
+            for (int i = lane; i < nAtoms; i += 64) {
+                const double dx = xyz[3 * i] - rx, dy = xyz[3 * i + 1] - ry, dz = xyz[3 * i + 2] - rz;
+                R[i] = sqrt(dx * dx + dy * dy + dz * dz);
+            }
+        __syncthreads();
+        if (active)
+            for (int i = lane; i < nAtoms; i += 64) {
+                double prod = 1.0;
+                int nz = 0;
+                for (int j = 0; j < nAtoms; j++) {
+                    if (j == i) continue;
+                    const int ij = pair_index(i, j);
+                    const double s = becke_factor_of(i > j, R[i], R[j], invR[ij], aij[ij]);
+                    if (s == 0.0) nz++;
+                    else prod *= s;
+                }
+                prodNZ[i] = prod;
+                zeros[i] = static_cast<double>(nz);
+            }
+        __syncthreads();
+        if (active) {
+            const int g = pointAtom[p];
+            double sumA = 0.0;
+            for (int i = lane; i < nAtoms; i += 64) sumA += zeros[i] == 0.0 ? prodNZ[i] : 0.0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) sumA += __shfl_xor(sumA, o, 64);
+            const double Ag = zeros[g] == 0.0 ? prodNZ[g] : 0.0;
+            const double pW = Ag / sumA;
+            const double ew = eXC[p] * pw[p] / (pW * sumA);
+            double gs[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int a = 0; a < kLaneAtoms; a++) {
+                const int k = lane + 64 * a;
+                if (k >= nAtoms) break;
+                const double xk = xyz[3 * k], yk = xyz[3 * k + 1], zk = xyz[3 * k + 2];
+                const double fk = k == g ? ew * (1.0 - pW) : -ew * pW;
+                const double pk = prodNZ[k];
+                const int zk0 = static_cast<int>(zeros[k]);
+                for (int m = 0; m < nAtoms; m++) {
+                    if (m == k) continue;
+                    const int km = pair_index(k, m);
+                    const double r = invR[km], am = aij[km];
+                    const int hi = k > m ? k : m, lo = k > m ? m : k;
+                    const double muP = (R[hi] - R[lo]) * r;
+                    const double nu = muP + am * (1.0 - muP * muP), nu2 = nu * nu;
+                    double dnu = 1.0, dsum = 0.0;
+#pragma unroll
+                    for (int tt = 0; tt <= 10; tt++) {
+                        dsum += c_beckeC[tt] * dnu * static_cast<double>(2 * tt + 1);
+                        dnu *= nu2;
+                    }
+                    dsum *= -kBeckeA * (1.0 - 2.0 * am * muP);
+                    const double skm = becke_factor_of(k > m, R[k], R[m], r, am);
+                    const double smk = becke_factor_of(m > k, R[m], R[k], r, am);
+                    const double fm = m == g ? ew * (1.0 - pW) : -ew * pW;
+                    const double ifKM = dsum * excluded_product(pk, zk0, skm) * fk;
+                    const double ifMK = dsum * excluded_product(prodNZ[m], static_cast<int>(zeros[m]), smk) * fm;
+                    const double xm = xyz[3 * m], ym = xyz[3 * m + 1], zm = xyz[3 * m + 2];
+                    const double mu = (R[k] - R[m]) * r;
+                    const double tx = (xk - xm) * mu * r * r, ty = (yk - ym) * mu * r * r, tz = (zk - zm) * mu * r * r;
+                    // (i, j) = (k, m): atom k moves through dxi, the point through dxg.
+                    const double ux = (xk - rx) * r / R[k], uy = (yk - ry) * r / R[k], uz = (zk - rz) * r / R[k];
+                    G[a][0] += ifKM * (ux - tx);
+                    G[a][1] += ifKM * (uy - ty);
+                    G[a][2] += ifKM * (uz - tz);
+                    // The point's term of the pair (k, m): -(dmu/dr_k + dmu/dr_m) before the atom-pair part.
+                    gs[0] -= ifKM * (ux - (xm - rx) * r / R[m]);
+                    gs[1] -= ifKM * (uy - (ym - ry) * r / R[m]);
+                    gs[2] -= ifKM * (uz - (zm - rz) * r / R[m]);
+                    // (i, j) = (m, k): atom k moves through dxj of that pair.
+                    G[a][0] += ifMK * (tx - ux);
+                    G[a][1] += ifMK * (ty - uy);
+                    G[a][2] += ifMK * (tz - uz);
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) gs[d] += __shfl_xor(gs[d], o, 64);
+            if (lane == (g & 63)) {
+#pragma unroll
+                for (int a = 0; a < kLaneAtoms; a++)
+                    if (a == (g >> 6)) { G[a][0] += gs[0]; G[a][1] += gs[1]; G[a][2] += gs[2]; }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < kLaneAtoms; a++) {
+        const int k = lane + 64 * a;
+        if (k < nAtoms) {
+            atomicAdd(&grad[3 * k], G[a][0]);
+            atomicAdd(&grad[3 * k + 1], G[a][1]);
+            atomicAdd(&grad[3 * k + 2], G[a][2]);
+        }
+    }
+}
+
 }  // namespace
 
 int build_grid(xq_grid *g) {
@@ -379,17 +533,8 @@ int build_grid(xq_grid *g) {
     atomFirst[nA] = nCand;
     // Becke pair data (DFTGridWeights.c:43-80): 1/R_ij and the size adjustment a_ij, for i > j.
     const size_t nPairs = nA > 1 ? static_cast<size_t>(nA) * (nA - 1) / 2 : 1;
-    std::vector<double> invR(nPairs, 0.0), aij(nPairs, 0.0);
-    for (int i = 0, ij = 0; i < nA; i++) {
-        for (int j = 0; j < i; j++, ij++) {
-            double dx = g->xyz[3 * i] - g->xyz[3 * j], dy = g->xyz[3 * i + 1] - g->xyz[3 * j + 1],
-                   dz = g->xyz[3 * i + 2] - g->xyz[3 * j + 2];
-            double chi = bragg_radius(g->Z[i]) / bragg_radius(g->Z[j]);
-            double u = (chi - 1.0) / (chi + 1.0), a = u / (u * u - 1.0);
-            invR[ij] = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
-            aij[ij] = a > 0.5 ? 0.5 : (a < -0.5 ? -0.5 : a);
-        }
-    }
+    std::vector<double> invR, aij;
+    becke_pairs(g, invR, aij);
     // Device work.
     RadialShell *d_shells = nullptr;
     double *d_rules = nullptr, *d_xyz = nullptr, *d_invR = nullptr, *d_aij = nullptr;
@@ -546,6 +691,54 @@ long long estimated_points(int accuracy, int nAtoms, const int *Z) {
         }
     }
     return p;
+}
+
+
+// XC gradient weight terms of every point with a non-zero energy density eXC (device arrays; grad is incremented).
+int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_t s) {
+    const int nA = g->nAtoms, G = g->nPoints;
+    if (G <= 0 || nA < 2) return kOK;
+    const size_t lds = 4 * 3 * sizeof(double) * static_cast<size_t>(nA);
+    if (lds > 64 * 1024) {
+        set_error("XC gradients support at most 682 atoms");
+        return kInvalidArgument;
+    }
+    std::vector<double> invR, aij;
+    becke_pairs(g, invR, aij);
+    std::vector<int> pointAtom(static_cast<size_t>(G), 0);
+    for (const DevBlock &b : g->blocks)
+        for (int k = 0; k < b.nPoints; k++) pointAtom[static_cast<size_t>(b.pStart) + k] = b.atom;
+    double *d_invR = nullptr, *d_aij = nullptr, *d_xyz = nullptr;
+    int *d_pointAtom = nullptr;
+    int status = kOK;
+    do {
+        if ((status = dev_alloc(g, &d_invR, invR.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_aij, aij.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_xyz, g->xyz.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_pointAtom, pointAtom.size())) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_invR, invR.data(), invR.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy invR")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_aij, aij.data(), aij.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy aij")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_xyz, g->xyz.data(), g->xyz.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy xyz")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_pointAtom, pointAtom.data(), pointAtom.size() * sizeof(int),
+                                           hipMemcpyHostToDevice, s), "copy point atoms")) != kOK) break;
+        const int groups = std::min(2048, (G + 63) / 64);
+        int perGroup = (G + groups - 1) / groups;
+        perGroup = (perGroup + 3) / 4 * 4;
+        const int nGroups = (G + perGroup - 1) / perGroup;
+        hipLaunchKernelGGL(weight_gradient_kernel, dim3(nGroups), dim3(256), lds, s, G, perGroup, g->d_x, g->d_y, g->d_z,
+                           g->d_w, d_pointAtom, d_eXC, d_xyz, nA, d_invR, d_aij, d_grad);
+        if ((status = check(hipGetLastError(), "weight_gradient_kernel")) != kOK) break;
+        status = check(hipStreamSynchronize(s), "gradient sync");
+    } while (false);
+    (void)hipStreamSynchronize(s);
+    dev_free(g, &d_invR);
+    dev_free(g, &d_aij);
+    dev_free(g, &d_xyz);
+    dev_free(g, &d_pointAtom);
+    return status;
 }
 
 }  // namespace xq

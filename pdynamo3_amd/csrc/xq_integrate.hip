@@ -55,6 +55,18 @@ __global__ void zero_kernel(size_t n, double *__restrict__ a) {
 
 }  // namespace
 
+// Dense n x n P on the device from a packed lower triangle on the host (through the handle's pinned staging buffer).
+int upload_density(xq_grid *g, const double *Ppacked, hipStream_t s) {
+    XQ_TRY(ensure_work(g));
+    const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
+    std::memcpy(g->h_stage, Ppacked, nPacked * sizeof(double));
+    XQ_CHECK(hipMemcpyAsync(g->d_packed, g->h_stage, nPacked * sizeof(double), hipMemcpyHostToDevice, s));
+    const int grid = static_cast<int>(std::min<size_t>((n * n + 255) / 256, 16384));
+    hipLaunchKernelGGL(expand_packed_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, g->nFunctions, g->d_packed, g->d_P);
+    XQ_CHECK(hipGetLastError());
+    return kOK;
+}
+
 void free_work(xq_grid *g) {
     if (g->h_job != nullptr) { cache_host_free(g->h_job); g->h_job = nullptr; }
     if (g->h_stage != nullptr) { cache_host_free(g->h_stage); g->h_stage = nullptr; }
@@ -68,7 +80,7 @@ void free_work(xq_grid *g) {
     dev_free(g, &g->d_skip);
 }
 
-static int ensure_work(xq_grid *g) {
+int ensure_work(xq_grid *g) {
     if (g->d_P != nullptr) return kOK;
     const size_t n = static_cast<size_t>(g->nFunctions);
     XQ_TRY(dev_alloc(g, &g->d_P, n * n));

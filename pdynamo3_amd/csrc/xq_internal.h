@@ -104,6 +104,9 @@ struct xq_grid {
     int *d_pow = nullptr;
     long long nValueRows = 0;             // sum over blocks of kept functions
     int *d_idx = nullptr;                 // kept-function indices, per block
+    std::vector<int> h_shellOf;           // shell of each function (host), and its device copy for the XC gradients
+    std::vector<int> h_funcAtom;          // atom of each function (host)
+    int *d_shellOf = nullptr;
     void *d_f = nullptr, *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;   // values and gradients, rows of kBlockSize
     // Mixed precision (layouts in xq_mixed.hip): chunk tables, Fock tile list, digit planes and exponents of Phi in two
     // orders, fixed-point gradients, per-step W digits, P digits and W coefficients.
@@ -211,6 +214,15 @@ void free_f64(xq_grid *g);
 void free_basis(xq_grid *g);
 void free_work(xq_grid *g);
 int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz);
+// Ten FP64 planes of basis data (phi, its three first and six second derivatives), rows of kBlockSize points.
+struct Planes10 {
+    double *p[10];
+};
+int eval_basis2(xq_grid *g, int b0, int b1, long long rowBase, const Planes10 &out, hipStream_t s);
+int gradients(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, double *grad);
+int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_t s);
+int ensure_work(xq_grid *g);
+int upload_density(xq_grid *g, const double *Ppacked, hipStream_t s);
 void free_mixed(xq_grid *g);
 
 }  // namespace xq

@@ -31,6 +31,7 @@ class QuadratureGrid:
 
     def __init__(self, atomic_numbers, xyz_bohr, accuracy=DFTGridAccuracy.Medium):
         self.atomic_numbers = _lib.i32(atomic_numbers)
+        self.number_of_atoms = len(self.atomic_numbers)
         self.xyz = _lib.f64(xyz_bohr).reshape(-1, 3)
         self.accuracy = DFTGridAccuracy(accuracy)
         self._h = ctypes.c_void_p()
@@ -76,6 +77,20 @@ class QuadratureGrid:
         return e.value, ne.value, {"bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2],
                                    "rho_ms": stats[3], "fock_ms": stats[4], "bf_squares": stats[5],
                                    "rho_bytes": stats[6], "fock_bytes": stats[7]}
+
+    def gradients(self, ids, density, gradients=None):
+        """XC nuclear gradients (n_atoms x 3, hartree/bohr) at the density; `gradients` (contiguous float64, n_atoms x 3)
+        is incremented in place when given, as the reference increments its Coordinates3."""
+        ids = _lib.i32(ids)
+        P = _lib.f64(density)
+        n = self.number_of_functions
+        if P.size != n * (n + 1) // 2:
+            raise QCModelError("Density has {:d} entries; {:d} expected.".format(P.size, n * (n + 1) // 2))
+        G = gradients if gradients is not None else np.zeros((self.number_of_atoms, 3))
+        if G.dtype != np.float64 or not G.flags.c_contiguous or G.size != 3 * self.number_of_atoms:
+            raise QCModelError("Gradients must be a contiguous float64 array of n_atoms x 3.")
+        _check(_lib.lib.xq_gradients(self._h, ids.size, _lib.iptr(ids), _lib.dptr(P), _lib.dptr(G)), "gradients")
+        return G
 
     def device_bytes(self):
         b = ctypes.c_longlong()
@@ -247,9 +262,17 @@ class DFTGridIntegrator:
         return eQuad
 
     def Gradients(self, target):
-        """XC nuclear gradients (DFTGridIntegrator.py:72-88) are not provided by this library."""
-        if getattr(target.scratch, "doGradients", False):
-            raise QCModelError("Error in DFT integrator (gradients are not available on this path).")
+        """Add the XC contributions to the nuclear gradients (DFTGridIntegrator.py:72-88): scratch.qcGradients3AU
+        (n_atoms x 3, float64) is incremented."""
+        scratch = target.scratch
+        if getattr(scratch, "doGradients", False):
+            if not target.electronicState.isSpinRestricted:
+                raise QCModelError("Error in DFT integrator (spin-unrestricted densities are not supported).")
+            grid = scratch.dftGrid
+            bases = target.qcState.orbitalBases
+            if grid.number_of_functions == 0:
+                grid.set_basis(bases.spec, bases.spherical, self.precision)
+            grid.gradients(target.qcModel.functionalModel.ids, scratch.onePDMP.density, scratch.qcGradients3AU)
 
     def SummaryItems(self):
         """Summary items (DFTGridIntegrator.py:90-93)."""
