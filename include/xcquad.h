@@ -68,7 +68,7 @@ int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const
 !  doGradients branch :405-487 with DFTGridWeights_Derivatives, DFTGridWeights.c:98-215), as DFTGridIntegrator.Gradients
 !  calls it (DFTGridIntegrator.py:72-88).  Restricted closed shell, GGA functionals, FP64 on either precision's handle (the
 !  basis values and second derivatives are evaluated afresh).  gradients3: nAtoms x 3 in hartree/bohr, INCREMENTED as the
-!  reference's Coordinates3 is.  At most 682 atoms. */
+!  reference's Coordinates3 is.  At most 655 atoms. */
 int         xq_gradients        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *gradients3 ) ;
 
 /* Peak device bytes of one handle for a molecule with the given grid size, function count and mean kept functions per

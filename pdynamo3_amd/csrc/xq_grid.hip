@@ -340,24 +340,38 @@ void becke_pairs(const xq_grid *g, std::vector<double> &invR, std::vector<double
     }
 }
 
-// Atoms per lane of the weight-derivative kernel (at most 64 x 16 = 1024 atoms).
-constexpr int kLaneAtoms = 16;
+// Largest number of atoms per lane of the weight-derivative kernel (64 x 11 = 704 atoms; the LDS bounds it at 655).
+constexpr int kMaxLaneAtoms = 11;
 
-// Product of atom k's cell factors with the factor of partner j left out, from the product of its non-zero factors and
-// the count of its zero factors (no division by a zero factor).
-__device__ __forceinline__ double excluded_product(double prodNZ, int zeros, double s) {
-    if (zeros == 0) return prodNZ / s;
+// Derivative polynomial of the cell function's polynomial: d_t = (2 t + 1) c_t, so that d/dnu sum_t c_t nu^(2t+1) =
+// sum_t d_t nu^(2t).
+__constant__ double c_beckeD[11] = {1.0 * 1.000000000000000e+00, 3.0 * -3.333333333333333e+00,
+                                    5.0 * 9.000000000000000e+00, 7.0 * -17.142857142857142e+00,
+                                    9.0 * 23.333333333333332e+00, 11.0 * -22.909090909090910e+00,
+                                    13.0 * 16.153846153846153e+00, 15.0 * -8.000000000000000e+00,
+                                    17.0 * 2.647058823529412e+00, 19.0 * -0.526315789473684e+00,
+                                    21.0 * 0.047619047619048e+00};
+
+// Atom k's cell-factor product with the factor s of one partner left out, from the product of its non-zero factors (of
+// either sign: a factor can round to a tiny negative value) and the count of its zero factors.  invS = 1 / s (used only
+// when no factor vanished, so s != 0).
+__device__ __forceinline__ double excluded_product(double prodNZ, int zeros, double s, double invS) {
+    if (zeros == 0) return prodNZ * invS;
     return (zeros == 1 && s == 0.0) ? prodNZ : 0.0;
 }
 
 // XC gradient weight terms (DFTGridWeights_Derivatives, DFTGridWeights.c:98-215).  One wave per point, four points per
-// round of a work-group; lane l owns atoms l, l + 64, ... and accumulates their gradient in registers.  Per point (grid
-// atom g, weight w, energy density eXC = eps rho): A_i = prod_j s_ij; p = A_g / sum_i A_i; ew = eXC w / (p sum_i A_i);
-// dA_i/dmu_ij = ds_ij prod_{k != j} s_ik; for every ordered pair (i, j) the reference adds ifac = dA_i/dmu_ij fac_i,
-// fac_i = ew (1 - p) for i = g and -ew p otherwise, times the derivatives of mu_ij with respect to atom i, atom j and the
-// point (which moves with atom g).  Each lane adds, for its atom k, the terms of the pairs (k, m) and (m, k) that move k,
-// and the point terms of the pairs (k, m); the reference's O(nAtoms^3) update of dA/dmu becomes O(nAtoms^2) per point.
-// LDS per wave: R, the non-zero products and the zero counts (3 nAtoms doubles).
+// round of a work-group; lane l owns atoms l, l + 64, ... (at most LA of them) and accumulates their gradient in
+// registers.  Per point (grid atom g, weight w, energy density eXC = eps rho): A_i = prod_j s_ij; p = A_g / sum_i A_i;
+// ew = eXC w / (p sum_i A_i); dA_i/dmu_ij = ds_ij prod_{k != j} s_ik; for every ordered pair (i, j) the reference adds
+// ifac = dA_i/dmu_ij fac_i, fac_i = ew (1 - p) for i = g and -ew p otherwise, times the derivatives of mu_ij with respect
+// to atom i, atom j and the point (which moves with atom g).  Each lane adds, for its atom k, the terms of the pairs
+// (k, m) and (m, k) that move k, and the point terms of the pairs (k, m); the reference's O(nAtoms^3) update of dA/dmu
+// becomes O(nAtoms^2) per point.  Both factors of a pair come from one polynomial (s_km and s_mk = 1 - s_km share nu),
+// and one division gives both 1 / s_km and 1 / s_mk.
+// LDS per wave: the distances R, their inverses and the non-zero products (3 nAtoms doubles), and the zero counts (nAtoms
+// bytes, after the doubles of all four waves).
+template <int LA>
 __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int pointsPerGroup,
                                                               const double *__restrict__ px, const double *__restrict__ py,
                                                               const double *__restrict__ pz, const double *__restrict__ pw,
@@ -367,10 +381,11 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                                                               const double *__restrict__ aij, double *__restrict__ grad) {
     extern __shared__ double lds[];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    double *R = lds + static_cast<size_t>(wv) * 3 * nAtoms, *prodNZ = R + nAtoms, *zeros = prodNZ + nAtoms;
-    double G[kLaneAtoms][3];
+    double *R = lds + static_cast<size_t>(wv) * 3 * nAtoms, *iR = R + nAtoms, *prodNZ = iR + nAtoms;
+    unsigned char *zeros = reinterpret_cast<unsigned char *>(lds + static_cast<size_t>(12) * nAtoms) + wv * nAtoms;
+    double G[LA][3];
 #pragma unroll
-    for (int a = 0; a < kLaneAtoms; a++) G[a][0] = G[a][1] = G[a][2] = 0.0;
+    for (int a = 0; a < LA; a++) G[a][0] = G[a][1] = G[a][2] = 0.0;
     const int p0 = blockIdx.x * pointsPerGroup;
     const int p1 = p0 + pointsPerGroup < nPoints ? p0 + pointsPerGroup : nPoints;
     for (int base = p0; base < p1; base += 4) {
@@ -381,6 +396,7 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
             for (int i = lane; i < nAtoms; i += 64) {
                 const double dx = xyz[3 * i] - rx, dy = xyz[3 * i + 1] - ry, dz = xyz[3 * i + 2] - rz;
                 R[i] = sqrt(dx * dx + dy * dy + dz * dz);
+                iR[i] = 1.0 / R[i];
             }
         __syncthreads();
         if (active)
@@ -395,63 +411,73 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                     else prod *= s;
                 }
                 prodNZ[i] = prod;
-                zeros[i] = static_cast<double>(nz);
+                zeros[i] = static_cast<unsigned char>(nz < 2 ? nz : 2);
             }
         __syncthreads();
         if (active) {
             const int g = pointAtom[p];
             double sumA = 0.0;
-            for (int i = lane; i < nAtoms; i += 64) sumA += zeros[i] == 0.0 ? prodNZ[i] : 0.0;
+            for (int i = lane; i < nAtoms; i += 64) sumA += zeros[i] == 0 ? prodNZ[i] : 0.0;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) sumA += __shfl_xor(sumA, o, 64);
-            const double Ag = zeros[g] == 0.0 ? prodNZ[g] : 0.0;
+            const double Ag = zeros[g] == 0 ? prodNZ[g] : 0.0;
             const double pW = Ag / sumA;
             const double ew = eXC[p] * pw[p] / (pW * sumA);
             double gs[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-            for (int a = 0; a < kLaneAtoms; a++) {
+            for (int a = 0; a < LA; a++) {
                 const int k = lane + 64 * a;
                 if (k >= nAtoms) break;
                 const double xk = xyz[3 * k], yk = xyz[3 * k + 1], zk = xyz[3 * k + 2];
+                const double Rk = R[k], pk = prodNZ[k];
+                const int nzk = zeros[k];
+                const double ekx = (xk - rx) * iR[k], eky = (yk - ry) * iR[k], ekz = (zk - rz) * iR[k];
                 const double fk = k == g ? ew * (1.0 - pW) : -ew * pW;
-                const double pk = prodNZ[k];
-                const int zk0 = static_cast<int>(zeros[k]);
+                double sk = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
                 for (int m = 0; m < nAtoms; m++) {
                     if (m == k) continue;
                     const int km = pair_index(k, m);
-                    const double r = invR[km], am = aij[km];
-                    const int hi = k > m ? k : m, lo = k > m ? m : k;
-                    const double muP = (R[hi] - R[lo]) * r;
+                    const double r = invR[km], am = aij[km], Rm = R[m];
+                    const bool kHigh = k > m;
+                    // mu and nu of the pair stored as (max, min), as the cell factors take them.
+                    const double muP = (kHigh ? Rk - Rm : Rm - Rk) * r;
                     const double nu = muP + am * (1.0 - muP * muP), nu2 = nu * nu;
-                    double dnu = 1.0, dsum = 0.0;
+                    const double S = kBeckeA * becke_poly(nu);
+                    const double skm = kHigh ? 0.5 - S : 0.5 + S, smk = kHigh ? 0.5 + S : 0.5 - S;
+                    double dsum = c_beckeD[10];
 #pragma unroll
-                    for (int tt = 0; tt <= 10; tt++) {
-                        dsum += c_beckeC[tt] * dnu * static_cast<double>(2 * tt + 1);
-                        dnu *= nu2;
-                    }
+                    for (int tt = 9; tt >= 0; tt--) dsum = dsum * nu2 + c_beckeD[tt];
                     dsum *= -kBeckeA * (1.0 - 2.0 * am * muP);
-                    const double skm = becke_factor_of(k > m, R[k], R[m], r, am);
-                    const double smk = becke_factor_of(m > k, R[m], R[k], r, am);
+                    const double sProd = skm * smk;
+                    double invKM, invMK;
+                    if (fabs(sProd) >= 0x1p-1000) {
+                        const double inv = 1.0 / sProd;
+                        invKM = smk * inv;
+                        invMK = skm * inv;
+                    } else {
+                        invKM = 1.0 / skm;
+                        invMK = 1.0 / smk;
+                    }
                     const double fm = m == g ? ew * (1.0 - pW) : -ew * pW;
-                    const double ifKM = dsum * excluded_product(pk, zk0, skm) * fk;
-                    const double ifMK = dsum * excluded_product(prodNZ[m], static_cast<int>(zeros[m]), smk) * fm;
+                    const double ifKM = dsum * excluded_product(pk, nzk, skm, invKM) * fk;
+                    const double ifMK = dsum * excluded_product(prodNZ[m], zeros[m], smk, invMK) * fm;
                     const double xm = xyz[3 * m], ym = xyz[3 * m + 1], zm = xyz[3 * m + 2];
-                    const double mu = (R[k] - R[m]) * r;
-                    const double tx = (xk - xm) * mu * r * r, ty = (yk - ym) * mu * r * r, tz = (zk - zm) * mu * r * r;
-                    // (i, j) = (k, m): atom k moves through dxi, the point through dxg.
-                    const double ux = (xk - rx) * r / R[k], uy = (yk - ry) * r / R[k], uz = (zk - rz) * r / R[k];
-                    G[a][0] += ifKM * (ux - tx);
-                    G[a][1] += ifKM * (uy - ty);
-                    G[a][2] += ifKM * (uz - tz);
-                    // The point's term of the pair (k, m): -(dmu/dr_k + dmu/dr_m) before the atom-pair part.
-                    gs[0] -= ifKM * (ux - (xm - rx) * r / R[m]);
-                    gs[1] -= ifKM * (uy - (ym - ry) * r / R[m]);
-                    gs[2] -= ifKM * (uz - (zm - rz) * r / R[m]);
-                    // (i, j) = (m, k): atom k moves through dxj of that pair.
-                    G[a][0] += ifMK * (tx - ux);
-                    G[a][1] += ifMK * (ty - uy);
-                    G[a][2] += ifMK * (tz - uz);
+                    const double mu = kHigh ? muP : -muP, mr2 = mu * r * r;
+                    // Pair (k, m) moves atom k through d mu / d r_k and the point through -(d mu / d r_k + d mu / d r_m);
+                    // pair (m, k) moves atom k through its second atom: both along r (e_k) - mu r^2 (r_k - r_m).
+                    const double d = ifKM - ifMK;
+                    G[a][0] += d * (ekx * r - (xk - xm) * mr2);
+                    G[a][1] += d * (eky * r - (yk - ym) * mr2);
+                    G[a][2] += d * (ekz * r - (zk - zm) * mr2);
+                    const double wr = ifKM * r, iRm = iR[m];
+                    sk += wr;
+                    vx += wr * ((xm - rx) * iRm);
+                    vy += wr * ((ym - ry) * iRm);
+                    vz += wr * ((zm - rz) * iRm);
                 }
+                gs[0] += vx - sk * ekx;
+                gs[1] += vy - sk * eky;
+                gs[2] += vz - sk * ekz;
             }
 #pragma unroll
             for (int d = 0; d < 3; d++)
@@ -459,14 +485,14 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                 for (int o = 32; o > 0; o >>= 1) gs[d] += __shfl_xor(gs[d], o, 64);
             if (lane == (g & 63)) {
 #pragma unroll
-                for (int a = 0; a < kLaneAtoms; a++)
+                for (int a = 0; a < LA; a++)
                     if (a == (g >> 6)) { G[a][0] += gs[0]; G[a][1] += gs[1]; G[a][2] += gs[2]; }
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < kLaneAtoms; a++) {
+    for (int a = 0; a < LA; a++) {
         const int k = lane + 64 * a;
         if (k < nAtoms) {
             atomicAdd(&grad[3 * k], G[a][0]);
@@ -698,9 +724,9 @@ long long estimated_points(int accuracy, int nAtoms, const int *Z) {
 int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_t s) {
     const int nA = g->nAtoms, G = g->nPoints;
     if (G <= 0 || nA < 2) return kOK;
-    const size_t lds = 4 * 3 * sizeof(double) * static_cast<size_t>(nA);
+    const size_t lds = 4 * (3 * sizeof(double) + 1) * static_cast<size_t>(nA);
     if (lds > 64 * 1024) {
-        set_error("XC gradients support at most 682 atoms");
+        set_error("XC gradients support at most 655 atoms");
         return kInvalidArgument;
     }
     std::vector<double> invR, aij;
@@ -728,8 +754,17 @@ int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_
         int perGroup = (G + groups - 1) / groups;
         perGroup = (perGroup + 3) / 4 * 4;
         const int nGroups = (G + perGroup - 1) / perGroup;
-        hipLaunchKernelGGL(weight_gradient_kernel, dim3(nGroups), dim3(256), lds, s, G, perGroup, g->d_x, g->d_y, g->d_z,
-                           g->d_w, d_pointAtom, d_eXC, d_xyz, nA, d_invR, d_aij, d_grad);
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(nGroups), dim3(256), lds, s, G, perGroup, g->d_x, g->d_y, g->d_z, g->d_w,
+                               d_pointAtom, d_eXC, d_xyz, nA, d_invR, d_aij, d_grad);
+        };
+        // Atoms per lane: the smallest instance that covers nAtoms (fewer registers for small systems).
+        if (nA <= 64) launch(weight_gradient_kernel<1>);
+        else if (nA <= 128) launch(weight_gradient_kernel<2>);
+        else if (nA <= 192) launch(weight_gradient_kernel<3>);
+        else if (nA <= 256) launch(weight_gradient_kernel<4>);
+        else if (nA <= 384) launch(weight_gradient_kernel<6>);
+        else launch(weight_gradient_kernel<kMaxLaneAtoms>);
         if ((status = check(hipGetLastError(), "weight_gradient_kernel")) != kOK) break;
         status = check(hipStreamSynchronize(s), "gradient sync");
     } while (false);
