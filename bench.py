@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--cpu-baseline", default="reference", choices=["reference", "port", "none"])
     p.add_argument("--cpu-sample-waters", type=int, default=8)
     p.add_argument("--skip-fp64", action="store_true")
+    p.add_argument("--grad-steps", type=int, default=2,
+                   help="timed XC nuclear-gradient calls on the workload after the quadrature legs (0 skips them)")
     p.add_argument("--serve-requests", type=int, default=256,
                    help="requests in the serving stream (0 skips it)")
     p.add_argument("--serve-slots", type=int, default=16, help="jobs set up or running at once in the serving loop")
@@ -136,6 +138,39 @@ def cpu_baseline(kind, n_waters, functional, spherical, threads=1):
     return {"value": h.npoints * reps / total, "unit": "grid points/s", "cores": threads, "kind": kind,
             "sample": "(H2O){:d} cluster, same generator, {:s}, Medium grid ({:d} points), {:d} in-core quadrature calls, "
                       "{:.1f} s".format(n_waters, functional.upper(), h.npoints, reps, total)}
+
+
+def gradient_leg(grid, ids, P, steps, n_waters, functional, spherical, cpu_kind):
+    """XC nuclear gradients (xq_gradients, FP64) on the workload: wall time per call (P upload and the gradient download
+    included), the translational sum of the gradient, and the CPU's time for the same call on the bounded sample."""
+    G = grid.gradients(ids, P)
+    wall = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        G = grid.gradients(ids, P)
+        wall.append((time.perf_counter() - t) * 1e3)
+    out = {"ms_per_call": float(np.mean(wall)), "calls": steps, "atoms": int(G.shape[0]),
+           "max_abs_gradient": float(np.max(np.abs(G))), "max_abs_translational_sum": float(np.max(np.abs(G.sum(0))))}
+    if cpu_kind != "none":
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            from pdynamo3_amd import basis, densities, molecules
+            from pdynamo3_amd.definitions import functional_ids
+            kind = "reference" if cpu_kind == "reference" and oracle.reference_available() else "port"
+            lib = oracle.reference() if kind == "reference" else oracle.oracle()
+            Z, xa, rots, _ = molecules.water_cluster(n_waters, 9.0 * (n_waters / 16.0) ** (1.0 / 3.0), 1)
+            h = lib.setup(2, Z, molecules.to_bohr(xa), basis.element_spec(basis.load_basis(), Z), spherical,
+                          functional_ids(functional))
+            Ps = densities.cluster_density(rots, spherical=spherical)
+            t = time.perf_counter()
+            h.gradients(Ps, 1)
+            out["cpu_baseline"] = {"seconds_per_call": time.perf_counter() - t, "cores": 1, "kind": kind,
+                                   "sample": "(H2O){:d} cluster, same generator, {:s}, Medium grid ({:d} points), one "
+                                             "gradient call".format(n_waters, functional.upper(), h.npoints)}
+        except Exception as exc:
+            out["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    return out
 
 
 def serving_leg(args, log_fn):
@@ -234,6 +269,10 @@ def main():
         out["speedup_mixed_vs_fp64"] = ms64 / ms
         out["dExc_mixed_vs_fp64"] = mixed["E"] - fp64["E"]
         out["max_dVxc_mixed_vs_fp64"] = float(np.max(np.abs(mixed["V"] - fp64["V"])))
+    if args.grad_steps > 0:
+        out["xc_gradients"] = gradient_leg(grid, ids, P, args.grad_steps, args.cpu_sample_waters, functional, spherical,
+                                           args.cpu_baseline)
+        log("gradients:", json.dumps(out["xc_gradients"]))
     grid.close()
     if args.serve_requests > 0:
         serve = serving_leg(args, log)

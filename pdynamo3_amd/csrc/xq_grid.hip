@@ -343,6 +343,14 @@ void becke_pairs(const xq_grid *g, std::vector<double> &invR, std::vector<double
 // Largest number of atoms per lane of the weight-derivative kernel (64 x 11 = 704 atoms; the LDS bounds it at 655).
 constexpr int kMaxLaneAtoms = 11;
 
+// Screen of the weight-derivative terms.  The term of the ordered pair (k, m) is ifac = (ds_km/dmu) (A_k / s_km) fac_k, and
+// moves an atom or the point by at most |ifac| 2 / R_km.  |ds/dmu| <= 1.6875 x 2 = 3.375 (|ds/dnu| peaks at nu = 0 with
+// (3/2)^3 / 2, |dnu/dmu| = |1 - 2 a mu| <= 2); |A_k / s_km| <= Q_k, the product of k's factors without its smallest one;
+// |fac_k| = |eXC w| / sum_i A_i for k != g.  Atoms with Q_k < 2^-80 sum_i A_i (k != g) are outside the set K: the terms of
+// pairs (k, m) with k outside K and of pairs (m, k) with k outside K are each below 3.375 2^-80 |eXC w| 2 / R_km, and
+// are left out (below 1e-15 hartree/bohr in all for any molecule this kernel takes); every other pair term is computed.
+constexpr double kWeightScreen = 0x1p-80;
+
 // Derivative polynomial of the cell function's polynomial: d_t = (2 t + 1) c_t, so that d/dnu sum_t c_t nu^(2t+1) =
 // sum_t d_t nu^(2t).
 __constant__ double c_beckeD[11] = {1.0 * 1.000000000000000e+00, 3.0 * -3.333333333333333e+00,
@@ -367,8 +375,9 @@ __device__ __forceinline__ double excluded_product(double prodNZ, int zeros, dou
 // ifac = dA_i/dmu_ij fac_i, fac_i = ew (1 - p) for i = g and -ew p otherwise, times the derivatives of mu_ij with respect
 // to atom i, atom j and the point (which moves with atom g).  Each lane adds, for its atom k, the terms of the pairs
 // (k, m) and (m, k) that move k, and the point terms of the pairs (k, m); the reference's O(nAtoms^3) update of dA/dmu
-// becomes O(nAtoms^2) per point.  Both factors of a pair come from one polynomial (s_km and s_mk = 1 - s_km share nu),
-// and one division gives both 1 / s_km and 1 / s_mk.
+// becomes O(nAtoms^2) per point, and the screen (kWeightScreen) leaves O(|K| nAtoms): the atoms whose cell product is
+// not negligible at the point.  Both factors of a pair come from one polynomial (s_km and s_mk = 1 - s_km share nu), and
+// one division gives both 1 / s_km and 1 / s_mk.
 // LDS per wave: the distances R, their inverses and the non-zero products (3 nAtoms doubles), and the zero counts (nAtoms
 // bytes, after the doubles of all four waves).
 template <int LA>
@@ -399,43 +408,62 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                 iR[i] = 1.0 / R[i];
             }
         __syncthreads();
-        if (active)
-            for (int i = lane; i < nAtoms; i += 64) {
-                double prod = 1.0;
+        // Products of the cell factors, their zero counts, and the screened set K (flag 0x80 of the count).
+        double sumA = 0.0;
+        int g = 0;
+        if (active) {
+            g = pointAtom[p];
+            double Q[LA];
+#pragma unroll
+            for (int a = 0; a < LA; a++) {
+                const int i = lane + 64 * a;
+                Q[a] = 0.0;
+                if (i >= nAtoms) continue;
+                double prod = 1.0, minS = 1.0;
                 int nz = 0;
                 for (int j = 0; j < nAtoms; j++) {
                     if (j == i) continue;
                     const int ij = pair_index(i, j);
                     const double s = becke_factor_of(i > j, R[i], R[j], invR[ij], aij[ij]);
                     if (s == 0.0) nz++;
-                    else prod *= s;
+                    else {
+                        prod *= s;
+                        minS = fmin(minS, fabs(s));
+                    }
                 }
                 prodNZ[i] = prod;
+                Q[a] = nz == 0 ? fabs(prod) / minS : (nz == 1 ? fabs(prod) : 0.0);
+                if (nz == 0) sumA += prod;
                 zeros[i] = static_cast<unsigned char>(nz < 2 ? nz : 2);
             }
-        __syncthreads();
-        if (active) {
-            const int g = pointAtom[p];
-            double sumA = 0.0;
-            for (int i = lane; i < nAtoms; i += 64) sumA += zeros[i] == 0 ? prodNZ[i] : 0.0;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) sumA += __shfl_xor(sumA, o, 64);
-            const double Ag = zeros[g] == 0 ? prodNZ[g] : 0.0;
+#pragma unroll
+            for (int a = 0; a < LA; a++) {
+                const int i = lane + 64 * a;
+                if (i < nAtoms && (i == g || !(Q[a] < kWeightScreen * sumA))) zeros[i] |= 0x80;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const double Ag = (zeros[g] & 0x7f) == 0 ? prodNZ[g] : 0.0;
             const double pW = Ag / sumA;
             const double ew = eXC[p] * pw[p] / (pW * sumA);
             double gs[3] = {0.0, 0.0, 0.0};
+            // Pairs with a partner in K, for every atom k of the lane.
 #pragma unroll
             for (int a = 0; a < LA; a++) {
                 const int k = lane + 64 * a;
                 if (k >= nAtoms) break;
                 const double xk = xyz[3 * k], yk = xyz[3 * k + 1], zk = xyz[3 * k + 2];
                 const double Rk = R[k], pk = prodNZ[k];
-                const int nzk = zeros[k];
+                const int nzk = zeros[k] & 0x7f;
                 const double ekx = (xk - rx) * iR[k], eky = (yk - ry) * iR[k], ekz = (zk - rz) * iR[k];
                 const double fk = k == g ? ew * (1.0 - pW) : -ew * pW;
                 double sk = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
                 for (int m = 0; m < nAtoms; m++) {
-                    if (m == k) continue;
+                    const int zm = zeros[m];
+                    if (!(zm & 0x80) || m == k) continue;
                     const int km = pair_index(k, m);
                     const double r = invR[km], am = aij[km], Rm = R[m];
                     const bool kHigh = k > m;
@@ -460,24 +488,74 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                     }
                     const double fm = m == g ? ew * (1.0 - pW) : -ew * pW;
                     const double ifKM = dsum * excluded_product(pk, nzk, skm, invKM) * fk;
-                    const double ifMK = dsum * excluded_product(prodNZ[m], zeros[m], smk, invMK) * fm;
-                    const double xm = xyz[3 * m], ym = xyz[3 * m + 1], zm = xyz[3 * m + 2];
+                    const double ifMK = dsum * excluded_product(prodNZ[m], zm & 0x7f, smk, invMK) * fm;
+                    const double xm = xyz[3 * m], ym = xyz[3 * m + 1], zmc = xyz[3 * m + 2];
                     const double mu = kHigh ? muP : -muP, mr2 = mu * r * r;
                     // Pair (k, m) moves atom k through d mu / d r_k and the point through -(d mu / d r_k + d mu / d r_m);
                     // pair (m, k) moves atom k through its second atom: both along r (e_k) - mu r^2 (r_k - r_m).
                     const double d = ifKM - ifMK;
                     G[a][0] += d * (ekx * r - (xk - xm) * mr2);
                     G[a][1] += d * (eky * r - (yk - ym) * mr2);
-                    G[a][2] += d * (ekz * r - (zk - zm) * mr2);
+                    G[a][2] += d * (ekz * r - (zk - zmc) * mr2);
                     const double wr = ifKM * r, iRm = iR[m];
                     sk += wr;
                     vx += wr * ((xm - rx) * iRm);
                     vy += wr * ((ym - ry) * iRm);
-                    vz += wr * ((zm - rz) * iRm);
+                    vz += wr * ((zmc - rz) * iRm);
                 }
                 gs[0] += vx - sk * ekx;
                 gs[1] += vy - sk * eky;
                 gs[2] += vz - sk * ekz;
+            }
+            // Pairs (k2, m) with k2 in K and m outside it (their pair (m, k2) terms are below the screen): lanes own m,
+            // the terms that move atom k2 are summed over the wave.
+            for (int k2 = 0; k2 < nAtoms; k2++) {
+                const int z2 = zeros[k2];
+                if (!(z2 & 0x80)) continue;
+                const int nz2 = z2 & 0x7f;
+                const double x2 = xyz[3 * k2], y2 = xyz[3 * k2 + 1], w2 = xyz[3 * k2 + 2];
+                const double R2 = R[k2], p2 = prodNZ[k2];
+                const double e2x = (x2 - rx) * iR[k2], e2y = (y2 - ry) * iR[k2], e2z = (w2 - rz) * iR[k2];
+                const double f2 = k2 == g ? ew * (1.0 - pW) : -ew * pW;
+                double hx = 0.0, hy = 0.0, hz = 0.0;
+#pragma unroll
+                for (int a = 0; a < LA; a++) {
+                    const int m = lane + 64 * a;
+                    if (m >= nAtoms) break;
+                    if (zeros[m] & 0x80) continue;
+                    const int km = pair_index(k2, m);
+                    const double r = invR[km], am = aij[km], Rm = R[m];
+                    const bool kHigh = k2 > m;
+                    const double muP = (kHigh ? R2 - Rm : Rm - R2) * r;
+                    const double nu = muP + am * (1.0 - muP * muP), nu2 = nu * nu;
+                    const double S = kBeckeA * becke_poly(nu);
+                    const double s2m = kHigh ? 0.5 - S : 0.5 + S;
+                    double dsum = c_beckeD[10];
+#pragma unroll
+                    for (int tt = 9; tt >= 0; tt--) dsum = dsum * nu2 + c_beckeD[tt];
+                    dsum *= -kBeckeA * (1.0 - 2.0 * am * muP);
+                    const double ifKM = dsum * excluded_product(p2, nz2, s2m, nz2 == 0 ? 1.0 / s2m : 0.0) * f2;
+                    const double xm = xyz[3 * m], ym = xyz[3 * m + 1], zmc = xyz[3 * m + 2];
+                    const double mu = kHigh ? muP : -muP, mr2 = mu * r * r;
+                    hx += ifKM * (e2x * r - (x2 - xm) * mr2);
+                    hy += ifKM * (e2y * r - (y2 - ym) * mr2);
+                    hz += ifKM * (e2z * r - (w2 - zmc) * mr2);
+                    const double wr = ifKM * r, iRm = iR[m];
+                    gs[0] += wr * ((xm - rx) * iRm) - wr * e2x;
+                    gs[1] += wr * ((ym - ry) * iRm) - wr * e2y;
+                    gs[2] += wr * ((zmc - rz) * iRm) - wr * e2z;
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    hx += __shfl_xor(hx, o, 64);
+                    hy += __shfl_xor(hy, o, 64);
+                    hz += __shfl_xor(hz, o, 64);
+                }
+                if (lane == (k2 & 63)) {
+#pragma unroll
+                    for (int a = 0; a < LA; a++)
+                        if (a == (k2 >> 6)) { G[a][0] += hx; G[a][1] += hy; G[a][2] += hz; }
+                }
             }
 #pragma unroll
             for (int d = 0; d < 3; d++)
