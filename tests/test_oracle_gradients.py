@@ -57,7 +57,8 @@ def test_gradients_are_finite_differences_of_exc():
                 x[atom, c] += sgn * h
                 e.append(lib.setup(2, Z, x, spec, False, ids("blyp")).integrate(P, 8)[0])
             fd = (e[0] - e[1]) / (2.0 * h)
-            assert abs(fd - g[atom, c]) <= 2e-6 * max(1.0, abs(g[atom, c])), (atom, c, fd, g[atom, c])
+            # Central differences with h = 1e-4 bohr agree to about 2e-9 here; a dropped weight term is 1e-3 or more.
+            assert abs(fd - g[atom, c]) <= 2e-8 * max(1.0, abs(g[atom, c])), (atom, c, fd, g[atom, c])
 
 
 def test_gradients_sum_to_zero():
