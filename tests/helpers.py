@@ -24,10 +24,17 @@ def random_density(n, nocc, seed, scale=0.3):
     return packed(2.0 * C @ C.T)
 
 
-def water_system(spherical=True):
+def water_system(spherical=True, label="6-31g_st"):
     Z, xyz = molecules.water()
-    b = basis.load_basis("6-31g_st")
+    b = basis.load_basis(label)
     return Z, xyz, basis.element_spec(b, Z), basis.number_of_functions(b, Z, spherical)
+
+
+def methane_angstrom():
+    """CH4, C at the origin, C-H 1.09 Angstrom along the tetrahedral directions (slightly distorted)."""
+    d = 1.09 / np.sqrt(3.0)
+    xyz = np.array([[0.0, 0.0, 0.0], [d, d, d], [-d, -d, d], [-d, d, -d], [d, -d, -d * 1.02]])
+    return np.array([6, 1, 1, 1, 1], dtype=np.int32), xyz
 
 
 def cluster_system(n_waters, cube, seed, spherical=True):

@@ -71,3 +71,19 @@ def test_gradients_increment_and_zero_density():
     g.gradients(ids("blyp"), P, G2)
     g.close()
     assert np.max(np.abs(G2 - 0.25 - G1)) <= 1e-14
+
+
+def test_methane_def2svp_gradients_match_oracle(oracle_lib):
+    from pdynamo3_amd import basis, molecules
+    from helpers import methane_angstrom
+    Zm, xa = methane_angstrom()
+    xyz = molecules.to_bohr(xa)
+    b = basis.load_basis("def2-svp")
+    spec, n = basis.element_spec(b, Zm), basis.number_of_functions(b, Zm, True)
+    P = random_density(n, 5, seed=24)
+    g0 = oracle_lib.setup(2, Zm, xyz, spec, True, ids("blyp")).gradients(P, 8)
+    g = _grid(Zm, xyz, spec, True, "fp64")
+    G = g.gradients(ids("blyp"), P)
+    g.close()
+    print("max dG", np.max(np.abs(G - g0)))
+    assert np.max(np.abs(G - g0)) <= 1e-9

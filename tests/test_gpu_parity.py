@@ -139,3 +139,32 @@ def test_large_cluster_mixed_against_fp64(waters, cube, seed, functional):
     assert abs(ne - n0) <= 1e-6
     assert dv <= 1e-7
     g.close()
+
+
+def _methane(label, spherical):
+    from pdynamo3_amd import basis, molecules
+    from helpers import methane_angstrom
+    Z, xa = methane_angstrom()
+    b = basis.load_basis(label)
+    return Z, molecules.to_bohr(xa), basis.element_spec(b, Z), basis.number_of_functions(b, Z, spherical)
+
+
+@pytest.mark.parametrize("system,label,spherical", [("water", "def2-svp", True), ("water", "def2-svp", False),
+                                                   ("methane", "def2-svp", True), ("methane", "6-31g_st", False)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_other_bases_and_carbon_match_oracle(oracle_lib, system, label, spherical, precision):
+    """def2-SVP (p functions on H, spherical or Cartesian d) and carbon, against the oracle (which the CPU tests pin to
+    the compiled reference for the same inputs)."""
+    I = _integrator()
+    Z, xyz, spec, n = water_system(spherical, label) if system == "water" else _methane(label, spherical)
+    P = random_density(n, 5, seed=23)
+    e0, n0, V0, _ = oracle_lib.setup(2, Z, xyz, spec, spherical, ids("pbe")).integrate(P)
+    g = I.QuadratureGrid(Z, xyz, 2)
+    g.set_basis(spec, spherical, precision)
+    V = np.zeros_like(P)
+    e, ne, _ = g.integrate(ids("pbe"), P, V)
+    g.close()
+    tolE, tolV = (1e-9, 1e-9) if precision == "fp64" else (1e-6, 1e-7)
+    print(system, label, spherical, precision, "dE", e - e0, "max dV", np.max(np.abs(V - V0)))
+    assert abs(e - e0) <= tolE and abs(ne - n0) <= tolE
+    assert np.max(np.abs(V - V0)) <= tolV
