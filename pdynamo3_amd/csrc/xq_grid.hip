@@ -175,7 +175,9 @@ __device__ __forceinline__ int pair_index(int k, int m) {
 }
 
 // Candidate points of one radial shell per work-group (thread = angular point) and their Becke weights
-// w = 4 pi w_r w_ang P_A / sum_i P_i (DFTGrid.c:158-211, DFTGridWeights.c:220-258).  The atom coordinates, the pair
+// w = 4 pi w_r w_ang P_A / sum_i P_i (DFTGrid.c:158-211, DFTGridWeights.c:220-258).  P_A and the product of the atom
+// nearest the point come first; when even P_A / (P_A + P_near) leaves the weight below half the cull tolerance (or
+// P_A = 0), the point is culled (as the reference culls it) without the other products.  The atom coordinates, the pair
 // data of the shell's atom A (row A of 1/R and a) and the atoms in order of distance from A are staged in LDS.  P_A is
 // computed in full; each other P_i takes its factors nearest atom first and stops once it falls below 2^-64 P_A, where
 // it no longer changes the sum beyond rounding.  Each factor is the reference's cell function of the same mu and nu,
@@ -221,9 +223,38 @@ __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell
             PA *= becke_factor_of(A > j, rA, dist(j, px, py, pz), sInv[j], sA[j]);
         }
         const double floor = PA * 0x1p-64;
+        const double wru = sh.wr * u[3];
+        const int o = sh.out + k;
+        cx[o] = px;
+        cy[o] = py;
+        cz[o] = pz;
+        // The atom nearest the point (other than A) gets its full product first: sum_i P_i >= P_A + P_near, so a point
+        // whose weight cannot exceed half the cull tolerance even then is culled without the other products.
+        int near = -1;
+        double dNear = 0.0;
+        for (int j = 0; j < nAtoms; j++) {
+            if (j == A) continue;
+            const double dx = sx[j] - px, dy = sy[j] - py, dz = sz[j] - pz, d2 = dx * dx + dy * dy + dz * dz;
+            if (near < 0 || d2 < dNear) { near = j; dNear = d2; }
+        }
         double sum = PA;
+        if (near >= 0) {
+            const double rn = dist(near, px, py, pz);
+            double Pn = becke_factor_of(near > A, rn, rA, sInv[near], sA[near]);
+            for (int j = 0; j < nAtoms; j++) {
+                if (j == near || j == A) continue;
+                const int ij = pair_index(near, j);
+                Pn *= becke_factor_of(near > j, rn, dist(j, px, py, pz), invR[ij], aij[ij]);
+            }
+            sum += Pn;
+            if (PA == 0.0 || (sum > 0.0 && fabs(wru) * (fabs(PA) / sum) < 0.5 * kWeightTol)) {
+                cw[o] = 0.0;
+                keep[o] = 0;
+                continue;
+            }
+        }
         for (int i = 0; i < nAtoms; i++) {
-            if (i == A) continue;
+            if (i == A || i == near) continue;
             const double ri = dist(i, px, py, pz);
             double Pi = becke_factor_of(i > A, ri, rA, sInv[i], sA[i]);
             for (int jj = 0; jj < nAtoms && fabs(Pi) >= floor; jj++) {
@@ -234,11 +265,7 @@ __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell
             }
             if (fabs(Pi) >= floor) sum += Pi;
         }
-        const double w = sh.wr * u[3] * (PA / sum);
-        const int o = sh.out + k;
-        cx[o] = px;
-        cy[o] = py;
-        cz[o] = pz;
+        const double w = wru * (PA / sum);
         cw[o] = w;
         keep[o] = fabs(w) > kWeightTol ? 1 : 0;
     }
