@@ -53,7 +53,8 @@ int         xq_estimated_points ( int accuracy, int nAtoms, const int *atomicNum
 !  every grid block on the device (the reference's inCore mode).  precision selects the storage (XQ_PRECISION_*). */
 int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, int spherical, int precision, int *nFunctions ) ;
 
-/* One quadrature.  ids: libxc functional IDs (101 PBE-x, 106 B88, 130 PBE-c, 131 LYP; pMolecule/QCModel/DFTDefinitions.py).
+/* One quadrature.  ids: libxc functional IDs (1 Slater exchange, 7 VWN, 8 VWN-RPA, 101 PBE-x, 106 B88, 130 PBE-c, 131 LYP,
+!  402 B3LYP's XC part; pMolecule/QCModel/DFTDefinitions.py), all LDA or all GGA.
 !  densityP: packed total density (n(n+1)/2).  fockV: packed, incremented by V_xc.  Statistics (optional, may be NULL, else
 !  XQ_NSTATS doubles): [0] sum over blocks of kept functions x points ("Quadrature BF Values"), [1] blocks skipped by the
 !  density tolerance, [2] device ms from P expansion to packed V, [3] device ms of the density/functional kernel, [4] device

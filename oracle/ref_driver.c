@@ -245,7 +245,12 @@ int xcref_functional ( int nIDs, const int *ids, int n, const double *rho, const
     {
         xc_func_type func ;
         if ( xc_func_init ( &func, ids[f], XC_UNPOLARIZED ) < 0 ) { free ( e ) ; free ( vr ) ; free ( vs ) ; return 3 ; }
-        xc_gga_exc_vxc ( &func, n, rho, sigma, e, vr, vs ) ;
+        if ( func.info->family == XC_FAMILY_LDA )
+        {
+            xc_lda_exc_vxc ( &func, n, rho, e, vr ) ;
+            for ( i = 0 ; i < n ; i++ ) vs[i] = 0.0 ;
+        }
+        else xc_gga_exc_vxc ( &func, n, rho, sigma, e, vr, vs ) ;
         for ( i = 0 ; i < n ; i++ ) { eXC[i] += e[i] ; vRho[i] += vr[i] ; vSigma[i] += vs[i] ; }
         xc_func_end ( &func ) ;
     }

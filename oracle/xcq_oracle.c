@@ -639,13 +639,85 @@ static void CorrelationGGA ( int which, double rho, double sigma, double *eps, d
     *vs += 2.0 * dens * fxs * ( CBRT2 * dxtds ) ;
 }
 
-/* Functional IDs (pMolecule/QCModel/DFTDefinitions.py:49,54,78-79): 101 PBE-x, 106 B88, 130 PBE-c, 131 LYP. */
-static int ValidID ( int id ) { return id == 101 || id == 106 || id == 130 || id == 131 ; }
+/*----------------------------------------------------------------------------------------------------------------------------------
+! LDA pieces (libxc 1.2.0 as vendored: work_lda.i:44-86 for the rs variable and vrho, lda_x.c:121-140 for Slater exchange,
+! lda_c_vwn.c:42-63,103-160,178-236 for Vosko-Wilk-Nusair correlation at zeta = 0).  Below XC_MIN_DENS both vanish.
+!   rs = (3/(4 pi))^(1/3) rho^(-1/3); vrho = eps + rho (d eps/d rs)(d rs/d rho), d rs/d rho = -rs/(3 rho).
+!   Slater: eps = a_x / rs, a_x = -(3/4)(3/(2 pi))^(2/3).
+!   VWN (x = sqrt(rs), X(x) = x^2 + b x + c, Q = sqrt(4 c - b^2)): eps = A [ln(x^2/X) + (2b/Q - f2 f3) atan(Q/(2x + b))
+!   - f2 ln((x - x0)^2/X)], f2 = b x0 / X(x0), f3 = 2 (2 x0 + b)/Q; parameters of VWN (functional 7) and of its RPA fit
+!   (functional 8), in hartree.
+!---------------------------------------------------------------------------------------------------------------------------------*/
+# define RS_FACTOR 0.6203504908994000866              /* (3/(4 pi))^(1/3) */
+# define SLATER_AX (-0.458165293283142893475554485052) /* -(3/4) (3/(2 pi))^(2/3) */
+
+static void LdaFinish ( double dens, double rs, double e, double dedrs, double *eps, double *vr )
+{
+    *eps = e ;
+    *vr  = e + dens * dedrs * ( - rs / ( 3.0 * dens ) ) ;
+}
+
+static void ExchangeSlater ( double rho, double *eps, double *vr, double *vs )
+{
+    const double dens = rho > 0.0 ? rho : 0.0 ;
+    double rs ;
+    *eps = *vr = *vs = 0.0 ;
+    if ( dens < XC_MIN_DENS ) return ;
+    rs = RS_FACTOR * pow ( dens, - 1.0 / 3.0 ) ;
+    LdaFinish ( dens, rs, SLATER_AX / rs, - SLATER_AX / ( rs * rs ), eps, vr ) ;
+}
+
+static void CorrelationVWN ( int rpa, double rho, double *eps, double *vr, double *vs )
+{
+    /* A, b, c, x0 of the paramagnetic fit (VWN 1980, divided by two for hartree). */
+    static const double A = 0.0310907 ;
+    static const double bP[2] = { 3.72744, 13.0720 }, cP[2] = { 12.9352, 42.7198 }, x0P[2] = { -0.10498, -0.409286 } ;
+    const double dens = rho > 0.0 ? rho : 0.0 ;
+    double b, c, x0, Q, f1, f2, f3, rs, x, X, at, xx0, t1, t2, t3, dedx ;
+    *eps = *vr = *vs = 0.0 ;
+    if ( dens < XC_MIN_DENS ) return ;
+    b  = bP[rpa] ; c = cP[rpa] ; x0 = x0P[rpa] ;
+    Q  = sqrt ( 4.0 * c - b * b ) ;
+    f1 = 2.0 * b / Q ;
+    f2 = b * x0 / ( x0 * x0 + b * x0 + c ) ;
+    f3 = 2.0 * ( 2.0 * x0 + b ) / Q ;
+    rs  = RS_FACTOR * pow ( dens, - 1.0 / 3.0 ) ;
+    x   = sqrt ( rs ) ;
+    X   = x * x + b * x + c ;
+    at  = atan ( Q / ( 2.0 * x + b ) ) ;
+    xx0 = x - x0 ;
+    t1  = 2.0 * x + b ;
+    t2  = 2.0 * c + b * x ;
+    t3  = t1 * t1 + Q * Q ;
+    dedx = A * ( - 2.0 * f2 / xx0 + ( f2 * t1 + t2 / x ) / X - 2.0 * Q * ( f1 - f2 * f3 ) / t3 ) ;
+    LdaFinish ( dens, rs, A * ( log ( x * x / X ) + ( f1 - f2 * f3 ) * at - f2 * log ( xx0 * xx0 / X ) ), dedx / ( 2.0 * x ),
+                eps, vr ) ;
+}
+
+/* B3LYP (hyb_gga_xc_b3lyp.c:50-56 with mix_func.c:28-107): 0.08 Slater + 0.72 B88 + 0.19 VWN(RPA) + 0.81 LYP, each
+   component with its own thresholds, summed in that order; the 0.20 of exact exchange is the caller's. */
+static void B3LYP ( double rho, double sigma, double *eps, double *vr, double *vs )
+{
+    static const double coef[4] = { 1.0 - 0.20 - 0.72, 0.72, 1.0 - 0.81, 0.81 } ;
+    double e, a, b ;
+    *eps = *vr = *vs = 0.0 ;
+    ExchangeSlater ( rho, &e, &a, &b ) ;             *eps += coef[0] * e ; *vr += coef[0] * a ;
+    ExchangeGGA    ( 0, rho, sigma, &e, &a, &b ) ;   *eps += coef[1] * e ; *vr += coef[1] * a ; *vs += coef[1] * b ;
+    CorrelationVWN ( 1, rho, &e, &a, &b ) ;          *eps += coef[2] * e ; *vr += coef[2] * a ;
+    CorrelationGGA ( 0, rho, sigma, &e, &a, &b ) ;   *eps += coef[3] * e ; *vr += coef[3] * a ; *vs += coef[3] * b ;
+}
+
+/* Functional IDs (pMolecule/QCModel/DFTDefinitions.py:22,28-29,49,54,78-79,173): 1 Slater exchange, 7 VWN, 8 VWN (RPA),
+   101 PBE-x, 106 B88, 130 PBE-c, 131 LYP, 402 B3LYP. */
+static int ValidID ( int id ) { return id == 1 || id == 7 || id == 8 || id == 101 || id == 106 || id == 130 || id == 131 || id == 402 ; }
+static int IsLDA ( int id ) { return id == 1 || id == 7 || id == 8 ; }
 
 /* eps (per particle, summed over functionals), vrho and vsigma for n points (DFTFunctionalModel.c:108-154). */
 int xqo_functional ( int nIDs, const int *ids, int n, const double *rho, const double *sigma, double *eps, double *vr, double *vs )
 {
-    for ( int f = 0 ; f < nIDs ; f++ ) if ( ! ValidID ( ids[f] ) ) return 3 ;
+    int nLDA = 0 ;
+    for ( int f = 0 ; f < nIDs ; f++ ) { if ( ! ValidID ( ids[f] ) ) return 3 ; nLDA += IsLDA ( ids[f] ) ; }
+    if ( nLDA > 0 && nLDA < nIDs ) return 3 ;   /* LDA and GGA functionals in one model: not taken */
     for ( int p = 0 ; p < n ; p++ )
     {
         double e = 0.0, a = 0.0, b = 0.0 ;
@@ -658,6 +730,10 @@ int xqo_functional ( int nIDs, const int *ids, int n, const double *rho, const d
                 case 101: ExchangeGGA    ( 1, rho[p], sigma[p], &ed, &v1, &v2 ) ; break ;
                 case 131: CorrelationGGA ( 0, rho[p], sigma[p], &ed, &v1, &v2 ) ; break ;
                 case 130: CorrelationGGA ( 1, rho[p], sigma[p], &ed, &v1, &v2 ) ; break ;
+                case   1: ExchangeSlater ( rho[p], &ed, &v1, &v2 ) ; break ;
+                case   7: CorrelationVWN ( 0, rho[p], &ed, &v1, &v2 ) ; break ;
+                case   8: CorrelationVWN ( 1, rho[p], &ed, &v1, &v2 ) ; break ;
+                case 402: B3LYP          ( rho[p], sigma[p], &ed, &v1, &v2 ) ; break ;
             }
             e += ed ; a += v1 ; b += v2 ;
         }

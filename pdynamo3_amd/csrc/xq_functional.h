@@ -1,5 +1,6 @@
-// GGA exchange-correlation functionals on the device, unpolarised, FP64: Becke 88 and PBE exchange, LYP and PBE correlation
-// (the latter on the "modified" PW92 parameters).  The thresholds and variable choices are those of libxc 1.2.0 as vendored
+// Exchange-correlation functionals on the device, unpolarised, FP64: Becke 88 and PBE exchange, LYP and PBE correlation
+// (the latter on the "modified" PW92 parameters), Slater exchange and VWN correlation (both fits), and B3LYP's XC part
+// (lda_x.c:121-140, lda_c_vwn.c:42-63,103-160,178-236, work_lda.i:44-86, hyb_gga_xc_b3lyp.c:50-56, mix_func.c:28-107).  The thresholds and variable choices are those of libxc 1.2.0 as vendored
 // by pDynamo3 (pMolecule/QCModel/__extensions__/cxc/work_gga_x.i:37-183, work_gga_c.i:25-245, gga_x_b88.c:48,109-148,
 // gga_x_pbe.c:48,64,128-162, gga_c_lyp.c:40,65-223, gga_c_pbe.c:40,75,80-264, lda_c_pw.c:60-236, util.h:67-84).  The
 // cube roots are pow(x, 1/3), as in that build (cxc/xc_config.h falls back to pow because config.h is not included).
@@ -126,7 +127,55 @@ __device__ __forceinline__ void gga_correlation(int which, double rho, double si
     vs = dens * fxt * dxtds + 2.0 * dens * fxs * (kCbrt2 * dxtds);
 }
 
-// Sum over up to four functionals given by libxc ID (101 PBE-x, 106 B88, 130 PBE-c, 131 LYP).
+// LDA pieces: rs = (3/(4 pi))^(1/3) rho^(-1/3), vrho = eps + rho (d eps/d rs)(-rs/(3 rho)); zero below the density threshold.
+constexpr double kRsFactor = 0.6203504908994000866;              // (3/(4 pi))^(1/3)
+constexpr double kSlaterAx = -0.458165293283142893475554485052;  // -(3/4) (3/(2 pi))^(2/3)
+
+__device__ __forceinline__ void lda_finish(double dens, double rs, double e, double dedrs, double &eps, double &vr) {
+    eps = e;
+    vr = e + dens * dedrs * (-rs / (3.0 * dens));
+}
+
+__device__ __forceinline__ void slater_exchange(double rho, double &eps, double &vr, double &vs) {
+    eps = 0.0; vr = 0.0; vs = 0.0;
+    const double dens = rho > 0.0 ? rho : 0.0;
+    if (dens < kXMinDens) return;
+    const double rs = kRsFactor * pow(dens, -1.0 / 3.0);
+    lda_finish(dens, rs, kSlaterAx / rs, -kSlaterAx / (rs * rs), eps, vr);
+}
+
+// Vosko-Wilk-Nusair correlation at zeta = 0 in x = sqrt(rs): eps = A [ln(x^2/X) + (2b/Q - f2 f3) atan(Q/(2x + b))
+// - f2 ln((x - x0)^2/X)], X = x^2 + b x + c, Q = sqrt(4c - b^2), f2 = b x0 / X(x0), f3 = 2 (2 x0 + b)/Q; rpa selects the
+// fit to the RPA energies (functional 8) over the default fit (functional 7).
+__device__ __forceinline__ void vwn_correlation(int rpa, double rho, double &eps, double &vr, double &vs) {
+    eps = 0.0; vr = 0.0; vs = 0.0;
+    const double dens = rho > 0.0 ? rho : 0.0;
+    if (dens < kXMinDens) return;
+    const double A = 0.0310907;
+    const double b = rpa ? 13.0720 : 3.72744, c = rpa ? 42.7198 : 12.9352, x0 = rpa ? -0.409286 : -0.10498;
+    const double Q = sqrt(4.0 * c - b * b);
+    const double f1 = 2.0 * b / Q, f2 = b * x0 / (x0 * x0 + b * x0 + c), f3 = 2.0 * (2.0 * x0 + b) / Q;
+    const double rs = kRsFactor * pow(dens, -1.0 / 3.0), x = sqrt(rs);
+    const double X = x * x + b * x + c, at = atan(Q / (2.0 * x + b)), xx0 = x - x0;
+    const double t1 = 2.0 * x + b, t2 = 2.0 * c + b * x, t3 = t1 * t1 + Q * Q;
+    const double dedx = A * (-2.0 * f2 / xx0 + (f2 * t1 + t2 / x) / X - 2.0 * Q * (f1 - f2 * f3) / t3);
+    lda_finish(dens, rs, A * (log(x * x / X) + (f1 - f2 * f3) * at - f2 * log(xx0 * xx0 / X)), dedx / (2.0 * x), eps, vr);
+}
+
+// B3LYP's XC part: 0.08 Slater + 0.72 B88 + 0.19 VWN (RPA fit) + 0.81 LYP, summed in that order (the exact exchange,
+// 0.20, is the caller's).
+__device__ __forceinline__ void b3lyp(double rho, double sigma, double &eps, double &vr, double &vs) {
+    const double c0 = 1.0 - 0.20 - 0.72, c1 = 0.72, c2 = 1.0 - 0.81, c3 = 0.81;
+    double e, a, b;
+    eps = 0.0; vr = 0.0; vs = 0.0;
+    slater_exchange(rho, e, a, b);          eps += c0 * e; vr += c0 * a;
+    gga_exchange(0, rho, sigma, e, a, b);    eps += c1 * e; vr += c1 * a; vs += c1 * b;
+    vwn_correlation(1, rho, e, a, b);        eps += c2 * e; vr += c2 * a;
+    gga_correlation(0, rho, sigma, e, a, b); eps += c3 * e; vr += c3 * a; vs += c3 * b;
+}
+
+// Sum over up to four functionals given by libxc ID (1 Slater, 7 VWN, 8 VWN-RPA, 101 PBE-x, 106 B88, 130 PBE-c, 131 LYP,
+// 402 B3LYP).
 struct FunctionalSet {
     int n;
     int id[4];
@@ -141,7 +190,11 @@ __device__ __forceinline__ void evaluate_functional(const FunctionalSet &fs, dou
             case 106: gga_exchange(0, rho, sigma, e, a, b); break;
             case 101: gga_exchange(1, rho, sigma, e, a, b); break;
             case 131: gga_correlation(0, rho, sigma, e, a, b); break;
-            default: gga_correlation(1, rho, sigma, e, a, b); break;
+            case 130: gga_correlation(1, rho, sigma, e, a, b); break;
+            case 1: slater_exchange(rho, e, a, b); break;
+            case 7: vwn_correlation(0, rho, e, a, b); break;
+            case 8: vwn_correlation(1, rho, e, a, b); break;
+            default: b3lyp(rho, sigma, e, a, b); break;
         }
         eps += e; vr += a; vs += b;
     }

@@ -98,12 +98,20 @@ int ensure_work(xq_grid *g) {
 int functional_set(int nIDs, const int *ids, FunctionalSet *fs) {
     if (nIDs < 1 || nIDs > 4) { set_error("between one and four functionals are supported"); return kInvalidArgument; }
     fs->n = nIDs;
+    int nLDA = 0;
     for (int k = 0; k < nIDs; k++) {
-        if (ids[k] != 101 && ids[k] != 106 && ids[k] != 130 && ids[k] != 131) {
-            set_error("unsupported functional ID (supported: 101, 106, 130, 131)");
+        const int id = ids[k];
+        if (id != 1 && id != 7 && id != 8 && id != 101 && id != 106 && id != 130 && id != 131 && id != 402) {
+            set_error("unsupported functional ID (supported: 1, 7, 8, 101, 106, 130, 131, 402)");
             return kInvalidArgument;   // DFTFunctionalModel.c:196-202
         }
-        fs->id[k] = ids[k];
+        nLDA += (id == 1 || id == 7 || id == 8) ? 1 : 0;
+        fs->id[k] = id;
+    }
+    if (nLDA > 0 && nLDA < nIDs) {
+        // The named models are all LDA or all GGA (DFTDefinitions.py:212-222); a model mixing the two families is refused.
+        set_error("LDA and GGA functionals cannot be combined in one model");
+        return kInvalidArgument;
     }
     for (int k = nIDs; k < 4; k++) fs->id[k] = 0;
     return kOK;

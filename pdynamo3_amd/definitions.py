@@ -16,14 +16,28 @@ class DFTGridAccuracy(enum.IntEnum):
     VeryHigh = 4
 
 
-# . libxc functional IDs used by the supported functionals (DFTDefinitions.py:49,54,78-79).
+# . libxc functional IDs used by the supported functionals (DFTDefinitions.py:22,28-29,49,54,78-79,173).
+LDA_X = 1
+LDA_C_VWN = 7
+LDA_C_VWN_RPA = 8
 GGA_X_PBE = 101
 GGA_X_B88 = 106
 GGA_C_PBE = 130
 GGA_C_LYP = 131
+HYB_GGA_XC_B3LYP = 402
 
-# . Functional name -> libxc IDs (DFTDefinitions.py:212-222). Only GGAs on the quadrature path are supported.
-FUNCTIONAL_IDS = {"blyp": (GGA_X_B88, GGA_C_LYP), "pbe": (GGA_X_PBE, GGA_C_PBE)}
+# . Functional name -> libxc IDs (DFTDefinitions.py:212-222): the named models whose functionals the library evaluates.
+FUNCTIONAL_IDS = {"b3lyp": (HYB_GGA_XC_B3LYP,), "blyp": (GGA_X_B88, GGA_C_LYP), "lda": (LDA_X, LDA_C_VWN),
+                  "pbe": (GGA_X_PBE, GGA_C_PBE)}
+
+# . Fraction of exact (Hartree-Fock) exchange of the hybrid functionals (hyb_gga_xc_b3lyp.c:56); the caller's Fock build
+#   adds it, as QCModelDFT does from DFTFunctionalModel's scaling (DFTFunctionalModel.c:82-103, QCModelDFT.py:225-243).
+EXACT_EXCHANGE = {HYB_GGA_XC_B3LYP: 0.20}
+
+
+def exact_exchange_scaling(ids):
+    """The largest exact-exchange fraction among the functional IDs (0 for pure functionals)."""
+    return max([EXACT_EXCHANGE.get(int(i), 0.0) for i in ids] + [0.0])
 
 # . Bohr radius in metres (pScientific/__extensions__/cinclude/Constants.h:15).
 BOHR_RADIUS_M = 5.291772083e-11

@@ -12,7 +12,8 @@ import numpy as np
 
 from . import _lib
 from .basis import element_spec, load_basis, number_of_functions
-from .definitions import DFTGridAccuracy, functional_ids
+from .definitions import (DFTGridAccuracy, LDA_C_VWN, LDA_C_VWN_RPA, LDA_X, exact_exchange_scaling,
+                          functional_ids)
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
@@ -207,13 +208,18 @@ class OrbitalBasis:
 
 
 class FunctionalModel:
-    """Functional by name ('blyp', 'pbe'): libxc IDs; order 1 and sigma for the GGAs (DFTFunctionalModel.c:183-186)."""
+    """Functional by name ('b3lyp', 'blyp', 'lda', 'pbe'): libxc IDs; order and sigma as DFTFunctionalModel sets them
+    (DFTFunctionalModel.c:175-192: order 0 without sigma for LDAs, 1 with sigma for GGAs); exchangeScaling, the fraction of
+    exact exchange the caller's Fock build adds (DFTFunctionalModel.c:82-103, read by QCModelDFT.py:343).  The library
+    evaluates every model on its GGA path: for an LDA, v_sigma is zero, which leaves E_xc, V_xc and the gradients those of
+    the LDA path."""
 
     def __init__(self, name):
         self.name = name
         self.ids = functional_ids(name)
-        self.order = 1
-        self.hasSigma = True
+        self.hasSigma = any(i not in (LDA_X, LDA_C_VWN, LDA_C_VWN_RPA) for i in self.ids)
+        self.order = 1 if self.hasSigma else 0
+        self.exchangeScaling = exact_exchange_scaling(self.ids)
 
 
 class DFTGridIntegrator:

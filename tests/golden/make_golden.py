@@ -8,7 +8,8 @@ them through oracle/ref_driver.c and stores inputs and outputs only (numbers, no
                             the EHT monomer density, and the kept functions and values of three blocks.
   reference_cluster.json    (H2O)4 (6 A cube, seed 5): grid size and checksums, E_xc, N_e and packed V_xc (PBE, spherical);
                             (H2O)16 (9 A cube, seed 1, the benchmark cluster): grid size, E_xc and N_e (PBE, Cartesian).
-  reference_functionals.json  libxc-1.2.0 (as vendored) eps, vrho, vsigma of BLYP and PBE on bulk and threshold (rho, sigma).
+  reference_functionals.json  libxc-1.2.0 (as vendored) eps, vrho, vsigma of BLYP, PBE, their parts, Slater exchange, VWN
+                            (both fits), LDA and B3LYP on bulk and threshold (rho, sigma).
   reference_c2s.json        the reference's own Cartesian -> spherical fixture (examples/pMolecule/data/yaml/
                             C2STransformations_0to6.yaml, read with SafeLoader) converted to JSON.
 
@@ -104,7 +105,8 @@ def functionals():
     rho = [5.0e-13, 4.9e-13, 1.0e-12, 1.0e-10, 1.0e-6, 1.0e-3, 0.1, 1.0, 10.0, 300.0, 1.0e-3, 1.0, 1.0, 0.5, 2.0e-5]
     sigma = [0.0, 1.0e-20, 1.0e-30, 1.0e-18, 1.0e-9, 1.0e-5, 0.01, 1.0, 100.0, 1.0e5, 0.0, 2.5e-25, 1.0e-26, 7.3, 4.0e-8]
     out = {"rho": rho, "sigma": sigma}
-    for name, ids in (("blyp", BLYP), ("pbe", PBE), ("b88", [106]), ("lyp", [131]), ("pbex", [101]), ("pbec", [130])):
+    for name, ids in (("blyp", BLYP), ("pbe", PBE), ("b88", [106]), ("lyp", [131]), ("pbex", [101]), ("pbec", [130]),
+                      ("slater", [1]), ("vwn", [7]), ("vwnrpa", [8]), ("lda", [1, 7]), ("b3lyp", [402])):
         e, vr, vs = ref.functional(ids, np.asarray(rho), np.asarray(sigma))
         out[name] = {"ids": ids, "eps": e.tolist(), "vrho": vr.tolist(), "vsigma": vs.tolist()}
     with open(os.path.join(HERE, "reference_functionals.json"), "w") as f:

@@ -87,3 +87,17 @@ def test_methane_def2svp_gradients_match_oracle(oracle_lib):
     g.close()
     print("max dG", np.max(np.abs(G - g0)))
     assert np.max(np.abs(G - g0)) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["lda", "b3lyp"])
+def test_lda_and_b3lyp_gradients_match_oracle(oracle_lib, name):
+    from pdynamo3_amd.definitions import functional_ids
+    ids = list(functional_ids(name))
+    Z, xyz, spec, n = water_system(True)
+    P = random_density(n, 5, seed=26)
+    g0 = oracle_lib.setup(2, Z, xyz, spec, True, ids).gradients(P, 4)
+    g = _grid(Z, xyz, spec, True, "mixed")
+    G = g.gradients(ids, P)
+    g.close()
+    print(name, "max dG", np.max(np.abs(G - g0)))
+    assert np.max(np.abs(G - g0)) <= 1e-9

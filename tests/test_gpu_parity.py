@@ -168,3 +168,28 @@ def test_other_bases_and_carbon_match_oracle(oracle_lib, system, label, spherica
     print(system, label, spherical, precision, "dE", e - e0, "max dV", np.max(np.abs(V - V0)))
     assert abs(e - e0) <= tolE and abs(ne - n0) <= tolE
     assert np.max(np.abs(V - V0)) <= tolV
+
+
+@pytest.mark.parametrize("name", ["lda", "b3lyp"])
+@pytest.mark.parametrize("spherical", [True, False])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lda_and_b3lyp_match_oracle(oracle_lib, name, spherical, precision):
+    """Slater + VWN and B3LYP's XC part (pinned in the oracle against the compiled reference's libxc)."""
+    from pdynamo3_amd.definitions import functional_ids
+    I = _integrator()
+    ids = list(functional_ids(name))
+    Z, xyz, spec, n = water_system(spherical)
+    P = random_density(n, 5, seed=25)
+    e0, n0, V0, _ = oracle_lib.setup(2, Z, xyz, spec, spherical, ids).integrate(P)
+    g = I.QuadratureGrid(Z, xyz, 2)
+    g.set_basis(spec, spherical, precision)
+    V = np.zeros_like(P)
+    e, ne, _ = g.integrate(ids, P, V)
+    tolE, tolV = (1e-9, 1e-9) if precision == "fp64" else (1e-6, 1e-7)
+    print(name, spherical, precision, "dE", e - e0, "max dV", np.max(np.abs(V - V0)))
+    assert abs(e - e0) <= tolE and abs(ne - n0) <= tolE
+    assert np.max(np.abs(V - V0)) <= tolV
+    # An LDA and a GGA in one model are refused with the invalid-argument status.
+    with pytest.raises(I.QCModelError):
+        g.integrate([1, 131], P, np.zeros_like(P))
+    g.close()

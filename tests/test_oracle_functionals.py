@@ -1,0 +1,81 @@
+"""LDA (Slater + VWN) and B3LYP in the oracle, on the CPU.
+
+- Water quadrature and XC gradients against the compiled reference (DFTFunctionalModel with these libxc IDs): E_xc and
+  N_e to 1e-11 Eh, V_xc to 1e-12, gradients to 1e-11 (the functional values themselves are pinned point by point against
+  libxc in test_oracle_reference.py);
+- finite differences of the new functionals (vrho = d(rho eps)/d rho, and v_sigma = 0 for the LDAs);
+- closed forms: Slater exchange is -(3/4)(3/pi)^(1/3) rho^(4/3) in energy density, and B3LYP is the 0.08/0.72/0.19/0.81
+  mix of its parts;
+- the model rules: LDA and GGA IDs are not combined, B3LYP carries 0.20 exact exchange.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import random_density, water_system
+from pdynamo3_amd.definitions import exact_exchange_scaling, functional_ids
+
+
+def _oracle():
+    import oracle
+    return oracle.oracle()
+
+
+@pytest.mark.parametrize("name", ["lda", "b3lyp"])
+@pytest.mark.parametrize("spherical", [True, False])
+def test_water_matches_reference(name, spherical):
+    import oracle
+    if not oracle.reference_available():
+        pytest.skip("compiled reference not built (make -C oracle ref)")
+    ids = list(functional_ids(name))
+    Z, xyz, spec, n = water_system(spherical)
+    P = random_density(n, 5, seed=11)
+    e0, n0, V0, _ = oracle.reference().setup(2, Z, xyz, spec, spherical, ids).integrate(P, 1)
+    e, ne, V, _ = _oracle().setup(2, Z, xyz, spec, spherical, ids).integrate(P, 4)
+    assert abs(e - e0) <= 1e-11 and abs(ne - n0) <= 1e-11
+    assert np.max(np.abs(V - V0)) <= 1e-12
+    g0 = oracle.reference().setup(2, Z, xyz, spec, spherical, ids).gradients(P, 1)
+    g = _oracle().setup(2, Z, xyz, spec, spherical, ids).gradients(P, 4)
+    assert np.max(np.abs(g - g0)) <= 1e-11 and np.max(np.abs(g0)) > 1e-2
+
+
+@pytest.mark.parametrize("ids", [[1], [7], [8], [1, 7], [402]])
+def test_finite_differences(ids):
+    lib = _oracle()
+    rho = np.array([1e-4, 3e-3, 0.05, 0.7, 4.0, 60.0])
+    sigma = np.array([1e-9, 1e-6, 2e-3, 0.4, 9.0, 500.0])
+    e, vr, vs = lib.functional(ids, rho, sigma)
+    h = 1e-6
+    ep, _, _ = lib.functional(ids, rho * (1 + h), sigma)
+    em, _, _ = lib.functional(ids, rho * (1 - h), sigma)
+    fd = (rho * (1 + h) * ep - rho * (1 - h) * em) / (2 * h * rho)
+    assert np.allclose(vr, fd, rtol=1e-7, atol=1e-12)
+    if 402 in ids:
+        sp, _, _ = lib.functional(ids, rho, sigma * (1 + h))
+        sm, _, _ = lib.functional(ids, rho, sigma * (1 - h))
+        # (h = 1e-6 relative in sigma: cancellation leaves about 3e-6 at the largest sigma, for B88 and LYP alike)
+        assert np.allclose(vs, rho * (sp - sm) / (2 * h * sigma), rtol=2e-5, atol=1e-14)
+    else:
+        assert np.all(vs == 0.0)
+
+
+def test_slater_closed_form_and_b3lyp_mix():
+    lib = _oracle()
+    rho = np.array([1e-3, 0.2, 5.0])
+    sigma = np.array([1e-5, 0.3, 20.0])
+    e, _, _ = lib.functional([1], rho, sigma)
+    assert np.allclose(rho * e, -0.75 * (3.0 / np.pi) ** (1.0 / 3.0) * rho ** (4.0 / 3.0), rtol=1e-14)
+    parts = [lib.functional([i], rho, sigma) for i in (1, 106, 8, 131)]
+    coef = (1.0 - 0.20 - 0.72, 0.72, 1.0 - 0.81, 0.81)
+    e3, vr3, vs3 = lib.functional([402], rho, sigma)
+    for k, out in enumerate((e3, vr3, vs3)):
+        assert np.allclose(out, sum(c * p[k] for c, p in zip(coef, parts)), rtol=1e-15, atol=0.0)
+
+
+def test_model_rules():
+    lib = _oracle()
+    with pytest.raises(ValueError):
+        lib.functional([1, 131], np.ones(2), np.ones(2))   # LDA and GGA in one model
+    assert exact_exchange_scaling(functional_ids("b3lyp")) == 0.20
+    assert exact_exchange_scaling(functional_ids("blyp")) == 0.0
+    assert exact_exchange_scaling(functional_ids("lda")) == 0.0

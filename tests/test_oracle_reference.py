@@ -84,7 +84,7 @@ def test_cluster16_grid_matches_reference_count():
     assert c["npoints"] == 625927 and c["nblocks"] == 4913
 
 
-@pytest.mark.parametrize("name", ["blyp", "pbe", "b88", "lyp", "pbex", "pbec"])
+@pytest.mark.parametrize("name", ["blyp", "pbe", "b88", "lyp", "pbex", "pbec", "slater", "vwn", "vwnrpa", "lda", "b3lyp"])
 def test_functionals_match_reference(oracle_lib, name):
     d = load("reference_functionals.json")
     ref = d[name]
