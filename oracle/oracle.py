@@ -58,6 +58,8 @@ class _Library:
         f("block_values").argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
         f("functional").restype = ctypes.c_int
         f("functional").argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        f("functional_pol").restype = ctypes.c_int
+        f("functional_pol").argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         f("c2s").restype = None
         f("c2s").argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         f("free").restype = None
@@ -73,6 +75,18 @@ class _Library:
         n = rho.size
         e, vr, vs = np.zeros(n), np.zeros(n), np.zeros(n)
         st = self._fn("functional")(len(ids), _i(ids), n, _d(rho), _d(sigma), _d(e), _d(vr), _d(vs))
+        if st != 0:
+            raise ValueError("functional status {:d}".format(st))
+        return e, vr, vs
+
+    def functional_pol(self, ids, rho, sigma):
+        """Spin-polarised: rho (n, 2), sigma (n, 3) -> eps (n), vrho (n, 2), vsigma (n, 3)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
+        n = rho.shape[0]
+        e, vr, vs = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
+        st = self._fn("functional_pol")(len(ids), _i(ids), n, _d(rho), _d(sigma), _d(e), _d(vr), _d(vs))
         if st != 0:
             raise ValueError("functional status {:d}".format(st))
         return e, vr, vs

@@ -258,6 +258,35 @@ int xcref_functional ( int nIDs, const int *ids, int n, const double *rho, const
     return 0 ;
 }
 
+/* . The same, spin-polarised: rho [n][2], sigma [n][3] -> eXC [n], vRho [n][2], vSigma [n][3]. */
+int xcref_functional_pol ( int nIDs, const int *ids, int n, const double *rho, const double *sigma,
+                           double *eXC, double *vRho, double *vSigma )
+{
+    int f, i ;
+    double *e = ( double * ) malloc ( n * sizeof ( double ) ), *vr = ( double * ) malloc ( 2 * n * sizeof ( double ) ),
+           *vs = ( double * ) malloc ( 3 * n * sizeof ( double ) ) ;
+    for ( i = 0 ; i < n ; i++ ) eXC[i] = 0.0 ;
+    for ( i = 0 ; i < 2 * n ; i++ ) vRho[i] = 0.0 ;
+    for ( i = 0 ; i < 3 * n ; i++ ) vSigma[i] = 0.0 ;
+    for ( f = 0 ; f < nIDs ; f++ )
+    {
+        xc_func_type func ;
+        if ( xc_func_init ( &func, ids[f], XC_POLARIZED ) < 0 ) { free ( e ) ; free ( vr ) ; free ( vs ) ; return 3 ; }
+        if ( func.info->family == XC_FAMILY_LDA )
+        {
+            xc_lda_exc_vxc ( &func, n, rho, e, vr ) ;
+            for ( i = 0 ; i < 3 * n ; i++ ) vs[i] = 0.0 ;
+        }
+        else xc_gga_exc_vxc ( &func, n, rho, sigma, e, vr, vs ) ;
+        for ( i = 0 ; i < n ; i++ ) eXC[i] += e[i] ;
+        for ( i = 0 ; i < 2 * n ; i++ ) vRho[i] += vr[i] ;
+        for ( i = 0 ; i < 3 * n ; i++ ) vSigma[i] += vs[i] ;
+        xc_func_end ( &func ) ;
+    }
+    free ( e ) ; free ( vr ) ; free ( vs ) ;
+    return 0 ;
+}
+
 /* . Cartesian to spherical transformation for lLow..lHigh (nc x ns, row-major). */
 void xcref_c2s ( int lLow, int lHigh, double *out )
 {

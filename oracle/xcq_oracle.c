@@ -712,6 +712,301 @@ static void B3LYP ( double rho, double sigma, double *eps, double *vr, double *v
 static int ValidID ( int id ) { return id == 1 || id == 7 || id == 8 || id == 101 || id == 106 || id == 130 || id == 131 || id == 402 ; }
 static int IsLDA ( int id ) { return id == 1 || id == 7 || id == 8 ; }
 
+/*----------------------------------------------------------------------------------------------------------------------------------
+! Spin-polarised forms (libxc 1.2.0 with nspin = 2, as DFTFunctionalModel_MakeFromIDs sets it for unrestricted models):
+! per point rho = (rho_a, rho_b), sigma = (s_aa, s_ab, s_bb); outputs eps, vrho (a, b), vsigma (aa, ab, bb).
+!   exchange: spin by spin on (rho_s, s_ss) with no factor 1/2 (work_gga_x.i:78-158), eps = sum / (rho_a + rho_b);
+!   correlation: rs, zeta = (rho_a - rho_b)/rho, xt = |grad rho|/rho^(4/3), xs_s = sqrt(s_ss)/rho_s^(4/3) with the
+!   thresholds of work_gga_c.i:55-100 and the derivative chain of :115-131; LYP and PBE-c at general zeta
+!   (gga_c_lyp.c:65-158, gga_c_pbe.c:80-218 with PW92 interpolated in zeta, lda_c_pw.c:62-216); LDA parts through
+!   rs and zeta (work_lda.i:44-86, lda_x.c:121-149, lda_c_vwn.c:42-63,103-236 with its default VWN interpolation).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+# define XC_MIN_ZETA 5.0e-13
+# define FZETA_DEN   0.519842099789746380   /* 2^(4/3) - 2 */
+
+static double FZeta  ( double z ) { return ( pow ( 1.0 + z, 4.0 / 3.0 ) + pow ( 1.0 - z, 4.0 / 3.0 ) - 2.0 ) / FZETA_DEN ; }
+static double DFZeta ( double z ) { return ( Cbrt ( 1.0 + z ) - Cbrt ( 1.0 - z ) ) * ( 4.0 / 3.0 ) / FZETA_DEN ; }
+
+/* Total density and zeta (util.c rho2dzeta, polarised branch). */
+static void DensZeta ( double ra, double rb, double *dens, double *zeta )
+{
+    *dens = ra + rb > 0.0 ? ra + rb : 0.0 ;
+    *zeta = *dens > XC_MIN_DENS ? ( ra - rb ) / *dens : 0.0 ;
+    if ( *zeta >  1.0 ) *zeta =  1.0 ;
+    if ( *zeta < -1.0 ) *zeta = -1.0 ;
+}
+
+/* An LDA energy e(rs, zeta) with de/drs and de/dzeta turned into vrho (a, b) (work_lda.i:69-76). */
+static void LdaFinishPol ( double dens, double rs, double zeta, double e, double dedrs, double dedz, double *eps,
+                           double *va, double *vb )
+{
+    double v = e + dens * dedrs * ( - rs / ( 3.0 * dens ) ) ;
+    *eps = e ;
+    *vb  = v - ( zeta + 1.0 ) * dedz ;
+    *va  = v - ( zeta - 1.0 ) * dedz ;
+}
+
+static void SlaterPol ( double ra, double rb, double *eps, double *va, double *vb )
+{
+    double dens, zeta, rs, fz, dfz ;
+    *eps = *va = *vb = 0.0 ;
+    DensZeta ( ra, rb, &dens, &zeta ) ;
+    if ( dens < XC_MIN_DENS ) return ;
+    rs  = RS_FACTOR * pow ( dens, - 1.0 / 3.0 ) ;
+    fz  = 0.5 * ( pow ( 1.0 + zeta, 4.0 / 3.0 ) + pow ( 1.0 - zeta, 4.0 / 3.0 ) ) ;
+    dfz = 2.0 / 3.0 * ( Cbrt ( 1.0 + zeta ) - Cbrt ( 1.0 - zeta ) ) ;
+    LdaFinishPol ( dens, rs, zeta, SLATER_AX / rs * fz, - SLATER_AX / ( rs * rs ) * fz, SLATER_AX / rs * dfz, eps, va, vb ) ;
+}
+
+/* One VWN fit k (0 paramagnetic, 1 ferromagnetic, 2 spin stiffness) at x = sqrt(rs): e and de/drs (VWN 1980 eq. 4.4). */
+static void VWNFit ( int rpa, int k, double x, double *e, double *dedrs )
+{
+    static const double bT[2][3]  = { { 3.72744, 7.06042, 1.13107 }, { 13.0720, 20.1231, 1.06835 } } ;
+    static const double cT[2][3]  = { { 12.9352, 18.0578, 13.0045 }, { 42.7198, 101.578, 11.4813 } } ;
+    static const double x0T[2][3] = { { -0.10498, -0.32500, -0.0047584 }, { -0.409286, -0.743294, -0.228344 } } ;
+    const double A = k == 0 ? 0.0310907 : ( k == 1 ? 0.01554535 : - 1.0 / ( 6.0 * M_PI * M_PI ) ) ;
+    const double b = bT[rpa][k], c = cT[rpa][k], x0 = x0T[rpa][k] ;
+    const double Q = sqrt ( 4.0 * c - b * b ) ;
+    const double f1 = 2.0 * b / Q, f2 = b * x0 / ( x0 * x0 + b * x0 + c ), f3 = 2.0 * ( 2.0 * x0 + b ) / Q ;
+    const double X = x * x + b * x + c, at = atan ( Q / ( 2.0 * x + b ) ), xx0 = x - x0 ;
+    const double t1 = 2.0 * x + b, t2 = 2.0 * c + b * x, t3 = t1 * t1 + Q * Q ;
+    *e     = A * ( log ( x * x / X ) + ( f1 - f2 * f3 ) * at - f2 * log ( xx0 * xx0 / X ) ) ;
+    *dedrs = A * ( - 2.0 * f2 / xx0 + ( f2 * t1 + t2 / x ) / X - 2.0 * Q * ( f1 - f2 * f3 ) / t3 ) / ( 2.0 * x ) ;
+}
+
+/* hl = 1: the interpolation in zeta that B3LYP sets for its VWN part (hyb_gga_xc_b3lyp.c:55, lda_c_vwn.c:203-207),
+   e = e_P + (e_F - e_P) f(zeta); hl = 0: VWN's own, with the spin stiffness. */
+static void VWNPol ( int rpa, int hl, double ra, double rb, double *eps, double *va, double *vb )
+{
+    double dens, zeta, rs, x, e1, e2, e3, v1, v2, v3, fz, dfz, z3, z4, fpp, t1, t2, dt1, dt2 ;
+    *eps = *va = *vb = 0.0 ;
+    DensZeta ( ra, rb, &dens, &zeta ) ;
+    if ( dens < XC_MIN_DENS ) return ;
+    rs = RS_FACTOR * pow ( dens, - 1.0 / 3.0 ) ;
+    x  = sqrt ( rs ) ;
+    VWNFit ( rpa, 0, x, &e1, &v1 ) ;
+    VWNFit ( rpa, 1, x, &e2, &v2 ) ;
+    VWNFit ( rpa, 2, x, &e3, &v3 ) ;
+    fpp = 4.0 / ( 9.0 * ( Cbrt ( 2.0 ) - 1.0 ) ) ;
+    fz  = FZeta ( zeta ) ; dfz = DFZeta ( zeta ) ;
+    z3  = pow ( zeta, 3 ) ; z4 = z3 * zeta ;
+    if ( hl ) { t1 = 0.0 ; t2 = fz ; dt1 = 0.0 ; dt2 = dfz ; }
+    else
+    {
+        t1  = ( fz / fpp ) * ( 1.0 - z4 ) ; t2 = fz * z4 ;
+        dt1 = ( dfz * ( 1.0 - z4 ) - 4.0 * fz * z3 ) / fpp ; dt2 = dfz * z4 + 4.0 * fz * z3 ;
+    }
+    LdaFinishPol ( dens, rs, zeta, e1 + e3 * t1 + ( e2 - e1 ) * t2, v1 + v3 * t1 + ( v2 - v1 ) * t2,
+                   e3 * dt1 + ( e2 - e1 ) * dt2, eps, va, vb ) ;
+}
+
+/* GGA exchange, polarised: sum over spins of (-X_FACTOR_C) rho_s^(4/3) F(x_s), x_s = max(sqrt(s_ss), MIN_GRAD)/rho_s^(4/3). */
+static void ExchangeGGAPol ( int which, double ra, double rb, double saa, double sbb, double *eps, double *va, double *vb,
+                             double *vsaa, double *vsbb )
+{
+    const double rr[2] = { ra, rb }, ss[2] = { saa, sbb } ;
+    double v[2] = { 0.0, 0.0 }, vs[2] = { 0.0, 0.0 }, zk = 0.0, dens = ra + rb ;
+    *eps = *va = *vb = *vsaa = *vsbb = 0.0 ;
+    if ( dens < XC_MIN_DENS ) return ;
+    for ( int s = 0 ; s < 2 ; s++ )
+    {
+        double gdm, ds, rl, x, F, dF ;
+        if ( rr[s] < XC_MIN_DENS ) continue ;
+        gdm = sqrt ( ss[s] ) ;
+        if ( gdm < XC_MIN_GRAD ) gdm = XC_MIN_GRAD ;
+        ds = rr[s] ; rl = pow ( ds, 4.0 / 3.0 ) ; x = gdm / pow ( ds, 4.0 / 3.0 ) ;
+        if ( which == 0 ) B88 ( x, &F, &dF ) ; else PBEX ( x, &F, &dF ) ;
+        zk  += ( - X_FACTOR_C ) * rl * F ;
+        v[s] += ( - X_FACTOR_C ) * ( rl / ds ) * ( ( 4.0 / 3.0 ) * F - ( 4.0 / 3.0 ) * dF * x ) ;
+        if ( gdm > XC_MIN_GRAD ) vs[s] = ( - X_FACTOR_C ) * rl * ( dF * x / ( 2.0 * ss[s] ) ) ;
+    }
+    *eps = zk / dens ; *va = v[0] ; *vb = v[1] ; *vsaa = vs[0] ; *vsbb = vs[1] ;
+}
+
+/* LYP at general zeta (Miehlich et al. 1989 form): f and its derivatives in rs, zeta, xt, xs_a, xs_b. */
+static void LYPz ( double rs, double zeta, double xt, const double *xs, double *f, double *frs, double *fz, double *fxt,
+                   double *fxs )
+{
+    const double A = 0.04918, B = 0.132 ;
+    const double crs = Cbrt ( 4.0 * M_PI / 3.0 ), Cf = 3.0 * pow ( 3.0 * M_PI * M_PI, 2.0 / 3.0 ) / 10.0 ;
+    const double c = 0.2533 * crs, d = 0.349 * crs ;
+    const double xt2 = xt * xt, z2 = zeta * zeta, opz = 1.0 + zeta, omz = 1.0 - zeta ;
+    const double opz23 = pow ( opz, 2.0 / 3.0 ), omz23 = pow ( omz, 2.0 / 3.0 ) ;
+    const double opz53 = opz * opz23, omz53 = omz * omz23, opz83 = opz * opz53, omz83 = omz * omz53 ;
+    const double xa2 = xs[0] * xs[0], xb2 = xs[1] * xs[1] ;
+    const double opd = 1.0 / ( 1.0 + d * rs ), om = B * exp ( - c * rs ) * opd, de = ( c + d * opd ) * rs ;
+    const double a6 = 1.0 / pow ( 2.0, 8.0 / 3.0 ), a4 = a6 / 4.0, a5 = a4 / ( 9.0 * 2.0 ) ;
+    const double S8 = xa2 * opz83 + xb2 * omz83, S11 = xa2 * opz * opz83 + xb2 * omz * omz83 ;
+    const double t1 = - ( 1.0 - z2 ) / ( 1.0 + d * rs ) ;
+    const double t2 = - xt2 * ( ( 1.0 - z2 ) * ( 47.0 - 7.0 * de ) / ( 4.0 * 18.0 ) - 2.0 / 3.0 ) ;
+    const double t3 = - Cf / 2.0 * ( 1.0 - z2 ) * ( opz83 + omz83 ) ;
+    const double t4 = a4 * ( 1.0 - z2 ) * ( 5.0 / 2.0 - de / 18.0 ) * S8 ;
+    const double t5 = a5 * ( 1.0 - z2 ) * ( de - 11.0 ) * S11 ;
+    const double t6 = - a6 * ( 2.0 / 3.0 * S8 - opz * opz * xb2 * omz83 / 4.0 - omz * omz * xa2 * opz83 / 4.0 ) ;
+    const double tt = t2 + t3 + t4 + t5 + t6 ;
+    const double dom = - om * ( c + d * opd ), dde = c + d * opd * opd ;
+    *f   = A * ( t1 + om * tt ) ;
+    *frs = A * ( - d * t1 / ( 1.0 + d * rs ) + dom * tt
+                 + om * ( xt2 * ( 1.0 - z2 ) * dde * 7.0 / ( 4.0 * 18.0 ) - a4 * ( 1.0 - z2 ) * dde / 18.0 * S8
+                          + a5 * ( 1.0 - z2 ) * dde * S11 ) ) ;
+    *fz  = A * ( 2.0 * zeta / ( 1.0 + d * rs )
+                 + om * ( xt2 * 2.0 * zeta * ( 47.0 - 7.0 * de ) / ( 4.0 * 18.0 )
+                          - Cf / 2.0 * ( - 2.0 * zeta * ( opz83 + omz83 ) + ( 1.0 - z2 ) * 8.0 / 3.0 * ( opz53 - omz53 ) )
+                          + a4 * ( 5.0 / 2.0 - de / 18.0 ) * ( - 2.0 * zeta * S8 + ( 1.0 - z2 ) * 8.0 / 3.0 * ( xa2 * opz53 - xb2 * omz53 ) )
+                          + a5 * ( de - 11.0 ) * ( - 2.0 * zeta * S11 + ( 1.0 - z2 ) * 11 / 3.0 * ( xa2 * opz83 - xb2 * omz83 ) )
+                          - a6 * ( 16.0 / 9.0 * ( xa2 * opz53 - xb2 * omz53 ) - 1.0 / 2.0 * ( opz * xb2 * omz83 - omz * xa2 * opz83 )
+                                   + 2.0 / 3.0 * ( opz * opz * xb2 * omz53 - omz * omz * xa2 * opz53 ) ) ) ) ;
+    *fxt = - 2.0 * A * om * xt * ( ( 1.0 - z2 ) * ( 47.0 - 7.0 * de ) / ( 4.0 * 18.0 ) - 2.0 / 3.0 ) ;
+    fxs[0] = A * om * 2.0 * xs[0] * ( a4 * ( 1.0 - z2 ) * ( 5.0 / 2.0 - de / 18.0 ) * opz83 + a5 * ( 1.0 - z2 ) * ( de - 11.0 ) * opz * opz83
+                                      - a6 * ( 2.0 / 3.0 * opz83 - omz * omz * opz83 / 4.0 ) ) ;
+    fxs[1] = A * om * 2.0 * xs[1] * ( a4 * ( 1.0 - z2 ) * ( 5.0 / 2.0 - de / 18.0 ) * omz83 + a5 * ( 1.0 - z2 ) * ( de - 11.0 ) * omz * omz83
+                                      - a6 * ( 2.0 / 3.0 * omz83 - opz * opz * omz83 / 4.0 ) ) ;
+}
+
+/* PW92 "modified" fit k (0 paramagnetic, 1 ferromagnetic, 2 minus the spin stiffness): e and de/drs (PW 1992 eq. 10, A5). */
+static void PW92Fit ( int k, double rs, double *e, double *dedrs )
+{
+    static const double aT[3] = { 0.0310907, 0.01554535, 0.0168869 }, alT[3] = { 0.21370, 0.20548, 0.11125 } ;
+    static const double bT[3][4] = { { 7.5957, 3.5876, 1.6382, 0.49294 }, { 14.1189, 6.1977, 3.3662, 0.62517 },
+                                     { 10.357, 3.6231, 0.88026, 0.49671 } } ;
+    const double a = aT[k], al = alT[k], srs = sqrt ( rs ) ;
+    const double q0 = - 2.0 * a * ( 1.0 + al * rs ) ;
+    const double q1 = 2.0 * a * ( bT[k][0] * srs + bT[k][1] * rs + bT[k][2] * srs * rs + bT[k][3] * pow ( rs, 2.0 ) ) ;
+    const double q2 = log ( 1.0 + 1.0 / q1 ) ;
+    const double dq1 = a * ( bT[k][0] / srs + 2.0 * bT[k][1] + 3.0 * bT[k][2] * srs + 2.0 * 2.0 * bT[k][3] * pow ( rs, 1.0 ) ) ;
+    *e     = q0 * q2 ;
+    *dedrs = - 2.0 * a * al * q2 - q0 * dq1 / ( q1 * ( 1.0 + q1 ) ) ;
+}
+
+/* PBE correlation at general zeta: f = ec(rs, zeta) + H(rs, zeta, t), t = xt / (4 2^(1/3) phi sqrt(rs)). */
+static void PBECz ( double rs, double zeta, double xt, double *f, double *frs, double *fz, double *fxt )
+{
+    const double beta = 0.06672455060314922, gam = ( 1.0 - log ( 2.0 ) ) / ( M_PI * M_PI ) ;
+    const double fz20 = 1.709920934161365617563962776245 ;
+    double ecp, vcp, ecf, vcf, al, dal, fzz, dfz, z3, z4, ec, decdrs, decdz ;
+    PW92Fit ( 0, rs, &ecp, &vcp ) ;
+    PW92Fit ( 1, rs, &ecf, &vcf ) ;
+    PW92Fit ( 2, rs, &al, &dal ) ; al = - al ; dal = - dal ;
+    fzz = FZeta ( zeta ) ; dfz = DFZeta ( zeta ) ;
+    z3 = zeta * zeta * zeta ; z4 = zeta * z3 ;
+    ec     = ecp + z4 * fzz * ( ecf - ecp - al / fz20 ) + fzz * al / fz20 ;
+    decdrs = vcp + z4 * fzz * ( vcf - vcp - dal / fz20 ) + fzz * dal / fz20 ;
+    decdz  = ( 4.0 * z3 * fzz + z4 * dfz ) * ( ecf - ecp - al / fz20 ) + dfz * al / fz20 ;
+    {
+        const double tconv = 4.0 * CBRT2, auxp = Cbrt ( 1.0 + zeta ), auxm = Cbrt ( 1.0 - zeta ) ;
+        const double phi = 0.5 * ( auxp * auxp + auxm * auxm ), phi3 = pow ( phi, 3 ), srs = sqrt ( rs ) ;
+        const double t = xt / ( tconv * phi * srs ), t2 = t * t ;
+        /* eq. 8: A(ec, phi) */
+        const double e1 = ec / ( gam * phi3 ), e2 = exp ( - e1 ), e3 = e2 - 1.0 ;
+        const double Aa = beta / ( gam * e3 ), dx = Aa * e2 / e3 ;
+        const double dAdec = dx / ( gam * phi3 ), dAdphi = dx * ( - 3.0 * e1 / phi ) ;
+        /* eq. 7: H(phi, t, A) */
+        const double g1 = t2 + Aa * t2 * t2, g3 = 1.0 + Aa * g1, g2 = beta * g1 / ( gam * g3 ) ;
+        const double H = gam * phi3 * log ( 1.0 + g2 ) ;
+        const double dHdphi = 3.0 * H / phi ;
+        const double dHdt = gam * phi3 * ( beta / ( gam * g3 * g3 ) * t * ( 2.0 + 4.0 * Aa * t2 ) ) / ( 1.0 + g2 ) ;
+        const double dHdA = gam * phi3 * ( beta / ( gam * g3 * g3 ) * ( t2 * t2 - g1 * g1 ) ) / ( 1.0 + g2 ) ;
+        const double dfdphi = dHdphi + dHdA * dAdphi, dfdec = 1.0 + dHdA * dAdec ;
+        double dphidz = 0.0 ;
+        if ( auxp > XC_MIN_ZETA ) dphidz += 1 / auxp ;
+        if ( auxm > XC_MIN_ZETA ) dphidz -= 1 / auxm ;
+        dphidz *= 1.0 / 3.0 ;
+        *f   = ec + H ;
+        *frs = dfdec * decdrs + dHdt * ( - xt / ( 2.0 * tconv * phi * rs * srs ) ) ;
+        *fz  = dfdec * decdz + ( dfdphi + dHdt * ( - t / phi ) ) * dphidz ;
+        *fxt = dHdt * ( t / xt ) ;
+    }
+}
+
+/* GGA correlation, polarised (work_gga_c.i): eps, vrho (a, b), vsigma (aa, ab, bb). */
+static void CorrelationGGAPol ( int which, double ra, double rb, double saa, double sab, double sbb, double *eps,
+                                double *va, double *vb, double *vsaa, double *vsab, double *vsbb )
+{
+    const double mg2 = XC_MIN_GRAD * XC_MIN_GRAD ;
+    double dens, zeta, rs, ds[2], st, s0, s2, xt, xs[2], f, frs, fz, fxt, fxs[2] = { 0.0, 0.0 } ;
+    *eps = *va = *vb = *vsaa = *vsab = *vsbb = 0.0 ;
+    DensZeta ( ra, rb, &dens, &zeta ) ;
+    if ( dens < XC_MIN_DENS ) return ;
+    rs    = Cbrt ( 3.0 / ( 4.0 * M_PI * dens ) ) ;
+    ds[0] = ra > XC_MIN_DENS ? ra : XC_MIN_DENS ;
+    ds[1] = rb > XC_MIN_DENS ? rb : XC_MIN_DENS ;
+    st = saa + 2.0 * sab + sbb ; if ( st < mg2 ) st = mg2 ;
+    xt = sqrt ( st ) / pow ( dens, 4.0 / 3.0 ) ;
+    s0 = saa > mg2 ? saa : mg2 ;
+    s2 = sbb > mg2 ? sbb : mg2 ;
+    xs[0] = sqrt ( s0 ) / pow ( ds[0], 4.0 / 3.0 ) ;
+    xs[1] = sqrt ( s2 ) / pow ( ds[1], 4.0 / 3.0 ) ;
+    if ( which == 0 ) LYPz ( rs, zeta, xt, xs, &f, &frs, &fz, &fxt, fxs ) ;
+    else              PBECz ( rs, zeta, xt, &f, &frs, &fz, &fxt ) ;
+    {
+        const double drs = - rs / ( 3.0 * dens ), dxt = - 4.0 * xt / ( 3.0 * dens ), dxtds = xt / ( 2.0 * st ) ;
+        const double dxsdn0 = - 4.0 / 3.0 * xs[0] / ds[0], dxsdn1 = - 4.0 / 3.0 * xs[1] / ds[1] ;
+        const double dxsds0 = xs[0] / ( 2.0 * s0 ), dxsds1 = xs[1] / ( 2.0 * s2 ) ;
+        const double v0 = f + dens * ( frs * drs + fxt * dxt ), vs0 = dens * fxt * dxtds ;
+        *eps  = f ;
+        *vb   = v0 + fz * ( - ( zeta + 1.0 ) ) + dens * fxs[1] * dxsdn1 ;
+        *va   = v0 + fz * ( - ( zeta - 1.0 ) ) + dens * fxs[0] * dxsdn0 ;
+        *vsbb = vs0 + dens * fxs[1] * dxsds1 ;
+        *vsab = 2.0 * vs0 ;
+        *vsaa = vs0 + dens * fxs[0] * dxsds0 ;
+    }
+}
+
+/* eps, vrho (a, b) and vsigma (aa, ab, bb) of one functional, polarised. */
+static void FunctionalPol ( int id, double ra, double rb, double saa, double sab, double sbb, double *e, double *v,
+                            double *vs )
+{
+    v[0] = v[1] = vs[0] = vs[1] = vs[2] = 0.0 ;
+    switch ( id )
+    {
+        case 106: ExchangeGGAPol ( 0, ra, rb, saa, sbb, e, v, v + 1, vs, vs + 2 ) ; break ;
+        case 101: ExchangeGGAPol ( 1, ra, rb, saa, sbb, e, v, v + 1, vs, vs + 2 ) ; break ;
+        case 131: CorrelationGGAPol ( 0, ra, rb, saa, sab, sbb, e, v, v + 1, vs, vs + 1, vs + 2 ) ; break ;
+        case 130: CorrelationGGAPol ( 1, ra, rb, saa, sab, sbb, e, v, v + 1, vs, vs + 1, vs + 2 ) ; break ;
+        case   1: SlaterPol ( ra, rb, e, v, v + 1 ) ; break ;
+        case   7: VWNPol ( 0, 0, ra, rb, e, v, v + 1 ) ; break ;
+        case   8: VWNPol ( 1, 0, ra, rb, e, v, v + 1 ) ; break ;
+        case 402:
+        {
+            static const int parts[4] = { 1, 106, 8, 131 } ;
+            static const double coef[4] = { 1.0 - 0.20 - 0.72, 0.72, 1.0 - 0.81, 0.81 } ;
+            double pe, pv[2], ps[3] ;
+            *e = 0.0 ;
+            for ( int k = 0 ; k < 4 ; k++ )
+            {
+                ps[0] = ps[1] = ps[2] = 0.0 ;
+                if ( k == 2 ) VWNPol ( 1, 1, ra, rb, &pe, pv, pv + 1 ) ;
+                else          FunctionalPol ( parts[k], ra, rb, saa, sab, sbb, &pe, pv, ps ) ;
+                *e += coef[k] * pe ; v[0] += coef[k] * pv[0] ; v[1] += coef[k] * pv[1] ;
+                if ( ! IsLDA ( parts[k] ) ) { vs[0] += coef[k] * ps[0] ; vs[1] += coef[k] * ps[1] ; vs[2] += coef[k] * ps[2] ; }
+            }
+            break ;
+        }
+    }
+}
+
+/* Polarised evaluation for n points: rho [n][2], sigma [n][3] -> eps [n], vrho [n][2], vsigma [n][3], summed over the
+   functionals (DFTFunctionalModel.c:108-154 with an unrestricted model). */
+int xqo_functional_pol ( int nIDs, const int *ids, int n, const double *rho, const double *sigma, double *eps, double *vr,
+                         double *vs )
+{
+    int nLDA = 0 ;
+    for ( int f = 0 ; f < nIDs ; f++ ) { if ( ! ValidID ( ids[f] ) ) return 3 ; nLDA += IsLDA ( ids[f] ) ; }
+    if ( nLDA > 0 && nLDA < nIDs ) return 3 ;
+    for ( int p = 0 ; p < n ; p++ )
+    {
+        double e = 0.0, a[2] = { 0.0, 0.0 }, b[3] = { 0.0, 0.0, 0.0 } ;
+        for ( int f = 0 ; f < nIDs ; f++ )
+        {
+            double ed, v[2], s[3] ;
+            FunctionalPol ( ids[f], rho[2*p], rho[2*p+1], sigma[3*p], sigma[3*p+1], sigma[3*p+2], &ed, v, s ) ;
+            e += ed ; a[0] += v[0] ; a[1] += v[1] ; b[0] += s[0] ; b[1] += s[1] ; b[2] += s[2] ;
+        }
+        eps[p] = e ; vr[2*p] = a[0] ; vr[2*p+1] = a[1] ; vs[3*p] = b[0] ; vs[3*p+1] = b[1] ; vs[3*p+2] = b[2] ;
+    }
+    return 0 ;
+}
+
 /* eps (per particle, summed over functionals), vrho and vsigma for n points (DFTFunctionalModel.c:108-154). */
 int xqo_functional ( int nIDs, const int *ids, int n, const double *rho, const double *sigma, double *eps, double *vr, double *vs )
 {

@@ -10,6 +10,7 @@ them through oracle/ref_driver.c and stores inputs and outputs only (numbers, no
                             (H2O)16 (9 A cube, seed 1, the benchmark cluster): grid size, E_xc and N_e (PBE, Cartesian).
   reference_functionals.json  libxc-1.2.0 (as vendored) eps, vrho, vsigma of BLYP, PBE, their parts, Slater exchange, VWN
                             (both fits), LDA and B3LYP on bulk and threshold (rho, sigma).
+  reference_functionals_pol.json  the same functionals spin-polarised (rho_a, rho_b; s_aa, s_ab, s_bb).
   reference_c2s.json        the reference's own Cartesian -> spherical fixture (examples/pMolecule/data/yaml/
                             C2STransformations_0to6.yaml, read with SafeLoader) converted to JSON.
 
@@ -113,6 +114,24 @@ def functionals():
         json.dump(out, f)
 
 
+def functionals_pol():
+    """Spin-polarised values (libxc with nspin = 2): bulk, one spin empty or below threshold, fully polarised, negative
+    s_ab."""
+    ref = oracle.reference()
+    rho = [[0.3, 0.1], [1e-3, 2e-3], [2.0, 2.0], [0.5, 0.0], [1e-13, 0.3], [0.7, 1e-14], [0.2, 0.2], [5e-13, 5e-13],
+           [0.4, 0.4], [10.0, 3.0], [1e-6, 3e-6], [0.25, 0.25]]
+    sigma = [[0.01, 0.004, 0.003], [1e-8, -1e-8, 3e-8], [3.0, 2.5, 3.0], [0.2, 0.0, 0.0], [0.0, 0.0, 0.1],
+             [0.3, 0.0, 0.0], [1e-30, 0.0, 1e-30], [1e-20, 0.0, 1e-20], [0.5, 0.5, 0.5], [40.0, 15.0, 9.0],
+             [1e-12, 1e-12, 2e-12], [0.0, 0.0, 0.0]]
+    out = {"rho": rho, "sigma": sigma}
+    for name, ids in (("blyp", BLYP), ("pbe", PBE), ("b88", [106]), ("lyp", [131]), ("pbex", [101]), ("pbec", [130]),
+                      ("slater", [1]), ("vwn", [7]), ("vwnrpa", [8]), ("lda", [1, 7]), ("b3lyp", [402])):
+        e, vr, vs = ref.functional_pol(ids, np.asarray(rho), np.asarray(sigma))
+        out[name] = {"ids": ids, "eps": e.tolist(), "vrho": vr.tolist(), "vsigma": vs.tolist()}
+    with open(os.path.join(HERE, "reference_functionals_pol.json"), "w") as f:
+        json.dump(out, f)
+
+
 def c2s():
     import yaml
     path = os.path.join(REFERENCE, "examples", "pMolecule", "data", "yaml", "C2STransformations_0to6.yaml")
@@ -129,6 +148,7 @@ def c2s():
 
 if __name__ == "__main__":
     functionals()
+    functionals_pol()
     c2s()
     water()
     cluster()

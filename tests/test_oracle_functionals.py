@@ -79,3 +79,53 @@ def test_model_rules():
     assert exact_exchange_scaling(functional_ids("b3lyp")) == 0.20
     assert exact_exchange_scaling(functional_ids("blyp")) == 0.0
     assert exact_exchange_scaling(functional_ids("lda")) == 0.0
+
+
+# ------------------------------------------------------------------------------------------------------------------------
+# Spin-polarised forms (the functionals of unrestricted models).
+# ------------------------------------------------------------------------------------------------------------------------
+POL_NAMES = ["blyp", "pbe", "b88", "lyp", "pbex", "pbec", "slater", "vwn", "vwnrpa", "lda", "b3lyp"]
+
+
+@pytest.mark.parametrize("name", POL_NAMES)
+def test_polarised_match_reference_fixture(name):
+    import json, os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_functionals_pol.json")) as f:
+        d = json.load(f)
+    ref = d[name]
+    e, vr, vs = _oracle().functional_pol(ref["ids"], np.asarray(d["rho"]), np.asarray(d["sigma"]))
+    for mine, key in ((e, "eps"), (vr, "vrho"), (vs, "vsigma")):
+        r = np.asarray(ref[key])
+        assert np.allclose(mine, r, rtol=1e-13, atol=1e-300), (key, mine, r)
+
+
+@pytest.mark.parametrize("name", POL_NAMES)
+def test_polarised_equal_spins_is_unpolarised(name):
+    lib = _oracle()
+    ids = {"b88": [106], "lyp": [131], "pbex": [101], "pbec": [130], "slater": [1], "vwn": [7],
+           "vwnrpa": [8]}.get(name) or list(functional_ids(name))
+    rho = np.array([1e-4, 3e-3, 0.05, 0.7, 4.0, 60.0])
+    sigma = np.array([1e-9, 1e-6, 2e-3, 0.4, 9.0, 500.0])
+    e, vr, vs = lib.functional(ids, rho, sigma)
+    ep, vrp, vsp = lib.functional_pol(ids, np.stack([rho / 2, rho / 2], 1), np.stack([sigma / 4] * 3, 1))
+    assert np.allclose(ep, e, rtol=1e-12, atol=0.0)
+    assert np.allclose(vrp[:, 0], vr, rtol=1e-12, atol=0.0) and np.allclose(vrp[:, 1], vr, rtol=1e-12, atol=0.0)
+    assert np.allclose(vsp.sum(1) / 4.0, vs, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("ids", [[106, 131], [101, 130], [1, 7], [402]])
+def test_polarised_finite_differences(ids):
+    lib = _oracle()
+    rho = np.array([[0.3, 0.1], [0.02, 0.05], [2.0, 1.2], [0.6, 0.6]])
+    sigma = np.array([[0.01, 0.004, 0.003], [1e-4, 2e-5, 3e-4], [3.0, 2.0, 2.5], [0.2, 0.1, 0.2]])
+    e, vr, vs = lib.functional_pol(ids, rho, sigma)
+    h = 1e-6
+    E = lambda r, s: r.sum(1) * lib.functional_pol(ids, r, s)[0]
+    for k in range(2):
+        dr = np.zeros_like(rho); dr[:, k] = h * rho[:, k]
+        fd = (E(rho + dr, sigma) - E(rho - dr, sigma)) / (2.0 * dr[:, k])
+        assert np.allclose(vr[:, k], fd, rtol=1e-6, atol=1e-10), (k, vr[:, k], fd)
+    for k in range(3):
+        ds = np.zeros_like(sigma); ds[:, k] = h * sigma[:, k]
+        fd = (E(rho, sigma + ds) - E(rho, sigma - ds)) / (2.0 * ds[:, k])
+        assert np.allclose(vs[:, k], fd, rtol=1e-5, atol=1e-10), (k, vs[:, k], fd)
