@@ -58,6 +58,8 @@ class _Library:
         f("block_values").argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
         f("functional").restype = ctypes.c_int
         f("functional").argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        f("integrate_uks").restype = ctypes.c_int
+        f("integrate_uks").argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         f("functional_pol").restype = ctypes.c_int
         f("functional_pol").argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         f("c2s").restype = None
@@ -138,6 +140,20 @@ class _Handle:
         if st != 0:
             raise ValueError("integrate status {:d}".format(st))
         return float(e[0]), float(ne[0]), V, float(sec[0])
+
+    def integrate_uks(self, Pa_packed, Pb_packed, threads=1):
+        """Unrestricted: E_xc, N_e, V_alpha, V_beta (packed, from zero) and the call's wall time."""
+        n = self.nfunctions
+        Pa = np.ascontiguousarray(Pa_packed, dtype=np.float64)
+        Pb = np.ascontiguousarray(Pb_packed, dtype=np.float64)
+        if Pa.size != n * (n + 1) // 2 or Pb.size != Pa.size:
+            raise ValueError("P has {:d} entries, expected {:d}".format(Pa.size, n * (n + 1) // 2))
+        Va, Vb = np.zeros_like(Pa), np.zeros_like(Pb)
+        e, ne, sec = np.zeros(1), np.zeros(1), np.zeros(1)
+        st = self.lib._fn("integrate_uks")(self.h, _d(Pa), _d(Pb), int(threads), _d(e), _d(ne), _d(Va), _d(Vb), _d(sec))
+        if st != 0:
+            raise ValueError("integrate_uks status {:d}".format(st))
+        return float(e[0]), float(ne[0]), Va, Vb, float(sec[0])
 
     def gradients(self, P_packed, threads=1):
         """XC nuclear gradients (nAtoms x 3, hartree/bohr) of E_xc at the density P (the oracle's threads, or the reference's

@@ -38,6 +38,7 @@ typedef struct {
     GaussianBasis         **unique      ;
     Integer                 nUnique     ;
     DFTFunctionalModel     *functional  ;
+    DFTFunctionalModel     *functionalU ;   /* the same IDs, spin-polarised (unrestricted models) */
     IntegerArray1D         *cfp, *fc    ;
 } RefHandle ;
 
@@ -120,7 +121,8 @@ void *xcref_setup ( int accuracy, int nAtoms, const int *Z, const double *xyzBoh
     {
         IntegerArray1D *idArray = IntegerArray1D_AllocateWithExtent ( nIDs, &st ) ;
         for ( i = 0 ; i < nIDs ; i++ ) Array1D_Item ( idArray, i ) = ids[i] ;
-        h->functional = DFTFunctionalModel_MakeFromIDs ( idArray, True, &st ) ;
+        h->functional  = DFTFunctionalModel_MakeFromIDs ( idArray, True , &st ) ;
+        h->functionalU = DFTFunctionalModel_MakeFromIDs ( idArray, False, &st ) ;
         IntegerArray1D_Deallocate ( &idArray ) ;
     }
     /* . Grid. */
@@ -178,6 +180,29 @@ int xcref_integrate ( void *vh, const double *P, int inCore, double *eXC, double
     if ( seconds != NULL ) *seconds = ( t1.tv_sec - t0.tv_sec ) + 1.0e-9 * ( t1.tv_nsec - t0.tv_nsec ) ;
     SymmetricMatrix_Deallocate ( &dP ) ;
     SymmetricMatrix_Deallocate ( &fA ) ;
+    return ( int ) st ;
+}
+
+/* . One unrestricted quadrature (DFTIntegrator.pyx:82-107: alpha and beta densities, alpha and beta Fock matrices): E_xc and
+!    N_e; Va and Vb (packed) are incremented in place. */
+int xcref_integrate_uks ( void *vh, const double *Pa, const double *Pb, int inCore, double *eXC, double *rho, double *Va,
+                          double *Vb, double *seconds )
+{
+    RefHandle *h = ( RefHandle * ) vh ;
+    Status st = Status_OK ;
+    int n = h->nFunctions, k, size = ( n * ( n + 1 ) ) / 2 ;
+    struct timespec t0, t1 ;
+    SymmetricMatrix *dP = SymmetricMatrix_AllocateWithExtent ( n, &st ), *dQ = SymmetricMatrix_AllocateWithExtent ( n, &st ) ;
+    SymmetricMatrix *fA = SymmetricMatrix_AllocateWithExtent ( n, &st ), *fB = SymmetricMatrix_AllocateWithExtent ( n, &st ) ;
+    for ( k = 0 ; k < size ; k++ ) { dP->data[k] = Pa[k] ; dQ->data[k] = Pb[k] ; fA->data[k] = Va[k] ; fB->data[k] = Vb[k] ; }
+    clock_gettime ( CLOCK_MONOTONIC, &t0 ) ;
+    DFTIntegrator_Integrate ( h->functionalU, h->grid, h->bases, h->coordinates, dP, dQ, ( inCore ? True : False ), True,
+                              eXC, rho, fA, fB, NULL, &st ) ;
+    clock_gettime ( CLOCK_MONOTONIC, &t1 ) ;
+    for ( k = 0 ; k < size ; k++ ) { Va[k] = fA->data[k] ; Vb[k] = fB->data[k] ; }
+    if ( seconds != NULL ) *seconds = ( t1.tv_sec - t0.tv_sec ) + 1.0e-9 * ( t1.tv_nsec - t0.tv_nsec ) ;
+    SymmetricMatrix_Deallocate ( &dP ) ; SymmetricMatrix_Deallocate ( &dQ ) ;
+    SymmetricMatrix_Deallocate ( &fA ) ; SymmetricMatrix_Deallocate ( &fB ) ;
     return ( int ) st ;
 }
 
@@ -306,6 +331,7 @@ void xcref_free ( void *vh )
     if ( h == NULL ) return ;
     DFTGrid_Deallocate ( &h->grid, &st ) ;
     DFTFunctionalModel_Deallocate ( &h->functional ) ;
+    DFTFunctionalModel_Deallocate ( &h->functionalU ) ;
     for ( u = 0 ; u < h->nUnique ; u++ ) GaussianBasis_Deallocate ( &h->unique[u] ) ;
     free ( h->unique ) ;
     free ( h->bases->entries ) ;

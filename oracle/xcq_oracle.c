@@ -1291,6 +1291,144 @@ int xqo_integrate ( void *vh, const double *Ppacked, int nThreads, double *eXC, 
 }
 
 /*----------------------------------------------------------------------------------------------------------------------------------
+! Unrestricted quadrature (DFTIntegrator.c:281-402 with isSpinUnrestricted; DFTIntegrator.pyx:82-107): alpha and beta densities
+! Pa, Pb give rho_s and grad rho_s from P_s,red Phi as in the restricted case; sigma = (|grad rho_a|^2, grad rho_a . grad rho_b,
+! |grad rho_b|^2); a block is skipped when no |rho_s| exceeds rhoTol; eXC (rho_a + rho_b) w and (rho_a + rho_b) w are summed;
+! V_s = Phi diag(w vrho_s) Phi^T + Phi Z_s^T + Z_s Phi^T with Z_s = 2 w vsigma_ss (grad rho_s . grad Phi)
+! + w vsigma_ab (grad rho_t . grad Phi), t the other spin (DFTIntegrator_Fock and _FockSigma, :517-600).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+int xqo_integrate_uks ( void *vh, const double *Pa, const double *Pb, int nThreads, double *eXC, double *nE, double *Va,
+                        double *Vb, double *seconds )
+{
+    OHandle *h = ( OHandle * ) vh ;
+    int nF = h->nFunctions ;
+    size_t nPacked = ( ( size_t ) nF * ( nF + 1 ) ) / 2 ;
+    struct timespec t0, t1 ;
+    clock_gettime ( CLOCK_MONOTONIC, &t0 ) ;
+    if ( nThreads < 1 ) nThreads = 1 ;
+# ifdef _OPENMP
+    omp_set_num_threads ( nThreads ) ;
+# endif
+    double *Vt = ( double * ) calloc ( 2 * ( size_t ) nThreads * nPacked, sizeof ( double ) ) ;
+    double *eT = ( double * ) calloc ( nThreads, sizeof ( double ) ), *nT = ( double * ) calloc ( nThreads, sizeof ( double ) ) ;
+    int status = 0 ;
+    # pragma omp parallel num_threads(nThreads)
+    {
+        int tid = 0 ;
+# ifdef _OPENMP
+        tid = omp_get_thread_num ( ) ;
+# endif
+        double *V[2] = { Vt + ( 2 * ( size_t ) tid ) * nPacked, Vt + ( 2 * ( size_t ) tid + 1 ) * nPacked } ;
+        const double *P[2] = { Pa, Pb } ;
+        double *buf = ( double * ) malloc ( 4 * ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *X   = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *Z   = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *Pr  = ( double * ) malloc ( ( size_t ) nF * nF * sizeof ( double ) ) ;
+        int    *idx = ( int * ) malloc ( nF * sizeof ( int ) ) ;
+        double rho[2][BLOCK_SIZE], g[2][3][BLOCK_SIZE], rr[2 * BLOCK_SIZE], ss[3 * BLOCK_SIZE], eps[BLOCK_SIZE] ;
+        double vr[2 * BLOCK_SIZE], vs[3 * BLOCK_SIZE] ;
+        # pragma omp for schedule(dynamic,1)
+        for ( int b = 0 ; b < h->nBlocks ; b++ )
+        {
+            int nP = h->blockCount[b] ;
+            const double *pts = h->points + 4 * ( size_t ) h->blockStart[b] ;
+            double *f = buf, *fx = buf + ( size_t ) nF * nP, *fy = buf + 2 * ( size_t ) nF * nP, *fz = buf + 3 * ( size_t ) nF * nP ;
+            EvaluateBasis ( h, nP, pts, f, fx, fy, fz ) ;
+            int nb = FilterRows ( nF, nP, h->bfTol, idx, f, fx, fy, fz ) ;
+            if ( nb <= 0 ) continue ;
+            double mx = 0.0 ;
+            for ( int s = 0 ; s < 2 ; s++ )
+            {
+                for ( int i = 0 ; i < nb ; i++ )
+                    for ( int j = 0 ; j <= i ; j++ )
+                    {
+                        int m = idx[i], n = idx[j] ;
+                        double v = P[s][( ( size_t ) m * ( m + 1 ) ) / 2 + n] ;
+                        Pr[i * nb + j] = v ; Pr[j * nb + i] = v ;
+                    }
+                for ( int i = 0 ; i < nb ; i++ )
+                    for ( int p = 0 ; p < nP ; p++ )
+                    {
+                        double t = 0.0 ;
+                        for ( int k = 0 ; k < nb ; k++ ) t += Pr[i * nb + k] * f[k * nP + p] ;
+                        X[i * nP + p] = t ;
+                    }
+                for ( int p = 0 ; p < nP ; p++ )
+                {
+                    double r = 0.0, ax = 0.0, ay = 0.0, az = 0.0 ;
+                    for ( int i = 0 ; i < nb ; i++ )
+                    {
+                        double xv = X[i * nP + p] ;
+                        r  += f [i * nP + p] * xv ;
+                        ax += fx[i * nP + p] * xv ;
+                        ay += fy[i * nP + p] * xv ;
+                        az += fz[i * nP + p] * xv ;
+                    }
+                    rho[s][p] = r ; g[s][0][p] = 2.0 * ax ; g[s][1][p] = 2.0 * ay ; g[s][2][p] = 2.0 * az ;
+                    if ( fabs ( r ) > mx ) mx = fabs ( r ) ;
+                }
+            }
+            if ( mx <= h->rhoTol ) continue ;
+            for ( int p = 0 ; p < nP ; p++ )
+            {
+                rr[2 * p] = rho[0][p] ; rr[2 * p + 1] = rho[1][p] ;
+                ss[3 * p]     = g[0][0][p] * g[0][0][p] + g[0][1][p] * g[0][1][p] + g[0][2][p] * g[0][2][p] ;
+                ss[3 * p + 1] = g[0][0][p] * g[1][0][p] + g[0][1][p] * g[1][1][p] + g[0][2][p] * g[1][2][p] ;
+                ss[3 * p + 2] = g[1][0][p] * g[1][0][p] + g[1][1][p] * g[1][1][p] + g[1][2][p] * g[1][2][p] ;
+            }
+            if ( xqo_functional_pol ( h->nIDs, h->ids, nP, rr, ss, eps, vr, vs ) != 0 ) { status = 3 ; continue ; }
+            for ( int p = 0 ; p < nP ; p++ )
+            {
+                double w = pts[4 * p + 3], rt = rho[0][p] + rho[1][p] ;
+                eT[tid] += eps[p] * rt * w ;
+                nT[tid] += rt * w ;
+            }
+            for ( int s = 0 ; s < 2 ; s++ )
+            {
+                const int o = 1 - s ;
+                for ( int i = 0 ; i < nb ; i++ )
+                    for ( int p = 0 ; p < nP ; p++ )
+                    {
+                        double w = pts[4 * p + 3] ;
+                        double gs = fx[i * nP + p] * g[s][0][p] + fy[i * nP + p] * g[s][1][p] + fz[i * nP + p] * g[s][2][p] ;
+                        double go = fx[i * nP + p] * g[o][0][p] + fy[i * nP + p] * g[o][1][p] + fz[i * nP + p] * g[o][2][p] ;
+                        Z[i * nP + p] = 2.0 * w * vs[3 * p + 2 * s] * gs + w * vs[3 * p + 1] * go ;
+                    }
+                for ( int i = 0 ; i < nb ; i++ )
+                {
+                    int m = idx[i] ;
+                    for ( int j = 0 ; j <= i ; j++ )
+                    {
+                        int n = idx[j] ;
+                        double t = 0.0 ;
+                        for ( int p = 0 ; p < nP ; p++ )
+                            t += f[i * nP + p] * ( vr[2 * p + s] * pts[4 * p + 3] ) * f[j * nP + p] + f[i * nP + p] * Z[j * nP + p]
+                               + Z[i * nP + p] * f[j * nP + p] ;
+                        V[s][( ( size_t ) m * ( m + 1 ) ) / 2 + n] += t ;
+                    }
+                }
+            }
+        }
+        free ( buf ) ; free ( X ) ; free ( Z ) ; free ( Pr ) ; free ( idx ) ;
+    }
+    double eTot = 0.0, nTot = 0.0 ;
+    for ( int t = 0 ; t < nThreads ; t++ )
+    {
+        eTot += eT[t] ; nTot += nT[t] ;
+        for ( size_t k = 0 ; k < nPacked ; k++ )
+        {
+            Va[k] += Vt[( 2 * ( size_t ) t ) * nPacked + k] ;
+            Vb[k] += Vt[( 2 * ( size_t ) t + 1 ) * nPacked + k] ;
+        }
+    }
+    free ( Vt ) ; free ( eT ) ; free ( nT ) ;
+    clock_gettime ( CLOCK_MONOTONIC, &t1 ) ;
+    *eXC = eTot ; *nE = nTot ;
+    if ( seconds != NULL ) *seconds = ( t1.tv_sec - t0.tv_sec ) + 1.0e-9 * ( t1.tv_nsec - t0.tv_nsec ) ;
+    return status ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
 ! XC nuclear gradients (DFTIntegrator.c:159-509 with doGradients, restricted closed shell, GGA).  Per block, with the basis values,
 ! gradients and second derivatives (order + 1, :203-207) after the |phi| > bfTol filter:
 !   - rho and its gradient as in the quadrature; skip the block if max rho <= rhoTol; vrho, vsigma *= w; eXC = eps rho (:332-350);
