@@ -9,6 +9,7 @@
 !   xq_basis_set         <- the GaussianBasisContainer argument of DFTIntegrator_Integrate, built as
 !                           GaussianBasis.pyx:114-137 + GaussianBasis_Finalize (GaussianBasis.c:391-398) build it
 !   xq_integrate         <- DFTIntegrator_Integrate   (DFTIntegrator.h:55-68; DFTIntegrator.c:159-509), restricted
+!   xq_integrate_uks     <- DFTIntegrator_Integrate   (the same, unrestricted: alpha and beta densities and Fock matrices)
 !                           closed shell, Fock only (gradients3 = NULL)
 !   xq_estimate_bytes    <- DFTGridIntegrator.EstimateMemory (DFTGridIntegrator.py:31-39), for this library's HBM layout
 !   xq_free              <- DFTGrid_Deallocate        (DFTGrid.h:63-64)
@@ -64,6 +65,15 @@ int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, 
 # define XQ_NSTATS 8
 int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
                                   double *nElectrons, double *fockV, double *stats ) ;
+
+/* One unrestricted quadrature: DFTIntegrator_Integrate with densityQ, fockB and isSpinUnrestricted (DFTIntegrator.c:159-509,
+!  the spin branches :301-330, :342-347, :366-402), as DFTIntegrator.pyx:82-107 calls it with alpha and beta matrices (the
+!  caller converts total and spin matrices: alpha = (T + S)/2, beta = (T - S)/2; F_T += (F_a + F_b)/2, F_S += (F_a - F_b)/2).
+!  densityA, densityB: packed alpha and beta densities; fockA, fockB: packed, INCREMENTED by V_xc of each spin; eXC and
+!  nElectrons as xq_integrate; stats as xq_integrate ([3], [4], [6], [7] are 0).  The functionals are evaluated
+!  spin-polarised (libxc with nspin = 2). */
+int         xq_integrate_uks    ( xq_grid *grid, int nIDs, const int *ids, const double *densityA, const double *densityB,
+                                  double *eXC, double *nElectrons, double *fockA, double *fockB, double *stats ) ;
 
 /* XC nuclear gradients: DFTIntegrator_Integrate with a gradients3 argument and no Fock matrix (DFTIntegrator.c:159-509, the
 !  doGradients branch :405-487 with DFTGridWeights_Derivatives, DFTGridWeights.c:98-215), as DFTGridIntegrator.Gradients

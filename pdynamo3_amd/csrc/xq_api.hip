@@ -98,6 +98,19 @@ int xq_basis_set(xq_grid *grid, int nSpec, const double *spec, int spherical, in
     return status;
 }
 
+int xq_integrate_uks(xq_grid *grid, int nIDs, const int *ids, const double *densityA, const double *densityB,
+                     double *eXC, double *nElectrons, double *fockA, double *fockB, double *stats) {
+    if (grid == nullptr || ids == nullptr || densityA == nullptr || densityB == nullptr || eXC == nullptr ||
+        nElectrons == nullptr || fockA == nullptr || fockB == nullptr) {
+        set_error("invalid integrate arguments");
+        return kInvalidArgument;
+    }
+    *eXC = 0.0;
+    *nElectrons = 0.0;
+    if (!grid->hasBasis) { set_error("no basis set on this grid"); return kInvalidArgument; }
+    return integrate_uks(grid, nIDs, ids, densityA, densityB, eXC, nElectrons, fockA, fockB, stats);
+}
+
 int xq_integrate(xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC, double *nElectrons,
                  double *fockV, double *stats) {
     if (grid == nullptr || ids == nullptr || densityP == nullptr || eXC == nullptr || nElectrons == nullptr ||

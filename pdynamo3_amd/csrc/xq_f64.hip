@@ -99,6 +99,8 @@ density_kernel_f64(const MixedJob *__restrict__ jobs, int nJobs, int groupsPerXc
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, l16 = lane & 15, l4 = lane >> 4;
     if (nb <= 0) {
         if (t == 0) { J.blockEN[2 * b] = 0.0; J.blockEN[2 * b + 1] = 0.0; J.skip[b] = 1; }
+        if (J.rhoOut != nullptr && t < np)
+            for (int c = 0; c < 4; c++) J.rhoOut[4 * (static_cast<size_t>(blk.pStart) + t) + c] = 0.0;
         return;
     }
     // The block's kept-function indices, cached in LDS for the P_red gathers (blocks with more kept functions than the
@@ -203,6 +205,13 @@ density_kernel_f64(const MixedJob *__restrict__ jobs, int nJobs, int groupsPerXc
     __syncthreads();
     double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
     if (t < np) { rho = rsum[0][t]; gx = 2.0 * rsum[1][t]; gy = 2.0 * rsum[2][t]; gz = 2.0 * rsum[3][t]; }
+    if (J.rhoOut != nullptr) {   // one spin of an unrestricted quadrature: the functional follows in its own kernel
+        if (t < np) {
+            double *o = J.rhoOut + 4 * (static_cast<size_t>(blk.pStart) + t);
+            o[0] = rho; o[1] = gx; o[2] = gy; o[3] = gz;
+        }
+        return;
+    }
     double mx = t < np ? fabs(rho) : 0.0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
@@ -355,7 +364,7 @@ void free_f64(xq_grid *g) {
 }
 
 int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-               MixedJob *h_table, MixedJob *d_table) {
+               MixedJob *h_table, MixedJob *d_table, int stages) {
     long long rows, blocks, chunks, groups;
     fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups);
     if (blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
@@ -364,13 +373,13 @@ int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *
     }
     XQ_CHECK(hipMemcpyAsync(d_table, h_table, nJobs * sizeof(MixedJob), hipMemcpyHostToDevice, s));
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
-    if (blocks > 0) {
+    if ((stages & kStageDensity) && blocks > 0) {
         const int perXcd = static_cast<int>((blocks + 7) / 8);
         hipLaunchKernelGGL(density_kernel_f64, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,
                            perXcd, static_cast<int>(blocks));
     }
     XQ_CHECK(hipEventRecord(ev.evRho1, s));
-    if (chunks > 0) {
+    if ((stages & kStageFock) && chunks > 0) {
         hipLaunchKernelGGL(w_kernel_f64, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, d_table, nJobs);
         const int perXcd = static_cast<int>((groups + 7) / 8);
         hipLaunchKernelGGL(fock_kernel_f64, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,

@@ -56,6 +56,27 @@ __global__ void zero_kernel(size_t n, double *__restrict__ a) {
 }  // namespace
 
 // Dense n x n P on the device from a packed lower triangle on the host (through the handle's pinned staging buffer).
+// The handle's packed matrix buffer to its dense P, its dense V to zero, its dense V to the packed buffer.
+void launch_expand_P(xq_grid *g, hipStream_t s) {
+    const size_t n = static_cast<size_t>(g->nFunctions);
+    const int grid = static_cast<int>(std::min<size_t>((n * n + 255) / 256, 16384));
+    hipLaunchKernelGGL(expand_packed_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, g->nFunctions, g->d_packed,
+                       g->d_P);
+}
+
+void launch_zero_V(xq_grid *g, hipStream_t s) {
+    const size_t n = static_cast<size_t>(g->nFunctions);
+    const int grid = static_cast<int>(std::min<size_t>((n * n + 255) / 256, 16384));
+    hipLaunchKernelGGL(zero_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, n * n, g->d_V);
+}
+
+void launch_pack_V(xq_grid *g, hipStream_t s) {
+    const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
+    const int grid = static_cast<int>(std::min<size_t>((nPacked + 255) / 256, 16384));
+    hipLaunchKernelGGL(pack_lower_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, g->nFunctions, g->d_V,
+                       g->d_packed);
+}
+
 int upload_density(xq_grid *g, const double *Ppacked, hipStream_t s) {
     XQ_TRY(ensure_work(g));
     const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;

@@ -61,6 +61,7 @@ struct MixedJob {
     int *eP;
     unsigned long long *Pq;
     double *wcoef, *blockEN, *V;
+    double *rhoOut;                    // when set, the density kernels store (rho, grad rho) per point here and stop
     int *skip;
     signed char *DW, *EW;
     const double *FPB, *GPB, *FPA;     // FP64 path: Phi and grad Phi in the density's operand order, Phi in the Fock's
@@ -119,6 +120,7 @@ struct xq_grid {
     unsigned long long *d_Pq = nullptr;
     int *d_eP = nullptr;
     double *d_wcoef = nullptr;
+    double *d_rhoOut = nullptr;        // set only while an unrestricted quadrature runs its density stage
     long long nChunks = 0, nTiles = 0;
     // Mixed path, Fock side (xq_mixed.hip): superblock table, union indices, union chunk tables, tiles, grad Phi in
     // the union layout; the union row map is kept only during the set-up.
@@ -201,10 +203,17 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
                    const double *const *P, double *eXC, double *nE, double *const *V, double *stats, MixedJob *h_table,
                    MixedJob *d_table);
 int functional_set(int nIDs, const int *ids, FunctionalSet *fs);
+// stages: kStageDensity (P digits and the density kernel), kStageFock (W and the Fock product), or both.
+constexpr int kStageDensity = 1, kStageFock = 2;
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-                 MixedJob *h_table, MixedJob *d_table);
+                 MixedJob *h_table, MixedJob *d_table, int stages = kStageDensity | kStageFock);
 int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-               MixedJob *h_table, MixedJob *d_table);
+               MixedJob *h_table, MixedJob *d_table, int stages = kStageDensity | kStageFock);
+int integrate_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const double *Pb, double *eXC, double *nE,
+                  double *Va, double *Vb, double *stats);
+void launch_expand_P(xq_grid *g, hipStream_t s);
+void launch_zero_V(xq_grid *g, hipStream_t s);
+void launch_pack_V(xq_grid *g, hipStream_t s);
 // Shared by both precisions: the job table entries of the given handles, the per-block chunk tables and Fock tile list.
 void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rows,
                long long *blocks, long long *chunks, long long *groups);

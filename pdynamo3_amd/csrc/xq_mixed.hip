@@ -364,6 +364,8 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
 // E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
 // point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  P_red is staged in LDS by row panels shared by
 // the four waves; the Phi operands are fetched one step ahead.
+// kRhoOut: the density stage of one spin of an unrestricted quadrature (the job's rhoOut receives rho and grad rho).
+template <bool kRhoOut>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
@@ -399,6 +401,8 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     const int lr = lane & 31, lh = lane >> 5;
     if (nb <= 0) {
         if (t == 0) { blockEN[2 * b] = 0.0; blockEN[2 * b + 1] = 0.0; skip[b] = 1; }
+        if (kRhoOut && t < np)
+            for (int c = 0; c < 4; c++) J.rhoOut[4 * (static_cast<size_t>(blk.pStart) + t) + c] = 0.0;
         return;
     }
     // The block's kept-function indices, cached in LDS for the P_red gathers (blocks with more kept functions than the
@@ -514,6 +518,13 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     // Functional, E and N, W coefficients.
     double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
     if (t < np) { rho = rsum[0][t]; gx = 2.0 * rsum[1][t]; gy = 2.0 * rsum[2][t]; gz = 2.0 * rsum[3][t]; }
+    if (kRhoOut) {   // one spin of an unrestricted quadrature: the functional follows in its own kernel
+        if (t < np) {
+            double *o = J.rhoOut + 4 * (static_cast<size_t>(blk.pStart) + t);
+            o[0] = rho; o[1] = gx; o[2] = gy; o[3] = gz;
+        }
+        return;
+    }
     double mx = t < np ? fabs(rho) : 0.0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
@@ -920,6 +931,7 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
         J.eP = g->d_eP;
         J.Pq = g->d_Pq;
         J.wcoef = g->d_wcoef;
+        J.rhoOut = g->d_rhoOut;
         J.blockEN = g->d_blockEN;
         J.V = g->d_V;
         J.skip = g->d_skip;
@@ -962,7 +974,7 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
 // The four stage launches of the mixed path, each one launch over all the jobs.  h_table (pinned) and d_table hold at
 // least nJobs entries; the caller synchronises the stream before h_table is written again.
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-                 MixedJob *h_table, MixedJob *d_table) {
+                 MixedJob *h_table, MixedJob *d_table, int stages) {
     long long rows, blocks, chunks, groups;
     fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups);
     if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
@@ -970,15 +982,21 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
         return kInvalidArgument;
     }
     XQ_CHECK(hipMemcpyAsync(d_table, h_table, nJobs * sizeof(MixedJob), hipMemcpyHostToDevice, s));
-    if (rows > 0) hipLaunchKernelGGL(p_slice_kernel, dim3(static_cast<unsigned>(rows)), dim3(256), 0, s, d_table, nJobs);
+    if ((stages & kStageDensity) && rows > 0)
+        hipLaunchKernelGGL(p_slice_kernel, dim3(static_cast<unsigned>(rows)), dim3(256), 0, s, d_table, nJobs);
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
-    if (blocks > 0) {
+    if ((stages & kStageDensity) && blocks > 0) {
         const int perXcd = static_cast<int>((blocks + 7) / 8);
-        hipLaunchKernelGGL(density_kernel_mixed, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,
-                           perXcd, static_cast<int>(blocks));
+        if (h_table[0].rhoOut != nullptr) {   // the density stage of one spin (single-job calls only)
+            hipLaunchKernelGGL(density_kernel_mixed<true>, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table,
+                               nJobs, perXcd, static_cast<int>(blocks));
+        } else {
+            hipLaunchKernelGGL(density_kernel_mixed<false>, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s,
+                               d_table, nJobs, perXcd, static_cast<int>(blocks));
+        }
     }
     XQ_CHECK(hipEventRecord(ev.evRho1, s));
-    if (chunks > 0) {
+    if ((stages & kStageFock) && chunks > 0) {
         hipLaunchKernelGGL(w_digits_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, d_table, nJobs);
         const int perXcd = static_cast<int>((groups + 7) / 8);
         hipLaunchKernelGGL(fock_kernel_mixed, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,

@@ -79,6 +79,27 @@ class QuadratureGrid:
                                    "rho_ms": stats[3], "fock_ms": stats[4], "bf_squares": stats[5],
                                    "rho_bytes": stats[6], "fock_bytes": stats[7]}
 
+    def integrate_uks(self, ids, density_a, density_b, fock_a=None, fock_b=None):
+        """One unrestricted quadrature (alpha and beta densities, packed): returns (E_xc, N_e, stats); fock_a and fock_b
+        (packed float64) are incremented in place when given."""
+        ids = _lib.i32(ids)
+        Pa, Pb = _lib.f64(density_a), _lib.f64(density_b)
+        n = self.number_of_functions
+        if Pa.size != n * (n + 1) // 2 or Pb.size != Pa.size:
+            raise QCModelError("Densities have {:d} and {:d} entries; {:d} expected.".format(Pa.size, Pb.size,
+                                                                                          n * (n + 1) // 2))
+        Va = fock_a if fock_a is not None else np.zeros_like(Pa)
+        Vb = fock_b if fock_b is not None else np.zeros_like(Pb)
+        for V in (Va, Vb):
+            if V.dtype != np.float64 or not V.flags.c_contiguous or V.size != Pa.size:
+                raise QCModelError("Fock matrices must be contiguous float64 arrays of the density's size.")
+        e, ne = ctypes.c_double(), ctypes.c_double()
+        stats = np.zeros(8)
+        _check(_lib.lib.xq_integrate_uks(self._h, ids.size, _lib.iptr(ids), _lib.dptr(Pa), _lib.dptr(Pb), ctypes.byref(e),
+                                         ctypes.byref(ne), _lib.dptr(Va), _lib.dptr(Vb), _lib.dptr(stats)),
+               "integrate_uks")
+        return e.value, ne.value, {"bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2]}
+
     def gradients(self, ids, density, gradients=None):
         """XC nuclear gradients (n_atoms x 3, hartree/bohr) at the density; `gradients` (contiguous float64, n_atoms x 3)
         is incremented in place when given, as the reference increments its Coordinates3."""
@@ -248,16 +269,28 @@ class DFTGridIntegrator:
         return estimate_bytes(p, n, float(n), self.precision)
 
     def Fock(self, target):
-        """Add V_xc into the Fock matrix and return E_xc (DFTGridIntegrator.py:41-70)."""
+        """Add V_xc into the Fock matrix (or matrices) and return E_xc (DFTGridIntegrator.py:41-70).  Unrestricted: the
+        total and spin densities (onePDMP, onePDMQ) become alpha = (T + S)/2 and beta = (T - S)/2, and the total and spin
+        Fock matrices receive (F_a + F_b)/2 and (F_a - F_b)/2 (DFTIntegrator.pyx:82-107)."""
         scratch = target.scratch
-        if not target.electronicState.isSpinRestricted:
-            raise QCModelError("Error in DFT integrator (spin-unrestricted densities are not supported).")
         grid = scratch.dftGrid
         bases = target.qcState.orbitalBases
         if grid.number_of_functions == 0 or grid.precision != self.precision:
             grid.set_basis(bases.spec, bases.spherical, self.precision)
-        eQuad, rhoQuad, stats = grid.integrate(target.qcModel.functionalModel.ids, scratch.onePDMP.density,
-                                               scratch.onePDMP.fock)
+        ids = target.qcModel.functionalModel.ids
+        if hasattr(scratch, "onePDMQ") and not target.electronicState.isSpinRestricted:
+            T, S = scratch.onePDMP.density, scratch.onePDMQ.density
+            Fa, Fb = np.zeros_like(T), np.zeros_like(T)
+            eQuad, rhoQuad, stats = grid.integrate_uks(ids, 0.5 * (T + S), -0.5 * (S - T), Fa, Fb)
+            fT, fS = scratch.onePDMP.fock, scratch.onePDMQ.fock
+            fT += 0.5 * Fa
+            fT += 0.5 * Fb
+            fS += 0.5 * Fa
+            fS += -0.5 * Fb
+        elif not target.electronicState.isSpinRestricted:
+            raise QCModelError("Error in DFT integrator (an unrestricted state needs the spin density onePDMQ).")
+        else:
+            eQuad, rhoQuad, stats = grid.integrate(ids, scratch.onePDMP.density, scratch.onePDMP.fock)
         scratch.qcEnergyReport["Quadrature Electron Count"] = (rhoQuad, "{:.3f}")
         scratch.qcEnergyReport["Quadrature Energy"] = eQuad
         n = stats["bf_values"]
