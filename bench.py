@@ -43,6 +43,9 @@ def parse():
     p.add_argument("--cpu-baseline", default="reference", choices=["reference", "port", "none"])
     p.add_argument("--cpu-sample-waters", type=int, default=8)
     p.add_argument("--skip-fp64", action="store_true")
+    p.add_argument("--uks-steps", type=int, default=3,
+                   help="timed unrestricted quadratures (alpha 0.55 P, beta 0.45 P) on the workload, each precision "
+                        "(0 skips them)")
     p.add_argument("--grad-steps", type=int, default=2,
                    help="timed XC nuclear-gradient calls on the workload after the quadrature legs (0 skips them)")
     p.add_argument("--serve-requests", type=int, default=256,
@@ -138,6 +141,31 @@ def cpu_baseline(kind, n_waters, functional, spherical, threads=1):
     return {"value": h.npoints * reps / total, "unit": "grid points/s", "cores": threads, "kind": kind,
             "sample": "(H2O){:d} cluster, same generator, {:s}, Medium grid ({:d} points), {:d} in-core quadrature calls, "
                       "{:.1f} s".format(n_waters, functional.upper(), h.npoints, reps, total)}
+
+
+def uks_leg(grid, ids, P, steps, spec, spherical):
+    """Unrestricted quadratures (xq_integrate_uks) with alpha = 0.55 P and beta = 0.45 P, both precisions: device ms from
+    the first upload to the last result (the two density stages, the spin-polarised functional and the two Fock stages,
+    with the host waits between them), and the mixed path's E_xc and V_xc against the FP64 path's."""
+    Pa, Pb = 0.55 * P, 0.45 * P
+    out, res = {}, {}
+    for precision in ("fp64", "mixed"):
+        grid.set_basis(spec, spherical, precision)
+        Va, Vb = np.zeros_like(P), np.zeros_like(P)
+        grid.integrate_uks(ids, Pa, Pb, Va, Vb)
+        dev = []
+        for _ in range(steps):
+            Va[:] = 0.0
+            Vb[:] = 0.0
+            E, N, st = grid.integrate_uks(ids, Pa, Pb, Va, Vb)
+            dev.append(st["device_ms"])
+        res[precision] = (E, N, Va.copy(), Vb.copy())
+        out[precision + "_ms_per_call"] = float(np.mean(dev))
+    (e0, n0, a0, b0), (e1, n1, a1, b1) = res["fp64"], res["mixed"]
+    out.update({"E_xc": e0, "N_e": n0, "dExc_mixed_vs_fp64": e1 - e0,
+                "max_dVxc_mixed_vs_fp64": float(max(np.max(np.abs(a1 - a0)), np.max(np.abs(b1 - b0)))),
+                "calls": steps, "densities": "alpha 0.55 P, beta 0.45 P of the workload's density"})
+    return out
 
 
 def gradient_leg(grid, ids, P, steps, n_waters, functional, spherical, cpu_kind):
@@ -269,6 +297,10 @@ def main():
         out["speedup_mixed_vs_fp64"] = ms64 / ms
         out["dExc_mixed_vs_fp64"] = mixed["E"] - fp64["E"]
         out["max_dVxc_mixed_vs_fp64"] = float(np.max(np.abs(mixed["V"] - fp64["V"])))
+    if args.uks_steps > 0:
+        out["uks"] = uks_leg(grid, ids, P, args.uks_steps, spec, spherical)
+        log("uks:", json.dumps(out["uks"]))
+        grid.set_basis(spec, spherical, "mixed")
     if args.grad_steps > 0:
         out["xc_gradients"] = gradient_leg(grid, ids, P, args.grad_steps, args.cpu_sample_waters, functional, spherical,
                                            args.cpu_baseline)
