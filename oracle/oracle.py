@@ -8,6 +8,7 @@ package.  Both libraries expose the same operations:
     grid()            -> (xyzw [G,4], block_atom [B], block_count [B])
     integrate(P)      -> (E_xc, N_e, V_packed, seconds)          (V starts at zero here)
     gradients(P)      -> XC nuclear gradients [nAtoms, 3] (from zero)
+    gradients_uks(Pa, Pb) -> the same, spin unrestricted
     block_values(b)   -> (indices, f, fx, fy, fz)
     functional(ids, rho, sigma) -> (eps, vrho, vsigma)
     c2s(lLow, lHigh)  -> matrix
@@ -54,6 +55,8 @@ class _Library:
         f("integrate").argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
         f("gradients").restype = ctypes.c_int
         f("gradients").argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
+        f("gradients_uks").restype = ctypes.c_int
+        f("gradients_uks").argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp]
         f("block_values").restype = ctypes.c_int
         f("block_values").argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
         f("functional").restype = ctypes.c_int
@@ -166,6 +169,19 @@ class _Handle:
         st = self.lib._fn("gradients")(self.h, _d(P), int(threads), _d(g))
         if st != 0:
             raise ValueError("gradients status {:d}".format(st))
+        return g
+
+    def gradients_uks(self, Pa_packed, Pb_packed, threads=1):
+        """Unrestricted XC nuclear gradients (nAtoms x 3) at the alpha and beta densities."""
+        n = self.nfunctions
+        Pa = np.ascontiguousarray(Pa_packed, dtype=np.float64)
+        Pb = np.ascontiguousarray(Pb_packed, dtype=np.float64)
+        if Pa.size != n * (n + 1) // 2 or Pb.size != Pa.size:
+            raise ValueError("P has {:d} entries, expected {:d}".format(Pa.size, n * (n + 1) // 2))
+        g = np.zeros((self.natoms, 3))
+        st = self.lib._fn("gradients_uks")(self.h, _d(Pa), _d(Pb), int(threads), _d(g))
+        if st != 0:
+            raise ValueError("gradients_uks status {:d}".format(st))
         return g
 
     def block_values(self, block):

@@ -231,6 +231,31 @@ int xcref_gradients ( void *vh, const double *P, int inCore, double *grad )
     return ( int ) st ;
 }
 
+/* . Unrestricted XC nuclear gradients: the same with the alpha and beta densities and the polarised functional
+!    (DFTIntegrator.c:405-487 with isSpinUnrestricted).  grad (nAtoms x 3) is incremented. */
+int xcref_gradients_uks ( void *vh, const double *Pa, const double *Pb, int inCore, double *grad )
+{
+    RefHandle *h = ( RefHandle * ) vh ;
+    Status st = Status_OK ;
+    int n = h->nFunctions, k, i, size = ( n * ( n + 1 ) ) / 2 ;
+    Real eXC = 0.0, rho = 0.0 ;
+    SymmetricMatrix *dP = SymmetricMatrix_AllocateWithExtent ( n, &st ), *dQ = SymmetricMatrix_AllocateWithExtent ( n, &st ) ;
+    Coordinates3 *g3 = Coordinates3_Allocate ( h->nAtoms, &st ) ;
+    for ( k = 0 ; k < size ; k++ ) { dP->data[k] = Pa[k] ; dQ->data[k] = Pb[k] ; }
+    for ( i = 0 ; i < h->nAtoms ; i++ ) Coordinates3_SetRow ( g3, i, grad[3*i], grad[3*i+1], grad[3*i+2] ) ;
+    DFTIntegrator_Integrate ( h->functionalU, h->grid, h->bases, h->coordinates, dP, dQ, ( inCore ? True : False ), True,
+                              &eXC, &rho, NULL, NULL, g3, &st ) ;
+    for ( i = 0 ; i < h->nAtoms ; i++ )
+    {
+        grad[3*i  ] = Coordinates3_Item ( g3, i, 0 ) ;
+        grad[3*i+1] = Coordinates3_Item ( g3, i, 1 ) ;
+        grad[3*i+2] = Coordinates3_Item ( g3, i, 2 ) ;
+    }
+    SymmetricMatrix_Deallocate ( &dP ) ; SymmetricMatrix_Deallocate ( &dQ ) ;
+    Coordinates3_Deallocate ( &g3 ) ;
+    return ( int ) st ;
+}
+
 /* . Basis values and first derivatives of one block after the reference's |phi| > bfTol filter.
 !    Outputs: nb, indices[nb], f/fx/fy/fz [nb x nPoints] row-major.  Returns nb. */
 int xcref_block_values ( void *vh, int record, int *indices, double *f, double *fx, double *fy, double *fz )

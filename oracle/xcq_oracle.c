@@ -1428,6 +1428,72 @@ int xqo_integrate_uks ( void *vh, const double *Pa, const double *Pb, int nThrea
     return status ;
 }
 
+/* The weight terms of one point (DFTGridWeights_Derivatives, DFTGridWeights.c:98-215): A_i = prod_j s(nu_ij) and
+! dA_i/dmu_ij, then the derivatives of mu_ij with respect to the point and to atoms i and j, weighted by eXC w / (p sum_i A_i)
+! where pt = (x, y, z, w), gAtom is the point's atom and eXC = eps rho.  A, R (nA) and dAdm (nA x nA) are scratch. */
+static void WeightTerms ( const OHandle *h, const BeckePairs *bpp, int gAtom, const double *pt, double eXC, double *A,
+                          double *R, double *dAdm, double *G )
+{
+    int nA = h->nAtoms ;
+    BeckePairs bp = *bpp ;
+    double rgX = pt[0], rgY = pt[1], rgZ = pt[2], w = pt[3] ;
+    for ( int i = 0 ; i < nA ; i++ )
+    {
+        double x = h->xyz[3 * i] - rgX, y = h->xyz[3 * i + 1] - rgY, z = h->xyz[3 * i + 2] - rgZ ;
+        A[i] = 1.0 ;
+        R[i] = sqrt ( x * x + y * y + z * z ) ;
+    }
+    for ( size_t k = 0 ; k < ( size_t ) nA * nA ; k++ ) dAdm[k] = 1.0 ;
+    for ( int i = 0, ij = 0 ; i < nA ; i++ )
+        for ( int j = 0 ; j < i ; j++, ij++ )
+        {
+            double mu = ( R[i] - R[j] ) * bp.invR[ij] ;
+            double nu = mu + bp.aij[ij] * ( 1.0 - mu * mu ), nu2 = nu * nu, dnu = 1.0, snu = nu, dsum = 0.0, sum = 0.0 ;
+            for ( int t = 0 ; t <= 10 ; t++ )
+            {
+                dsum += BECKE_C[t] * dnu * ( double ) ( 2 * t + 1 ) ;
+                sum  += BECKE_C[t] * snu ;
+                dnu  *= nu2 ;
+                snu  *= nu2 ;
+            }
+            double sij = 0.5 - BECKE_A * sum, sji = 0.5 + BECKE_A * sum ;
+            A[i] *= sij ;
+            A[j] *= sji ;
+            dsum *= - BECKE_A * ( 1.0 - 2.0 * bp.aij[ij] * mu ) ;
+            for ( int k = 0 ; k < nA ; k++ )
+            {
+                dAdm[( size_t ) i * nA + k] *= ( k == j ? dsum : sij ) ;
+                dAdm[( size_t ) j * nA + k] *= ( k == i ? dsum : sji ) ;
+            }
+        }
+    double sum = 0.0 ;
+    for ( int i = 0 ; i < nA ; i++ ) sum += A[i] ;
+    double pw = A[gAtom] / sum ;
+    double ew = eXC * w / ( pw * sum ) ;
+    for ( int i = 0 ; i < nA ; i++ )
+    {
+        double xi = h->xyz[3 * i], yi = h->xyz[3 * i + 1], zi = h->xyz[3 * i + 2] ;
+        double fac = ( i == gAtom ) ? ew * ( 1.0 - pw ) : - ew * pw ;
+        for ( int j = 0 ; j < nA ; j++ )
+        {
+            if ( i == j ) continue ;
+            double xj = h->xyz[3 * j], yj = h->xyz[3 * j + 1], zj = h->xyz[3 * j + 2] ;
+            int ij = i > j ? ( i * ( i - 1 ) ) / 2 + j : ( j * ( j - 1 ) ) / 2 + i ;
+            double rij = bp.invR[ij], mu = ( R[i] - R[j] ) * rij ;
+            double dxi = ( xi - rgX ) * rij / R[i], dyi = ( yi - rgY ) * rij / R[i], dzi = ( zi - rgZ ) * rij / R[i] ;
+            double dxj = - ( xj - rgX ) * rij / R[j], dyj = - ( yj - rgY ) * rij / R[j], dzj = - ( zj - rgZ ) * rij / R[j] ;
+            double dxg = - ( dxi + dxj ), dyg = - ( dyi + dyj ), dzg = - ( dzi + dzj ) ;
+            double tx = ( xi - xj ) * mu * rij * rij, ty = ( yi - yj ) * mu * rij * rij, tz = ( zi - zj ) * mu * rij * rij ;
+            dxi -= tx ; dyi -= ty ; dzi -= tz ;
+            dxj += tx ; dyj += ty ; dzj += tz ;
+            double ifac = dAdm[( size_t ) i * nA + j] * fac ;
+            G[3 * i] += ifac * dxi ; G[3 * i + 1] += ifac * dyi ; G[3 * i + 2] += ifac * dzi ;
+            G[3 * j] += ifac * dxj ; G[3 * j + 1] += ifac * dyj ; G[3 * j + 2] += ifac * dzj ;
+            G[3 * gAtom] += ifac * dxg ; G[3 * gAtom + 1] += ifac * dyg ; G[3 * gAtom + 2] += ifac * dzg ;
+        }
+    }
+}
+
 /*----------------------------------------------------------------------------------------------------------------------------------
 ! XC nuclear gradients (DFTIntegrator.c:159-509 with doGradients, restricted closed shell, GGA).  Per block, with the basis values,
 ! gradients and second derivatives (order + 1, :203-207) after the |phi| > bfTol filter:
@@ -1547,65 +1613,7 @@ int xqo_gradients ( void *vh, const double *Ppacked, int nThreads, double *grad 
                 for ( int d = 0 ; d < 3 ; d++ ) { G[3 * c + d] -= g3[d] ; G[3 * gAtom + d] += g3[d] ; }
             }
             /* Weight terms. */
-            for ( int p = 0 ; p < nP ; p++ )
-            {
-                double rgX = pts[4 * p], rgY = pts[4 * p + 1], rgZ = pts[4 * p + 2], w = pts[4 * p + 3] ;
-                for ( int i = 0 ; i < nA ; i++ )
-                {
-                    double x = h->xyz[3 * i] - rgX, y = h->xyz[3 * i + 1] - rgY, z = h->xyz[3 * i + 2] - rgZ ;
-                    A[i] = 1.0 ;
-                    R[i] = sqrt ( x * x + y * y + z * z ) ;
-                }
-                for ( size_t k = 0 ; k < ( size_t ) nA * nA ; k++ ) dAdm[k] = 1.0 ;
-                for ( int i = 0, ij = 0 ; i < nA ; i++ )
-                    for ( int j = 0 ; j < i ; j++, ij++ )
-                    {
-                        double mu = ( R[i] - R[j] ) * bp.invR[ij] ;
-                        double nu = mu + bp.aij[ij] * ( 1.0 - mu * mu ), nu2 = nu * nu, dnu = 1.0, snu = nu, dsum = 0.0, sum = 0.0 ;
-                        for ( int t = 0 ; t <= 10 ; t++ )
-                        {
-                            dsum += BECKE_C[t] * dnu * ( double ) ( 2 * t + 1 ) ;
-                            sum  += BECKE_C[t] * snu ;
-                            dnu  *= nu2 ;
-                            snu  *= nu2 ;
-                        }
-                        double sij = 0.5 - BECKE_A * sum, sji = 0.5 + BECKE_A * sum ;
-                        A[i] *= sij ;
-                        A[j] *= sji ;
-                        dsum *= - BECKE_A * ( 1.0 - 2.0 * bp.aij[ij] * mu ) ;
-                        for ( int k = 0 ; k < nA ; k++ )
-                        {
-                            dAdm[( size_t ) i * nA + k] *= ( k == j ? dsum : sij ) ;
-                            dAdm[( size_t ) j * nA + k] *= ( k == i ? dsum : sji ) ;
-                        }
-                    }
-                double sum = 0.0 ;
-                for ( int i = 0 ; i < nA ; i++ ) sum += A[i] ;
-                double pw = A[gAtom] / sum ;
-                double ew = eps[p] * rho[p] * w / ( pw * sum ) ;
-                for ( int i = 0 ; i < nA ; i++ )
-                {
-                    double xi = h->xyz[3 * i], yi = h->xyz[3 * i + 1], zi = h->xyz[3 * i + 2] ;
-                    double fac = ( i == gAtom ) ? ew * ( 1.0 - pw ) : - ew * pw ;
-                    for ( int j = 0 ; j < nA ; j++ )
-                    {
-                        if ( i == j ) continue ;
-                        double xj = h->xyz[3 * j], yj = h->xyz[3 * j + 1], zj = h->xyz[3 * j + 2] ;
-                        int ij = i > j ? ( i * ( i - 1 ) ) / 2 + j : ( j * ( j - 1 ) ) / 2 + i ;
-                        double rij = bp.invR[ij], mu = ( R[i] - R[j] ) * rij ;
-                        double dxi = ( xi - rgX ) * rij / R[i], dyi = ( yi - rgY ) * rij / R[i], dzi = ( zi - rgZ ) * rij / R[i] ;
-                        double dxj = - ( xj - rgX ) * rij / R[j], dyj = - ( yj - rgY ) * rij / R[j], dzj = - ( zj - rgZ ) * rij / R[j] ;
-                        double dxg = - ( dxi + dxj ), dyg = - ( dyi + dyj ), dzg = - ( dzi + dzj ) ;
-                        double tx = ( xi - xj ) * mu * rij * rij, ty = ( yi - yj ) * mu * rij * rij, tz = ( zi - zj ) * mu * rij * rij ;
-                        dxi -= tx ; dyi -= ty ; dzi -= tz ;
-                        dxj += tx ; dyj += ty ; dzj += tz ;
-                        double ifac = dAdm[( size_t ) i * nA + j] * fac ;
-                        G[3 * i] += ifac * dxi ; G[3 * i + 1] += ifac * dyi ; G[3 * i + 2] += ifac * dzi ;
-                        G[3 * j] += ifac * dxj ; G[3 * j + 1] += ifac * dyj ; G[3 * j + 2] += ifac * dzj ;
-                        G[3 * gAtom] += ifac * dxg ; G[3 * gAtom + 1] += ifac * dyg ; G[3 * gAtom + 2] += ifac * dzg ;
-                    }
-                }
-            }
+            for ( int p = 0 ; p < nP ; p++ ) WeightTerms ( h, &bp, gAtom, pts + 4 * p, eps[p] * rho[p], A, R, dAdm, G ) ;
         }
         free ( buf ) ; free ( X ) ; free ( T ) ; free ( Y ) ; free ( Pr ) ; free ( idx ) ; free ( A ) ; free ( R ) ; free ( dAdm ) ;
     }
@@ -1613,6 +1621,159 @@ int xqo_gradients ( void *vh, const double *Ppacked, int nThreads, double *grad 
         for ( int k = 0 ; k < 3 * nA ; k++ ) grad[k] += Gt[( size_t ) t * 3 * nA + k] ;
     free ( Gt ) ; free ( atomOf ) ; free ( radii ) ; free ( bp.invR ) ; free ( bp.aij ) ;
     return 0 ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! XC nuclear gradients, spin unrestricted (DFTIntegrator.c:405-487 with isSpinUnrestricted): Pa and Pb are the alpha and beta
+! densities (packed).  Per block, rho_s and grad rho_s for both spins; skip the block if max over both spins |rho_s| <= rhoTol; the
+! polarised functional (vrho_s, vsigma_ss', times w); eXC = eps (rho_a + rho_b).  Direct terms per spin s (t the other spin):
+!   DFTIntegrator_Gradients with P_s and vrho_s                           (factor 2),
+!   DFTIntegrator_GradientsSigma with P_s, grad rho_s and vsigma_ss      (isSelf, factor 4),
+!   DFTIntegrator_GradientsSigma with P_s, grad rho_t and vsigma_ab      (factor 2),
+! that is, with c_s = 4 vsigma_ss grad rho_s + 2 vsigma_ab grad rho_t:
+!   work = 2 vrho_s (P_s Phi) + P_s (grad Phi . c_s)  with grad Phi,   work = (P_s Phi) c_s,d  with d/dd grad Phi;
+! the weight terms as in the restricted case.  grad (nAtoms x 3) is incremented.
+!---------------------------------------------------------------------------------------------------------------------------------*/
+int xqo_gradients_uks ( void *vh, const double *Pa, const double *Pb, int nThreads, double *grad )
+{
+    OHandle *h = ( OHandle * ) vh ;
+    int nF = h->nFunctions, nA = h->nAtoms, status = 0 ;
+    if ( nThreads < 1 ) nThreads = 1 ;
+# ifdef _OPENMP
+    omp_set_num_threads ( nThreads ) ;
+# endif
+    int *atomOf = ( int * ) malloc ( ( nF > 0 ? nF : 1 ) * sizeof ( int ) ) ;
+    for ( int s = 0 ; s < h->nShells ; s++ ) for ( int b = 0 ; b < h->shells[s].nBasis ; b++ ) atomOf[h->shells[s].start + b] = h->shells[s].atom ;
+    double *radii = ( double * ) malloc ( nA * sizeof ( double ) ) ;
+    for ( int a = 0 ; a < nA ; a++ ) radii[a] = BraggRadius ( h->Z[a] ) ;
+    BeckePairs bp ;
+    BeckePairs_Make ( &bp, nA, h->xyz, radii ) ;
+    double *Gt = ( double * ) calloc ( ( size_t ) nThreads * 3 * nA, sizeof ( double ) ) ;
+    # pragma omp parallel num_threads(nThreads)
+    {
+        int tid = 0 ;
+# ifdef _OPENMP
+        tid = omp_get_thread_num ( ) ;
+# endif
+        double *G = Gt + ( size_t ) tid * 3 * nA ;
+        const double *P[2] = { Pa, Pb } ;
+        double *buf = ( double * ) malloc ( 10 * ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        double *X[2], *T, *Y, *Pr[2] ;
+        for ( int s = 0 ; s < 2 ; s++ )
+        {
+            X[s]  = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+            Pr[s] = ( double * ) malloc ( ( size_t ) nF * nF * sizeof ( double ) ) ;
+        }
+        T = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        Y = ( double * ) malloc ( ( size_t ) nF * BLOCK_SIZE * sizeof ( double ) ) ;
+        int    *idx = ( int * ) malloc ( nF * sizeof ( int ) ) ;
+        double *A = ( double * ) malloc ( nA * sizeof ( double ) ), *R = ( double * ) malloc ( nA * sizeof ( double ) ) ;
+        double *dAdm = ( double * ) malloc ( ( size_t ) nA * nA * sizeof ( double ) ) ;
+        double rho[2][BLOCK_SIZE], gr[2][3][BLOCK_SIZE], rr[2 * BLOCK_SIZE], ss[3 * BLOCK_SIZE], eps[BLOCK_SIZE] ;
+        double vr[2 * BLOCK_SIZE], vs[3 * BLOCK_SIZE], cs[3][BLOCK_SIZE] ;
+        # pragma omp for schedule(dynamic,1)
+        for ( int b = 0 ; b < h->nBlocks ; b++ )
+        {
+            int nP = h->blockCount[b], gAtom = h->blockAtom[b] ;
+            const double *pts = h->points + 4 * ( size_t ) h->blockStart[b] ;
+            double *a[10] ;
+            for ( int k = 0 ; k < 10 ; k++ ) a[k] = buf + ( size_t ) k * nF * nP ;
+            EvaluateBasisN ( h, nP, pts, a, 10 ) ;
+            int nb = FilterRowsN ( nF, nP, h->bfTol, idx, a, 10 ) ;
+            if ( nb <= 0 ) continue ;
+            double *f = a[0], *fx = a[1], *fy = a[2], *fz = a[3] ;
+            double mx = 0.0 ;
+            for ( int s = 0 ; s < 2 ; s++ )
+            {
+                for ( int i = 0 ; i < nb ; i++ )
+                    for ( int j = 0 ; j <= i ; j++ )
+                    {
+                        int m = idx[i], n = idx[j] ;
+                        double v = P[s][( ( size_t ) m * ( m + 1 ) ) / 2 + n] ;
+                        Pr[s][i * nb + j] = v ; Pr[s][j * nb + i] = v ;
+                    }
+                for ( int i = 0 ; i < nb ; i++ )
+                    for ( int p = 0 ; p < nP ; p++ )
+                    {
+                        double t = 0.0 ;
+                        for ( int k = 0 ; k < nb ; k++ ) t += Pr[s][i * nb + k] * f[k * nP + p] ;
+                        X[s][i * nP + p] = t ;
+                    }
+                for ( int p = 0 ; p < nP ; p++ )
+                {
+                    double r = 0.0, ax = 0.0, ay = 0.0, az = 0.0 ;
+                    for ( int i = 0 ; i < nb ; i++ )
+                    {
+                        double xv = X[s][i * nP + p] ;
+                        r  += f [i * nP + p] * xv ;
+                        ax += fx[i * nP + p] * xv ;
+                        ay += fy[i * nP + p] * xv ;
+                        az += fz[i * nP + p] * xv ;
+                    }
+                    rho[s][p] = r ; gr[s][0][p] = 2.0 * ax ; gr[s][1][p] = 2.0 * ay ; gr[s][2][p] = 2.0 * az ;
+                    if ( fabs ( r ) > mx ) mx = fabs ( r ) ;
+                }
+            }
+            if ( mx <= h->rhoTol ) continue ;
+            for ( int p = 0 ; p < nP ; p++ )
+            {
+                rr[2 * p] = rho[0][p] ; rr[2 * p + 1] = rho[1][p] ;
+                ss[3 * p]     = gr[0][0][p] * gr[0][0][p] + gr[0][1][p] * gr[0][1][p] + gr[0][2][p] * gr[0][2][p] ;
+                ss[3 * p + 1] = gr[0][0][p] * gr[1][0][p] + gr[0][1][p] * gr[1][1][p] + gr[0][2][p] * gr[1][2][p] ;
+                ss[3 * p + 2] = gr[1][0][p] * gr[1][0][p] + gr[1][1][p] * gr[1][1][p] + gr[1][2][p] * gr[1][2][p] ;
+            }
+            if ( xqo_functional_pol ( h->nIDs, h->ids, nP, rr, ss, eps, vr, vs ) != 0 ) { status = 3 ; continue ; }
+            for ( int p = 0 ; p < nP ; p++ )
+            {
+                double w = pts[4 * p + 3] ;
+                vr[2 * p] *= w ; vr[2 * p + 1] *= w ;
+                vs[3 * p] *= w ; vs[3 * p + 1] *= w ; vs[3 * p + 2] *= w ;
+            }
+            for ( int s = 0 ; s < 2 ; s++ )
+            {
+                const int o = 1 - s ;
+                for ( int p = 0 ; p < nP ; p++ )
+                    for ( int d = 0 ; d < 3 ; d++ )
+                        cs[d][p] = 4.0 * vs[3 * p + 2 * s] * gr[s][d][p] + 2.0 * vs[3 * p + 1] * gr[o][d][p] ;
+                for ( int i = 0 ; i < nb ; i++ )
+                    for ( int p = 0 ; p < nP ; p++ )
+                        T[i * nP + p] = fx[i * nP + p] * cs[0][p] + fy[i * nP + p] * cs[1][p] + fz[i * nP + p] * cs[2][p] ;
+                for ( int i = 0 ; i < nb ; i++ )
+                    for ( int p = 0 ; p < nP ; p++ )
+                    {
+                        double t = 0.0 ;
+                        for ( int k = 0 ; k < nb ; k++ ) t += Pr[s][i * nb + k] * T[k * nP + p] ;
+                        Y[i * nP + p] = t ;
+                    }
+                for ( int i = 0 ; i < nb ; i++ )
+                {
+                    double g3[3] = { 0.0, 0.0, 0.0 } ;
+                    for ( int p = 0 ; p < nP ; p++ )
+                    {
+                        int q = i * nP + p ;
+                        double x = X[s][q], w1 = 2.0 * vr[2 * p + s] * x + Y[q] ;
+                        double hx = a[4][q] * cs[0][p] + a[5][q] * cs[1][p] + a[6][q] * cs[2][p] ;
+                        double hy = a[5][q] * cs[0][p] + a[7][q] * cs[1][p] + a[8][q] * cs[2][p] ;
+                        double hz = a[6][q] * cs[0][p] + a[8][q] * cs[1][p] + a[9][q] * cs[2][p] ;
+                        g3[0] += w1 * fx[q] + x * hx ;
+                        g3[1] += w1 * fy[q] + x * hy ;
+                        g3[2] += w1 * fz[q] + x * hz ;
+                    }
+                    int c = atomOf[idx[i]] ;
+                    for ( int d = 0 ; d < 3 ; d++ ) { G[3 * c + d] -= g3[d] ; G[3 * gAtom + d] += g3[d] ; }
+                }
+            }
+            /* Weight terms. */
+            for ( int p = 0 ; p < nP ; p++ )
+                WeightTerms ( h, &bp, gAtom, pts + 4 * p, eps[p] * ( rho[0][p] + rho[1][p] ), A, R, dAdm, G ) ;
+        }
+        free ( buf ) ; free ( X[0] ) ; free ( X[1] ) ; free ( Pr[0] ) ; free ( Pr[1] ) ; free ( T ) ; free ( Y ) ; free ( idx ) ;
+        free ( A ) ; free ( R ) ; free ( dAdm ) ;
+    }
+    for ( int t = 0 ; t < nThreads ; t++ )
+        for ( int k = 0 ; k < 3 * nA ; k++ ) grad[k] += Gt[( size_t ) t * 3 * nA + k] ;
+    free ( Gt ) ; free ( atomOf ) ; free ( radii ) ; free ( bp.invR ) ; free ( bp.aij ) ;
+    return status ;
 }
 
 /* All functions and their gradients at n arbitrary points xyz[n x 3] (test access); outputs nFunctions x n row-major. */
