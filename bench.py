@@ -170,7 +170,8 @@ def uks_leg(grid, ids, P, steps, spec, spherical):
 
 def gradient_leg(grid, ids, P, steps, n_waters, functional, spherical, cpu_kind):
     """XC nuclear gradients (xq_gradients, FP64) on the workload: wall time per call (P upload and the gradient download
-    included), the translational sum of the gradient, and the CPU's time for the same call on the bounded sample."""
+    included), the translational sum of the gradient, the same for the unrestricted gradients (xq_gradients_uks), and the
+    CPU's time for the restricted call on the bounded sample."""
     G = grid.gradients(ids, P)
     wall = []
     for _ in range(steps):
@@ -179,6 +180,16 @@ def gradient_leg(grid, ids, P, steps, n_waters, functional, spherical, cpu_kind)
         wall.append((time.perf_counter() - t) * 1e3)
     out = {"ms_per_call": float(np.mean(wall)), "calls": steps, "atoms": int(G.shape[0]),
            "max_abs_gradient": float(np.max(np.abs(G))), "max_abs_translational_sum": float(np.max(np.abs(G.sum(0))))}
+    # Unrestricted (xq_gradients_uks) at alpha = 0.55 P, beta = 0.45 P.
+    Gu = grid.gradients_uks(ids, 0.55 * P, 0.45 * P)
+    wall = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        Gu = grid.gradients_uks(ids, 0.55 * P, 0.45 * P)
+        wall.append((time.perf_counter() - t) * 1e3)
+    out["unrestricted"] = {"ms_per_call": float(np.mean(wall)), "calls": steps,
+                           "densities": "alpha = 0.55 P, beta = 0.45 P",
+                           "max_abs_translational_sum": float(np.max(np.abs(Gu.sum(0))))}
     if cpu_kind != "none":
         try:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))

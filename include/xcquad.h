@@ -77,10 +77,17 @@ int         xq_integrate_uks    ( xq_grid *grid, int nIDs, const int *ids, const
 
 /* XC nuclear gradients: DFTIntegrator_Integrate with a gradients3 argument and no Fock matrix (DFTIntegrator.c:159-509, the
 !  doGradients branch :405-487 with DFTGridWeights_Derivatives, DFTGridWeights.c:98-215), as DFTGridIntegrator.Gradients
-!  calls it (DFTGridIntegrator.py:72-88).  Restricted closed shell, GGA functionals, FP64 on either precision's handle (the
-!  basis values and second derivatives are evaluated afresh).  gradients3: nAtoms x 3 in hartree/bohr, INCREMENTED as the
-!  reference's Coordinates3 is.  At most 655 atoms. */
+!  calls it (DFTGridIntegrator.py:72-88).  Restricted closed shell, FP64 on either precision's handle (the basis values and
+!  second derivatives are evaluated afresh).  gradients3: nAtoms x 3 in hartree/bohr, INCREMENTED as the reference's
+!  Coordinates3 is.  At most 655 atoms. */
 int         xq_gradients        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *gradients3 ) ;
+
+/* Spin-unrestricted XC nuclear gradients: the same call with isSpinUnrestricted, alpha and beta densities and the
+!  polarised functional (DFTIntegrator.c:405-487: DFTIntegrator_Gradients per spin and the two DFTIntegrator_GradientsSigma
+!  cross calls with vSigmaPQ; the weight terms from eps (rho_a + rho_b)).  densityA, densityB: packed alpha and beta
+!  densities (the caller converts total and spin matrices as for xq_integrate_uks); gradients3 as xq_gradients. */
+int         xq_gradients_uks    ( xq_grid *grid, int nIDs, const int *ids, const double *densityA, const double *densityB,
+                                  double *gradients3 ) ;
 
 /* Peak device bytes of one handle for a molecule with the given grid size, function count and mean kept functions per
 !  block, set-up included (admission control for the request loop); xq_device_bytes is what a handle holds now, and

@@ -27,6 +27,7 @@ SIGNATURES = {
     "xq_integrate": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp]),
     "xq_integrate_uks": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "xq_gradients": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp]),
+    "xq_gradients_uks": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp]),
     "xq_estimate_bytes": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, ctypes.c_double, ctypes.c_int, _llp]),
     "xq_device_bytes": (ctypes.c_int, [_vp, _llp]),
     "xq_device_memory": (ctypes.c_int, [_llp, _llp]),

@@ -133,6 +133,16 @@ int xq_gradients(xq_grid *grid, int nIDs, const int *ids, const double *densityP
     return gradients(grid, nIDs, ids, densityP, gradients3);
 }
 
+int xq_gradients_uks(xq_grid *grid, int nIDs, const int *ids, const double *densityA, const double *densityB,
+                     double *gradients3) {
+    if (grid == nullptr || ids == nullptr || densityA == nullptr || densityB == nullptr || gradients3 == nullptr) {
+        set_error("invalid gradients arguments");
+        return kInvalidArgument;
+    }
+    if (!grid->hasBasis) { set_error("no basis set on this grid"); return kInvalidArgument; }
+    return gradients_uks(grid, nIDs, ids, densityA, densityB, gradients3);
+}
+
 int xq_estimate_bytes(long long nPoints, int nFunctions, double meanKeptFunctions, int precision, long long *bytes) {
     if (bytes == nullptr || nPoints < 0 || nFunctions < 0 || meanKeptFunctions < 0.0 ||
         (precision != XQ_PRECISION_FP64 && precision != XQ_PRECISION_MIXED)) {

@@ -1,4 +1,4 @@
-// XC nuclear gradients on the device, FP64, restricted closed shell with GGA functionals: DFTIntegrator_Integrate with a
+// XC nuclear gradients on the device, FP64, restricted closed shell or spin unrestricted: DFTIntegrator_Integrate with a
 // gradients argument and no Fock matrix (DFTIntegrator.c:159-509, as DFTGridIntegrator.Gradients calls it,
 // DFTGridIntegrator.py:72-88).
 //
@@ -9,10 +9,13 @@
 //   X = P_red Phi and Y = P_red (grad Phi . grad rho); g_i is subtracted from the atom of function i and added to the
 //   block's atom (UGradientContributions :909-941, the grid moving with its atom).  The reference sums per function then
 //   per atom; here the terms of one atom's rows are summed per point first, so the last bits differ.
+// Spin unrestricted (DFTIntegrator.c:405-487): rho_s and grad rho_s of both spins, the block skipped when both are below
+//   rhoTol, the polarised functional, eXC = eps (rho_a + rho_b), and the direct terms of each spin with its own P_s and
+//   vrho_s and the sigma coefficient 4 vsigma_ss grad rho_s + 2 vsigma_ab grad rho_t (t the other spin).
 // The weight terms (DFTGridWeights_Derivatives) follow per point from eXC (weight_gradient_kernel, xq_grid.hip).
 //
 // Device layout: the basis data of a batch of consecutive blocks in ten planes of rows of 128 points (as the set-up's FP64
-// values), plus X and U planes; batches keep the scratch bounded.  The products P_red Phi and P_red U run on FP64 MFMA
+// values), plus an X plane per spin and a U plane; batches keep the scratch bounded.  The products P_red Phi and P_red U run on FP64 MFMA
 // with P_red staged in LDS, as in the FP64 quadrature's density kernel.
 #include <hip/hip_runtime.h>
 
@@ -109,20 +112,25 @@ __device__ __forceinline__ void reduced_product(const double *__restrict__ P, in
 
 // Direct terms of the blocks of one batch (one work-group of 256 threads per block; work-group g runs on XCD g % 8 and
 // each XCD takes a contiguous range of blocks, so that its L2 holds the P rows they gather).  B: the ten basis planes of
-// the batch; Xp, Up: scratch planes with the same rows; eXC[point] = eps rho (zero for skipped or empty blocks); the
-// gradient terms go to copy (work-group % kGradCopies) of gradCopies.
-//   pass 1: X = P_red Phi, rho, grad rho, the functional (vrho, vsigma times w);
-//   U = 2 vrho Phi + 4 vsigma (grad rho . grad Phi), so that P_red U = 2 vrho X + 4 vsigma Y;
-//   pass 2: Z = P_red U and per row g_i = sum_p [Z grad phi + X (Hess phi . 4 vsigma grad rho)]_ip.
+// the batch; Xp[s], Up: scratch planes with the same rows; eXC[point] = eps rho (zero for skipped or empty blocks); the
+// gradient terms go to copy (work-group % kGradCopies) of gradCopies.  kSpins = 1: P[0] is the total density; kSpins = 2:
+// P[0], P[1] are the alpha and beta densities.  Per spin s (t the other spin):
+//   pass 1: X_s = P_s Phi, rho_s, grad rho_s; then the functional (vrho, vsigma times w);
+//   restricted: u0 = 2 vrho, c = 4 vsigma grad rho; unrestricted: u0 = 2 vrho_s, c = 4 vsigma_ss grad rho_s +
+//   2 vsigma_ab grad rho_t (DFTIntegrator.c:405-487: _Gradients per spin and the two _GradientsSigma cross calls);
+//   U_s = u0 Phi + c . grad Phi, so that P_s U_s = 2 vrho X_s + P_s (grad Phi . c);
+//   pass 2: Z = P_s U_s and per row g_i = sum_p [Z grad phi + X_s (Hess phi . c)]_ip.
+template <int kSpins>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grad_direct_kernel(
     int b0, int nGroups, int groupsPerXcd, long long rowBase, const DevBlock *__restrict__ blocks,
-    const int *__restrict__ idxAll, const int *__restrict__ atomOf, int n, const double *__restrict__ P,
-    const double *__restrict__ w, FunctionalSet fs, double rhoTol, Planes10 B, double *__restrict__ Xp,
-    double *__restrict__ Up, double *__restrict__ eXC, double *__restrict__ gradCopies, int nAtoms) {
+    const int *__restrict__ idxAll, const int *__restrict__ atomOf, int n, const double *__restrict__ P0,
+    const double *__restrict__ P1, const double *__restrict__ w, FunctionalSet fs, double rhoTol, Planes10 B,
+    double *__restrict__ Xp0, double *__restrict__ Xp1, double *__restrict__ Up, double *__restrict__ eXC,
+    double *__restrict__ gradCopies, int nAtoms) {
     __shared__ double sPf[kC][kPanelPitch];
     __shared__ int sIdx[kIdxCache];
-    __shared__ double rsum[4][kBlockSize];
-    __shared__ double sCoef[4][kBlockSize];
+    __shared__ double rsum[kSpins][4][kBlockSize];
+    __shared__ double sCoef[4 * kSpins][kBlockSize];
     __shared__ double sRed[4][kC][3];
     __shared__ double smax[4];
     const int wg = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
@@ -141,53 +149,61 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
     const int *idx = nb <= kIdxCache ? sIdx : idxG;
     const size_t R0 = static_cast<size_t>(blk.vOffset - rowBase) * kBlockSize;
     const double *F = B.p[0] + R0, *Fx = B.p[1] + R0, *Fy = B.p[2] + R0, *Fz = B.p[3] + R0;
-    double *X = Xp + R0, *U = Up + R0;
+    double *U = Up + R0;
     double *grad = gradCopies + static_cast<size_t>(wg % kGradCopies) * 3 * nAtoms;
+    const double *Ps[2] = {P0, P1};
+    double *Xs[2] = {Xp0 + R0, kSpins == 2 ? Xp1 + R0 : nullptr};
 
-    // Pass 1: X, rho and grad rho (DFTIntegrator_GridPointRho, DFTIntegrator.c:814-877).
-    double pr[2] = {0.0, 0.0}, pgx[2] = {0.0, 0.0}, pgy[2] = {0.0, 0.0}, pgz[2] = {0.0, 0.0};
-    reduced_product(P, n, idx, nb, F, sPf, [&](int I, v4d (&acc)[2][2]) {
+    // Pass 1, per spin: X, rho and grad rho (DFTIntegrator_GridPointRho, DFTIntegrator.c:814-877) into rsum[s].
 #pragma unroll
-        for (int hm = 0; hm < 2; hm++)
+    for (int sp = 0; sp < kSpins; sp++) {
+        double *X = Xs[sp];
+        double pr[2] = {0.0, 0.0}, pgx[2] = {0.0, 0.0}, pgy[2] = {0.0, 0.0}, pgz[2] = {0.0, 0.0};
+        reduced_product(Ps[sp], n, idx, nb, F, sPf, [&](int I, v4d (&acc)[2][2]) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int i = kC * I + 16 * hm + 4 * v + l4;
-                if (i < nb) {
+            for (int hm = 0; hm < 2; hm++)
 #pragma unroll
-                    for (int hn = 0; hn < 2; hn++) {
-                        const size_t o = static_cast<size_t>(i) * kBlockSize + kC * wv + 16 * hn + l16;
-                        const double x = acc[hm][hn][v];
-                        X[o] = x;
-                        pr[hn] += F[o] * x;
-                        pgx[hn] += Fx[o] * x;
-                        pgy[hn] += Fy[o] * x;
-                        pgz[hn] += Fz[o] * x;
+                for (int v = 0; v < 4; v++) {
+                    const int i = kC * I + 16 * hm + 4 * v + l4;
+                    if (i < nb) {
+#pragma unroll
+                        for (int hn = 0; hn < 2; hn++) {
+                            const size_t o = static_cast<size_t>(i) * kBlockSize + kC * wv + 16 * hn + l16;
+                            const double x = acc[hm][hn][v];
+                            X[o] = x;
+                            pr[hn] += F[o] * x;
+                            pgx[hn] += Fx[o] * x;
+                            pgy[hn] += Fy[o] * x;
+                            pgz[hn] += Fz[o] * x;
+                        }
                     }
                 }
+        });
+#pragma unroll
+        for (int hn = 0; hn < 2; hn++) {
+#pragma unroll
+            for (int o = 16; o < 64; o <<= 1) {
+                pr[hn] += __shfl_xor(pr[hn], o, 64);
+                pgx[hn] += __shfl_xor(pgx[hn], o, 64);
+                pgy[hn] += __shfl_xor(pgy[hn], o, 64);
+                pgz[hn] += __shfl_xor(pgz[hn], o, 64);
             }
-    });
-#pragma unroll
-    for (int hn = 0; hn < 2; hn++) {
-#pragma unroll
-        for (int o = 16; o < 64; o <<= 1) {
-            pr[hn] += __shfl_xor(pr[hn], o, 64);
-            pgx[hn] += __shfl_xor(pgx[hn], o, 64);
-            pgy[hn] += __shfl_xor(pgy[hn], o, 64);
-            pgz[hn] += __shfl_xor(pgz[hn], o, 64);
-        }
-        if (l4 == 0) {
-            const int p = kC * wv + 16 * hn + l16;
-            rsum[0][p] = pr[hn];
-            rsum[1][p] = pgx[hn];
-            rsum[2][p] = pgy[hn];
-            rsum[3][p] = pgz[hn];
+            if (l4 == 0) {
+                const int p = kC * wv + 16 * hn + l16;
+                rsum[sp][0][p] = pr[hn];
+                rsum[sp][1][p] = pgx[hn];
+                rsum[sp][2][p] = pgy[hn];
+                rsum[sp][3][p] = pgz[hn];
+            }
         }
     }
     __syncthreads();
     const bool valid = t < np;           // np <= 128: threads 128 .. 255 hold no point here
-    const double rho = valid ? rsum[0][t] : 0.0, gx = valid ? 2.0 * rsum[1][t] : 0.0,
-                 gy = valid ? 2.0 * rsum[2][t] : 0.0, gz = valid ? 2.0 * rsum[3][t] : 0.0;
-    double mx = valid ? fabs(rho) : 0.0;
+    double mx = 0.0;
+    if (valid) {
+#pragma unroll
+        for (int sp = 0; sp < kSpins; sp++) mx = fmax(mx, fabs(rsum[sp][0][t]));
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) smax[wv] = mx;
@@ -196,91 +212,119 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
         if (valid) eXC[blk.pStart + t] = 0.0;
         return;
     }
-    double u0 = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+    double coef[kSpins][4];
+#pragma unroll
+    for (int sp = 0; sp < kSpins; sp++) coef[sp][0] = coef[sp][1] = coef[sp][2] = coef[sp][3] = 0.0;
     if (valid) {
-        double eps, vr, vs;
-        evaluate_functional(fs, rho, gx * gx + gy * gy + gz * gz, eps, vr, vs);
         const double ww = w[blk.pStart + t];
-        vr *= ww;
-        vs *= ww;
-        eXC[blk.pStart + t] = eps * rho;
-        u0 = 2.0 * vr;
-        ux = 4.0 * vs * gx;
-        uy = 4.0 * vs * gy;
-        uz = 4.0 * vs * gz;
-    }
-    if (t < kBlockSize) {
-        sCoef[0][t] = u0;
-        sCoef[1][t] = ux;
-        sCoef[2][t] = uy;
-        sCoef[3][t] = uz;
-    }
-    __syncthreads();
-    {
-        // Thread t builds U at point t % 128 for rows t / 128, + 2, ...
-        const int p = t & (kBlockSize - 1);
-        const double c0 = sCoef[0][p], cx = sCoef[1][p], cy = sCoef[2][p], cz = sCoef[3][p];
-        for (int i = t >> 7; i < nb; i += 2) {
-            const size_t o = static_cast<size_t>(i) * kBlockSize + p;
-            U[o] = c0 * F[o] + cx * Fx[o] + cy * Fy[o] + cz * Fz[o];
+        if constexpr (kSpins == 1) {
+            const double rho = rsum[0][0][t], gx = 2.0 * rsum[0][1][t], gy = 2.0 * rsum[0][2][t],
+                         gz = 2.0 * rsum[0][3][t];
+            double eps, vr, vs;
+            evaluate_functional(fs, rho, gx * gx + gy * gy + gz * gz, eps, vr, vs);
+            vr *= ww;
+            vs *= ww;
+            eXC[blk.pStart + t] = eps * rho;
+            coef[0][0] = 2.0 * vr;
+            coef[0][1] = 4.0 * vs * gx;
+            coef[0][2] = 4.0 * vs * gy;
+            coef[0][3] = 4.0 * vs * gz;
+        } else {
+            double ra = rsum[0][0][t], rb = rsum[1][0][t], ga[3], gb[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) { ga[d] = 2.0 * rsum[0][1 + d][t]; gb[d] = 2.0 * rsum[1][1 + d][t]; }
+            PolOut f;
+            evaluate_functional_pol(fs, ra, rb, ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2],
+                                    ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2],
+                                    gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2], f);
+            eXC[blk.pStart + t] = f.e * (ra + rb);
+            const double vsaa = f.vsaa * ww, vsab = f.vsab * ww, vsbb = f.vsbb * ww;
+            coef[0][0] = 2.0 * f.va * ww;
+            coef[1][0] = 2.0 * f.vb * ww;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                coef[0][1 + d] = 4.0 * vsaa * ga[d] + 2.0 * vsab * gb[d];
+                coef[1][1 + d] = 4.0 * vsbb * gb[d] + 2.0 * vsab * ga[d];
+            }
         }
     }
-    __syncthreads();   // U and X are complete for the work-group
+    if (t < kBlockSize) {
+#pragma unroll
+        for (int sp = 0; sp < kSpins; sp++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) sCoef[4 * sp + k][t] = coef[sp][k];
+    }
 
-    // Pass 2: Z = P_red U and the per-row terms (DFTIntegrator_Gradients :643-747, _GradientsSigma :752-808).
+    // Pass 2, per spin: Z = P_s U_s and the per-row terms (DFTIntegrator_Gradients :643-747, _GradientsSigma :752-808).
     const double *Hxx = B.p[4] + R0, *Hxy = B.p[5] + R0, *Hxz = B.p[6] + R0, *Hyy = B.p[7] + R0, *Hyz = B.p[8] + R0,
                  *Hzz = B.p[9] + R0;
     double gs[3] = {0.0, 0.0, 0.0};
-    reduced_product(P, n, idx, nb, U, sPf, [&](int I, v4d (&acc)[2][2]) {
-        double c[2][4][3];
+    for (int sp = 0; sp < kSpins; sp++) {
+        __syncthreads();   // sCoef is written; the previous spin's product has finished reading U
+        {
+            // Thread t builds U at point t % 128 for rows t / 128, + 2, ...
+            const int p = t & (kBlockSize - 1);
+            const double c0 = sCoef[4 * sp][p], cx = sCoef[4 * sp + 1][p], cy = sCoef[4 * sp + 2][p],
+                         cz = sCoef[4 * sp + 3][p];
+            for (int i = t >> 7; i < nb; i += 2) {
+                const size_t o = static_cast<size_t>(i) * kBlockSize + p;
+                U[o] = c0 * F[o] + cx * Fx[o] + cy * Fy[o] + cz * Fz[o];
+            }
+        }
+        __syncthreads();   // U and X are complete for the work-group
+        const double *X = Xs[sp];
+        const double *sA = sCoef[4 * sp + 1], *sB = sCoef[4 * sp + 2], *sC = sCoef[4 * sp + 3];
+        reduced_product(Ps[sp], n, idx, nb, U, sPf, [&](int I, v4d (&acc)[2][2]) {
+            double c[2][4][3];
 #pragma unroll
-        for (int hm = 0; hm < 2; hm++)
+            for (int hm = 0; hm < 2; hm++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                c[hm][v][0] = c[hm][v][1] = c[hm][v][2] = 0.0;
-                const int i = kC * I + 16 * hm + 4 * v + l4;
-                if (i < nb) {
+                for (int v = 0; v < 4; v++) {
+                    c[hm][v][0] = c[hm][v][1] = c[hm][v][2] = 0.0;
+                    const int i = kC * I + 16 * hm + 4 * v + l4;
+                    if (i < nb) {
 #pragma unroll
-                    for (int hn = 0; hn < 2; hn++) {
-                        const int p = kC * wv + 16 * hn + l16;
-                        if (p < np) {
-                            const size_t o = static_cast<size_t>(i) * kBlockSize + p;
-                            const double z = acc[hm][hn][v], x = X[o];
-                            const double ax = sCoef[1][p], ay = sCoef[2][p], az = sCoef[3][p];
-                            const double hxy = Hxy[o], hxz = Hxz[o], hyz = Hyz[o];
-                            c[hm][v][0] += z * Fx[o] + x * (Hxx[o] * ax + hxy * ay + hxz * az);
-                            c[hm][v][1] += z * Fy[o] + x * (hxy * ax + Hyy[o] * ay + hyz * az);
-                            c[hm][v][2] += z * Fz[o] + x * (hxz * ax + hyz * ay + Hzz[o] * az);
+                        for (int hn = 0; hn < 2; hn++) {
+                            const int p = kC * wv + 16 * hn + l16;
+                            if (p < np) {
+                                const size_t o = static_cast<size_t>(i) * kBlockSize + p;
+                                const double z = acc[hm][hn][v], x = X[o];
+                                const double ax = sA[p], ay = sB[p], az = sC[p];
+                                const double hxy = Hxy[o], hxz = Hxz[o], hyz = Hyz[o];
+                                c[hm][v][0] += z * Fx[o] + x * (Hxx[o] * ax + hxy * ay + hxz * az);
+                                c[hm][v][1] += z * Fy[o] + x * (hxy * ax + Hyy[o] * ay + hyz * az);
+                                c[hm][v][2] += z * Fz[o] + x * (hxz * ax + hyz * ay + Hzz[o] * az);
+                            }
                         }
                     }
                 }
-            }
-        // Sum over the 16 points of each lane group, then over the four waves.
+            // Sum over the 16 points of each lane group, then over the four waves.
 #pragma unroll
-        for (int hm = 0; hm < 2; hm++)
+            for (int hm = 0; hm < 2; hm++)
 #pragma unroll
-            for (int v = 0; v < 4; v++)
+                for (int v = 0; v < 4; v++)
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    double e = c[hm][v][d];
+                    for (int d = 0; d < 3; d++) {
+                        double e = c[hm][v][d];
 #pragma unroll
-                    for (int o = 1; o < 16; o <<= 1) e += __shfl_xor(e, o, 64);
-                    if (l16 == 0) sRed[wv][16 * hm + 4 * v + l4][d] = e;
+                        for (int o = 1; o < 16; o <<= 1) e += __shfl_xor(e, o, 64);
+                        if (l16 == 0) sRed[wv][16 * hm + 4 * v + l4][d] = e;
+                    }
+            __syncthreads();
+            if (t < kC) {
+                const int i = kC * I + t;
+                if (i < nb) {
+                    const int atom = atomOf[idxG[i]];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const double gi = ((sRed[0][t][d] + sRed[1][t][d]) + sRed[2][t][d]) + sRed[3][t][d];
+                        atomicAdd(&grad[3 * atom + d], -gi);
+                        gs[d] += gi;
+                    }
                 }
-        __syncthreads();
-        if (t < kC) {
-            const int i = kC * I + t;
-            if (i < nb) {
-                const int atom = atomOf[idxG[i]];
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const double gi = ((sRed[0][t][d] + sRed[1][t][d]) + sRed[2][t][d]) + sRed[3][t][d];
-                    atomicAdd(&grad[3 * atom + d], -gi);
-                    gs[d] += gi;
-                }
             }
-        }
-    });
+        });
+    }
     // The grid moves with the block's atom (UGradientContributions :909-941): it receives the sum of the row terms.
     if (wv == 0) {
 #pragma unroll
@@ -305,11 +349,12 @@ __global__ void __launch_bounds__(256) sum_copies_kernel(int nEntries, const dou
 
 }  // namespace
 
-int gradients(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, double *grad) {
-    FunctionalSet fs;
-    XQ_TRY(functional_set(nIDs, ids, &fs));
+namespace {
+
+// The direct and weight terms for one (kSpins = 1) or two (alpha, beta) dense densities already on the device.
+int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const double *d_P0, const double *d_P1,
+                        double *grad) {
     hipStream_t s = g->stream;
-    XQ_TRY(upload_density(g, Ppacked, s));
     const int nA = g->nAtoms, nB = g->nBlocks;
     // Function -> atom (the reference's functionCenters).
     std::vector<int> atomOf(g->h_funcAtom);
@@ -319,14 +364,19 @@ int gradients(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
     const long long batchRows = std::max(kBatchRows, maxRows);
     std::vector<double> zeros(3 * static_cast<size_t>(nA), 0.0);
     std::vector<double> zeroCopies(zeros.size() * kGradCopies, 0.0);
-    double *planes[12] = {};
+    // Ten basis planes, X per spin, U.
+    const int nPlanes = 11 + spins;
+    double *planes[13] = {};
     double *d_grad = nullptr, *d_eXC = nullptr, *d_copies = nullptr;
     int *d_atomOf = nullptr;
     int status = kOK;
     do {
         bool ok = true;
-        for (double *&p : planes)
-            if ((status = dev_alloc(g, &p, static_cast<size_t>(batchRows) * kBlockSize)) != kOK) { ok = false; break; }
+        for (int k = 0; k < nPlanes; k++)
+            if ((status = dev_alloc(g, &planes[k], static_cast<size_t>(batchRows) * kBlockSize)) != kOK) {
+                ok = false;
+                break;
+            }
         if (!ok) break;
         if ((status = dev_alloc(g, &d_grad, zeros.size())) != kOK) break;
         if ((status = dev_alloc(g, &d_copies, zeroCopies.size())) != kOK) break;
@@ -340,15 +390,21 @@ int gradients(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
                             "copy function atoms")) != kOK) break;
         Planes10 B;
         for (int k = 0; k < 10; k++) B.p[k] = planes[k];
+        double *Xa = planes[10], *Xb = spins == 2 ? planes[11] : nullptr, *U = planes[nPlanes - 1];
         for (int b0 = 0; b0 < nB && status == kOK;) {
             const long long rowBase = g->blocks[b0].vOffset;
             int b1 = b0;
             while (b1 < nB && g->blocks[b1].vOffset + g->blocks[b1].nKept - rowBase <= batchRows) b1++;
             if ((status = eval_basis2(g, b0, b1, rowBase, B, s)) != kOK) break;
             const int nGroups = b1 - b0, perXcd = (nGroups + 7) / 8;
-            hipLaunchKernelGGL(grad_direct_kernel, dim3(8 * perXcd), dim3(256), 0, s, b0, nGroups, perXcd, rowBase,
-                               g->d_blocks, g->d_idx, d_atomOf, g->nFunctions, g->d_P, g->d_w, fs, g->rhoTol, B, planes[10],
-                               planes[11], d_eXC, d_copies, nA);
+            if (spins == 1)
+                hipLaunchKernelGGL(grad_direct_kernel<1>, dim3(8 * perXcd), dim3(256), 0, s, b0, nGroups, perXcd, rowBase,
+                                   g->d_blocks, g->d_idx, d_atomOf, g->nFunctions, d_P0, d_P0, g->d_w, fs, g->rhoTol, B,
+                                   Xa, Xa, U, d_eXC, d_copies, nA);
+            else
+                hipLaunchKernelGGL(grad_direct_kernel<2>, dim3(8 * perXcd), dim3(256), 0, s, b0, nGroups, perXcd, rowBase,
+                                   g->d_blocks, g->d_idx, d_atomOf, g->nFunctions, d_P0, d_P1, g->d_w, fs, g->rhoTol, B,
+                                   Xa, Xb, U, d_eXC, d_copies, nA);
             status = check(hipGetLastError(), "grad_direct_kernel");
             b0 = b1;
         }
@@ -369,6 +425,38 @@ int gradients(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
     dev_free(g, &d_copies);
     dev_free(g, &d_eXC);
     dev_free(g, &d_atomOf);
+    return status;
+}
+
+}  // namespace
+
+int gradients(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, double *grad) {
+    FunctionalSet fs;
+    XQ_TRY(functional_set(nIDs, ids, &fs));
+    XQ_TRY(upload_density(g, Ppacked, g->stream));
+    return gradients_on_device(g, fs, 1, g->d_P, nullptr, grad);
+}
+
+// Unrestricted: the alpha density goes to a dense copy of its own, the beta density to the handle's P.
+int gradients_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const double *Pb, double *grad) {
+    FunctionalSet fs;
+    XQ_TRY(functional_set(nIDs, ids, &fs));
+    hipStream_t s = g->stream;
+    const size_t n = static_cast<size_t>(g->nFunctions);
+    double *d_Pa = nullptr;
+    int status = kOK;
+    do {
+        if ((status = dev_alloc(g, &d_Pa, n * n > 0 ? n * n : 1)) != kOK) break;
+        if ((status = upload_density(g, Pa, s)) != kOK) break;
+        if (n > 0 && (status = check(hipMemcpyAsync(d_Pa, g->d_P, n * n * sizeof(double), hipMemcpyDeviceToDevice, s),
+                                     "copy alpha density")) != kOK) break;
+        // upload_density refills the pinned staging buffer, which the previous copy reads when it executes.
+        if ((status = check(hipStreamSynchronize(s), "density sync")) != kOK) break;
+        if ((status = upload_density(g, Pb, s)) != kOK) break;
+        status = gradients_on_device(g, fs, 2, d_Pa, g->d_P, grad);
+    } while (false);
+    (void)hipStreamSynchronize(s);
+    dev_free(g, &d_Pa);
     return status;
 }
 

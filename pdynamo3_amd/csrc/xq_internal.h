@@ -229,6 +229,7 @@ struct Planes10 {
 };
 int eval_basis2(xq_grid *g, int b0, int b1, long long rowBase, const Planes10 &out, hipStream_t s);
 int gradients(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, double *grad);
+int gradients_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const double *Pb, double *grad);
 int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_t s);
 int ensure_work(xq_grid *g);
 int upload_density(xq_grid *g, const double *Ppacked, hipStream_t s);

@@ -114,6 +114,22 @@ class QuadratureGrid:
         _check(_lib.lib.xq_gradients(self._h, ids.size, _lib.iptr(ids), _lib.dptr(P), _lib.dptr(G)), "gradients")
         return G
 
+    def gradients_uks(self, ids, density_a, density_b, gradients=None):
+        """Unrestricted XC nuclear gradients (n_atoms x 3) at the alpha and beta densities (packed); `gradients` is
+        incremented in place when given."""
+        ids = _lib.i32(ids)
+        Pa, Pb = _lib.f64(density_a), _lib.f64(density_b)
+        n = self.number_of_functions
+        if Pa.size != n * (n + 1) // 2 or Pb.size != Pa.size:
+            raise QCModelError("Densities have {:d} and {:d} entries; {:d} expected.".format(Pa.size, Pb.size,
+                                                                                          n * (n + 1) // 2))
+        G = gradients if gradients is not None else np.zeros((self.number_of_atoms, 3))
+        if G.dtype != np.float64 or not G.flags.c_contiguous or G.size != 3 * self.number_of_atoms:
+            raise QCModelError("Gradients must be a contiguous float64 array of n_atoms x 3.")
+        _check(_lib.lib.xq_gradients_uks(self._h, ids.size, _lib.iptr(ids), _lib.dptr(Pa), _lib.dptr(Pb), _lib.dptr(G)),
+               "gradients_uks")
+        return G
+
     def device_bytes(self):
         b = ctypes.c_longlong()
         _check(_lib.lib.xq_device_bytes(self._h, ctypes.byref(b)), "device bytes")
@@ -305,13 +321,18 @@ class DFTGridIntegrator:
         (n_atoms x 3, float64) is incremented."""
         scratch = target.scratch
         if getattr(scratch, "doGradients", False):
-            if not target.electronicState.isSpinRestricted:
-                raise QCModelError("Error in DFT integrator (spin-unrestricted densities are not supported).")
             grid = scratch.dftGrid
             bases = target.qcState.orbitalBases
             if grid.number_of_functions == 0:
                 grid.set_basis(bases.spec, bases.spherical, self.precision)
-            grid.gradients(target.qcModel.functionalModel.ids, scratch.onePDMP.density, scratch.qcGradients3AU)
+            ids = target.qcModel.functionalModel.ids
+            if target.electronicState.isSpinRestricted:
+                grid.gradients(ids, scratch.onePDMP.density, scratch.qcGradients3AU)
+            elif hasattr(scratch, "onePDMQ"):
+                T, S = scratch.onePDMP.density, scratch.onePDMQ.density
+                grid.gradients_uks(ids, 0.5 * (T + S), -0.5 * (S - T), scratch.qcGradients3AU)
+            else:
+                raise QCModelError("Error in DFT integrator (an unrestricted state needs the spin density onePDMQ).")
 
     def SummaryItems(self):
         """Summary items (DFTGridIntegrator.py:90-93)."""
