@@ -376,7 +376,10 @@ constexpr int kMaxLaneAtoms = 11;
 // |fac_k| = |eXC w| / sum_i A_i for k != g.  Atoms with Q_k < 2^-80 sum_i A_i (k != g) are outside the set K: the terms of
 // pairs (k, m) with k outside K and of pairs (m, k) with k outside K are each below 3.375 2^-80 |eXC w| 2 / R_km, and
 // are left out (below 1e-15 hartree/bohr in all for any molecule this kernel takes); every other pair term is computed.
+// sum_i A_i is taken over the atoms whose bound passes half the screen at A_g (K lies among them); each atom left out of
+// it has A_i below 2^-81 A_g.
 constexpr double kWeightScreen = 0x1p-80;
+constexpr int kNearAtoms = 24;           // atoms whose factors bound a cell product (the point's atom and its nearest)
 
 // Derivative polynomial of the cell function's polynomial: d_t = (2 t + 1) c_t, so that d/dnu sum_t c_t nu^(2t+1) =
 // sum_t d_t nu^(2t).
@@ -414,7 +417,9 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                                                               const int *__restrict__ pointAtom,
                                                               const double *__restrict__ eXC, const double *__restrict__ xyz,
                                                               int nAtoms, const double *__restrict__ invR,
-                                                              const double *__restrict__ aij, double *__restrict__ grad) {
+                                                              const double *__restrict__ aij,
+                                                              const int *__restrict__ nearAtoms, int nNear,
+                                                              double *__restrict__ grad) {
     extern __shared__ double lds[];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     double *R = lds + static_cast<size_t>(wv) * 3 * nAtoms, *iR = R + nAtoms, *prodNZ = iR + nAtoms;
@@ -435,20 +440,20 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                 iR[i] = 1.0 / R[i];
             }
         __syncthreads();
-        // Products of the cell factors, their zero counts, and the screened set K (flag 0x80 of the count).
+        // Products of the cell factors, their zero counts, and the screened set K (flag 0x80 of the count).  A_g is a
+        // lower bound of sum_i A_i; the partial product of atom i over the atoms nearest g bounds every product of i
+        // with one factor left out (|s| <= 1), so an atom whose bound is below the screen at A_g is outside K.  The
+        // remaining candidates take their full products with the factors spread over the wave.  The others keep a zero
+        // product and a count of 2, which leaves their terms out, as the screen leaves out those of the atoms outside K.
         double sumA = 0.0;
         int g = 0;
         if (active) {
             g = pointAtom[p];
-            double Q[LA];
-#pragma unroll
-            for (int a = 0; a < LA; a++) {
-                const int i = lane + 64 * a;
-                Q[a] = 0.0;
-                if (i >= nAtoms) continue;
-                double prod = 1.0, minS = 1.0;
-                int nz = 0;
-                for (int j = 0; j < nAtoms; j++) {
+            auto full_product = [&](int i, double &prod, int &nz, double &minS) {
+                prod = 1.0;
+                minS = 1.0;
+                nz = 0;
+                for (int j = lane; j < nAtoms; j += 64) {
                     if (j == i) continue;
                     const int ij = pair_index(i, j);
                     const double s = becke_factor_of(i > j, R[i], R[j], invR[ij], aij[ij]);
@@ -458,17 +463,70 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                         minS = fmin(minS, fabs(s));
                     }
                 }
-                prodNZ[i] = prod;
-                Q[a] = nz == 0 ? fabs(prod) / minS : (nz == 1 ? fabs(prod) : 0.0);
-                if (nz == 0) sumA += prod;
-                zeros[i] = static_cast<unsigned char>(nz < 2 ? nz : 2);
-            }
+                // Butterfly reductions: every lane ends with the same bits.
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) sumA += __shfl_xor(sumA, o, 64);
+                for (int o = 32; o > 0; o >>= 1) {
+                    prod *= __shfl_xor(prod, o, 64);
+                    minS = fmin(minS, __shfl_xor(minS, o, 64));
+                    nz += __shfl_xor(nz, o, 64);
+                }
+            };
+            double pg, mg;
+            int zg;
+            full_product(g, pg, zg, mg);
+            const double bound = zg == 0 ? 0x1p-81 * fabs(pg) : 0.0;   // half the screen at A_g
+            const int *near = nearAtoms + static_cast<size_t>(g) * nNear;
+            bool cand[LA];
+            double Q[LA];
 #pragma unroll
             for (int a = 0; a < LA; a++) {
                 const int i = lane + 64 * a;
-                if (i < nAtoms && (i == g || !(Q[a] < kWeightScreen * sumA))) zeros[i] |= 0x80;
+                cand[a] = false;
+                Q[a] = 0.0;
+                if (i >= nAtoms) continue;
+                if (i != g) {
+                    double pp = 1.0, mp = 1.0;
+                    int zp = 0;
+                    for (int q = 0; q < nNear; q++) {
+                        const int j = near[q];
+                        if (j == i) continue;
+                        const int ij = pair_index(i, j);
+                        const double s = fabs(becke_factor_of(i > j, R[i], R[j], invR[ij], aij[ij]));
+                        if (s == 0.0) zp++;
+                        else {
+                            pp *= s;
+                            mp = fmin(mp, s);
+                        }
+                    }
+                    const double qp = zp == 0 ? pp / mp : (zp == 1 ? pp : 0.0);
+                    cand[a] = !(qp < bound);
+                }
+                prodNZ[i] = 0.0;
+                zeros[i] = 2;
+            }
+#pragma unroll
+            for (int a = 0; a < LA; a++) {
+                unsigned long long mask = __ballot(cand[a] || lane + 64 * a == g);
+                while (mask != 0ull) {
+                    const int bit = __ffsll(static_cast<long long>(mask)) - 1;
+                    mask &= mask - 1ull;
+                    const int c = 64 * a + bit;
+                    double pc = pg, mc = mg;
+                    int zc = zg;
+                    if (c != g) full_product(c, pc, zc, mc);
+                    if (zc == 0) sumA += pc;
+                    if (lane == bit) {
+                        cand[a] = true;
+                        prodNZ[c] = pc;
+                        zeros[c] = static_cast<unsigned char>(zc < 2 ? zc : 2);
+                        Q[a] = zc == 0 ? fabs(pc) / mc : (zc == 1 ? fabs(pc) : 0.0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < LA; a++) {
+                const int i = lane + 64 * a;
+                if (i < nAtoms && cand[a] && (i == g || !(Q[a] < kWeightScreen * sumA))) zeros[i] |= 0x80;
             }
         }
         __syncthreads();
@@ -839,14 +897,32 @@ int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_
     std::vector<int> pointAtom(static_cast<size_t>(G), 0);
     for (const DevBlock &b : g->blocks)
         for (int k = 0; k < b.nPoints; k++) pointAtom[static_cast<size_t>(b.pStart) + k] = b.atom;
+    // Per atom: itself and the atoms nearest it (the partial products that bound the cell products of a point).
+    const int nNear = std::min(kNearAtoms, nA);
+    std::vector<int> nearAtoms(static_cast<size_t>(nA) * nNear), order(nA);
+    for (int a = 0; a < nA; a++) {
+        std::vector<double> d2(nA);
+        for (int j = 0; j < nA; j++) {
+            const double dx = g->xyz[3 * j] - g->xyz[3 * a], dy = g->xyz[3 * j + 1] - g->xyz[3 * a + 1],
+                         dz = g->xyz[3 * j + 2] - g->xyz[3 * a + 2];
+            d2[j] = j == a ? -1.0 : dx * dx + dy * dy + dz * dz;
+            order[j] = j;
+        }
+        std::partial_sort(order.begin(), order.begin() + nNear, order.end(),
+                          [&](int u, int v) { return d2[u] < d2[v] || (d2[u] == d2[v] && u < v); });
+        for (int q = 0; q < nNear; q++) nearAtoms[static_cast<size_t>(a) * nNear + q] = order[q];
+    }
     double *d_invR = nullptr, *d_aij = nullptr, *d_xyz = nullptr;
-    int *d_pointAtom = nullptr;
+    int *d_pointAtom = nullptr, *d_near = nullptr;
     int status = kOK;
     do {
         if ((status = dev_alloc(g, &d_invR, invR.size())) != kOK) break;
         if ((status = dev_alloc(g, &d_aij, aij.size())) != kOK) break;
         if ((status = dev_alloc(g, &d_xyz, g->xyz.size())) != kOK) break;
         if ((status = dev_alloc(g, &d_pointAtom, pointAtom.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_near, nearAtoms.size())) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_near, nearAtoms.data(), nearAtoms.size() * sizeof(int),
+                                           hipMemcpyHostToDevice, s), "copy near atoms")) != kOK) break;
         if ((status = check(hipMemcpyAsync(d_invR, invR.data(), invR.size() * sizeof(double), hipMemcpyHostToDevice, s),
                             "copy invR")) != kOK) break;
         if ((status = check(hipMemcpyAsync(d_aij, aij.data(), aij.size() * sizeof(double), hipMemcpyHostToDevice, s),
@@ -861,7 +937,7 @@ int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_
         const int nGroups = (G + perGroup - 1) / perGroup;
         auto launch = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(nGroups), dim3(256), lds, s, G, perGroup, g->d_x, g->d_y, g->d_z, g->d_w,
-                               d_pointAtom, d_eXC, d_xyz, nA, d_invR, d_aij, d_grad);
+                               d_pointAtom, d_eXC, d_xyz, nA, d_invR, d_aij, d_near, nNear, d_grad);
         };
         // Atoms per lane: the smallest instance that covers nAtoms (fewer registers for small systems).
         if (nA <= 64) launch(weight_gradient_kernel<1>);
@@ -878,6 +954,7 @@ int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_
     dev_free(g, &d_aij);
     dev_free(g, &d_xyz);
     dev_free(g, &d_pointAtom);
+    dev_free(g, &d_near);
     return status;
 }
 

@@ -78,7 +78,7 @@ constexpr int kChunkBytesB = kDB * kBlockSize * kC;   // one chunk of DB: 5 digi
 constexpr int kChunkGrad = 3 * kBlockSize * kC;     // one chunk of GF, GHU or GLU (elements)
 constexpr int kGradBits = 31;           // GF: |value| < 2^EG is stored as a multiple of 2^(EG - 31)
 constexpr int kGradBitsW = 23;          // GHU, GLU: |value| < 2^EG is stored as a multiple of 2^(EG - 23)
-constexpr int kPanel = 8;               // chunks of P_red columns staged at once by the density kernel
+constexpr int kPanel = 4;               // chunks of P_red columns staged at once by the density kernel
 constexpr int kPanelDwords = 8 * kPanel + 4;        // one panel row in LDS (padded against bank conflicts)
 constexpr int kIdxCache = 1024;         // kept-function indices of a block cached in LDS by the density kernel
 constexpr int kSuper = 4;               // blocks per superblock of the Fock product
@@ -363,12 +363,13 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
 // Density kernel (per block, 256 threads = 4 waves): X = P_red Phi on int8 MFMA, rho and grad rho, the FP64 functional,
 // E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
 // point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  P_red is staged in LDS by row panels shared by
-// the four waves; the Phi operands are fetched one step ahead.
+// the four waves; the Phi operands are fetched one step ahead, and those of chunk I are kept in LDS for its epilogue.
 // kRhoOut: the density stage of one spin of an unrestricted quadrature (the job's rhoOut receives rho and grad rho).
 template <bool kRhoOut>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
+    __shared__ v4i sDI[kDB][256];   // per thread: its Phi digits of the current row chunk, for the epilogue
     __shared__ int sPe[2][kC];
     __shared__ int sIdx[kIdxCache];
     __shared__ double rsum[4][kBlockSize];
@@ -469,6 +470,10 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
                 v4i a[kDP], bq[kDB];
 #pragma unroll
                 for (int j = 0; j < kDB; j++) bq[j] = bnext[j];
+                if (c == I) {   // chunk I's digits, kept for the epilogue (each thread reads back its own slot)
+#pragma unroll
+                    for (int j = 0; j < kDB; j++) sDI[j][t] = bq[j];
+                }
                 if (c + 1 < nChunks) fetchB(c + 1, bnext);
                 else if (I + 1 < nChunks) fetchB(0, bnext);
 #pragma unroll
@@ -490,7 +495,8 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
             x[v] = ldexp(static_cast<double>(combine<kXOrders>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 7 * (kXOrders + 1));
         }
         v4i dI[kDB];
-        fetchB(I, dI);
+#pragma unroll
+        for (int j = 0; j < kDB; j++) dI[j] = sDI[j][t];
         double sumR = 0.0, sumG[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int v = 0; v < 16; v++) sumR += static_cast<double>(digits5(dI, v)) * x[v];
