@@ -11,10 +11,14 @@ them through oracle/ref_driver.c and stores inputs and outputs only (numbers, no
   reference_functionals.json  libxc-1.2.0 (as vendored) eps, vrho, vsigma of BLYP, PBE, their parts, Slater exchange, VWN
                             (both fits), LDA and B3LYP on bulk and threshold (rho, sigma).
   reference_functionals_pol.json  the same functionals spin-polarised (rho_a, rho_b; s_aa, s_ab, s_bb).
+  reference_accuracy.json   water at the five grid accuracies (VeryLow .. VeryHigh): grid size, block count and checksums,
+                            E_xc, N_e, packed V_xc and the XC nuclear gradients (BLYP, spherical, EHT monomer density);
+                            and at Medium the unrestricted E_xc, N_e, V_xc of both spins and gradients for alpha = 0.6 P,
+                            beta = 0.4 P (PBE).
   reference_c2s.json        the reference's own Cartesian -> spherical fixture (examples/pMolecule/data/yaml/
                             C2STransformations_0to6.yaml, read with SafeLoader) converted to JSON.
 
-Run: make -C oracle ref && python tests/golden/make_golden.py
+Run: make -C oracle ref && python tests/golden/make_golden.py [names of the functions below; default all]
 """
 
 import json
@@ -146,9 +150,33 @@ def c2s():
         json.dump(out, f)
 
 
+def accuracies():
+    ref = oracle.reference()
+    b = basis.load_basis()
+    Z, xyz = molecules.water()
+    spec = basis.element_spec(b, Z)
+    P1, _ = densities.load_monomer_density(spherical=True)
+    P = P1[np.tril_indices(P1.shape[0])]
+    out = {"Z": Z.tolist(), "xyz_bohr": xyz.tolist(), "functional": "blyp", "spherical": True, "levels": []}
+    for accuracy in range(5):
+        h = ref.setup(accuracy, Z, xyz, spec, True, BLYP)
+        xyzw, ba, bc = h.grid()
+        e, ne, V, _ = h.integrate(P, 0)
+        g = h.gradients(P, 0)
+        out["levels"].append({"accuracy": accuracy, "npoints": h.npoints, "nblocks": h.nblocks,
+                              "checksums": checksums(xyzw), "E_xc": e, "N_e": ne, "V": V.tolist(),
+                              "gradients": g.tolist()})
+    h = ref.setup(2, Z, xyz, spec, True, PBE)
+    e, ne, Va, Vb, _ = h.integrate_uks(0.6 * P, 0.4 * P, 0)
+    g = h.gradients_uks(0.6 * P, 0.4 * P, 0)
+    out["uks"] = {"accuracy": 2, "functional": "pbe", "alpha": 0.6, "beta": 0.4, "E_xc": e, "N_e": ne,
+                  "Va": Va.tolist(), "Vb": Vb.tolist(), "gradients": g.tolist()}
+    with open(os.path.join(HERE, "reference_accuracy.json"), "w") as f:
+        json.dump(out, f)
+
+
 if __name__ == "__main__":
-    functionals()
-    functionals_pol()
-    c2s()
-    water()
-    cluster()
+    steps = {"functionals": functionals, "functionals_pol": functionals_pol, "c2s": c2s, "water": water,
+             "cluster": cluster, "accuracies": accuracies}
+    for name in sys.argv[1:] or list(steps):
+        steps[name]()

@@ -193,3 +193,60 @@ def test_lda_and_b3lyp_match_oracle(oracle_lib, name, spherical, precision):
     with pytest.raises(I.QCModelError):
         g.integrate([1, 131], P, np.zeros_like(P))
     g.close()
+
+
+# The five grid accuracies against the compiled reference's outputs (tests/golden/reference_accuracy.json, pinned against
+# the oracle in test_oracle_accuracy.py): water, BLYP, spherical 6-31G*, EHT monomer density.
+
+def _accuracy_golden():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_accuracy.json")) as f:
+        return json.load(f)
+
+
+def _accuracy_system():
+    from pdynamo3_amd import basis, densities, molecules
+    Z, xyz = molecules.water()
+    spec = basis.element_spec(basis.load_basis(), Z)
+    P1, _ = densities.load_monomer_density(spherical=True)
+    return Z, xyz, spec, P1[np.tril_indices(P1.shape[0])]
+
+
+@pytest.mark.parametrize("accuracy", range(5))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_every_grid_accuracy_matches_reference(accuracy, precision):
+    I = _integrator()
+    ref = _accuracy_golden()["levels"][accuracy]
+    Z, xyz, spec, P = _accuracy_system()
+    g = I.QuadratureGrid(Z, xyz, accuracy)
+    assert g.number_of_points == ref["npoints"] and g.number_of_blocks == ref["nblocks"]
+    g.set_basis(spec, True, precision)
+    V = np.zeros_like(P)
+    e, ne, _ = g.integrate(ids("blyp"), P, V)
+    G = g.gradients(ids("blyp"), P)
+    g.close()
+    tolE, tolV = (1e-9, 1e-9) if precision == "fp64" else (1e-6, 1e-7)
+    print(accuracy, precision, "dE", e - ref["E_xc"], "max dV", np.max(np.abs(V - np.asarray(ref["V"]))),
+          "max dG", np.max(np.abs(G - np.asarray(ref["gradients"]))))
+    assert abs(e - ref["E_xc"]) <= tolE and abs(ne - ref["N_e"]) <= tolE
+    assert np.max(np.abs(V - np.asarray(ref["V"]))) <= tolV
+    assert np.max(np.abs(G - np.asarray(ref["gradients"]))) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_unrestricted_matches_reference_fixture(precision):
+    I = _integrator()
+    ref = _accuracy_golden()["uks"]
+    Z, xyz, spec, P = _accuracy_system()
+    g = I.QuadratureGrid(Z, xyz, ref["accuracy"])
+    g.set_basis(spec, True, precision)
+    Pa, Pb = ref["alpha"] * P, ref["beta"] * P
+    Va, Vb = np.zeros_like(P), np.zeros_like(P)
+    e, ne, _ = g.integrate_uks(ids(ref["functional"]), Pa, Pb, Va, Vb)
+    G = g.gradients_uks(ids(ref["functional"]), Pa, Pb)
+    g.close()
+    tolE, tolV = (1e-9, 1e-9) if precision == "fp64" else (1e-6, 1e-7)
+    assert abs(e - ref["E_xc"]) <= tolE and abs(ne - ref["N_e"]) <= tolE
+    assert np.max(np.abs(Va - np.asarray(ref["Va"]))) <= tolV and np.max(np.abs(Vb - np.asarray(ref["Vb"]))) <= tolV
+    assert np.max(np.abs(G - np.asarray(ref["gradients"]))) <= 1e-9
