@@ -447,6 +447,9 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
         // product and a count of 2, which leaves their terms out, as the screen leaves out those of the atoms outside K.
         double sumA = 0.0;
         int g = 0;
+        unsigned long long maskK[LA];   // the atoms of K among atoms 64 a .. 64 a + 63 (the same in every lane)
+#pragma unroll
+        for (int a = 0; a < LA; a++) maskK[a] = 0ull;
         if (active) {
             g = pointAtom[p];
             auto full_product = [&](int i, double &prod, int &nz, double &minS) {
@@ -526,7 +529,9 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
 #pragma unroll
             for (int a = 0; a < LA; a++) {
                 const int i = lane + 64 * a;
-                if (i < nAtoms && cand[a] && (i == g || !(Q[a] < kWeightScreen * sumA))) zeros[i] |= 0x80;
+                const bool inK = i < nAtoms && cand[a] && (i == g || !(Q[a] < kWeightScreen * sumA));
+                if (inK) zeros[i] |= 0x80;
+                maskK[a] = __ballot(inK);
             }
         }
         __syncthreads();
@@ -546,9 +551,11 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                 const double ekx = (xk - rx) * iR[k], eky = (yk - ry) * iR[k], ekz = (zk - rz) * iR[k];
                 const double fk = k == g ? ew * (1.0 - pW) : -ew * pW;
                 double sk = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
-                for (int m = 0; m < nAtoms; m++) {
+                for (int a2 = 0; a2 < LA; a2++)
+                for (unsigned long long mk = maskK[a2]; mk != 0ull; mk &= mk - 1ull) {
+                    const int m = 64 * a2 + __ffsll(static_cast<long long>(mk)) - 1;
+                    if (m == k) continue;
                     const int zm = zeros[m];
-                    if (!(zm & 0x80) || m == k) continue;
                     const int km = pair_index(k, m);
                     const double r = invR[km], am = aij[km], Rm = R[m];
                     const bool kHigh = k > m;
@@ -594,9 +601,10 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
             }
             // Pairs (k2, m) with k2 in K and m outside it (their pair (m, k2) terms are below the screen): lanes own m,
             // the terms that move atom k2 are summed over the wave.
-            for (int k2 = 0; k2 < nAtoms; k2++) {
+            for (int a2 = 0; a2 < LA; a2++)
+            for (unsigned long long mk = maskK[a2]; mk != 0ull; mk &= mk - 1ull) {
+                const int k2 = 64 * a2 + __ffsll(static_cast<long long>(mk)) - 1;
                 const int z2 = zeros[k2];
-                if (!(z2 & 0x80)) continue;
                 const int nz2 = z2 & 0x7f;
                 const double x2 = xyz[3 * k2], y2 = xyz[3 * k2 + 1], w2 = xyz[3 * k2 + 2];
                 const double R2 = R[k2], p2 = prodNZ[k2];
