@@ -12,9 +12,10 @@
 //
 // Becke weight: P_A / sum_i P_i with P_i = prod_{j != i} s(nu_ij).  P_A (the point's own atom) is formed in full.  For
 // i != A the product is formed factor by factor, starting with the factor from atom A and then nearest atom to A first
-// (the factors that can make it vanish come early), and stops once it falls below 2^-64 P_A: all later factors lie in
-// [0, 1], so the dropped part of sum_i P_i is below nAtoms * 2^-64 of P_A <= the sum.  This turns the reference's
-// O(nAtoms^2) pair loop per point into O(nAtoms) for points that sit in their own atom's cell.
+// (the factors that can make it vanish come early), and stops once it falls below 2^-64 max(|P_A|, |P_near|), P_near
+// the product of the atom nearest the point: all later factors lie in [0, 1], so the dropped part of sum_i P_i is below
+// nAtoms * 2^-64 of max(P_A, P_near) <= the sum.  This turns the reference's O(nAtoms^2) pair loop per point into
+// O(nAtoms) for points that sit in their own atom's cell or in the cell of the atom nearest them.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -179,8 +180,8 @@ __device__ __forceinline__ int pair_index(int k, int m) {
 // nearest the point come first; when even P_A / (P_A + P_near) leaves the weight below half the cull tolerance (or
 // P_A = 0), the point is culled (as the reference culls it) without the other products.  The atom coordinates, the pair
 // data of the shell's atom A (row A of 1/R and a) and the atoms in order of distance from A are staged in LDS.  P_A is
-// computed in full; each other P_i takes its factors nearest atom first and stops once it falls below 2^-64 P_A, where
-// it no longer changes the sum beyond rounding.  Each factor is the reference's cell function of the same mu and nu,
+// computed in full; each other P_i takes its factors nearest atom first and stops once it falls below 2^-64 of
+// max(|P_A|, |P_near|) (both <= the sum), where it no longer changes the sum beyond rounding.  Each factor is the reference's cell function of the same mu and nu,
 // rounded as the reference rounds it (so factors that vanish there vanish here); the products are taken in another
 // order, so weights agree with the reference's to rounding and the same points pass the cull.
 __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell *__restrict__ shells,
@@ -222,7 +223,8 @@ __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell
             if (j == A) continue;
             PA *= becke_factor_of(A > j, rA, dist(j, px, py, pz), sInv[j], sA[j]);
         }
-        const double floor = PA * 0x1p-64;
+        // Products below 2^-64 of max(|P_A|, |P_near|) <= sum_i P_i no longer change the sum beyond rounding.
+        double floor = fabs(PA) * 0x1p-64;
         const double wru = sh.wr * u[3];
         const int o = sh.out + k;
         cx[o] = px;
@@ -247,6 +249,7 @@ __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell
                 Pn *= becke_factor_of(near > j, rn, dist(j, px, py, pz), invR[ij], aij[ij]);
             }
             sum += Pn;
+            floor = fmax(floor, fabs(Pn) * 0x1p-64);
             if (PA == 0.0 || (sum > 0.0 && fabs(wru) * (fabs(PA) / sum) < 0.5 * kWeightTol)) {
                 cw[o] = 0.0;
                 keep[o] = 0;
@@ -376,8 +379,8 @@ constexpr int kMaxLaneAtoms = 11;
 // |fac_k| = |eXC w| / sum_i A_i for k != g.  Atoms with Q_k < 2^-80 sum_i A_i (k != g) are outside the set K: the terms of
 // pairs (k, m) with k outside K and of pairs (m, k) with k outside K are each below 3.375 2^-80 |eXC w| 2 / R_km, and
 // are left out (below 1e-15 hartree/bohr in all for any molecule this kernel takes); every other pair term is computed.
-// sum_i A_i is taken over the atoms whose bound passes half the screen at A_g (K lies among them); each atom left out of
-// it has A_i below 2^-81 A_g.
+// sum_i A_i is taken over the atoms whose bound passes half the screen at max(A_g, A_near), A_near the product of the
+// atom nearest the point (K lies among them); each atom left out of it has A_i below 2^-81 max(A_g, A_near).
 constexpr double kWeightScreen = 0x1p-80;
 constexpr int kNearAtoms = 24;           // atoms whose factors bound a cell product (the point's atom and its nearest)
 
@@ -440,9 +443,10 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                 iR[i] = 1.0 / R[i];
             }
         __syncthreads();
-        // Products of the cell factors, their zero counts, and the screened set K (flag 0x80 of the count).  A_g is a
-        // lower bound of sum_i A_i; the partial product of atom i over the atoms nearest g bounds every product of i
-        // with one factor left out (|s| <= 1), so an atom whose bound is below the screen at A_g is outside K.  The
+        // Products of the cell factors, their zero counts, and the screened set K (flag 0x80 of the count).  A_g and the
+        // product of the atom nearest the point are lower bounds of sum_i A_i; the partial product of atom i over the
+        // atoms nearest g bounds every product of i with one factor left out (|s| <= 1), so an atom whose bound is below
+        // the screen at the larger of the two is outside K.  The
         // remaining candidates take their full products with the factors spread over the wave.  The others keep a zero
         // product and a count of 2, which leaves their terms out, as the screen leaves out those of the atoms outside K.
         double sumA = 0.0;
@@ -477,8 +481,23 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
             double pg, mg;
             int zg;
             full_product(g, pg, zg, mg);
-            const double bound = zg == 0 ? 0x1p-81 * fabs(pg) : 0.0;   // half the screen at A_g
-            const int *near = nearAtoms + static_cast<size_t>(g) * nNear;
+            // The atom nearest the point other than g (the wave's arg-min of R, ties to the lower index) and its product:
+            // sum_i A_i >= max(A_g, A_near), which bounds the screen for points deep in another atom's cell too.
+            double dn = 0.0;
+            int near = -1;
+            for (int j = lane; j < nAtoms; j += 64)
+                if (j != g && (near < 0 || R[j] < dn)) { dn = R[j]; near = j; }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const double od = __shfl_xor(dn, o, 64);
+                const int on = __shfl_xor(near, o, 64);
+                if (on >= 0 && (near < 0 || od < dn || (od == dn && on < near))) { dn = od; near = on; }
+            }
+            double pn = 0.0, mn = 1.0;
+            int zn = 2;
+            if (near >= 0) full_product(near, pn, zn, mn);
+            const double bound = 0x1p-81 * fmax(zg == 0 ? fabs(pg) : 0.0, zn == 0 ? fabs(pn) : 0.0);   // half the screen
+            const int *nearList = nearAtoms + static_cast<size_t>(g) * nNear;
             bool cand[LA];
             double Q[LA];
 #pragma unroll
@@ -491,7 +510,7 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                     double pp = 1.0, mp = 1.0;
                     int zp = 0;
                     for (int q = 0; q < nNear; q++) {
-                        const int j = near[q];
+                        const int j = nearList[q];
                         if (j == i) continue;
                         const int ij = pair_index(i, j);
                         const double s = fabs(becke_factor_of(i > j, R[i], R[j], invR[ij], aij[ij]));
@@ -509,14 +528,15 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
             }
 #pragma unroll
             for (int a = 0; a < LA; a++) {
-                unsigned long long mask = __ballot(cand[a] || lane + 64 * a == g);
+                unsigned long long mask = __ballot(cand[a] || lane + 64 * a == g || lane + 64 * a == near);
                 while (mask != 0ull) {
                     const int bit = __ffsll(static_cast<long long>(mask)) - 1;
                     mask &= mask - 1ull;
                     const int c = 64 * a + bit;
                     double pc = pg, mc = mg;
                     int zc = zg;
-                    if (c != g) full_product(c, pc, zc, mc);
+                    if (c == near) { pc = pn; mc = mn; zc = zn; }
+                    else if (c != g) full_product(c, pc, zc, mc);
                     if (zc == 0) sumA += pc;
                     if (lane == bit) {
                         cand[a] = true;
