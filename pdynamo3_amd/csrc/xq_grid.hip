@@ -11,8 +11,8 @@
 // reference orders them, so the order-preserving cull reproduces the reference's point and block order.
 //
 // Becke weight: P_A / sum_i P_i with P_i = prod_{j != i} s(nu_ij).  P_A (the point's own atom) is formed in full.  For
-// i != A the product is formed factor by factor, starting with the factor from atom A and then nearest atom to A first
-// (the factors that can make it vanish come early), and stops once it falls below 2^-64 max(|P_A|, |P_near|), P_near
+// i != A the product is formed factor by factor, starting with the factors from atom A and from the atom nearest the
+// point, then the others nearest atom to A first (the factors that can make it vanish come early), and stops once it falls below 2^-64 max(|P_A|, |P_near|), P_near
 // the product of the atom nearest the point: all later factors lie in [0, 1], so the dropped part of sum_i P_i is below
 // nAtoms * 2^-64 of max(P_A, P_near) <= the sum.  This turns the reference's O(nAtoms^2) pair loop per point into
 // O(nAtoms) for points that sit in their own atom's cell or in the cell of the atom nearest them.
@@ -180,9 +180,9 @@ __device__ __forceinline__ int pair_index(int k, int m) {
 // nearest the point come first; when even P_A / (P_A + P_near) leaves the weight below half the cull tolerance (or
 // P_A = 0), the point is culled (as the reference culls it) without the other products.  The atom coordinates, the pair
 // data of the shell's atom A (row A of 1/R and a) and the atoms in order of distance from A are staged in LDS.  P_A is
-// computed in full; each other P_i takes its factors nearest atom first and stops once it falls below 2^-64 of
-// max(|P_A|, |P_near|) (both <= the sum), where it no longer changes the sum beyond rounding.  Each factor is the reference's cell function of the same mu and nu,
-// rounded as the reference rounds it (so factors that vanish there vanish here); the products are taken in another
+// computed in full; each other P_i takes its factors with A and the nearest atom, then the others nearest A first, and
+// stops once it falls below 2^-64 of max(|P_A|, |P_near|) (both <= the sum), where it no longer changes the sum beyond
+// rounding.  Each factor is the reference's cell function of the same mu and nu, rounded as the reference rounds it (so factors that vanish there vanish here); the products are taken in another
 // order, so weights agree with the reference's to rounding and the same points pass the cull.
 __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell *__restrict__ shells,
                                                                const double *__restrict__ rules,
@@ -239,9 +239,9 @@ __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell
             const double dx = sx[j] - px, dy = sy[j] - py, dz = sz[j] - pz, d2 = dx * dx + dy * dy + dz * dz;
             if (near < 0 || d2 < dNear) { near = j; dNear = d2; }
         }
-        double sum = PA;
+        double sum = PA, rn = 0.0;
         if (near >= 0) {
-            const double rn = dist(near, px, py, pz);
+            rn = dist(near, px, py, pz);
             double Pn = becke_factor_of(near > A, rn, rA, sInv[near], sA[near]);
             for (int j = 0; j < nAtoms; j++) {
                 if (j == near || j == A) continue;
@@ -260,9 +260,13 @@ __global__ void __launch_bounds__(256) candidate_points_kernel(const RadialShell
             if (i == A || i == near) continue;
             const double ri = dist(i, px, py, pz);
             double Pi = becke_factor_of(i > A, ri, rA, sInv[i], sA[i]);
+            if (near >= 0 && fabs(Pi) >= floor) {   // then the atom nearest the point
+                const int in = pair_index(i, near);
+                Pi *= becke_factor_of(i > near, ri, rn, invR[in], aij[in]);
+            }
             for (int jj = 0; jj < nAtoms && fabs(Pi) >= floor; jj++) {
                 const int j = sOrd[jj];
-                if (j == i || j == A) continue;
+                if (j == i || j == A || j == near) continue;
                 const int ij = pair_index(i, j);
                 Pi *= becke_factor_of(i > j, ri, dist(j, px, py, pz), invR[ij], aij[ij]);
             }
