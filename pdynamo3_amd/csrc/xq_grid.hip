@@ -410,10 +410,11 @@ __device__ __forceinline__ double excluded_product(double prodNZ, int zeros, dou
 // registers.  Per point (grid atom g, weight w, energy density eXC = eps rho): A_i = prod_j s_ij; p = A_g / sum_i A_i;
 // ew = eXC w / (p sum_i A_i); dA_i/dmu_ij = ds_ij prod_{k != j} s_ik; for every ordered pair (i, j) the reference adds
 // ifac = dA_i/dmu_ij fac_i, fac_i = ew (1 - p) for i = g and -ew p otherwise, times the derivatives of mu_ij with respect
-// to atom i, atom j and the point (which moves with atom g).  Each lane adds, for its atom k, the terms of the pairs
-// (k, m) and (m, k) that move k, and the point terms of the pairs (k, m); the reference's O(nAtoms^3) update of dA/dmu
-// becomes O(nAtoms^2) per point, and the screen (kWeightScreen) leaves O(|K| nAtoms): the atoms whose cell product is
-// not negligible at the point.  Both factors of a pair come from one polynomial (s_km and s_mk = 1 - s_km share nu), and
+// to atom i, atom j and the point (which moves with atom g).  The pair loop runs over m in K (wave-uniform) and the
+// lane's atoms k, evaluating each pair once: the terms of (k, m) and (m, k) that move k, the point term of (k, m), and
+// for k outside K the terms of (m, k) that move m (summed over the wave) and the point; the reference's O(nAtoms^3)
+// update of dA/dmu becomes O(nAtoms^2) per point, and the screen (kWeightScreen) leaves O(|K| nAtoms): the atoms whose
+// cell product is not negligible at the point.  Both factors of a pair come from one polynomial (s_km and s_mk = 1 - s_km share nu), and
 // one division gives both 1 / s_km and 1 / s_mk.
 // LDS per wave: the distances R, their inverses and the non-zero products (3 nAtoms doubles), and the zero counts (nAtoms
 // bytes, after the doubles of all four waves).
@@ -564,27 +565,40 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
             const double pW = Ag / sumA;
             const double ew = eXC[p] * pw[p] / (pW * sumA);
             double gs[3] = {0.0, 0.0, 0.0};
-            // Pairs with a partner in K, for every atom k of the lane.
+            // The lane's atoms k (their data held in registers across the pair loop).
+            double kR[LA], kP[LA], kf[LA];
+            int kNZ[LA];
+            bool kIn[LA];
 #pragma unroll
             for (int a = 0; a < LA; a++) {
-                const int k = lane + 64 * a;
-                if (k >= nAtoms) break;
-                const double xk = xyz[3 * k], yk = xyz[3 * k + 1], zk = xyz[3 * k + 2];
-                const double Rk = R[k], pk = prodNZ[k];
-                const int nzk = zeros[k] & 0x7f;
-                const double ekx = (xk - rx) * iR[k], eky = (yk - ry) * iR[k], ekz = (zk - rz) * iR[k];
-                const double fk = k == g ? ew * (1.0 - pW) : -ew * pW;
-                double sk = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
-                for (int a2 = 0; a2 < LA; a2++)
-                for (unsigned long long mk = maskK[a2]; mk != 0ull; mk &= mk - 1ull) {
-                    const int m = 64 * a2 + __ffsll(static_cast<long long>(mk)) - 1;
-                    if (m == k) continue;
-                    const int zm = zeros[m];
+                const int k = lane + 64 * a < nAtoms ? lane + 64 * a : 0;
+                kR[a] = R[k];
+                kP[a] = prodNZ[k];
+                kNZ[a] = zeros[k] & 0x7f;
+                kIn[a] = (zeros[k] & 0x80) != 0;
+                kf[a] = k == g ? ew * (1.0 - pW) : -ew * pW;
+            }
+            // Every pair (k, m) with m in K and k a lane atom, each evaluated once: the terms of (k, m) and (m, k) that
+            // move k and the point term of (k, m); for k outside K also the terms of (m, k) that move m and the point
+            // (the terms of (k, m) that move m are below the screen), summed over the wave for m.
+            for (int a2 = 0; a2 < LA; a2++)
+            for (unsigned long long mk = maskK[a2]; mk != 0ull; mk &= mk - 1ull) {
+                const int m = 64 * a2 + __ffsll(static_cast<long long>(mk)) - 1;
+                const int nzm = zeros[m] & 0x7f;
+                const double xm = xyz[3 * m], ym = xyz[3 * m + 1], zmc = xyz[3 * m + 2];
+                const double Rm = R[m], pm = prodNZ[m], iRm = iR[m];
+                const double emx = (xm - rx) * iRm, emy = (ym - ry) * iRm, emz = (zmc - rz) * iRm;
+                const double fm = m == g ? ew * (1.0 - pW) : -ew * pW;
+                double hx = 0.0, hy = 0.0, hz = 0.0;
+#pragma unroll
+                for (int a = 0; a < LA; a++) {
+                    const int k = lane + 64 * a;
+                    if (k >= nAtoms || k == m) continue;
                     const int km = pair_index(k, m);
-                    const double r = invR[km], am = aij[km], Rm = R[m];
+                    const double r = invR[km], am = aij[km];
                     const bool kHigh = k > m;
                     // mu and nu of the pair stored as (max, min), as the cell factors take them.
-                    const double muP = (kHigh ? Rk - Rm : Rm - Rk) * r;
+                    const double muP = (kHigh ? kR[a] - Rm : Rm - kR[a]) * r;
                     const double nu = muP + am * (1.0 - muP * muP), nu2 = nu * nu;
                     const double S = kBeckeA * becke_poly(nu);
                     const double skm = kHigh ? 0.5 - S : 0.5 + S, smk = kHigh ? 0.5 + S : 0.5 - S;
@@ -602,65 +616,29 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                         invKM = 1.0 / skm;
                         invMK = 1.0 / smk;
                     }
-                    const double fm = m == g ? ew * (1.0 - pW) : -ew * pW;
-                    const double ifKM = dsum * excluded_product(pk, nzk, skm, invKM) * fk;
-                    const double ifMK = dsum * excluded_product(prodNZ[m], zm & 0x7f, smk, invMK) * fm;
-                    const double xm = xyz[3 * m], ym = xyz[3 * m + 1], zmc = xyz[3 * m + 2];
+                    const double ifKM = dsum * excluded_product(kP[a], kNZ[a], skm, invKM) * kf[a];
+                    const double ifMK = dsum * excluded_product(pm, nzm, smk, invMK) * fm;
                     const double mu = kHigh ? muP : -muP, mr2 = mu * r * r;
+                    const double xk = xyz[3 * k], yk = xyz[3 * k + 1], zk = xyz[3 * k + 2], iRk = iR[k];
+                    const double kex = (xk - rx) * iRk, key = (yk - ry) * iRk, kez = (zk - rz) * iRk;
+                    const double dx = xk - xm, dy = yk - ym, dz = zk - zmc;
                     // Pair (k, m) moves atom k through d mu / d r_k and the point through -(d mu / d r_k + d mu / d r_m);
                     // pair (m, k) moves atom k through its second atom: both along r (e_k) - mu r^2 (r_k - r_m).
                     const double d = ifKM - ifMK;
-                    G[a][0] += d * (ekx * r - (xk - xm) * mr2);
-                    G[a][1] += d * (eky * r - (yk - ym) * mr2);
-                    G[a][2] += d * (ekz * r - (zk - zmc) * mr2);
-                    const double wr = ifKM * r, iRm = iR[m];
-                    sk += wr;
-                    vx += wr * ((xm - rx) * iRm);
-                    vy += wr * ((ym - ry) * iRm);
-                    vz += wr * ((zmc - rz) * iRm);
-                }
-                gs[0] += vx - sk * ekx;
-                gs[1] += vy - sk * eky;
-                gs[2] += vz - sk * ekz;
-            }
-            // Pairs (k2, m) with k2 in K and m outside it (their pair (m, k2) terms are below the screen): lanes own m,
-            // the terms that move atom k2 are summed over the wave.
-            for (int a2 = 0; a2 < LA; a2++)
-            for (unsigned long long mk = maskK[a2]; mk != 0ull; mk &= mk - 1ull) {
-                const int k2 = 64 * a2 + __ffsll(static_cast<long long>(mk)) - 1;
-                const int z2 = zeros[k2];
-                const int nz2 = z2 & 0x7f;
-                const double x2 = xyz[3 * k2], y2 = xyz[3 * k2 + 1], w2 = xyz[3 * k2 + 2];
-                const double R2 = R[k2], p2 = prodNZ[k2];
-                const double e2x = (x2 - rx) * iR[k2], e2y = (y2 - ry) * iR[k2], e2z = (w2 - rz) * iR[k2];
-                const double f2 = k2 == g ? ew * (1.0 - pW) : -ew * pW;
-                double hx = 0.0, hy = 0.0, hz = 0.0;
-#pragma unroll
-                for (int a = 0; a < LA; a++) {
-                    const int m = lane + 64 * a;
-                    if (m >= nAtoms) break;
-                    if (zeros[m] & 0x80) continue;
-                    const int km = pair_index(k2, m);
-                    const double r = invR[km], am = aij[km], Rm = R[m];
-                    const bool kHigh = k2 > m;
-                    const double muP = (kHigh ? R2 - Rm : Rm - R2) * r;
-                    const double nu = muP + am * (1.0 - muP * muP), nu2 = nu * nu;
-                    const double S = kBeckeA * becke_poly(nu);
-                    const double s2m = kHigh ? 0.5 - S : 0.5 + S;
-                    double dsum = c_beckeD[10];
-#pragma unroll
-                    for (int tt = 9; tt >= 0; tt--) dsum = dsum * nu2 + c_beckeD[tt];
-                    dsum *= -kBeckeA * (1.0 - 2.0 * am * muP);
-                    const double ifKM = dsum * excluded_product(p2, nz2, s2m, nz2 == 0 ? 1.0 / s2m : 0.0) * f2;
-                    const double xm = xyz[3 * m], ym = xyz[3 * m + 1], zmc = xyz[3 * m + 2];
-                    const double mu = kHigh ? muP : -muP, mr2 = mu * r * r;
-                    hx += ifKM * (e2x * r - (x2 - xm) * mr2);
-                    hy += ifKM * (e2y * r - (y2 - ym) * mr2);
-                    hz += ifKM * (e2z * r - (w2 - zmc) * mr2);
-                    const double wr = ifKM * r, iRm = iR[m];
-                    gs[0] += wr * ((xm - rx) * iRm) - wr * e2x;
-                    gs[1] += wr * ((ym - ry) * iRm) - wr * e2y;
-                    gs[2] += wr * ((zmc - rz) * iRm) - wr * e2z;
+                    G[a][0] += d * (kex * r - dx * mr2);
+                    G[a][1] += d * (key * r - dy * mr2);
+                    G[a][2] += d * (kez * r - dz * mr2);
+                    double wr = ifKM * r;
+                    if (!kIn[a]) {
+                        // Pair (m, k) moves atom m along r (e_m) - (-mu) r^2 (r_m - r_k) = r (e_m) - mu r^2 (r_k - r_m).
+                        hx += ifMK * (emx * r - dx * mr2);
+                        hy += ifMK * (emy * r - dy * mr2);
+                        hz += ifMK * (emz * r - dz * mr2);
+                        wr -= ifMK * r;
+                    }
+                    gs[0] += wr * (emx - kex);
+                    gs[1] += wr * (emy - key);
+                    gs[2] += wr * (emz - kez);
                 }
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) {
@@ -668,10 +646,10 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
                     hy += __shfl_xor(hy, o, 64);
                     hz += __shfl_xor(hz, o, 64);
                 }
-                if (lane == (k2 & 63)) {
+                if (lane == (m & 63)) {
 #pragma unroll
                     for (int a = 0; a < LA; a++)
-                        if (a == (k2 >> 6)) { G[a][0] += hx; G[a][1] += hy; G[a][2] += hz; }
+                        if (a == (m >> 6)) { G[a][0] += hx; G[a][1] += hy; G[a][2] += hz; }
                 }
             }
 #pragma unroll
