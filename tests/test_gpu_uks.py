@@ -148,3 +148,25 @@ def test_drop_in_fock_unrestricted(oracle_lib):
     assert abs(e - e0) <= 1e-9
     assert np.max(np.abs(target.scratch.onePDMP.fock - 0.5 * (Va0 + Vb0))) <= 1e-9
     assert np.max(np.abs(target.scratch.onePDMQ.fock - 0.5 * (Va0 - Vb0))) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fully_polarised_hydrogen_matches_oracle(oracle_lib, precision):
+    """One hydrogen atom with all density in the alpha spin (zeta = 1); its XC gradient vanishes."""
+    from pdynamo3_amd import basis
+    Z = np.array([1], dtype=np.int32)
+    xyz = np.zeros((1, 3))
+    spec = basis.element_spec(basis.load_basis(), Z)
+    Pa, Pb = np.array([0.30, 0.25, 0.45]), np.zeros(3)
+    tolE, tolV = _tol(precision)
+    g = _grid(Z, xyz, spec, True, precision)
+    for name in ("blyp", "pbe", "lda", "b3lyp"):
+        ids = list(functional_ids(name))
+        e0, n0, Va0, Vb0, _ = oracle_lib.setup(2, Z, xyz, spec, True, ids).integrate_uks(Pa, Pb, 2)
+        Va, Vb = np.zeros(3), np.zeros(3)
+        e, ne, _ = g.integrate_uks(ids, Pa, Pb, Va, Vb)
+        print(name, precision, "dE", e - e0, "dVa", np.max(np.abs(Va - Va0)), "dVb", np.max(np.abs(Vb - Vb0)))
+        assert abs(e - e0) <= tolE and abs(ne - n0) <= tolE
+        assert np.max(np.abs(Va - Va0)) <= tolV and np.max(np.abs(Vb - Vb0)) <= tolV
+        assert np.max(np.abs(g.gradients_uks(ids, Pa, Pb))) <= 1e-12
+    g.close()

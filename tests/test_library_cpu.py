@@ -51,6 +51,12 @@ def test_argument_errors_return_status_codes():
     ids = np.array([106, 131], dtype=np.int32)
     assert _lib.lib.xq_integrate(None, 2, _lib.iptr(ids), _lib.dptr(x), ctypes.byref(e), ctypes.byref(n),
                                  _lib.dptr(x), None) == 3
+    # The unrestricted entry points and the gradients check their arguments before any device work.
+    assert _lib.lib.xq_integrate_uks(None, 2, _lib.iptr(ids), _lib.dptr(x), _lib.dptr(x), ctypes.byref(e),
+                                     ctypes.byref(n), _lib.dptr(x), _lib.dptr(x), None) == 3
+    assert _lib.lib.xq_gradients(None, 2, _lib.iptr(ids), _lib.dptr(x), _lib.dptr(x)) == 3
+    assert _lib.lib.xq_gradients_uks(None, 2, _lib.iptr(ids), _lib.dptr(x), _lib.dptr(x), _lib.dptr(x)) == 3
+    assert "invalid" in _lib.last_error()
 
 
 def test_memory_estimate_scales():

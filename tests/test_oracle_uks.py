@@ -146,3 +146,24 @@ def test_uks_gradients_sum_to_zero():
     g = _oracle().setup(2, Z, xyz, spec, True, list(functional_ids("b3lyp"))).gradients_uks(Pa, Pb, 8)
     assert np.max(np.abs(g.sum(axis=0))) <= 1e-11
     assert np.max(np.abs(g)) > 1e-3
+
+
+def test_fully_polarised_hydrogen_matches_reference():
+    """One hydrogen atom with all density in the alpha spin (zeta = 1, the functionals' threshold branch)."""
+    import oracle
+    if not oracle.reference_available():
+        pytest.skip("compiled reference not built (make -C oracle ref)")
+    from pdynamo3_amd import basis
+    Z = np.array([1], dtype=np.int32)
+    xyz = np.zeros((1, 3))
+    spec = basis.element_spec(basis.load_basis(), Z)
+    Pa = np.array([0.30, 0.25, 0.45])    # packed 2 x 2 (two functions on H), positive definite
+    Pb = np.zeros(3)
+    for name in ("blyp", "pbe", "lda", "b3lyp"):
+        ids = list(functional_ids(name))
+        e0, n0, Va0, Vb0, _ = oracle.reference().setup(2, Z, xyz, spec, True, ids).integrate_uks(Pa, Pb, 1)
+        h = _oracle().setup(2, Z, xyz, spec, True, ids)
+        e, ne, Va, Vb, _ = h.integrate_uks(Pa, Pb, 2)
+        assert abs(e - e0) <= 1e-12 and abs(ne - n0) <= 1e-12, name
+        assert np.max(np.abs(Va - Va0)) <= 1e-12 and np.max(np.abs(Vb - Vb0)) <= 1e-12, name
+        assert np.max(np.abs(h.gradients_uks(Pa, Pb, 2))) == 0.0
