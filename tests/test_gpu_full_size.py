@@ -1,0 +1,60 @@
+"""The benchmark workloads at their full size on the GPU, through properties that do not need the oracle's time:
+
+- (H2O)64 (14 A cube, seed 2), 6-31G* Cartesian, BLYP and (H2O)128 (18 A cube, seed 4), PBE: the mixed-precision
+  quadrature against the FP64 path on the same grid and density, |dE_xc| <= 1e-6 Eh and max|dV_xc| <= 1e-7 (the stated
+  tolerance of the mixed path); the electron count within 1e-5 per electron of 10 per water;
+- (H2O)64: the XC nuclear gradients (restricted, and unrestricted at alpha = 0.55 P, beta = 0.45 P) sum to zero over the
+  atoms (the grid moves with its atoms), and the closed-shell limit of the unrestricted gradients is the restricted one.
+The FP64 path itself is pinned against the oracle and the compiled reference on smaller systems (test_gpu_parity.py).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cluster(n, cube, seed):
+    from pdynamo3_amd import basis, densities, molecules
+    Z, xa, rots, _ = molecules.water_cluster(n, cube, seed)
+    spec = basis.element_spec(basis.load_basis("6-31g_st"), Z)
+    return Z, molecules.to_bohr(xa), spec, densities.cluster_density(rots, spherical=False)
+
+
+@pytest.mark.parametrize("n, cube, seed, name", [(64, 14.0, 2, "blyp"), (128, 18.0, 4, "pbe")])
+def test_mixed_against_fp64_at_full_size(n, cube, seed, name):
+    from pdynamo3_amd import integrator
+    from pdynamo3_amd.definitions import functional_ids
+    Z, xyz, spec, P = _cluster(n, cube, seed)
+    ids = list(functional_ids(name))
+    g = integrator.QuadratureGrid(Z, xyz, 2)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        g.set_basis(spec, False, precision)
+        V = np.zeros_like(P)
+        e, ne, _ = g.integrate(ids, P, V)
+        out[precision] = (e, ne, V)
+    g.close()
+    (e0, n0, V0), (e1, n1, V1) = out["fp64"], out["mixed"]
+    print(n, name, "E", e0, "dE", e1 - e0, "max dV", np.max(np.abs(V1 - V0)), "N", n0)
+    assert abs(e1 - e0) <= 1e-6
+    assert np.max(np.abs(V1 - V0)) <= 1e-7
+    assert abs(n0 - 10.0 * n) <= 1e-5 * 10.0 * n and abs(n1 - n0) <= 1e-6   # grid quadrature error, ~1e-6 per electron
+
+
+def test_h2o64_gradients_are_translation_invariant():
+    from pdynamo3_amd import integrator
+    from pdynamo3_amd.definitions import functional_ids
+    Z, xyz, spec, P = _cluster(64, 14.0, 2)
+    ids = list(functional_ids("blyp"))
+    g = integrator.QuadratureGrid(Z, xyz, 2)
+    g.set_basis(spec, False, "mixed")
+    G = g.gradients(ids, P)
+    Gu = g.gradients_uks(ids, 0.55 * P, 0.45 * P)
+    Gh = g.gradients_uks(ids, 0.5 * P, 0.5 * P)
+    g.close()
+    print("max G", np.max(np.abs(G)), "sum", np.max(np.abs(G.sum(0))), "uks sum", np.max(np.abs(Gu.sum(0))),
+          "closed-shell limit", np.max(np.abs(Gh - G)))
+    assert np.max(np.abs(G)) > 1e-2
+    assert np.max(np.abs(G.sum(0))) <= 1e-9 and np.max(np.abs(Gu.sum(0))) <= 1e-9
+    assert np.max(np.abs(Gh - G)) <= 1e-9
