@@ -41,7 +41,11 @@ def parse():
     p.add_argument("--workload", default="h2o64", choices=sorted(WORKLOADS))
     p.add_argument("--spherical", action="store_true", help="spherical d (reference default); default Cartesian d")
     p.add_argument("--cpu-baseline", default="reference", choices=["reference", "port", "none"])
-    p.add_argument("--cpu-sample-waters", type=int, default=8)
+    p.add_argument("--cpu-sample", default="h2o16", choices=sorted(WORKLOADS),
+                   help="workload the CPU baseline is timed on (default the (H2O)16 PBE cluster, the configuration whose "
+                        "quadrature is compared with the reference's CPU path)")
+    p.add_argument("--cpu-sample-waters", type=int, default=8,
+                   help="waters of the extra, smaller CPU sample and of the gradient leg's CPU sample (0 skips the extra)")
     p.add_argument("--skip-fp64", action="store_true")
     p.add_argument("--uks-steps", type=int, default=3,
                    help="timed unrestricted quadratures (alpha 0.55 P, beta 0.45 P) on the workload, each precision "
@@ -76,15 +80,19 @@ def copy_calibration():
 
 
 def measured_traffic(kernel, workload):
-    """HBM bytes per launch of a kernel from its committed rocprofv3 counter summary, or None."""
+    """HBM bytes per launch of a kernel from its committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE runs, each
+    scaled by a plain copy's reading) and the commit the counters were taken at, or (None, None)."""
     path = os.path.join(ROOT, "profiles", "{:s}_{:s}_pmc.txt".format(kernel, workload))
     if not os.path.exists(path):
-        return None
+        return None, None
+    value, commit = None, None
     with open(path) as f:
         for line in f:
             if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    return None
+                value = float(line.split(":", 1)[1])
+            elif line.startswith("profiled_at_commit:"):
+                commit = line.split(":", 1)[1].strip()
+    return value, commit
 
 
 def log(*a):
@@ -114,9 +122,9 @@ def run_leg(grid, ids, P, precision, steps, warmup, spec, spherical):
             "device_bytes": grid.device_bytes()}
 
 
-def cpu_baseline(kind, n_waters, functional, spherical, threads=1):
-    """CPU quadrature of a (H2O)n sample cut from the same generator; points/s.  The compiled reference is
-    single-threaded; the port (the oracle's C restatement) runs its blocks on `threads` OpenMP threads."""
+def cpu_baseline(kind, n_waters, cube, seed, functional, spherical, threads=1, seconds=10.0):
+    """CPU quadrature of a (H2O)n cluster from the same generator (cube side in angstrom, seed); points/s.  The compiled
+    reference is single-threaded; the port (the oracle's C restatement) runs its blocks on `threads` OpenMP threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from pdynamo3_amd import basis, densities, molecules
@@ -124,23 +132,26 @@ def cpu_baseline(kind, n_waters, functional, spherical, threads=1):
     if kind == "reference" and not oracle.reference_available():
         kind = "port"
     lib = oracle.reference() if kind == "reference" else oracle.oracle()
-    Z, xa, rots, _ = molecules.water_cluster(n_waters, 9.0 * (n_waters / 16.0) ** (1.0 / 3.0), 1)
+    Z, xa, rots, _ = molecules.water_cluster(n_waters, cube, seed)
     xyz = molecules.to_bohr(xa)
     spec = basis.element_spec(basis.load_basis(), Z)
     P = densities.cluster_density(rots, spherical=spherical)
     h = lib.setup(2, Z, xyz, spec, spherical, functional_ids(functional))
     # The reference's first call evaluates and caches the basis values (inCore); the in-core calls after it are timed,
-    # as the GPU's are, repeated to about 10 s of CPU work.
+    # as the GPU's are, repeated to about `seconds` of CPU work.
     threads = 1 if kind == "reference" else threads
     h.integrate(P, threads)
     _, _, _, sec = h.integrate(P, threads)
-    reps = max(1, int(np.ceil(10.0 / max(sec, 1e-3))))
-    total = 0.0
+    reps = max(1, int(np.ceil((seconds - sec) / max(sec, 1e-3))))
+    total = sec
     for _ in range(reps):
         total += h.integrate(P, threads)[3]
+    reps += 1
     return {"value": h.npoints * reps / total, "unit": "grid points/s", "cores": threads, "kind": kind,
-            "sample": "(H2O){:d} cluster, same generator, {:s}, Medium grid ({:d} points), {:d} in-core quadrature calls, "
-                      "{:.1f} s".format(n_waters, functional.upper(), h.npoints, reps, total)}
+            "sample": "(H2O){:d} cluster, {:.0f} A cube, seed {:d}, {:s}, 6-31G* {:s}, Medium grid ({:d} points), "
+                      "{:d} in-core quadrature calls after the first, {:.1f} s".format(
+                          n_waters, cube, seed, functional.upper(), "spherical" if spherical else "Cartesian",
+                          h.npoints, reps, total)}
 
 
 def uks_leg(grid, ids, P, steps, spec, spherical):
@@ -275,19 +286,28 @@ def main():
     ms = float(np.mean(mixed["device_ms"]))
     value = G / (ms * 1e-3)
     # Roofline of the dominant kernel: density_kernel_mixed (one kernel between the rho events; the Fock stage is two
-    # kernels).  Both its HBM time and its int8 MFMA time at peak are computed; the larger names the bound.
+    # kernels).  Algorithmic bytes: every value of a kept function at a point read once (17 bytes: 5 Phi digits and
+    # three 32-bit fixed-point gradients), P digits, weights, W coefficients and block sums (stats[6]).  Its int8 matrix
+    # work at peak is far shorter than its bytes at peak, so the HBM roof is the one that binds.
     rho_ms = float(np.mean(mixed["rho_ms"]))
     rho_bytes = st["rho_bytes"]
-    int8_ops = 2.0 * 14.0 * st["bf_squares"]          # 14 digit products per element of P_red Phi
+    flops = 2.0 * st["bf_squares"]                    # X = P_red Phi: 2 nb^2 G per block, as algorithmic flops
+    int8_ops = 2.0 * 15.0 * st["bf_squares_padded"]   # what the matrix cores issue: 15 digit products, rows padded to 32
     t_hbm, t_mfma = rho_bytes / (PEAK_HBM_GBS * 1e9), int8_ops / (PEAK_INT8_TOPS * 1e12)
     if t_hbm >= t_mfma:
         achieved, peak, unit, bound = rho_bytes / (rho_ms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s", "hbm"
     else:
         achieved, peak, unit, bound = int8_ops / (rho_ms * 1e-3) / 1e12, PEAK_INT8_TOPS, "TOP/s", "mfma"
+    traffic, traffic_commit = measured_traffic("density_kernel_mixed", args.workload)
     roofline = {"kernel": "density_kernel_mixed", "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                "frac": achieved / peak, "traffic": measured_traffic("density_kernel_mixed", args.workload),
-                "algorithmic_bytes": rho_bytes, "algorithmic_int8_ops": int8_ops, "kernel_ms": rho_ms,
-                "fock_stage_ms": float(np.mean(mixed["fock_ms"])), "fock_stage_bytes": st["fock_bytes"]}
+                "frac": achieved / peak, "traffic": traffic,
+                "traffic_source": None if traffic is None else
+                "profiles/density_kernel_mixed_{:s}_pmc.txt (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, at commit "
+                "{:s})".format(args.workload, traffic_commit or "unrecorded"),
+                "algorithmic_bytes": rho_bytes, "algorithmic_flops": flops,
+                "algorithmic_tflops_per_s": flops / (rho_ms * 1e-3) / 1e12, "hardware_int8_ops": int8_ops,
+                "kernel_ms": rho_ms, "fock_stage_ms": float(np.mean(mixed["fock_ms"])),
+                "fock_stage_bytes": st["fock_bytes"], "fock_algorithmic_flops": 4.0 * st["bf_squares"]}
     out = {"metric": METRIC, "value": value, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "int8 digits x int8 -> int32 (exact) with f64 functional and sums", "data": "synthetic",
@@ -323,16 +343,25 @@ def main():
         out["latency_ms_p99"] = 1e3 * serve["latency_s_p99"]
         out["serving"] = serve
     if args.cpu_baseline != "none":
+        cn, ccube, cseed, cfun = WORKLOADS[args.cpu_sample]
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_baseline, args.cpu_sample_waters, functional, spherical)
+            out["cpu_baseline"] = cpu_baseline(args.cpu_baseline, cn, ccube, cseed, cfun, spherical)
         except Exception as exc:  # the bench line is still printed; the failure is named
             out["cpu_baseline"] = {"value": None, "error": repr(exc)}
         try:
             # The restatement on the host cores this job may use (16 per GPU on the box).
-            out["cpu_baseline_all_cores"] = cpu_baseline("port", args.cpu_sample_waters, functional, spherical,
+            out["cpu_baseline_all_cores"] = cpu_baseline("port", cn, ccube, cseed, cfun, spherical,
                                                          threads=min(16, os.cpu_count() or 1))
         except Exception as exc:
             out["cpu_baseline_all_cores"] = {"value": None, "error": repr(exc)}
+        if args.cpu_sample_waters > 0:
+            try:
+                # A smaller sample too: per-point cost grows with the kept functions per block, so the rate differs.
+                w = args.cpu_sample_waters
+                out["cpu_baseline_small_sample"] = cpu_baseline(args.cpu_baseline, w, 9.0 * (w / 16.0) ** (1.0 / 3.0), 1,
+                                                                functional, spherical, seconds=5.0)
+            except Exception as exc:
+                out["cpu_baseline_small_sample"] = {"value": None, "error": repr(exc)}
     print(json.dumps(out), flush=True)
     if args.copy_calibration:
         copy_calibration()

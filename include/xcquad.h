@@ -60,9 +60,11 @@ int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, 
 !  XQ_NSTATS doubles): [0] sum over blocks of kept functions x points ("Quadrature BF Values"), [1] blocks skipped by the
 !  density tolerance, [2] device ms from P expansion to packed V, [3] device ms of the density/functional kernel, [4] device
 !  ms of the Fock stage, [5] sum over blocks of kept functions^2 x points (the contractions' algorithmic flops are
-!  2 [5] each), [6] and [7] algorithmic HBM bytes of the density and Fock stages (each stored operand read once, each
-!  result written once; for FP64 both stages are one kernel and [7] is 0). */
-# define XQ_NSTATS 8
+!  2 [5] each), [6] and [7] algorithmic HBM bytes of the density and Fock stages (each value of a kept function at a
+!  point read once in the stage's storage format, each result written once), [8] sum over blocks of (kept functions
+!  rounded up to 32)^2 x 128, the shape the matrix-core products run on (the mixed path's X = P_red Phi issues
+!  2 x 15 x [8] int8 operations: 15 digit products). */
+# define XQ_NSTATS 9
 int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
                                   double *nElectrons, double *fockV, double *stats ) ;
 
@@ -99,6 +101,11 @@ int         xq_device_memory    ( long long *freeBytes, long long *totalBytes ) 
 
 void        xq_free             ( xq_grid *grid ) ;
 const char *xq_last_error       ( void ) ;
+
+/* Test hook, no counterpart in the reference: after `count` more device allocations by the library, the next one fails
+!  as hipMalloc does when the device is full (once); a negative count disarms it.  Lets a test drive the out-of-memory
+!  paths (status 7) and check that the handle is usable afterwards. */
+void        xq_test_fail_allocations ( long long count ) ;
 
 /* Batched quadratures for a request loop.  No counterpart in the reference (its integrator takes one molecule per call):
 !  the quadratures of several handles (one per job, each with its own P, V and functional) run on the batch's stream, and

@@ -37,6 +37,7 @@ SIGNATURES = {
     "xq_integrate_batch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), _ip, _ip, ctypes.POINTER(_dp), _dp, _dp,
                                           ctypes.POINTER(_dp), _dp]),
     "xq_batch_free": (None, [_vp]),
+    "xq_test_fail_allocations": (None, [ctypes.c_longlong]),
 }
 
 

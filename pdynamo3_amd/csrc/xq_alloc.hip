@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <unordered_map>
+#include <utility>
 
 #include "xq_internal.h"
 
@@ -22,8 +23,9 @@ namespace {
 
 std::mutex g_mutex;
 std::multimap<size_t, void *> g_cached;            // size -> idle block
-std::unordered_map<void *, size_t> g_live;         // block -> size
+std::unordered_map<void *, std::pair<size_t, size_t>> g_live;   // block -> (size, bytes requested)
 size_t g_cachedBytes = 0;
+long long g_failAfter = -1;                        // test hook: allocations left before one fails (-1: none fails)
 std::multimap<size_t, void *> g_hostCached;        // pinned host blocks, as above
 std::unordered_map<void *, size_t> g_hostLive;
 
@@ -44,10 +46,14 @@ void release_cached_locked() {
 hipError_t cache_alloc(void **ptr, size_t bytes) {
     const size_t size = rounded(bytes);
     std::lock_guard<std::mutex> lock(g_mutex);
+    if (g_failAfter >= 0 && g_failAfter-- == 0) {
+        *ptr = nullptr;
+        return hipErrorOutOfMemory;
+    }
     auto it = g_cached.lower_bound(size);
     if (it != g_cached.end() && it->first <= size + size / 4) {
         *ptr = it->second;
-        g_live[it->second] = it->first;
+        g_live[it->second] = std::make_pair(it->first, bytes);
         g_cachedBytes -= it->first;
         g_cached.erase(it);
         return hipSuccess;
@@ -62,18 +68,20 @@ hipError_t cache_alloc(void **ptr, size_t bytes) {
         *ptr = nullptr;
         return e;
     }
-    g_live[*ptr] = size;
+    g_live[*ptr] = std::make_pair(size, bytes);
     return hipSuccess;
 }
 
-void cache_free(void *ptr) {
-    if (ptr == nullptr) return;
+size_t cache_free(void *ptr) {
+    if (ptr == nullptr) return 0;
     std::lock_guard<std::mutex> lock(g_mutex);
     auto it = g_live.find(ptr);
-    if (it == g_live.end()) return;
-    g_cached.emplace(it->second, ptr);
-    g_cachedBytes += it->second;
+    if (it == g_live.end()) return 0;
+    const size_t requested = it->second.second;
+    g_cached.emplace(it->second.first, ptr);
+    g_cachedBytes += it->second.first;
     g_live.erase(it);
+    return requested;
 }
 
 hipError_t cache_host_alloc(void **ptr, size_t bytes) {
@@ -113,6 +121,11 @@ void cache_host_free(void *ptr) {
 size_t cache_idle_bytes() {
     std::lock_guard<std::mutex> lock(g_mutex);
     return g_cachedBytes;
+}
+
+void cache_fail_after(long long count) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    g_failAfter = count;
 }
 
 void cache_release() {

@@ -268,4 +268,6 @@ void xq_free(xq_grid *grid) {
 
 const char *xq_last_error(void) { return g_lastError.c_str(); }
 
+void xq_test_fail_allocations(long long count) { cache_fail_after(count); }
+
 }  // extern "C"

@@ -448,10 +448,11 @@ void free_basis(xq_grid *g) {
     dev_free(g, &g->d_shellOf);
     g->h_shellOf.clear();
     g->h_funcAtom.clear();
-    if (g->d_f) { cache_free(g->d_f); g->d_f = nullptr; }
-    if (g->d_fx) { cache_free(g->d_fx); g->d_fx = nullptr; }
-    if (g->d_fy) { cache_free(g->d_fy); g->d_fy = nullptr; }
-    if (g->d_fz) { cache_free(g->d_fz); g->d_fz = nullptr; }
+    void **values[4] = {&g->d_f, &g->d_fx, &g->d_fy, &g->d_fz};
+    for (void **a : values) {
+        g->deviceBytes -= static_cast<long long>(cache_free(*a));
+        *a = nullptr;
+    }
     g->hasBasis = false;
 }
 
@@ -571,7 +572,6 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
         XQ_CHECK(hipMemcpyAsync(g->d_trans, trans.data(), trans.size() * sizeof(double), hipMemcpyHostToDevice, s));
     // Pass 1: kept functions per block.
     int *tmpIdx = nullptr, *counts = nullptr, *d_shellOf = nullptr;
-    const long long base = g->deviceBytes;
     int status = kOK;
     do {
         if ((status = dev_alloc(g, &tmpIdx, static_cast<size_t>(g->nBlocks) * f)) != kOK) break;
@@ -593,9 +593,11 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
                             "copy counts")) != kOK) break;
         if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
         long long rows = 0;
-        double bfv = 0.0, bfs = 0.0;
+        double bfv = 0.0, bfs = 0.0, bfp = 0.0;
         for (int b = 0; b < g->nBlocks; b++) {
             bfs += static_cast<double>(h_counts[b]) * h_counts[b] * g->blocks[b].nPoints;
+            const double padded = 32.0 * ((h_counts[b] + 31) / 32);
+            bfp += padded * padded * kBlockSize;
             g->blocks[b].nKept = h_counts[b];
             g->blocks[b].vOffset = rows;
             g->blocks[b].iOffset = static_cast<int>(rows);
@@ -605,6 +607,7 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
         g->nValueRows = rows;
         g->bfValues = bfv;
         g->bfSquares = bfs;
+        g->bfSquaresPadded = bfp;
         if ((status = check(hipMemcpyAsync(g->d_blocks, g->blocks.data(), g->nBlocks * sizeof(DevBlock), hipMemcpyHostToDevice, s),
                             "copy blocks")) != kOK) break;
         const size_t nVal = static_cast<size_t>(rows > 0 ? rows : 1) * kBlockSize;
@@ -634,20 +637,15 @@ int set_basis(xq_grid *g, int nSpec, const double *spec, int spherical, int prec
         if (status != kOK) break;
         if ((status = check(hipStreamSynchronize(s), "basis sync")) != kOK) break;
         for (void **a : arrays) {
-            cache_free(*a);
+            g->deviceBytes -= static_cast<long long>(cache_free(*a));
             *a = nullptr;
-            g->deviceBytes -= static_cast<long long>(nVal * elt);
         }
         status = check(hipStreamSynchronize(s), "basis sync");
     } while (false);
-    const long long temp = static_cast<long long>(sizeof(int)) * (static_cast<long long>(g->nBlocks) * f + g->nBlocks +
-                                                                 static_cast<long long>(shellOf.size()));
     (void)hipStreamSynchronize(s);   // the kernels that use the scratch arrays are done before they are reused
     dev_free(g, &tmpIdx);
     dev_free(g, &counts);
     dev_free(g, &d_shellOf);
-    g->deviceBytes -= temp > 0 ? temp : 0;
-    (void)base;
     if (status != kOK) { free_basis(g); return status; }
     g->h_shellOf = shellOf;     // kept for the second-derivative evaluation of the XC gradients
     g->h_funcAtom.resize(shellOf.size());

@@ -771,7 +771,6 @@ int build_grid(xq_grid *g) {
         dev_free(g, &cx); dev_free(g, &cy); dev_free(g, &cz); dev_free(g, &cw); dev_free(g, &keep); dev_free(g, &dst); dev_free(g, &total);
         dev_free(g, &tileSums);
     };
-    const long long before = g->deviceBytes;
     do {
         if ((status = dev_alloc(g, &d_shells, shells.size())) != kOK) break;
         if ((status = dev_alloc(g, &d_rules, rules.size())) != kOK) break;
@@ -860,10 +859,7 @@ int build_grid(xq_grid *g) {
                                            hipMemcpyHostToDevice, s), "copy blocks")) != kOK) break;
         status = check(hipStreamSynchronize(s), "grid sync");
     } while (false);
-    cleanup();
-    // Only the kept points and the block table stay resident.
-    g->deviceBytes = before + static_cast<long long>(4 * sizeof(double) * (size_t)(g->nPoints > 0 ? g->nPoints : 1) +
-                                                     sizeof(DevBlock) * (g->blocks.empty() ? 1 : g->blocks.size()));
+    cleanup();   // only the kept points and the block table stay resident (and counted on the handle)
     return status;
 }
 

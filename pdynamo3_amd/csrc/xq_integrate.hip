@@ -88,12 +88,12 @@ int upload_density(xq_grid *g, const double *Ppacked, hipStream_t s) {
     return kOK;
 }
 
-void free_work(xq_grid *g) {
-    if (g->h_job != nullptr) { cache_host_free(g->h_job); g->h_job = nullptr; }
+// The per-call work arrays and the one-entry job table.  Each set is complete or absent: an allocation that fails part
+// way releases what the set already holds, so the next call allocates it again from scratch.
+static void free_work_arrays(xq_grid *g) {
     if (g->h_stage != nullptr) { cache_host_free(g->h_stage); g->h_stage = nullptr; }
     if (g->h_blockEN != nullptr) { cache_host_free(g->h_blockEN); g->h_blockEN = nullptr; }
     if (g->h_skip != nullptr) { cache_host_free(g->h_skip); g->h_skip = nullptr; }
-    dev_free(g, &g->d_job);
     dev_free(g, &g->d_P);
     dev_free(g, &g->d_V);
     dev_free(g, &g->d_packed);
@@ -101,8 +101,17 @@ void free_work(xq_grid *g) {
     dev_free(g, &g->d_skip);
 }
 
-int ensure_work(xq_grid *g) {
-    if (g->d_P != nullptr) return kOK;
+static void free_job_table(xq_grid *g) {
+    if (g->h_job != nullptr) { cache_host_free(g->h_job); g->h_job = nullptr; }
+    dev_free(g, &g->d_job);
+}
+
+void free_work(xq_grid *g) {
+    free_job_table(g);
+    free_work_arrays(g);
+}
+
+static int alloc_work_arrays(xq_grid *g) {
     const size_t n = static_cast<size_t>(g->nFunctions);
     XQ_TRY(dev_alloc(g, &g->d_P, n * n));
     XQ_TRY(dev_alloc(g, &g->d_V, n * n));
@@ -114,6 +123,25 @@ int ensure_work(xq_grid *g) {
     XQ_CHECK(cache_host_alloc(reinterpret_cast<void **>(&g->h_blockEN), 2 * nb * sizeof(double)));
     XQ_CHECK(cache_host_alloc(reinterpret_cast<void **>(&g->h_skip), nb * sizeof(int)));
     return kOK;
+}
+
+int ensure_work(xq_grid *g) {
+    if (g->d_P != nullptr && g->d_V != nullptr && g->d_packed != nullptr && g->d_blockEN != nullptr &&
+        g->d_skip != nullptr && g->h_stage != nullptr && g->h_blockEN != nullptr && g->h_skip != nullptr)
+        return kOK;
+    free_work_arrays(g);
+    const int status = alloc_work_arrays(g);
+    if (status != kOK) free_work_arrays(g);
+    return status;
+}
+
+int ensure_job_table(xq_grid *g) {
+    if (g->h_job != nullptr && g->d_job != nullptr) return kOK;
+    free_job_table(g);
+    int status = check(cache_host_alloc(reinterpret_cast<void **>(&g->h_job), sizeof(MixedJob)), "hipHostMalloc");
+    if (status == kOK) status = dev_alloc(g, &g->d_job, 1);
+    if (status != kOK) free_job_table(g);
+    return status;
 }
 
 int functional_set(int nIDs, const int *ids, FunctionalSet *fs) {
@@ -138,32 +166,33 @@ int functional_set(int nIDs, const int *ids, FunctionalSet *fs) {
     return kOK;
 }
 
-// Algorithmic HBM bytes of the two stages of one handle's quadrature: every stored operand read once, every result
+// Algorithmic HBM bytes of the two stages of one handle's quadrature: every stored value of a kept function at a grid
+// point (sum over blocks of kept functions x points, not the rows padded to chunks of 32) read once, every result
 // written once.
 static void stage_bytes(const xq_grid *g, double *rho, double *fock) {
     const double n = static_cast<double>(g->nFunctions), nn = n * n, G = static_cast<double>(g->nPoints),
-                 B = static_cast<double>(g->nBlocks);
-    const double ch = static_cast<double>(g->nChunks);
+                 B = static_cast<double>(g->nBlocks), values = g->bfValues;
     if (g->precision == XQ_PRECISION_FP64) {
-        // Density: Phi and grad Phi planes (4 x 32 KB per chunk), P, weights in, W coefficients out, block sums.
-        // Fock stage: W formation (the same planes and the coefficients in, W out) and the Fock product (Phi and W in,
-        // V read and written).
-        *rho = ch * 4.0 * 32768.0 + 8.0 * nn + 40.0 * G + 20.0 * B;
-        *fock = ch * (4.0 * 32768.0 + 32768.0) + 32.0 * G + ch * 2.0 * 32768.0 + 16.0 * nn;
+        // Density: Phi and grad Phi (32 bytes per value), P, weights in and W coefficients out (40 bytes per point),
+        // block sums.  Fock stage: W formation (Phi and grad Phi in, W out: 40 bytes per value; the coefficients in) and
+        // the Fock product (Phi and W in: 16 bytes per value; V read and written).
+        *rho = 32.0 * values + 8.0 * nn + 40.0 * G + 20.0 * B;
+        *fock = 56.0 * values + 32.0 * G + 16.0 * nn;
     } else {
-        // Density: Phi digits (20 KB per chunk), fixed-point gradients (48 KB per chunk), P digits, point exponents,
-        // weights in, W coefficients out, block sums.  Fock stage: W formation (Phi digits and fixed-point gradients
-        // per union chunk, coefficients in, W digits out) and the Fock product (Phi and W digits per union chunk in, V
-        // read and written).
-        const double chU = static_cast<double>(g->nChunksU);
-        *rho = ch * (20480.0 + 49152.0) + 8.0 * nn + 512.0 * B + 40.0 * G + 20.0 * B;
-        *fock = chU * (16384.0 + 36864.0 + 16384.0 + 32.0) + 32.0 * G + 384.0 * B + chU * 2.0 * 16384.0 + 16.0 * nn;
+        // Density: Phi as 5 digits and grad Phi as 3 x 32-bit fixed point (17 bytes per value), P digits (8 bytes per
+        // entry), weights in, W coefficients out and the point exponents (44 bytes per point), block sums.  Fock stage,
+        // over the superblocks' union rows: W formation (Phi digits and 24-bit grad Phi in, W digits out: 17 bytes per
+        // union value; coefficients and gradient exponents in) and the Fock product (Phi and W digits in: 8 bytes per
+        // union value; V read and written).
+        const double union_values = g->bfUnion;
+        *rho = 17.0 * values + 8.0 * nn + 44.0 * G + 20.0 * B;
+        *fock = 25.0 * union_values + 35.0 * G + 16.0 * nn;
     }
 }
 
-int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-                   const double *const *Ppacked, double *eXC, double *nE, double *const *V, double *stats,
-                   MixedJob *h_table, MixedJob *d_table) {
+static int integrate_jobs_queued(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids,
+                                 const FunctionalSet *fs, const double *const *Ppacked, double *eXC, double *nE,
+                                 double *const *V, double *stats, MixedJob *h_table, MixedJob *d_table) {
     const int precision = grids[0]->precision;
     for (int j = 0; j < nJobs; j++) {
         if (grids[j]->precision != precision) {
@@ -216,7 +245,7 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
         }
     }
     XQ_CHECK(hipStreamSynchronize(s));
-    double bfv = 0.0, bfs = 0.0, rhoBytes = 0.0, fockBytes = 0.0;
+    double bfv = 0.0, bfs = 0.0, bfp = 0.0, rhoBytes = 0.0, fockBytes = 0.0;
     long long skipped = 0;
     for (int j = 0; j < nJobs; j++) {
         const xq_grid *g = grids[j];
@@ -233,6 +262,7 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
         nE[j] = ne;
         bfv += g->bfValues;
         bfs += g->bfSquares;
+        bfp += g->bfSquaresPadded;
         double rb, fb;
         stage_bytes(g, &rb, &fb);
         rhoBytes += rb;
@@ -251,18 +281,26 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
         stats[5] = bfs;
         stats[6] = rhoBytes;
         stats[7] = fockBytes;
+        stats[8] = bfp;
     }
     return kOK;
+}
+
+// A failure part way leaves work queued on the stream that uses the handles' arrays; the stream is drained before the
+// error returns, so that no array is released (or reused by another handle) while a kernel still reads it.
+int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
+                   const double *const *Ppacked, double *eXC, double *nE, double *const *V, double *stats,
+                   MixedJob *h_table, MixedJob *d_table) {
+    const int status = integrate_jobs_queued(s, ev, nJobs, grids, fs, Ppacked, eXC, nE, V, stats, h_table, d_table);
+    if (status != kOK) (void)hipStreamSynchronize(s);
+    return status;
 }
 
 int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, double *eXC, double *nE, double *V,
               double *stats) {
     FunctionalSet fs;
     XQ_TRY(functional_set(nIDs, ids, &fs));
-    if (g->h_job == nullptr) {
-        XQ_CHECK(cache_host_alloc(reinterpret_cast<void **>(&g->h_job), sizeof(MixedJob)));
-        XQ_TRY(dev_alloc(g, &g->d_job, 1));
-    }
+    XQ_TRY(ensure_job_table(g));
     const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1};
     xq_grid *grids[1] = {g};
     const double *P[1] = {Ppacked};

@@ -133,6 +133,8 @@ struct xq_grid {
     double *d_FPB = nullptr, *d_GPB = nullptr, *d_FPA = nullptr, *d_WPA = nullptr;
     double bfValues = 0.0;                // sum over blocks of kept functions x points
     double bfSquares = 0.0;               // sum over blocks of kept functions^2 x points
+    double bfSquaresPadded = 0.0;         // sum over blocks of (kept functions rounded up to 32)^2 x 128
+    double bfUnion = 0.0;                 // mixed path: sum over blocks of union functions of the superblock x points
 
     // Work arrays for a quadrature.
     double *d_P = nullptr, *d_V = nullptr, *d_packed = nullptr;   // dense n x n P and V, packed staging
@@ -157,13 +159,15 @@ namespace xq {
 
 // Process-wide caching allocator (xq_alloc.hip): blocks freed by the library are kept for reuse instead of being
 // returned with hipFree, which synchronises the device.  The caller must have completed all work on a block before it
-// frees it.  cache_idle_bytes: bytes held idle; cache_release: hipFree every idle block.
+// frees it; cache_free returns the bytes that were requested for the block (0 for a pointer it does not know).
+// cache_idle_bytes: bytes held idle; cache_release: hipFree every idle block.
 hipError_t cache_alloc(void **ptr, size_t bytes);
-void cache_free(void *ptr);
+size_t cache_free(void *ptr);
 hipError_t cache_host_alloc(void **ptr, size_t bytes);   // pinned host memory, cached the same way
 void cache_host_free(void *ptr);
 size_t cache_idle_bytes();
 void cache_release();
+void cache_fail_after(long long count);   // test hook: after `count` more device allocations, one fails (-1 disarms)
 
 // Allocation helpers that account device bytes on the handle.  Every stage synchronises the handle's stream before it
 // frees what it allocated.
@@ -171,20 +175,22 @@ template <typename T>
 int dev_alloc(xq_grid *g, T **ptr, size_t count) {
     *ptr = nullptr;
     if (count == 0) count = 1;
-    hipError_t e = cache_alloc(reinterpret_cast<void **>(ptr), count * sizeof(T));
+    const size_t bytes = count * sizeof(T);
+    hipError_t e = cache_alloc(reinterpret_cast<void **>(ptr), bytes);
     if (e != hipSuccess) {
         set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
         *ptr = nullptr;
         return kOutOfMemory;
     }
-    g->deviceBytes += static_cast<long long>(count * sizeof(T));
+    g->deviceBytes += static_cast<long long>(bytes);
     return kOK;
 }
 
+// Returns the array to the cache and takes its bytes off the handle's count.
 template <typename T>
-void dev_free(xq_grid *, T **ptr) {
+void dev_free(xq_grid *g, T **ptr) {
     if (*ptr != nullptr) {
-        cache_free(reinterpret_cast<void *>(*ptr));
+        g->deviceBytes -= static_cast<long long>(cache_free(reinterpret_cast<void *>(*ptr)));
         *ptr = nullptr;
     }
 }
@@ -231,7 +237,10 @@ int eval_basis2(xq_grid *g, int b0, int b1, long long rowBase, const Planes10 &o
 int gradients(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, double *grad);
 int gradients_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const double *Pb, double *grad);
 int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_t s);
+// The per-call work arrays (dense P and V, packed staging, block sums and flags) and the one-entry job table of single
+// calls: each is allocated complete or not at all.
 int ensure_work(xq_grid *g);
+int ensure_job_table(xq_grid *g);
 int upload_density(xq_grid *g, const double *Ppacked, hipStream_t s);
 void free_mixed(xq_grid *g);
 

@@ -779,6 +779,7 @@ static int union_tables(xq_grid *g) {
     std::vector<int> uIdx, uChunkOffset(static_cast<size_t>(nB) + 1, 0), uChunkBlock, uRow;
     std::vector<int> stamp(static_cast<size_t>(g->nFunctions > 0 ? g->nFunctions : 1), -1);
     long long chunks = 0;
+    double unionValues = 0.0;
     for (int b0 = 0; b0 < nB;) {
         int b1 = b0 + 1;
         while (b1 < nB && b1 - b0 < kSuper && g->blocks[b1].atom == g->blocks[b0].atom) b1++;
@@ -795,6 +796,7 @@ static int union_tables(xq_grid *g) {
         supers.push_back(make_int4(b0, b1 - b0, static_cast<int>(u0), nu));
         const int T = (nu + kC - 1) / kC;
         for (int b = b0; b < b1; b++) {
+            unionValues += static_cast<double>(nu) * g->blocks[b].nPoints;
             // Both lists are sorted: one merge gives each union row's place in the block's kept list.
             const int *kept = idx.data() + g->blocks[b].iOffset;
             const int nk = g->blocks[b].nKept;
@@ -826,6 +828,7 @@ static int union_tables(xq_grid *g) {
     g->nChunksU = chunks;
     g->nTilesU = static_cast<long long>(uTiles.size());
     g->nUnionRows = static_cast<long long>(uIdx.size());
+    g->bfUnion = unionValues;
     XQ_TRY(dev_alloc(g, &g->d_supers, supers.size()));
     XQ_TRY(dev_alloc(g, &g->d_uTiles, uTiles.size()));
     XQ_TRY(dev_alloc(g, &g->d_uIdx, uIdx.size()));
@@ -912,6 +915,7 @@ void free_mixed(xq_grid *g) {
     g->nChunksU = 0;
     g->nTilesU = 0;
     g->nUnionRows = 0;
+    g->bfUnion = 0.0;
 }
 
 // The job table entries of the given handles (each with its functional) and each stage's total work-groups.

@@ -88,10 +88,7 @@ int integrate_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const 
     FunctionalSet fs;
     XQ_TRY(functional_set(nIDs, ids, &fs));
     XQ_TRY(ensure_work(g));
-    if (g->h_job == nullptr) {
-        XQ_CHECK(cache_host_alloc(reinterpret_cast<void **>(&g->h_job), sizeof(MixedJob)));
-        XQ_TRY(dev_alloc(g, &g->d_job, 1));
-    }
+    XQ_TRY(ensure_job_table(g));
     hipStream_t s = g->stream;
     const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1};
     const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
@@ -170,6 +167,7 @@ int integrate_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const 
             stats[1] = static_cast<double>(skipped);
             stats[2] = static_cast<double>(msTotal);
             stats[5] = g->bfSquares;
+            stats[8] = g->bfSquaresPadded;
         }
     } while (false);
     g->d_rhoOut = nullptr;

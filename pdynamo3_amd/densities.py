@@ -6,7 +6,8 @@ whose matrix in the cluster basis is P_k = D_k^T P D_k, where D_k acts on each s
 A(R_k^T u) = D_k A(u).  D_k is fitted on random unit vectors (the angular functions of one shell span an invariant space, so
 the fit is exact to rounding).
 
-The monomer density itself is a committed fixture (tests/golden/water_density_<basis>_<sph|cart>.json).
+The monomer density itself is committed data (pdynamo3_amd/data/water_density_<basis>_<sph|cart>.json): an
+extended-Hueckel density of the water monomer, written once by tests/golden/make_density.py.
 """
 
 import json
@@ -17,7 +18,7 @@ import numpy as np
 
 from .basis import load_basis
 
-_FIXTURES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden")
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
 def _cart_powers(l):
@@ -111,7 +112,7 @@ def monomer_rotation(R, basis, spherical, atomic_numbers=(8, 1, 1)):
 def load_monomer_density(label="6-31g_st", spherical=True, path=None):
     """The committed monomer density (dense n x n) and its description."""
     if path is None:
-        path = os.path.join(_FIXTURES, "water_density_{:s}_{:s}.json".format(label, "sph" if spherical else "cart"))
+        path = os.path.join(_DATA, "water_density_{:s}_{:s}.json".format(label, "sph" if spherical else "cart"))
     with open(path) as f:
         data = json.load(f)
     return np.asarray(data["P"], dtype=np.float64), data

@@ -16,6 +16,7 @@ from .definitions import (DFTGridAccuracy, LDA_C_VWN, LDA_C_VWN_RPA, LDA_X, exac
                           functional_ids)
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
+NSTATS = 9          # XQ_NSTATS of include/xcquad.h
 
 
 class QCModelError(Exception):
@@ -72,12 +73,12 @@ class QuadratureGrid:
         if V.dtype != np.float64 or not V.flags.c_contiguous or V.size != P.size:
             raise QCModelError("Fock matrix must be a contiguous float64 array of the density's size.")
         e, ne = ctypes.c_double(), ctypes.c_double()
-        stats = np.zeros(8)
+        stats = np.zeros(NSTATS)
         _check(_lib.lib.xq_integrate(self._h, ids.size, _lib.iptr(ids), _lib.dptr(P), ctypes.byref(e), ctypes.byref(ne),
                                      _lib.dptr(V), _lib.dptr(stats)), "integrate")
         return e.value, ne.value, {"bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2],
                                    "rho_ms": stats[3], "fock_ms": stats[4], "bf_squares": stats[5],
-                                   "rho_bytes": stats[6], "fock_bytes": stats[7]}
+                                   "rho_bytes": stats[6], "fock_bytes": stats[7], "bf_squares_padded": stats[8]}
 
     def integrate_uks(self, ids, density_a, density_b, fock_a=None, fock_b=None):
         """One unrestricted quadrature (alpha and beta densities, packed): returns (E_xc, N_e, stats); fock_a and fock_b
@@ -94,7 +95,7 @@ class QuadratureGrid:
             if V.dtype != np.float64 or not V.flags.c_contiguous or V.size != Pa.size:
                 raise QCModelError("Fock matrices must be contiguous float64 arrays of the density's size.")
         e, ne = ctypes.c_double(), ctypes.c_double()
-        stats = np.zeros(8)
+        stats = np.zeros(NSTATS)
         _check(_lib.lib.xq_integrate_uks(self._h, ids.size, _lib.iptr(ids), _lib.dptr(Pa), _lib.dptr(Pb), ctypes.byref(e),
                                          ctypes.byref(ne), _lib.dptr(Va), _lib.dptr(Vb), _lib.dptr(stats)),
                "integrate_uks")
@@ -185,12 +186,13 @@ class QuadratureBatch:
             keep += [P, V]
             P_ptr[j] = _lib.dptr(P)
             V_ptr[j] = _lib.dptr(V)
-        e, ne, stats = np.zeros(n), np.zeros(n), np.zeros(8)
+        e, ne, stats = np.zeros(n), np.zeros(n), np.zeros(NSTATS)
         _check(_lib.lib.xq_integrate_batch(self._h, n, grids, _lib.iptr(n_ids), _lib.iptr(ids), P_ptr, _lib.dptr(e),
                                            _lib.dptr(ne), V_ptr, _lib.dptr(stats)), "batch integrate")
         return [(float(e[j]), float(ne[j])) for j in range(n)], {
             "bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2], "rho_ms": stats[3],
-            "fock_ms": stats[4], "bf_squares": stats[5], "rho_bytes": stats[6], "fock_bytes": stats[7]}
+            "fock_ms": stats[4], "bf_squares": stats[5], "rho_bytes": stats[6], "fock_bytes": stats[7],
+            "bf_squares_padded": stats[8]}
 
     def close(self):
         if self._h:

@@ -84,6 +84,8 @@ class Job:
     points: int = 0
     quadratures: int = 0
     energies: list = field(default_factory=list)
+    error: str = ""
+    fock: object = None             # the last quadrature's V_xc (packed), kept when the server is asked to
 
 
 def request_stream(n_requests=256, seed=3, min_waters=4, max_waters=64, rate=50.0, spherical=False):
@@ -133,13 +135,17 @@ class Server:
     def _set_up(self, job):
         r = job.request
         g = integrator.QuadratureGrid(r.Z, r.xyz_bohr, 2)
-        g.set_basis(r.spec, self.spherical, self.precision)
+        try:
+            g.set_basis(r.spec, self.spherical, self.precision)
+        except Exception:
+            g.close()
+            raise
         return g
 
     def run(self, requests, log=None):
         """Serve the requests (sorted by arrival); returns the finished jobs and loop statistics."""
         pending = sorted(requests, key=lambda r: r.arrival)
-        queue, setting_up, active, finished = [], {}, [], []
+        queue, setting_up, active, finished, failed = [], {}, [], [], []
         reserved = 0
         deferred_for_memory = 0
         batch_sizes = []
@@ -166,7 +172,18 @@ class Server:
                 setting_up[self.setup_pool.submit(self._set_up, job)] = job
             for fut in [f for f in setting_up if f.done()]:
                 job = setting_up.pop(fut)
-                job.grid = fut.result()
+                try:
+                    job.grid = fut.result()
+                except Exception as exc:
+                    # A set-up that fails (e.g. out of device memory) ends its request only: its reservation and slot
+                    # are released and the loop goes on.
+                    reserved -= job.reserved
+                    job.error = repr(exc)
+                    job.done = now()
+                    failed.append(job)
+                    if log:
+                        log("request {:d}: set-up failed: {:s}".format(job.request.rid, job.error))
+                    continue
                 job.points = job.grid.number_of_points
                 job.ready = now()
                 active.append(job)
@@ -205,7 +222,7 @@ class Server:
                     still.append(job)
             active = still
         wall = now()
-        return finished, {"wall_s": wall, "batches": len(batch_sizes),
+        return finished, {"wall_s": wall, "batches": len(batch_sizes), "failed": failed,
                           "mean_batch": float(np.mean(batch_sizes)) if batch_sizes else 0.0,
                           "max_batch": int(max(batch_sizes)) if batch_sizes else 0,
                           "deferred_for_memory": deferred_for_memory, "batch_call_s": t_batch,
@@ -217,10 +234,12 @@ class Server:
 
 
 def summarize(finished, loop):
+    loop = dict(loop)
+    failed = loop.pop("failed", [])
     lat = np.array([j.done - j.request.arrival for j in finished])
     wait = np.array([j.admitted - j.request.arrival for j in finished])
     points = sum(j.points * j.quadratures for j in finished)
     return {"requests": len(finished), "latency_s_p50": float(np.percentile(lat, 50)),
             "latency_s_p99": float(np.percentile(lat, 99)), "queue_wait_s_p50": float(np.percentile(wait, 50)),
             "requests_per_s": len(finished) / loop["wall_s"], "grid_points_per_s": points / loop["wall_s"],
-            "quadratures": int(sum(j.quadratures for j in finished)), **loop}
+            "quadratures": int(sum(j.quadratures for j in finished)), "failed": len(failed), **loop}

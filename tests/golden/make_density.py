@@ -1,4 +1,5 @@
-"""Writes the water monomer density fixtures tests/golden/water_density_<basis>_<sph|cart>.json.
+"""Writes the water monomer densities pdynamo3_amd/data/water_density_<basis>_<sph|cart>.json (synthetic input data of
+the benchmarks and tests; the script lives here because it takes the overlap matrix from the oracle).
 
 The reference's converged SCF densities cannot be produced here (its SCF needs the full Python package), so the monomer
 density is an extended-Hueckel density in the reference's own basis: H_ii from valence-orbital ionisation energies
@@ -72,7 +73,8 @@ def main():
                               "(+-0.757, 0.586, 0) A; basis 6-31g_st, {:s} functions.".format(
                                   "spherical" if spherical else "Cartesian"),
                "orbital_energies": e[:8].tolist(), "electrons_tr_PS": float(np.sum(P * S)), "P": P.tolist()}
-        name = os.path.join(HERE, "water_density_6-31g_st_{:s}.json".format("sph" if spherical else "cart"))
+        name = os.path.join(os.path.dirname(os.path.dirname(HERE)), "pdynamo3_amd", "data",
+                            "water_density_6-31g_st_{:s}.json".format("sph" if spherical else "cart"))
         with open(name, "w") as f:
             json.dump(out, f)
         print(name, "Tr(PS) =", out["electrons_tr_PS"], "e =", np.round(e[:6], 4))

@@ -103,3 +103,65 @@ def test_handle_reuse_matches_fresh_handles(precision):
         e0, n0, V0 = fresh(P)
         assert e == e0 and ne == n0
         assert np.max(np.abs(V - V0)) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_out_of_memory_leaves_the_handle_usable(precision):
+    """An allocation that fails inside a quadrature returns status 7 (OutOfMemory); the next call on the same handle
+    allocates its work arrays again and gives the fresh handle's result."""
+    from pdynamo3_amd import _lib, basis, integrator, molecules
+    Z, xyz = molecules.water()
+    spec = basis.element_spec(basis.load_basis(), Z)
+    n = basis.number_of_functions(basis.load_basis(), Z, False)
+    P = random_density(n, 5, seed=3)
+    g0 = integrator.QuadratureGrid(Z, xyz, 2)
+    g0.set_basis(spec, False, precision)
+    V0 = np.zeros_like(P)
+    e0, n0, _ = g0.integrate(ids("blyp"), P, V0)
+    g0.close()
+    failures = 0
+    for k in range(8):            # the k-th allocation of the first call fails: each work array and the job table
+        g = integrator.QuadratureGrid(Z, xyz, 2)
+        g.set_basis(spec, False, precision)
+        _lib.lib.xq_test_fail_allocations(k)
+        try:
+            g.integrate(ids("blyp"), P, np.zeros_like(P))
+        except integrator.QCModelError as exc:
+            assert "status 7" in str(exc)
+            failures += 1
+        finally:
+            _lib.lib.xq_test_fail_allocations(-1)
+        V = np.zeros_like(P)
+        e, ne, _ = g.integrate(ids("blyp"), P, V)
+        g.close()
+        print(precision, "fail at", k, "then E", e, "dE", e - e0)
+        assert e == e0 and ne == n0
+        assert np.max(np.abs(V - V0)) <= 1e-12
+    assert failures >= 6          # d_job, d_P, d_V, packed, block sums, skip flags
+
+
+def test_device_bytes_follow_the_precision():
+    """The handle's device byte count falls when the FP64 planes are released for the mixed ones and returns to the
+    same value when FP64 is chosen again."""
+    from pdynamo3_amd import basis, integrator, molecules
+    Z, xa, _, _ = molecules.water_cluster(4, 9.0 * (4 / 16.0) ** (1.0 / 3.0), 1)
+    xyz = molecules.to_bohr(xa)
+    spec = basis.element_spec(basis.load_basis(), Z)
+    n = basis.number_of_functions(basis.load_basis(), Z, False)
+    P = random_density(n, 20, seed=4)
+    g = integrator.QuadratureGrid(Z, xyz, 2)
+    grid_only = g.device_bytes()
+    g.set_basis(spec, False, "fp64")
+    g.integrate(ids("pbe"), P)
+    b64 = g.device_bytes()
+    g.set_basis(spec, False, "mixed")
+    g.integrate(ids("pbe"), P)
+    bmix = g.device_bytes()
+    g.set_basis(spec, False, "fp64")
+    g.integrate(ids("pbe"), P)
+    b64again = g.device_bytes()
+    g.close()
+    print("grid", grid_only, "fp64", b64, "mixed", bmix, "fp64 again", b64again)
+    assert grid_only > 0
+    assert bmix < b64
+    assert b64again == b64
