@@ -224,30 +224,34 @@ def gradient_leg(grid, ids, P, steps, n_waters, functional, spherical, cpu_kind)
 
 
 def serving_leg(args, log_fn):
-    """The serving stream: SCF requests on (H2O)N, N log-uniform in [4, 64] (seed 3), BLYP/PBE by coin flip, Poisson
-    arrivals at serve-load x the loop's capacity.  The capacity is measured first, on a calibration stream of 32 other
-    requests (seed 7) that all arrive at once, after a warm-up pass of 16 more (seed 8) that fills the device memory
-    cache and loads the kernels."""
+    """The serving stream in both precisions: SCF requests on (H2O)N, N log-uniform in [4, 64] (seed 3), BLYP/PBE by coin
+    flip, Poisson arrivals.  The mixed loop's capacity is measured first, on a calibration stream of 32 other requests
+    (seed 7) that all arrive at once, after a warm-up pass of 16 more (seed 8) that fills the device memory cache and
+    loads the kernels; the arrival rate is serve-load x that capacity.  The FP64 loop then serves the same stream (same
+    arrival times) after its own warm-up and calibration, so the two legs differ only in the precision."""
+    import dataclasses
     from pdynamo3_amd import serving
-    srv = serving.Server(precision="mixed", max_jobs=args.serve_slots)
-    srv.run(serving.request_stream(n_requests=16, seed=8, rate=1.0e6))
-    calib = serving.request_stream(n_requests=32, seed=7, rate=1.0e6)
-    done, cl = srv.run(calib)
-    capacity = len(done) / cl["wall_s"]
-    reqs = serving.request_stream(n_requests=args.serve_requests, seed=3, rate=1.0)
-    rate = args.serve_load * capacity
-    for r in reqs:
-        r.arrival /= rate
-    finished, loop = srv.run(reqs)
-    srv.close()
-    out = serving.summarize(finished, loop)
-    out.update({"arrival_rate_per_s": rate, "capacity_requests_per_s": capacity, "load": args.serve_load,
-                "slots": args.serve_slots,
-                "hbm_budget_bytes": srv.hbm_budget,
-                "stream": "{:d} requests, (H2O)N with N log-uniform in [4, 64] (seed 3), 6-31G* Cartesian, BLYP/PBE by "
-                          "coin flip, Poisson arrivals; each request is an SCF density sequence converging geometrically "
-                          "(ratio 0.4) until |dE_xc| < 1e-6 Eh".format(len(reqs))})
-    log_fn("serving:", json.dumps(out))
+    base = serving.request_stream(n_requests=args.serve_requests, seed=3, rate=1.0)
+    out, rate = {}, None
+    for precision in ("mixed", "fp64"):
+        srv = serving.Server(precision=precision, max_jobs=args.serve_slots)
+        srv.run(serving.request_stream(n_requests=16, seed=8, rate=1.0e6))
+        done, cl = srv.run(serving.request_stream(n_requests=32, seed=7, rate=1.0e6))
+        capacity = len(done) / cl["wall_s"]
+        if rate is None:
+            rate = args.serve_load * capacity
+        reqs = [dataclasses.replace(r, arrival=r.arrival / rate) for r in base]
+        finished, loop = srv.run(reqs)
+        srv.close()
+        leg = serving.summarize(finished, loop)
+        leg.update({"arrival_rate_per_s": rate, "capacity_requests_per_s": capacity, "load": rate / capacity,
+                    "slots": args.serve_slots, "hbm_budget_bytes": srv.hbm_budget, "precision": precision,
+                    "stream": "{:d} requests, (H2O)N with N log-uniform in [4, 64] (seed 3), 6-31G* Cartesian, BLYP/PBE "
+                              "by coin flip, Poisson arrivals at the mixed loop's capacity; each request is an SCF "
+                              "density sequence converging geometrically (ratio 0.4) until |dE_xc| < 1e-6 Eh".format(
+                                  len(reqs))})
+        out[precision] = leg
+        log_fn("serving {:s}:".format(precision), json.dumps(leg))
     return out
 
 
@@ -339,9 +343,10 @@ def main():
     grid.close()
     if args.serve_requests > 0:
         serve = serving_leg(args, log)
-        out["latency_ms_p50"] = 1e3 * serve["latency_s_p50"]
-        out["latency_ms_p99"] = 1e3 * serve["latency_s_p99"]
-        out["serving"] = serve
+        out["latency_ms_p50"] = 1e3 * serve["mixed"]["latency_s_p50"]
+        out["latency_ms_p99"] = 1e3 * serve["mixed"]["latency_s_p99"]
+        out["serving"] = serve["mixed"]
+        out["serving_fp64"] = serve["fp64"]
     if args.cpu_baseline != "none":
         cn, ccube, cseed, cfun = WORKLOADS[args.cpu_sample]
         try:

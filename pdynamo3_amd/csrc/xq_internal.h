@@ -77,6 +77,7 @@ struct MixedJob {
     double rhoTol;
     int n, nBlocks, nChunks, nTiles;
     int rowBase, blockBase, chunkBase, groupBase;
+    int superBase;                     // mixed path: the job's first work-group of the density stage (one per superblock)
 };
 
 }  // namespace xq
@@ -209,6 +210,8 @@ int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *con
                    const double *const *P, double *eXC, double *nE, double *const *V, double *stats, MixedJob *h_table,
                    MixedJob *d_table);
 int functional_set(int nIDs, const int *ids, FunctionalSet *fs);
+// Algorithmic HBM bytes of the density and Fock stages of one quadrature on the handle (stats[6] and [7]).
+void stage_bytes(const xq_grid *g, double *rho, double *fock);
 // stages: kStageDensity (P digits and the density kernel), kStageFock (W and the Fock product), or both.
 constexpr int kStageDensity = 1, kStageFock = 2;
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,

@@ -85,6 +85,7 @@ class Job:
     quadratures: int = 0
     energies: list = field(default_factory=list)
     error: str = ""
+    electrons: float = 0.0          # the last quadrature's N_e
     fock: object = None             # the last quadrature's V_xc (packed), kept when the server is asked to
 
 
@@ -121,8 +122,9 @@ class Server:
     """The request loop.  max_jobs: jobs set up or running at once (the batch size); hbm_budget: device bytes the loop may
     reserve (default 90% of the free memory when it starts)."""
 
-    def __init__(self, precision="mixed", max_jobs=16, hbm_budget=None, spherical=False):
+    def __init__(self, precision="mixed", max_jobs=16, hbm_budget=None, spherical=False, keep_fock=False):
         self.precision = precision
+        self.keep_fock = keep_fock          # keep each job's last V_xc (packed) on the finished job
         self.max_jobs = max_jobs
         self.spherical = spherical
         if hbm_budget is None:
@@ -199,15 +201,19 @@ class Server:
                 continue
             # One batched quadrature over every job that is set up.
             batch_sizes.append(len(active))
-            work = [(j.grid, functional_ids(j.request.functional), j.scf.density(), None) for j in active]
+            focks = [np.zeros(j.request.n_functions * (j.request.n_functions + 1) // 2) if self.keep_fock else None
+                     for j in active]
+            work = [(j.grid, functional_ids(j.request.functional), j.scf.density(), V) for j, V in zip(active, focks)]
             t_call = now()
             results, stats = self.batch.integrate(work)
             t_batch += now() - t_call
             t_device += 1e-3 * stats["device_ms"]
             still = []
-            for job, (E, N) in zip(active, results):
+            for job, (E, N), V in zip(active, results, focks):
                 job.quadratures += 1
                 job.energies.append(E)
+                job.electrons = N
+                job.fock = V
                 if job.scf.record(E):
                     job.done = now()
                     job.grid.close()
