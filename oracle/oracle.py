@@ -252,3 +252,61 @@ def reference_available():
 
 
 # . The compiled reference names its integrate "inCore" flag where the oracle takes a thread count; both are ints.
+
+
+class RefBasis:
+    """A basis-only handle of the compiled reference (orbital or fit basis on the given atoms), for the integrals of the
+    SCF wrapper: one-electron matrices (f1KOf1i + f2Cm1V), electron-fit Coulomb integrals (f1Xg2i) and the constrained fit
+    matrix (f1Xf1i_f1Oi)."""
+
+    def __init__(self, Z, xyz_bohr, spec, spherical):
+        lib = reference().lib
+        self.lib = lib
+        f = lib.xcref_basis_setup
+        f.restype = ctypes.c_void_p
+        f.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip]
+        for name, args in (("xcref_one_electron", [ctypes.c_void_p, _dp, _dp, _dp]),
+                           ("xcref_fit_integrals", [ctypes.c_void_p, ctypes.c_void_p, _dp, _ip]),
+                           ("xcref_fit_matrix", [ctypes.c_void_p, _dp])):
+            getattr(lib, name).restype = ctypes.c_int
+            getattr(lib, name).argtypes = args
+        lib.xcref_nfunctions.restype = ctypes.c_int
+        lib.xcref_nfunctions.argtypes = [ctypes.c_void_p]
+        lib.xcref_free.restype = None
+        lib.xcref_free.argtypes = [ctypes.c_void_p]
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        spec = np.ascontiguousarray(spec, dtype=np.float64)
+        st = np.zeros(1, dtype=np.int32)
+        self.h = f(Z.size, _i(Z), _d(xyz), spec.size, _d(spec), int(bool(spherical)), _i(st))
+        if not self.h or st[0] != 0:
+            raise ValueError("reference basis set-up failed (status {:d})".format(int(st[0])))
+        self.n = lib.xcref_nfunctions(self.h)
+
+    def one_electron(self):
+        """(S, T, V) packed: overlap, kinetic energy and electron-nuclear attraction (point nuclei of charge Z)."""
+        m = self.n * (self.n + 1) // 2
+        S, T, V = np.zeros(m), np.zeros(m), np.zeros(m)
+        if self.lib.xcref_one_electron(self.h, _d(S), _d(T), _d(V)) != 0:
+            raise ValueError("reference one-electron integrals failed")
+        return S, T, V
+
+    def fit_integrals(self, fit):
+        """Dense (mu nu|P) [packed orbital pair, fit function] as stored by the reference, and the stored count."""
+        dense = np.zeros((self.n * (self.n + 1) // 2, fit.n))
+        count = np.zeros(1, dtype=np.int32)
+        if self.lib.xcref_fit_integrals(self.h, fit.h, _d(dense), _i(count)) != 0:
+            raise ValueError("reference fit integrals failed")
+        return dense, int(count[0])
+
+    def fit_matrix(self):
+        """The constrained fit matrix, packed, extent n + 1 (this handle is the fit basis)."""
+        M = np.zeros((self.n + 1) * (self.n + 2) // 2)
+        if self.lib.xcref_fit_matrix(self.h, _d(M)) != 0:
+            raise ValueError("reference fit matrix failed")
+        return M
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.xcref_free(self.h)
+            self.h = None

@@ -17,6 +17,7 @@
 # include <string.h>
 # include <time.h>
 
+# include "BlockStorage.h"
 # include "Coordinates3.h"
 # include "DFTFunctionalModel.h"
 # include "DFTGrid.h"
@@ -24,6 +25,11 @@
 # include "GaussianBasis.h"
 # include "GaussianBasisContainer.h"
 # include "GaussianBasisContainerIntegrals_f1Op1.h"
+# include "GaussianBasisContainerIntegrals_f1X.h"
+# include "GaussianBasisContainerIntegrals_f1Xg1.h"
+# include "GaussianBasisContainerIntegrals_f1Xg2.h"
+# include "GaussianBasisContainerIntegrals_f2Cp1.h"
+# include "RealArray1D.h"
 # include "GridFunctionDataBlock.h"
 # include "IntegerArray1D.h"
 # include "SymmetricMatrix.h"
@@ -346,6 +352,117 @@ void xcref_c2s ( int lLow, int lHigh, double *out )
     for ( i = 0 ; i < View2D_Rows ( t ) ; i++ )
         for ( j = 0 ; j < View2D_Columns ( t ) ; j++ ) out[i*View2D_Columns ( t )+j] = Array2D_Item ( t, i, j ) ;
     RealArray2D_Deallocate ( &t ) ;
+}
+
+/*----------------------------------------------------------------------------------------------------------------------------------
+! . Integrals of the SCF wrapper (electronic structure around the quadrature), as QCModelDFT's energy closures call them
+!   (QCModelDFT.py:130-160 through GaussianBasisIntegralEvaluator.pyx).
+!---------------------------------------------------------------------------------------------------------------------------------*/
+/* . A basis-only handle: coordinates, container and index arrays (orbital or fit basis). */
+void *xcref_basis_setup ( int nAtoms, const int *Z, const double *xyzBohr, int nSpec, const double *spec, int spherical,
+                          int *status )
+{
+    RefHandle *h = ( RefHandle * ) calloc ( 1, sizeof ( RefHandle ) ) ;
+    Status     st = Status_OK ;
+    int        i, p, u ;
+    h->nAtoms        = nAtoms ;
+    h->coordinates   = Coordinates3_Allocate ( nAtoms, &st ) ;
+    h->atomicNumbers = IntegerArray1D_AllocateWithExtent ( nAtoms, &st ) ;
+    for ( i = 0 ; i < nAtoms ; i++ )
+    {
+        Coordinates3_SetRow ( h->coordinates, i, xyzBohr[3*i], xyzBohr[3*i+1], xyzBohr[3*i+2] ) ;
+        Array1D_Item ( h->atomicNumbers, i ) = Z[i] ;
+    }
+    h->unique  = ( GaussianBasis ** ) calloc ( 128, sizeof ( GaussianBasis * ) ) ;
+    h->nUnique = 0 ;
+    for ( p = 0 ; p < nSpec ; )
+    {
+        int used = 0 ;
+        h->unique[h->nUnique++] = MakeBasis ( &spec[p], &used, spherical ) ;
+        p += used ;
+    }
+    h->bases = GaussianBasisContainer_Allocate ( nAtoms, &st ) ;
+    for ( i = 0 ; i < nAtoms ; i++ )
+    {
+        GaussianBasis *b = NULL ;
+        for ( u = 0 ; u < h->nUnique ; u++ ) if ( h->unique[u]->atomicNumber == Z[i] ) b = h->unique[u] ;
+        if ( b == NULL ) { *status = 3 ; return NULL ; }
+        h->bases->entries[i] = b ;
+    }
+    h->nFunctions = GaussianBasisContainer_NumberOfFunctions ( h->bases ) ;
+    h->cfp = IntegerArray1D_AllocateWithExtent ( nAtoms + 1   , &st ) ;
+    h->fc  = IntegerArray1D_AllocateWithExtent ( h->nFunctions, &st ) ;
+    GaussianBasisContainer_MakeIndexArrays ( h->bases, h->cfp, h->fc, &st ) ;
+    *status = ( int ) st ;
+    return h ;
+}
+
+static SymmetricMatrix *NewSymmetric ( int n, Status *st )
+{
+    SymmetricMatrix *m = SymmetricMatrix_AllocateWithExtent ( n, st ) ;
+    SymmetricMatrix_Set ( m, 0.0 ) ;
+    return m ;
+}
+
+/* . Overlap, kinetic and electron-nuclear matrices (packed), as f1KOf1i and f2Cm1V fill them (point nuclei of charge Z). */
+int xcref_one_electron ( void *vh, double *S, double *T, double *V )
+{
+    RefHandle       *h  = ( RefHandle * ) vh ;
+    Status           st = Status_OK ;
+    int              i, n = h->nFunctions, size = n * ( n + 1 ) / 2 ;
+    SymmetricMatrix *s = NewSymmetric ( n, &st ), *t = NewSymmetric ( n, &st ), *v = NewSymmetric ( n, &st ) ;
+    RealArray1D     *charges = RealArray1D_AllocateWithExtent ( h->nAtoms, &st ) ;
+    for ( i = 0 ; i < h->nAtoms ; i++ ) Array1D_Item ( charges, i ) = ( double ) Array1D_Item ( h->atomicNumbers, i ) ;
+    GaussianBasisContainerIntegrals_f1KOf1i ( h->bases, h->coordinates, t, s, &st ) ;
+    GaussianBasisContainerIntegrals_f2Cm1V  ( h->bases, charges, NULL, NULL, h->coordinates, h->coordinates, NULL, v, &st ) ;
+    for ( i = 0 ; i < size ; i++ ) { S[i] = s->data[i] ; T[i] = t->data[i] ; V[i] = v->data[i] ; }
+    SymmetricMatrix_Deallocate ( &s ) ;
+    SymmetricMatrix_Deallocate ( &t ) ;
+    SymmetricMatrix_Deallocate ( &v ) ;
+    RealArray1D_Deallocate ( &charges ) ;
+    return ( int ) st ;
+}
+
+/* . Electron-fit Coulomb integrals (mu nu|P) as f1Xg2i stores them (the reference drops values below its underflow
+!    threshold), scattered into a dense array [packed orbital pair][fit function]; the number of stored values. */
+int xcref_fit_integrals ( void *vorb, void *vfit, double *dense, int *count )
+{
+    RefHandle    *o = ( RefHandle * ) vorb, *f = ( RefHandle * ) vfit ;
+    Status        st = Status_OK ;
+    BlockStorage *store = BlockStorage_Allocate ( &st ) ;
+    Block        *block ;
+    int           i, nFit = f->nFunctions, stored = 0 ;
+    GaussianBasisContainerIntegrals_f1Xg2i ( o->bases, f->bases, o->coordinates, GaussianBasisOperator_Coulomb, store, &st ) ;
+    List_Iterate_Initialize ( store->blocks ) ;
+    while ( ( block = BlockStorage_Iterate ( store ) ) != NULL )
+    {
+        for ( i = 0 ; i < block->count ; i++ )
+        {
+            dense[( size_t ) block->indices32[i] * nFit + block->indices16[i]] = block->data[i] ;
+            stored++ ;
+        }
+    }
+    *count = stored ;
+    BlockStorage_Deallocate ( &store ) ;
+    return ( int ) st ;
+}
+
+/* . The fit matrix of f1Xf1i_f1Oi with constraints (GaussianBasisIntegralEvaluator.pyx:260-302): (P|Q) and, in the last row,
+!    the fit functions' integrals; packed, extent nFit + 1. */
+int xcref_fit_matrix ( void *vfit, double *M )
+{
+    RefHandle       *f  = ( RefHandle * ) vfit ;
+    Status           st = Status_OK ;
+    int              i, n = f->nFunctions, size = ( n + 1 ) * ( n + 2 ) / 2 ;
+    SymmetricMatrix *eri = NewSymmetric ( n + 1, &st ) ;
+    RealArray1D     *fso = RealArray1D_AllocateWithExtent ( n, &st ) ;
+    GaussianBasisContainerIntegrals_f1Cf1i ( f->bases, f->coordinates, eri, &st ) ;
+    GaussianBasisContainerIntegrals_f1Oi   ( f->bases, fso, &st ) ;
+    for ( i = 0 ; i < n ; i++ ) SymmetricMatrix_Item ( eri, n, i ) = Array1D_Item ( fso, i ) ;
+    for ( i = 0 ; i < size ; i++ ) M[i] = eri->data[i] ;
+    SymmetricMatrix_Deallocate ( &eri ) ;
+    RealArray1D_Deallocate ( &fso ) ;
+    return ( int ) st ;
 }
 
 void xcref_free ( void *vh )

@@ -40,13 +40,27 @@ SIGNATURES = {
     "xq_test_fail_allocations": (None, [ctypes.c_longlong]),
 }
 
+# . The SCF wrapper's integrals (include/qcscf.h), in the same library.
+SCF_SIGNATURES = {
+    "qs_basis_create": (ctypes.c_int, [ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "qs_basis_size": (ctypes.c_int, [_vp, _ip]),
+    "qs_basis_free": (None, [_vp]),
+    "qs_one_electron": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "qs_fit_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "qs_fit_matrix": (ctypes.c_int, [_vp, _dp]),
+    "qs_fit_b": (ctypes.c_int, [_vp, _dp, _dp]),
+    "qs_fit_fock": (ctypes.c_int, [_vp, _dp, _dp]),
+    "qs_fit_stats": (ctypes.c_int, [_vp, _llp, _llp, _llp]),
+    "qs_fit_free": (None, [_vp]),
+}
+
 
 def load(path=LIBRARY):
     """Load the library and declare every entry point; raises OSError if it cannot be loaded."""
     if not os.path.exists(path):
         raise OSError("libxcquad.so not found at {:s}: build it with `python -m pdynamo3_amd.build`.".format(path))
     lib = ctypes.CDLL(path)
-    for name, (res, args) in SIGNATURES.items():
+    for name, (res, args) in list(SIGNATURES.items()) + list(SCF_SIGNATURES.items()):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args

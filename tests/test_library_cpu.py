@@ -13,9 +13,9 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "xcquad.h")).read()
-    return sorted(set(re.findall(r"\b(xq_[a-z_]+)\s*\(", text)))
+def declared_symbols(header="xcquad.h", prefix="xq_"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -25,6 +25,11 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(_lib.lib, name), name
     assert sorted(_lib.SIGNATURES) == names
+    scf = declared_symbols("qcscf.h", "qs_")
+    assert len(scf) >= 10
+    for name in scf:
+        assert hasattr(_lib.lib, name), name
+    assert sorted(_lib.SCF_SIGNATURES) == scf
 
 
 def test_estimated_points_match_reference_count():
