@@ -169,7 +169,7 @@ int functional_set(int nIDs, const int *ids, FunctionalSet *fs) {
 // Algorithmic HBM bytes of the two stages of one handle's quadrature: every stored value of a kept function at a grid
 // point (sum over blocks of kept functions x points, not the rows padded to chunks of 32) read once, every result
 // written once.
-static void stage_bytes(const xq_grid *g, double *rho, double *fock) {
+void stage_bytes(const xq_grid *g, double *rho, double *fock) {
     const double n = static_cast<double>(g->nFunctions), nn = n * n, G = static_cast<double>(g->nPoints),
                  B = static_cast<double>(g->nBlocks), values = g->bfValues;
     if (g->precision == XQ_PRECISION_FP64) {
