@@ -130,3 +130,46 @@ def cluster_density(rotations, label="6-31g_st", spherical=True):
         P[k * m:(k + 1) * m, k * m:(k + 1) * m] = D.T @ P1 @ D
     P = 0.5 * (P + P.T)
     return np.ascontiguousarray(P[np.tril_indices(n)])
+
+
+# Valence-orbital ionisation energies (hartree) of the extended-Hueckel densities: the oxygen core shell, then per element
+# and angular momentum.  Hydrogen and oxygen only (the water clusters).
+_EHT_CORE = {8: -20.6}
+_EHT_VALENCE = {(8, 0): -1.187, (8, 1): -0.544, (8, 2): -0.05, (1, 0): -0.4998, (1, 1): -0.05}
+
+
+def eht_diagonal_energies(Z, label="6-31g_st", spherical=True):
+    """H_ii of every basis function of the atoms Z, in the basis's function order."""
+    basis = load_basis(label)
+    h = []
+    for z in Z:
+        z = int(z)
+        for k, shell in enumerate(basis["elements"][str(z)]["shells"]):
+            for l in range(shell["lLow"], shell["lHigh"] + 1):
+                n = (2 * l + 1) if spherical else (l + 1) * (l + 2) // 2
+                if k == 0 and z in _EHT_CORE:
+                    e = _EHT_CORE[z]
+                elif (z, l) in _EHT_VALENCE:
+                    e = _EHT_VALENCE[(z, l)]
+                else:
+                    raise ValueError("no extended-Hueckel energy for element {:d}, l = {:d}".format(z, l))
+                h += [e] * n
+    return np.asarray(h)
+
+
+def eht_density(Z, S_packed, label="6-31g_st", spherical=True, charge=0):
+    """Packed extended-Hueckel density of a whole molecule (every pair of waters of a cluster coupled): H_ij =
+    1.75 S_ij (H_ii + H_jj) / 2, H_ii = e_i S_ii, the lowest (sum Z - charge) / 2 orbitals of H C = S C e doubly occupied.
+    S_packed is the overlap matrix (packed lower triangle), from the GPU integrals or any other source."""
+    import scipy.linalg
+    hd = eht_diagonal_energies(Z, label, spherical)
+    n = hd.size
+    S = np.zeros((n, n))
+    S[np.tril_indices(n)] = S_packed
+    S = S + np.tril(S, -1).T
+    H = 1.75 * S * 0.5 * (hd[:, None] + hd[None, :])
+    np.fill_diagonal(H, hd * np.diag(S))
+    n_occ = (int(np.sum(Z)) - charge) // 2
+    _, C = scipy.linalg.eigh(H, S, subset_by_index=[0, n_occ - 1])
+    P = 2.0 * C @ C.T
+    return np.ascontiguousarray(P[np.tril_indices(n)])

@@ -67,22 +67,28 @@ def nuclear_repulsion(Z, xyz):
 
 
 class GPUIntegrals:
-    """The integrals of one molecule on the GPU: one-electron matrices and the resident electron-fit integrals."""
+    """The integrals of one molecule on the GPU: one-electron matrices and the resident electron-fit integrals (none
+    when fit_spec is None: one-electron matrices only)."""
 
     def __init__(self, Z, xyz_bohr, orbital_spec, spherical, fit_spec, fit_spherical=True):
         self.Z = _lib.i32(Z)
         self.xyz = _lib.f64(xyz_bohr).reshape(-1, 3)
         self._orb, self._fitb, self._fit = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
         L = _lib.lib
-        o, f = _lib.f64(orbital_spec), _lib.f64(fit_spec)
+        o = _lib.f64(orbital_spec)
         _check(L.qs_basis_create(self.Z.size, _lib.iptr(self.Z), _lib.dptr(self.xyz), o.size, _lib.dptr(o),
                                  int(bool(spherical)), ctypes.byref(self._orb)), "orbital basis")
+        n = ctypes.c_int()
+        _check(L.qs_basis_size(self._orb, ctypes.byref(n)), "basis size")
+        self.n, self.n_fit = n.value, 0
+        if fit_spec is None:
+            return
+        f = _lib.f64(fit_spec)
         _check(L.qs_basis_create(self.Z.size, _lib.iptr(self.Z), _lib.dptr(self.xyz), f.size, _lib.dptr(f),
                                  int(bool(fit_spherical)), ctypes.byref(self._fitb)), "fit basis")
-        n, nf = ctypes.c_int(), ctypes.c_int()
-        _check(L.qs_basis_size(self._orb, ctypes.byref(n)), "basis size")
+        nf = ctypes.c_int()
         _check(L.qs_basis_size(self._fitb, ctypes.byref(nf)), "basis size")
-        self.n, self.n_fit = n.value, nf.value
+        self.n_fit = nf.value
         _check(L.qs_fit_create(self._orb, self._fitb, ctypes.byref(self._fit)), "fit integrals")
 
     def one_electron(self):

@@ -15,6 +15,8 @@ them through oracle/ref_driver.c and stores inputs and outputs only (numbers, no
                             E_xc, N_e, packed V_xc and the XC nuclear gradients (BLYP, spherical, EHT monomer density);
                             and at Medium the unrestricted E_xc, N_e, V_xc of both spins and gradients for alpha = 0.6 P,
                             beta = 0.4 P (PBE).
+  reference_cluster_h2o64.json  (H2O)64 BLYP (14 A cube, seed 2, Cartesian) at the whole-cluster extended-Hueckel
+                            density: grid size, E_xc, N_e, every 97th packed V_xc entry, and the same sample of P.
   reference_c2s.json        the reference's own Cartesian -> spherical fixture (examples/pMolecule/data/yaml/
                             C2STransformations_0to6.yaml, read with SafeLoader) converted to JSON.
 
@@ -105,6 +107,34 @@ def cluster():
         json.dump(out, f)
 
 
+def eht_cluster_density(Z, xyz, spec, spherical):
+    """The whole-cluster extended-Hueckel density (every pair of waters coupled), with the compiled reference's overlap."""
+    S, _, _ = oracle.RefBasis(Z, xyz, spec, spherical).one_electron()
+    return densities.eht_density(Z, S, spherical=spherical)
+
+
+def cluster_large():
+    """(H2O)64 BLYP (14 A cube, seed 2, Cartesian), the mixed-precision benchmark cluster, at its whole-cluster
+    extended-Hueckel density: E_xc, N_e and every 97th packed V_xc entry from the compiled reference, with checksums and
+    the same sample of P so that a test can confirm it rebuilt the same density."""
+    Z, xa, _, _ = molecules.water_cluster(64, 14.0, 2)
+    xyz = molecules.to_bohr(xa)
+    spec = basis.element_spec(basis.load_basis(), Z)
+    P = eht_cluster_density(Z, xyz, spec, False)
+    h = oracle.reference().setup(2, Z, xyz, spec, False, BLYP)
+    print("h2o64 grid", h.npoints, flush=True)
+    e, ne, V, sec = h.integrate(P, 0)
+    stride = 97
+    out = {"waters": 64, "cube": 14.0, "seed": 2, "spherical": False, "functional": "blyp",
+           "density": "whole-cluster extended-Hueckel (pdynamo3_amd.densities.eht_density, reference overlap)",
+           "npoints": h.npoints, "nblocks": h.nblocks, "E_xc": e, "N_e": ne, "stride": stride,
+           "V_sample": V[::stride].tolist(), "P_sample": P[::stride].tolist(), "V_norm": float(np.linalg.norm(V)),
+           "V_sum": float(np.sum(V)), "P_norm": float(np.linalg.norm(P)), "seconds": sec}
+    print("h2o64", h.npoints, e, ne, sec, flush=True)
+    with open(os.path.join(HERE, "reference_cluster_h2o64.json"), "w") as f:
+        json.dump(out, f)
+
+
 def functionals():
     ref = oracle.reference()
     rho = [5.0e-13, 4.9e-13, 1.0e-12, 1.0e-10, 1.0e-6, 1.0e-3, 0.1, 1.0, 10.0, 300.0, 1.0e-3, 1.0, 1.0, 0.5, 2.0e-5]
@@ -177,6 +207,6 @@ def accuracies():
 
 if __name__ == "__main__":
     steps = {"functionals": functionals, "functionals_pol": functionals_pol, "c2s": c2s, "water": water,
-             "cluster": cluster, "accuracies": accuracies}
+             "cluster": cluster, "cluster_large": cluster_large, "accuracies": accuracies}
     for name in sys.argv[1:] or list(steps):
         steps[name]()

@@ -250,3 +250,28 @@ def test_unrestricted_matches_reference_fixture(precision):
     assert abs(e - ref["E_xc"]) <= tolE and abs(ne - ref["N_e"]) <= tolE
     assert np.max(np.abs(Va - np.asarray(ref["Va"]))) <= tolV and np.max(np.abs(Vb - np.asarray(ref["Vb"]))) <= tolV
     assert np.max(np.abs(G - np.asarray(ref["gradients"]))) <= 1e-9
+
+
+def test_mixed_path_equals_its_cpu_emulation():
+    """The int8 path's density arithmetic (5 P digits per row, 5 Phi digits per point, digit orders <= 4, 32-bit grad Phi)
+    emulated on the CPU from the oracle's values (tests/precision_emulator.py) gives the GPU's mixed-precision E_xc and
+    N_e to 1e-10 on (H2O)4 PBE at its whole-cluster extended-Hueckel density; the FP64 result differs from both by the
+    rounding itself (about 5e-9 Eh here)."""
+    import oracle
+    from precision_emulator import Emulator
+    from pdynamo3_amd import densities, integrator
+    if not oracle.reference_available():
+        pytest.skip("needs the compiled reference (oracle/_ref) for the overlap matrix")
+    Z, xyz, spec, _ = cluster_system(4, 6.0, 5)
+    S, _, _ = oracle.RefBasis(Z, xyz, spec, True).one_electron()
+    P = densities.eht_density(Z, S, spherical=True)
+    h = oracle.oracle().setup(2, Z, xyz, spec, True, [101, 130])
+    em = Emulator(h, oracle.oracle(), [101, 130])
+    e_emu, n_emu = em.quadrature(P, phi_digits=5, p_digits=5, kept_orders=4, grad_bits=31)
+    g = integrator.QuadratureGrid(Z, xyz, 2)
+    g.set_basis(spec, True, "mixed")
+    V = np.zeros_like(P)
+    e, n, _ = g.integrate([101, 130], P, V)
+    g.close()
+    print("GPU mixed - emulation: dE {:.2e} dN {:.2e}".format(e - e_emu, n - n_emu))
+    assert abs(e - e_emu) < 1e-10 and abs(n - n_emu) < 1e-10
