@@ -52,6 +52,10 @@ def parse():
                         "(0 skips them)")
     p.add_argument("--grad-steps", type=int, default=2,
                    help="timed XC nuclear-gradient calls on the workload after the quadrature legs (0 skips them)")
+    p.add_argument("--scf", default="h2o16,h2o64",
+                   help="SCF legs, comma-separated workloads: h2o16 runs one full SCF in each precision, timed; h2o64 runs "
+                        "the FP64 SCF with the mixed-precision quadrature's E_xc / V_xc errors at every iteration; "
+                        "'none' skips them")
     p.add_argument("--serve-requests", type=int, default=256,
                    help="requests in the serving stream (0 skips it)")
     p.add_argument("--serve-slots", type=int, default=16, help="jobs set up or running at once in the serving loop")
@@ -223,18 +227,75 @@ def gradient_leg(grid, ids, P, steps, n_waters, functional, spherical, cpu_kind)
     return out
 
 
+def scf_leg(name, compare, log_fn):
+    """Restricted Kohn-Sham SCF of a workload cluster (scf.RKSDFT: one-electron and electron-fit integrals on the GPU, the
+    XC quadrature on the GPU, ODA then DIIS on the host, converged to 1e-6 Eh and an RMS density change of 1e-6), from
+    the diagonal guess.  compare=False: one full SCF per precision, wall time of the set-up and of the iterations with
+    the time per phase.  compare=True: the FP64 SCF, and at every iteration the mixed-precision quadrature at the same
+    density on a second handle, with its E_xc, N_e and V_xc differences from the FP64 quadrature."""
+    from pdynamo3_amd import basis, integrator, molecules, scf
+    n_w, cube, seed, functional = WORKLOADS[name]
+    Z, xa, _, _ = molecules.water_cluster(n_w, cube, seed)
+    xyz = molecules.to_bohr(xa)
+    spec = basis.element_spec(basis.load_basis(), Z)
+    out = {"workload": "(H2O){:d} cluster, {:.0f} A cube, seed {:d}, 6-31G* Cartesian, {:s}, Medium grid, J fitted in "
+                       "DGauss A1 (dgauss-a1-dftjfit)".format(n_w, cube, seed, functional.upper())}
+    for precision in (["fp64"] if compare else ["fp64", "mixed"]):
+        t0 = time.perf_counter()
+        s = scf.RKSDFT(Z, xyz, functional=functional, precision=precision, spherical=False)
+        setup_s = time.perf_counter() - t0
+        gm, rows, t_cmp = None, [], 0.0
+        if compare:
+            gm = integrator.QuadratureGrid(Z, xyz, 2)
+            gm.set_basis(spec, False, "mixed")
+        t1 = time.perf_counter()
+        while True:
+            P = s.density()
+            e, V = s.xc(P)
+            if gm is not None:
+                tc = time.perf_counter()
+                Vm = np.zeros_like(P)
+                em, nm, _ = gm.integrate(s.ids, P, Vm)
+                _, n64, _ = s.grid.integrate(s.ids, P, np.zeros_like(P))
+                t_cmp += time.perf_counter() - tc
+                rows.append({"iteration": s.iteration + 1, "dExc": em - e, "dNe": nm - n64,
+                             "max_dVxc": float(np.max(np.abs(Vm - V)))})
+            if s.advance(e, V):
+                break
+        iter_s = time.perf_counter() - t1 - t_cmp
+        res = s.summary()
+        leg = {"energy": res["energy"], "converged": res["converged"], "iterations": res["iterations"],
+               "setup_s": setup_s, "iterations_s": iter_s, "total_s": setup_s + iter_s,
+               "timings": res["timings"], "points": s.grid.number_of_points, "functions": s.n,
+               "fit_functions": s.integrals.n_fit, **s.integrals.stats()}
+        if rows:
+            leg["mixed_vs_fp64_per_iteration"] = rows
+            leg["max_abs_dExc"] = max(abs(r["dExc"]) for r in rows)
+            leg["max_dVxc"] = max(r["max_dVxc"] for r in rows)
+            leg["note"] = "iterations_s excludes the comparison quadratures"
+            gm.close()
+        s.close()
+        out[precision] = leg
+        log_fn("scf {:s} {:s}:".format(name, precision), json.dumps({k: v for k, v in leg.items()
+                                                                      if k != "mixed_vs_fp64_per_iteration"}))
+    if not compare:
+        out["dE_mixed_vs_fp64"] = out["mixed"]["energy"] - out["fp64"]["energy"]
+    return out
+
+
 def serving_leg(args, log_fn):
     """The serving stream in both precisions: SCF requests on (H2O)N, N log-uniform in [4, 64] (seed 3), BLYP/PBE by coin
-    flip, Poisson arrivals.  The mixed loop's capacity is measured first, on a calibration stream of 32 other requests
-    (seed 7) that all arrive at once, after a warm-up pass of 16 more (seed 8) that fills the device memory cache and
-    loads the kernels; the arrival rate is serve-load x that capacity.  The FP64 loop then serves the same stream (same
+    flip, Poisson arrivals, each a restricted Kohn-Sham SCF to convergence.  The mixed loop's capacity is measured
+    first, on a calibration stream of 32 other requests (seed 7) that all arrive at once, after a warm-up pass of 16
+    more (seed 8) that fills the device memory cache and loads the kernels; the arrival rate is serve-load x that
+    capacity.  The FP64 loop then serves the same stream (same
     arrival times) after its own warm-up and calibration, so the two legs differ only in the precision."""
     import dataclasses
     from pdynamo3_amd import serving
     base = serving.request_stream(n_requests=args.serve_requests, seed=3, rate=1.0)
     out, rate = {}, None
     for precision in ("mixed", "fp64"):
-        srv = serving.Server(precision=precision, max_jobs=args.serve_slots)
+        srv = serving.Server(precision=precision, max_jobs=args.serve_slots, scf="rks")
         srv.run(serving.request_stream(n_requests=16, seed=8, rate=1.0e6))
         done, cl = srv.run(serving.request_stream(n_requests=32, seed=7, rate=1.0e6))
         capacity = len(done) / cl["wall_s"]
@@ -247,9 +308,9 @@ def serving_leg(args, log_fn):
         leg.update({"arrival_rate_per_s": rate, "capacity_requests_per_s": capacity, "load": rate / capacity,
                     "slots": args.serve_slots, "hbm_budget_bytes": srv.hbm_budget, "precision": precision,
                     "stream": "{:d} requests, (H2O)N with N log-uniform in [4, 64] (seed 3), 6-31G* Cartesian, BLYP/PBE "
-                              "by coin flip, Poisson arrivals at the mixed loop's capacity; each request is an SCF "
-                              "density sequence converging geometrically (ratio 0.4) until |dE_xc| < 1e-6 Eh".format(
-                                  len(reqs))})
+                              "by coin flip, Poisson arrivals at the mixed loop's capacity; each request runs a "
+                              "restricted Kohn-Sham SCF (J fitted, XC quadrature on the GPU) from the diagonal guess to "
+                              "1e-6 Eh and an RMS density change of 1e-6".format(len(reqs))})
         out[precision] = leg
         log_fn("serving {:s}:".format(precision), json.dumps(leg))
     return out
@@ -341,6 +402,10 @@ def main():
                                            args.cpu_baseline)
         log("gradients:", json.dumps(out["xc_gradients"]))
     grid.close()
+    if args.scf != "none":
+        out["scf"] = {}
+        for name in [w for w in args.scf.split(",") if w]:
+            out["scf"][name] = scf_leg(name, name != "h2o16", log)
     if args.serve_requests > 0:
         serve = serving_leg(args, log)
         out["latency_ms_p50"] = 1e3 * serve["mixed"]["latency_s_p50"]

@@ -169,6 +169,7 @@ __global__ void __launch_bounds__(256) fit_fock_kernel(long long rows, int nFit,
 struct qs_basis : qs::QBasis {
     qs::QShell *d_shells = nullptr;
     double *d_exps = nullptr, *d_coefs = nullptr;
+    hipStream_t stream = nullptr;      // the one-electron kernel's stream
 };
 
 struct qs_fit {
@@ -186,10 +187,12 @@ struct qs_fit {
 namespace qs {
 namespace {
 
+// Device arrays come from the quadrature library's caching allocator: a request loop sets up and releases molecules
+// while other jobs compute, and hipFree would synchronise the whole device.
 template <typename T>
 int qs_alloc(T **p, size_t n) {
     *p = nullptr;
-    if (hipMalloc(reinterpret_cast<void **>(p), (n > 0 ? n : 1) * sizeof(T)) != hipSuccess) {
+    if (xq::cache_alloc(reinterpret_cast<void **>(p), (n > 0 ? n : 1) * sizeof(T)) != hipSuccess) {
         (void)hipGetLastError();
         *p = nullptr;
         set_error("hipMalloc failed (" + std::to_string((n > 0 ? n : 1) * sizeof(T)) + " bytes)");
@@ -200,8 +203,15 @@ int qs_alloc(T **p, size_t n) {
 
 template <typename T>
 void qs_release(T **p) {
-    if (*p != nullptr) { (void)hipFree(*p); *p = nullptr; }
+    if (*p != nullptr) { (void)xq::cache_free(*p); *p = nullptr; }
 }
+
+// The integral kernels hold their McMurchie-Davidson tables in private memory (about 14 KB per lane for the fit
+// integrals), and the runtime gives every hardware queue that runs one its own scratch area sized for the whole
+// device.  Two such kernels in flight at once on two queues (set-up threads of a request loop) made the second queue's
+// scratch allocation fail (HSA_STATUS_ERROR_OUT_OF_RESOURCES, queue abort), so at most one of them runs at a time in
+// the process: each holds this lock from its launch until its stream has drained.
+std::mutex g_integralKernels;
 
 }  // namespace
 }  // namespace qs
@@ -226,15 +236,17 @@ int qs_basis_create(int nAtoms, const int *Z, const double *xyzBohr, int nSpec, 
     const std::string err = build_shells(b, nSpec, spec);
     int status = kOK;
     if (!err.empty()) { set_error(err); status = kInvalidArgument; }
+    if (status == kOK) status = check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (status == kOK) status = qs_alloc(&b->d_shells, b->shells.size());
     if (status == kOK) status = qs_alloc(&b->d_exps, b->exps.size());
     if (status == kOK) status = qs_alloc(&b->d_coefs, b->coefs.size());
-    if (status == kOK) status = check(hipMemcpy(b->d_shells, b->shells.data(), b->shells.size() * sizeof(QShell),
-                                                hipMemcpyHostToDevice), "copy shells");
-    if (status == kOK) status = check(hipMemcpy(b->d_exps, b->exps.data(), b->exps.size() * sizeof(double),
-                                                hipMemcpyHostToDevice), "copy exponents");
-    if (status == kOK) status = check(hipMemcpy(b->d_coefs, b->coefs.data(), b->coefs.size() * sizeof(double),
-                                                hipMemcpyHostToDevice), "copy coefficients");
+    if (status == kOK) status = check(hipMemcpyAsync(b->d_shells, b->shells.data(), b->shells.size() * sizeof(QShell),
+                                                     hipMemcpyHostToDevice, b->stream), "copy shells");
+    if (status == kOK) status = check(hipMemcpyAsync(b->d_exps, b->exps.data(), b->exps.size() * sizeof(double),
+                                                     hipMemcpyHostToDevice, b->stream), "copy exponents");
+    if (status == kOK) status = check(hipMemcpyAsync(b->d_coefs, b->coefs.data(), b->coefs.size() * sizeof(double),
+                                                     hipMemcpyHostToDevice, b->stream), "copy coefficients");
+    if (status == kOK) status = check(hipStreamSynchronize(b->stream), "copy basis");
     if (status != kOK) { qs_basis_free(b); return status; }
     *basis = b;
     return kOK;
@@ -251,6 +263,7 @@ void qs_basis_free(qs_basis *basis) {
     qs_release(&basis->d_shells);
     qs_release(&basis->d_exps);
     qs_release(&basis->d_coefs);
+    if (basis->stream) (void)hipStreamDestroy(basis->stream);
     delete basis;
 }
 
@@ -273,20 +286,24 @@ int qs_one_electron(qs_basis *basis, double *S, double *T, double *V) {
     if (status == kOK) status = qs_alloc(&d_S, size);
     if (status == kOK) status = qs_alloc(&d_T, size);
     if (status == kOK) status = qs_alloc(&d_V, size);
-    if (status == kOK) status = check(hipMemcpy(d_pairs, pairs.data(), pairs.size() * sizeof(QPair), hipMemcpyHostToDevice),
-                                      "copy pairs");
-    if (status == kOK) status = check(hipMemcpy(d_atoms, atoms.data(), atoms.size() * sizeof(double),
-                                                hipMemcpyHostToDevice), "copy atoms");
+    const hipStream_t s = basis->stream;
+    if (status == kOK) status = check(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(QPair),
+                                                     hipMemcpyHostToDevice, s), "copy pairs");
+    if (status == kOK) status = check(hipMemcpyAsync(d_atoms, atoms.data(), atoms.size() * sizeof(double),
+                                                     hipMemcpyHostToDevice, s), "copy atoms");
     if (status == kOK && !pairs.empty()) {
+        std::lock_guard<std::mutex> lock(g_integralKernels);
         hipLaunchKernelGGL(one_electron_kernel, dim3((static_cast<unsigned>(pairs.size()) + kThreads - 1) / kThreads),
-                           dim3(kThreads), 0, 0, static_cast<int>(pairs.size()), d_pairs, basis->d_shells, basis->d_exps,
+                           dim3(kThreads), 0, s, static_cast<int>(pairs.size()), d_pairs, basis->d_shells, basis->d_exps,
                            basis->d_coefs, basis->nAtoms, d_atoms, basis->spherical, d_S, d_T, d_V);
         status = check(hipGetLastError(), "one_electron_kernel");
+        const int drained = check(hipStreamSynchronize(s), "one-electron integrals");
+        if (status == kOK) status = drained;
     }
-    if (status == kOK) status = check(hipMemcpy(S, d_S, size * sizeof(double), hipMemcpyDeviceToHost), "copy S");
-    if (status == kOK) status = check(hipMemcpy(T, d_T, size * sizeof(double), hipMemcpyDeviceToHost), "copy T");
-    if (status == kOK) status = check(hipMemcpy(V, d_V, size * sizeof(double), hipMemcpyDeviceToHost), "copy V");
-    (void)hipDeviceSynchronize();
+    if (status == kOK) status = check(hipMemcpyAsync(S, d_S, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy S");
+    if (status == kOK) status = check(hipMemcpyAsync(T, d_T, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy T");
+    if (status == kOK) status = check(hipMemcpyAsync(V, d_V, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy V");
+    (void)hipStreamSynchronize(s);
     qs_release(&d_pairs);
     qs_release(&d_atoms);
     qs_release(&d_S);
@@ -358,6 +375,7 @@ int qs_fit_create(const qs_basis *orbital, const qs_basis *fitBasis, qs_fit **ou
         if (status == kOK) status = check(hipMemcpy(d_pairs, pairs.data(), pairs.size() * sizeof(PairEntry),
                                                     hipMemcpyHostToDevice), "copy pairs");
         const long long nWork = F->nPairs * static_cast<long long>(fitBasis->shells.size());
+        std::lock_guard<std::mutex> lock(g_integralKernels);
         if (status == kOK && nWork > 0) {
             hipLaunchKernelGGL(fit_integrals_kernel, dim3(static_cast<unsigned>((nWork + kThreads - 1) / kThreads)),
                                dim3(kThreads), 0, F->stream, nWork, static_cast<int>(fitBasis->shells.size()), d_pairs,
@@ -366,7 +384,8 @@ int qs_fit_create(const qs_basis *orbital, const qs_basis *fitBasis, qs_fit **ou
                                F->d_B);
             status = check(hipGetLastError(), "fit_integrals_kernel");
         }
-        if (status == kOK) status = check(hipStreamSynchronize(F->stream), "fit integrals");
+        const int drained = check(hipStreamSynchronize(F->stream), "fit integrals");
+        if (status == kOK) status = drained;
     }
     qs_release(&d_pairs);
     if (status == kOK) F->fitPairs = shell_pairs(fitBasis, false);
@@ -385,13 +404,19 @@ int qs_fit_matrix(qs_fit *fit, double *M) {
     if (status == kOK) status = check(hipMemsetAsync(d_M, 0, size * sizeof(double), fit->stream), "clear M");
     if (status == kOK) status = check(hipMemcpy(d_pairs, fit->fitPairs.data(), fit->fitPairs.size() * sizeof(QPair),
                                                 hipMemcpyHostToDevice), "copy fit pairs");
-    if (status == kOK && !fit->fitPairs.empty()) {
-        hipLaunchKernelGGL(fit_matrix_kernel, dim3((static_cast<unsigned>(fit->fitPairs.size()) + kThreads - 1) / kThreads),
-                           dim3(kThreads), 0, fit->stream, static_cast<int>(fit->fitPairs.size()), d_pairs,
-                           fit->fit->d_shells, fit->fit->d_exps, fit->fit->d_coefs, fit->nFit, fit->fit->spherical, d_M);
-        status = check(hipGetLastError(), "fit_matrix_kernel");
+    {
+        std::lock_guard<std::mutex> lock(g_integralKernels);
+        if (status == kOK && !fit->fitPairs.empty()) {
+            hipLaunchKernelGGL(fit_matrix_kernel,
+                               dim3((static_cast<unsigned>(fit->fitPairs.size()) + kThreads - 1) / kThreads),
+                               dim3(kThreads), 0, fit->stream, static_cast<int>(fit->fitPairs.size()), d_pairs,
+                               fit->fit->d_shells, fit->fit->d_exps, fit->fit->d_coefs, fit->nFit, fit->fit->spherical,
+                               d_M);
+            status = check(hipGetLastError(), "fit_matrix_kernel");
+        }
+        const int drained = check(hipStreamSynchronize(fit->stream), "fit matrix");
+        if (status == kOK) status = drained;
     }
-    if (status == kOK) status = check(hipStreamSynchronize(fit->stream), "fit matrix");
     if (status == kOK) status = check(hipMemcpy(M, d_M, size * sizeof(double), hipMemcpyDeviceToHost), "copy M");
     qs_release(&d_M);
     qs_release(&d_pairs);

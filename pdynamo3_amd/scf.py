@@ -23,6 +23,7 @@ drives it alone.
 """
 
 import ctypes
+import time
 
 import numpy as np
 import scipy.linalg
@@ -178,6 +179,7 @@ class RKSDFT:
 
     # ---------------------------------------------------------------------------------------------------------------
     def _setup(self, guess):
+        self.timings = {"quadrature_s": 0.0, "coulomb_s": 0.0, "diagonalisation_s": 0.0, "diis_s": 0.0}
         S, T, V = self.integrals.one_electron()
         self.S_packed, self.H = S, T + V
         self.S = unpack(S, self.n)
@@ -208,10 +210,12 @@ class RKSDFT:
 
     def coulomb(self, P, F):
         """Add the fitted J(P) to F (packed); return E_fit."""
+        t = time.perf_counter()
         b = self.integrals.fit_b(P)
         rhs = np.append(b, self.n_electrons)
         c = scipy.linalg.lu_solve(self.fit_lu, rhs)
         self.integrals.fit_fock(c[:-1], F)
+        self.timings["coulomb_s"] += time.perf_counter() - t
         return 0.5 * float(np.dot(c, rhs))
 
     def fock(self, e_xc, v_xc):
@@ -226,11 +230,14 @@ class RKSDFT:
 
     def _aufbau(self, F):
         """The density of the n_occupied lowest orbitals of F (in the orthonormal basis X)."""
+        t = time.perf_counter()
         Fo = self.X.T @ unpack(F, self.n) @ self.X
         e, C = np.linalg.eigh(Fo)
         Cocc = self.X @ C[:, :self.n_occupied]
         self.orbital_energies = e
-        return pack(2.0 * Cocc @ Cocc.T)
+        P = pack(2.0 * Cocc @ Cocc.T)
+        self.timings["diagonalisation_s"] += time.perf_counter() - t
+        return P
 
     def commutator_error(self, F, P):
         Fd, Pd = unpack(F, self.n), unpack(P, self.n)
@@ -241,7 +248,9 @@ class RKSDFT:
         """Take the current density's XC energy and matrix; build its Fock matrix, test convergence and choose the next
         density.  Returns True once the SCF has converged or stopped."""
         F, energy, parts = self.fock(e_xc, v_xc)
+        t = time.perf_counter()
         err = self.commutator_error(F, self.P)
+        self.timings["diis_s"] += time.perf_counter() - t
         err_max = float(np.max(np.abs(err)))
         dE = None if self.last_energy is None else energy - self.last_energy
         dP = None if self.last_P is None else float(np.sqrt(np.mean((self.P - self.last_P) ** 2)))
@@ -264,7 +273,9 @@ class RKSDFT:
             self.history_e.append(err)
             self.history_F = self.history_F[-self.history:]
             self.history_e = self.history_e[-self.history:]
+            t = time.perf_counter()
             F_next = self._diis_extrapolate()
+            self.timings["diis_s"] += time.perf_counter() - t
             self.P = self._aufbau(F_next)
         else:
             self.P = self._oda_step(F, energy)
@@ -307,8 +318,10 @@ class RKSDFT:
 
     def xc(self, P):
         """The XC energy and matrix at P on the GPU (a batch of one)."""
+        t = time.perf_counter()
         V = np.zeros_like(P)
         e, _, _ = self.grid.integrate(self.ids, P, V)
+        self.timings["quadrature_s"] += time.perf_counter() - t
         return e, V
 
     def run(self, log=None):
@@ -326,7 +339,8 @@ class RKSDFT:
         return self.summary()
 
     def summary(self):
-        return {"energy": self.energy, "converged": self.converged, "iterations": self.iteration, **self.parts}
+        return {"energy": self.energy, "converged": self.converged, "iterations": self.iteration, **self.parts,
+                "timings": dict(self.timings)}
 
     def close(self):
         if isinstance(self.integrals, GPUIntegrals):

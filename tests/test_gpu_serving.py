@@ -16,7 +16,7 @@ def test_served_energies_equal_alone():
     from pdynamo3_amd import integrator, serving
     from pdynamo3_amd.definitions import functional_ids
     reqs = _stream(6, 11)
-    srv = serving.Server(precision="mixed", max_jobs=4, hbm_budget=16 << 30)
+    srv = serving.Server(precision="mixed", max_jobs=4, hbm_budget=16 << 30, scf="synthetic")
     finished, loop = srv.run(reqs)
     srv.close()
     assert len(finished) == len(reqs)

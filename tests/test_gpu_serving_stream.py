@@ -33,7 +33,7 @@ def _alone(request, precision, quadratures):
 def test_stream_results_equal_alone(precision):
     from pdynamo3_amd import serving
     reqs = serving.request_stream(n_requests=256, seed=3, rate=200.0)
-    srv = serving.Server(precision=precision, max_jobs=16, keep_fock=True)
+    srv = serving.Server(precision=precision, max_jobs=16, keep_fock=True, scf="synthetic")
     finished, loop = srv.run(reqs)
     srv.close()
     summary = serving.summarize(finished, loop)
