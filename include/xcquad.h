@@ -68,6 +68,18 @@ int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, 
 int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
                                   double *nElectrons, double *fockV, double *stats ) ;
 
+/* xq_integrate without allocation.  No counterpart in the reference: it is for callers that cannot allocate device or
+!  pinned host memory during a call (code under HIP graph capture, serving loops that forbid allocation).  Same arguments,
+!  semantics and results as xq_integrate (E_xc and N_e bit for bit; V_xc up to the order of the Fock stage's FP64 atomic
+!  additions), on a handle set up by xq_grid_build and xq_basis_set (which allocate).  The per-call work arrays live in one
+!  static device array of 64 MiB per GPU, enough for about 1,600 basis functions; a larger molecule is refused with
+!  status 3 before any work is queued.  P and V move through the caller's own (pageable) host arrays.  The handle's device
+!  must be current.  Every call on one GPU shares that GPU's array: calls that overlap on one GPU race.
+!  xq_noalloc_scratch reports the bytes a handle's call needs (neededBytes, 0 for a NULL handle) and the array's size. */
+int         xq_integrate_noalloc ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
+                                   double *nElectrons, double *fockV, double *stats ) ;
+int         xq_noalloc_scratch   ( const xq_grid *grid, long long *neededBytes, long long *limitBytes ) ;
+
 /* One unrestricted quadrature: DFTIntegrator_Integrate with densityQ, fockB and isSpinUnrestricted (DFTIntegrator.c:159-509,
 !  the spin branches :301-330, :342-347, :366-402), as DFTIntegrator.pyx:82-107 calls it with alpha and beta matrices (the
 !  caller converts total and spin matrices: alpha = (T + S)/2, beta = (T - S)/2; F_T += (F_a + F_b)/2, F_S += (F_a - F_b)/2).

@@ -25,6 +25,8 @@ SIGNATURES = {
     "xq_estimated_points": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _ip, _llp]),
     "xq_basis_set": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _ip]),
     "xq_integrate": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp]),
+    "xq_integrate_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp]),
+    "xq_noalloc_scratch": (ctypes.c_int, [_vp, _llp, _llp]),
     "xq_integrate_uks": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "xq_gradients": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp]),
     "xq_gradients_uks": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp]),

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBRARY = os.path.join(HERE, "libxcquad.so")
 SOURCES = ["xq_alloc.hip", "xq_api.hip", "xq_grid.hip", "xq_basis.hip", "xq_integrate.hip", "xq_f64.hip", "xq_mixed.hip",
-           "xq_gradients.hip", "xq_uks.hip", "qs_integrals.hip"]
+           "xq_gradients.hip", "xq_uks.hip", "xq_noalloc.hip", "qs_integrals.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          # Packed FP32 instructions stay off: hipcc would otherwise form v_pk_* FP32 ops from plain float code.

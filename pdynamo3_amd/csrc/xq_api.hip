@@ -46,7 +46,8 @@ int xq_grid_build(int accuracy, int nAtoms, const int *atomicNumbers, const doub
     g->nAtoms = nAtoms;
     g->Z.assign(atomicNumbers, atomicNumbers + nAtoms);
     g->xyz.assign(xyzBohr, xyzBohr + 3 * nAtoms);
-    int status = check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
+    int status = check(hipGetDevice(&g->device), "hipGetDevice");
+    if (status == kOK) status = check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (status == kOK) status = check(hipEventCreate(&g->ev0), "hipEventCreate");
     if (status == kOK) status = check(hipEventCreate(&g->ev1), "hipEventCreate");
     if (status == kOK) status = check(hipEventCreate(&g->evRho0), "hipEventCreate");
@@ -122,6 +123,25 @@ int xq_integrate(xq_grid *grid, int nIDs, const int *ids, const double *densityP
     *nElectrons = 0.0;
     if (!grid->hasBasis) { set_error("no basis set on this grid"); return kInvalidArgument; }
     return integrate(grid, nIDs, ids, densityP, eXC, nElectrons, fockV, stats);
+}
+
+int xq_integrate_noalloc(xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
+                         double *nElectrons, double *fockV, double *stats) {
+    if (grid == nullptr || ids == nullptr || densityP == nullptr || eXC == nullptr || nElectrons == nullptr ||
+        fockV == nullptr) {
+        set_error("invalid integrate arguments");
+        return kInvalidArgument;
+    }
+    *eXC = 0.0;
+    *nElectrons = 0.0;
+    if (!grid->hasBasis) { set_error("no basis set on this grid"); return kInvalidArgument; }
+    return integrate_noalloc(grid, nIDs, ids, densityP, eXC, nElectrons, fockV, stats);
+}
+
+int xq_noalloc_scratch(const xq_grid *grid, long long *neededBytes, long long *limitBytes) {
+    if (limitBytes != nullptr) *limitBytes = static_cast<long long>(noalloc_scratch_bytes());
+    if (neededBytes != nullptr) *neededBytes = grid == nullptr ? 0 : static_cast<long long>(noalloc_needed_bytes(grid));
+    return kOK;
 }
 
 int xq_gradients(xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *gradients3) {

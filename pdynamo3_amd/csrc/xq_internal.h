@@ -85,6 +85,7 @@ struct MixedJob {
 // The handle.  Device arrays are prefixed d_.
 struct xq_grid {
     int accuracy = 2;
+    int device = 0;                      // the GPU that was current when the handle was built
     int nAtoms = 0;
     std::vector<int> Z;
     std::vector<double> xyz;
@@ -246,6 +247,11 @@ int ensure_work(xq_grid *g);
 int ensure_job_table(xq_grid *g);
 int upload_density(xq_grid *g, const double *Ppacked, hipStream_t s);
 void free_mixed(xq_grid *g);
+// xq_integrate_noalloc (xq_noalloc.hip): the scratch array's size and what one call on the handle needs of it.
+int integrate_noalloc(xq_grid *g, int nIDs, const int *ids, const double *P, double *eXC, double *nE, double *V,
+                      double *stats);
+size_t noalloc_scratch_bytes();
+size_t noalloc_needed_bytes(const xq_grid *g);
 
 }  // namespace xq
 
