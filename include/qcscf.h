@@ -16,6 +16,7 @@
 !   qs_fit_b          <- the b vector of Fock_MakeCoefficientsFromFitIntegrals (FockConstruction.c:31-69), without the
 !                        constraint entry (the caller appends the total charge)
 !   qs_fit_fock       <- Fock_MakeFockFromFitIntegrals (FockConstruction.c:74-96)
+!   qs_fit_estimate_bytes  (no counterpart: the device bytes of qs_fit_create, for admission control)
 ! Conventions as include/xcquad.h: bohr, packed lower triangles (i(i+1)/2 + j), the reference's Status codes, host pointers.
 !=================================================================================================================================*/
 # ifndef QCSCF_H
@@ -46,6 +47,11 @@ int  qs_fit_matrix   ( qs_fit *fit, double *M ) ;
 int  qs_fit_b        ( qs_fit *fit, const double *densityP, double *b ) ;
 /* F_mn += sum_P c_P (mn|P) (packed F, INCREMENTED; c holds nFit values). */
 int  qs_fit_fock     ( qs_fit *fit, const double *c, double *fockF ) ;
+/* Device bytes qs_fit_create would allocate for the orbital basis (spec, spherical) and the fit basis (fitSpec,
+!  fitSpherical) on the given atoms, computed on the host with the same pair screen: admission control for a request loop.
+!  No counterpart in the reference (its fit integrals are computed in host memory). */
+int  qs_fit_estimate_bytes ( int nAtoms, const int *atomicNumbers, const double *xyzBohr, int nSpec, const double *spec,
+                             int spherical, int nFitSpec, const double *fitSpec, int fitSpherical, long long *bytes ) ;
 /* Rows of stored integrals (function pairs of the kept shell pairs), kept shell pairs, device bytes of the integrals. */
 int  qs_fit_stats    ( const qs_fit *fit, long long *rows, long long *pairs, long long *deviceBytes ) ;
 void qs_fit_free     ( qs_fit *fit ) ;

@@ -53,6 +53,8 @@ SCF_SIGNATURES = {
     "qs_fit_b": (ctypes.c_int, [_vp, _dp, _dp]),
     "qs_fit_fock": (ctypes.c_int, [_vp, _dp, _dp]),
     "qs_fit_stats": (ctypes.c_int, [_vp, _llp, _llp, _llp]),
+    "qs_fit_estimate_bytes": (ctypes.c_int, [ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                             ctypes.c_int, _llp]),
     "qs_fit_free": (None, [_vp]),
 }
 

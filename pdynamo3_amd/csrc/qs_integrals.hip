@@ -460,6 +460,35 @@ int qs_fit_fock(qs_fit *fit, const double *c, double *Fock) {
     return kOK;
 }
 
+int qs_fit_estimate_bytes(int nAtoms, const int *Z, const double *xyzBohr, int nSpec, const double *spec, int spherical,
+                          int nFitSpec, const double *fitSpec, int fitSpherical, long long *bytes) {
+    if (bytes == nullptr || nAtoms <= 0 || Z == nullptr || xyzBohr == nullptr || spec == nullptr || nSpec <= 0 ||
+        fitSpec == nullptr || nFitSpec <= 0) {
+        set_error("invalid fit estimate arguments");
+        return kInvalidArgument;
+    }
+    *bytes = 0;
+    QBasis orb, fit;
+    for (QBasis *b : {&orb, &fit}) {
+        b->nAtoms = nAtoms;
+        b->Z.assign(Z, Z + nAtoms);
+        b->xyz.assign(xyzBohr, xyzBohr + 3 * nAtoms);
+    }
+    orb.spherical = spherical ? 1 : 0;
+    fit.spherical = fitSpherical ? 1 : 0;
+    std::string err = build_shells(&orb, nSpec, spec);
+    if (err.empty()) err = build_shells(&fit, nFitSpec, fitSpec);
+    if (!err.empty()) { set_error(err); return kInvalidArgument; }
+    // The arrays qs_fit_create allocates: the integrals (rows x nFit), the row maps, P, F, b, c and the partial sums of b.
+    long long rows = 0;
+    for (const QPair &pq : shell_pairs(&orb, true))
+        rows += static_cast<long long>(orb.shells[pq.x].nf) * orb.shells[pq.y].nf;
+    const long long nFit = fit.nFunctions, n = orb.nFunctions, nPacked = n * (n + 1) / 2;
+    const long long nChunks = (rows + kRowChunk - 1) / kRowChunk;
+    *bytes = 8 * rows * nFit + 9 * rows + 16 * nPacked + 16 * nFit + 8 * (nChunks > 0 ? nChunks : 1) * nFit;
+    return kOK;
+}
+
 int qs_fit_stats(const qs_fit *fit, long long *rows, long long *pairs, long long *deviceBytes) {
     if (fit == nullptr) { set_error("invalid arguments"); return kInvalidArgument; }
     if (rows) *rows = fit->rows;

@@ -77,7 +77,6 @@ struct MixedJob {
     double rhoTol;
     int n, nBlocks, nChunks, nTiles;
     int rowBase, blockBase, chunkBase, groupBase;
-    int superBase;                     // mixed path: the job's first work-group of the density stage (one per superblock)
 };
 
 }  // namespace xq

@@ -78,10 +78,10 @@ constexpr int kChunkBytesB = kDB * kBlockSize * kC;   // one chunk of DB: 5 digi
 constexpr int kChunkGrad = 3 * kBlockSize * kC;     // one chunk of GF, GHU or GLU (elements)
 constexpr int kGradBits = 31;           // GF: |value| < 2^EG is stored as a multiple of 2^(EG - 31)
 constexpr int kGradBitsW = 23;          // GHU, GLU: |value| < 2^EG is stored as a multiple of 2^(EG - 23)
-constexpr int kPanelS = 7;              // union chunks of P_red columns staged at once by the density kernel
-constexpr int kPanelDwordsS = 8 * kPanelS + 4;      // one panel row in LDS (dwords, padded against bank conflicts)
-constexpr int kUIdxCache = 512;         // union function indices of a superblock cached in LDS by the density kernel
-constexpr int kSuper = 4;               // blocks per superblock (density and Fock products)
+constexpr int kPanel = 4;               // chunks of P_red columns staged at once by the density kernel
+constexpr int kPanelDwords = 8 * kPanel + 4;        // one panel row in LDS (padded against bank conflicts)
+constexpr int kIdxCache = 1024;         // kept-function indices of a block cached in LDS by the density kernel
+constexpr int kSuper = 4;               // blocks per superblock of the Fock product
 
 // Stored exponents are int8.  A group whose largest magnitude is below 2^-100 (contributions below 1e-30 of unit-sized
 // terms) gets exponent 0, which rounds all its digits to zero.  Magnitudes here stay far below 2^127.
@@ -285,24 +285,19 @@ __global__ void __launch_bounds__(256) fock_operand_prepare_kernel(
     }
 }
 
-// One union chunk per work-group (256 threads): DB and GF from the FP64 values of the block's kept rows among the
-// chunk's 32 union rows (zero for the rows the block does not keep).
+// One chunk per work-group (256 threads): DB and GF from the FP64 values of the chunk's rows.
 __global__ void __launch_bounds__(256) density_operand_prepare_kernel(
-    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ uChunkBlock, const int *__restrict__ uRow,
+    const xq::DevBlock *__restrict__ blocks, const int *__restrict__ chunkOffset, const int *__restrict__ chunkBlock,
     const double *__restrict__ f, const double *__restrict__ fx, const double *__restrict__ fy,
     const double *__restrict__ fz, const signed char *__restrict__ EB, const signed char *__restrict__ EG,
     signed char *__restrict__ DB, int *__restrict__ GF) {
-    __shared__ int sRow[kC];
     const size_t C = blockIdx.x;
-    const int b = uChunkBlock[C];
+    const int b = chunkBlock[C];
     const xq::DevBlock blk = blocks[b];
-    const int t = threadIdx.x;
+    const int nb = blk.nKept, rc = static_cast<int>(C) - chunkOffset[b], t = threadIdx.x;
     const size_t R = static_cast<size_t>(blk.vOffset);
-    if (t < kC) sRow[t] = uRow[C * kC + t];
-    __syncthreads();
     auto value = [&](const double *a, int row, int p) -> double {
-        const int k = sRow[row];
-        return k >= 0 ? a[(R + k) * kBlockSize + p] : 0.0;
+        return row < nb ? a[(R + row) * kBlockSize + p] : 0.0;
     };
     // DB and GF: thread = (point p, half h) with the 16 rows chunk_fun(h, 0..15).
     {
@@ -312,7 +307,7 @@ __global__ void __launch_bounds__(256) density_operand_prepare_kernel(
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             signed char dd[kDB];
-            slice<kDB>(value(f, chunk_fun(h, v), p), e, dd);
+            slice<kDB>(value(f, kC * rc + chunk_fun(h, v), p), e, dd);
 #pragma unroll
             for (int j = 0; j < kDB; j++) d[j][v] = dd[j];
         }
@@ -326,7 +321,7 @@ __global__ void __launch_bounds__(256) density_operand_prepare_kernel(
             v4i gq[4];
 #pragma unroll
             for (int v = 0; v < 16; v++)
-                gq[v >> 2][v & 3] = grad_encode<kGradBits>(value(grads[c], chunk_fun(h, v), p), eg);
+                gq[v >> 2][v & 3] = grad_encode<kGradBits>(value(grads[c], kC * rc + chunk_fun(h, v), p), eg);
             const size_t o = C * kChunkGrad + (c * kBlockSize + p) * kC + 16 * h;
 #pragma unroll
             for (int w = 0; w < 4; w++) *reinterpret_cast<v4i *>(GF + o + 4 * w) = gq[w];
@@ -365,31 +360,28 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
     if (threadIdx.x == 0) eP[i] = e;
 }
 
-// Density kernel (one superblock per work-group, 256 threads = 4 waves): X = P_red Phi on int8 MFMA over the union rows of
-// the superblock, rho and grad rho, the FP64 functional, E_xc / N_e block sums, and the four W coefficients per point, for
-// each of its blocks.  Wave wv owns points 32 wv .. + 31 of every block; lane (lr, lh) owns point 32 wv + lr and the rows
-// chunk_fun(lh, v) of each union row chunk I.  For each I, the P_red panel (rows 32 I .. + 31, all union columns) is
-// gathered once into LDS with coalesced loads and serves all blocks of the superblock (union rows that a block does not
-// keep have zero Phi, so they add nothing to its X and rho); a superblock wider than kPanelS chunks re-stages its column
-// panels per block.  The Phi operands are fetched one step ahead, and those of chunk I are kept in LDS for the epilogue;
-// per-point partial sums of rho and grad rho accumulate in LDS over the row chunks.
+// Density kernel (per block, 256 threads = 4 waves): X = P_red Phi on int8 MFMA, rho and grad rho, the FP64 functional,
+// E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
+// point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  P_red is staged in LDS by row panels shared by
+// the four waves; the Phi operands are fetched one step ahead, and those of chunk I are kept in LDS for its epilogue.
 // kRhoOut: the density stage of one spin of an unrestricted quadrature (the job's rhoOut receives rho and grad rho).
 template <bool kRhoOut>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
-    __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwordsS];   // [digit][row][panel positions]
+    __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
     __shared__ v4i sDI[kDB][256];   // per thread: its Phi digits of the current row chunk, for the epilogue
-    __shared__ double rsum[kSuper][4][kBlockSize];   // per block and point: rho and the three grad rho / 2 sums
-    __shared__ int sIdx[kUIdxCache];
-    __shared__ int sPe[kC];
-    __shared__ int sCU[kSuper];
+    __shared__ int sPe[2][kC];
+    __shared__ int sIdx[kIdxCache];
+    __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
-    // Work-group g runs on XCD g % 8; each XCD takes a contiguous range of superblocks (one region of the molecule), so
-    // that its L2 holds the P rows those superblocks gather.
+    // Work-group g runs on XCD g % 8; each XCD takes a contiguous range of blocks (one region of the molecule), so that
+    // its L2 holds the P rows those blocks gather.
     const int wg = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
     if (wg >= nGroups) return;
-    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, wg, [](const xq::MixedJob &j) { return j.superBase; })];
+    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, wg, [](const xq::MixedJob &j) { return j.blockBase; })];
     const xq::DevBlock *__restrict__ blocks = J.blocks;
+    const int *__restrict__ chunkOffset = J.chunkOffset;
+    const int *__restrict__ idxAll = J.idx;
     const signed char *__restrict__ DB = J.DB;
     const signed char *__restrict__ EB = J.EB;
     const int *__restrict__ GF = J.GF;
@@ -403,196 +395,175 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     double *__restrict__ wcoef = J.wcoef;
     double *__restrict__ blockEN = J.blockEN;
     int *__restrict__ skip = J.skip;
-    const int4 sb = J.supers[wg - J.superBase];   // first block, blocks, union offset, union size
-    const int b0 = sb.x, nbk = sb.y, nu = sb.w;
-    const int *uIdxG = J.uIdx + sb.z;
-    const int T = (nu + kC - 1) / kC;
-    const int nPanels = (T + kPanelS - 1) / kPanelS;
+    const int b = wg - J.blockBase;
+    const xq::DevBlock blk = blocks[b];
+    const int nb = blk.nKept, np = blk.nPoints;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int lr = lane & 31, lh = lane >> 5;
-    for (int k = t; k < nu && k < kUIdxCache; k += 256) sIdx[k] = uIdxG[k];
-    for (int e = t; e < nbk * 4 * kBlockSize; e += 256) (&rsum[0][0][0])[e] = 0.0;
-    if (t < nbk) sCU[t] = J.uChunkOffset[b0 + t];
+    if (nb <= 0) {
+        if (t == 0) { blockEN[2 * b] = 0.0; blockEN[2 * b + 1] = 0.0; skip[b] = 1; }
+        if (kRhoOut && t < np)
+            for (int c = 0; c < 4; c++) J.rhoOut[4 * (static_cast<size_t>(blk.pStart) + t) + c] = 0.0;
+        return;
+    }
+    // The block's kept-function indices, cached in LDS for the P_red gathers (blocks with more kept functions than the
+    // cache read them from memory).
+    const int *idxG = idxAll + blk.iOffset;
+    for (int k = t; k < nb && k < kIdxCache; k += 256) sIdx[k] = idxG[k];
     __syncthreads();
-    const int *uidx = nu <= kUIdxCache ? sIdx : uIdxG;
+    const int *idx = nb <= kIdxCache ? sIdx : idxG;
+    const size_t C = static_cast<size_t>(chunkOffset[b]);
+    const int nChunks = (nb + kC - 1) / kC;
     const int p = kC * wv + lr;
-    // B operand of union chunk c of block bi of the superblock, for this lane's point and half.
-    auto fetchB = [&](int bi, int c, v4i (&bq)[kDB]) {
-        const signed char *src = DB + (static_cast<size_t>(sCU[bi]) + c) * kChunkBytesB + p * kC + 16 * lh;
+    const signed char *DBb = DB + C * kChunkBytesB;
+    // P_red is staged one row panel at a time (rows 32 I .. + 31, up to kPanel chunks of columns), all of the panel's
+    // words fetched before the barrier; the panel's MFMA steps then run without barriers.  Thread (row sr, slot s8)
+    // fetches columns 4 s8 .. + 3 of every chunk of the panel and writes them as one word per digit at chunk_pos.
+    const int sr = t >> 3, s8 = t & 7, sdw = 4 * (s8 & 1) + (s8 >> 1);
+    auto fetchB = [&](int c, v4i (&bq)[kDB]) {
 #pragma unroll
-        for (int j = 0; j < kDB; j++) bq[j] = *reinterpret_cast<const v4i *>(src + j * (kBlockSize * kC));
+        for (int j = 0; j < kDB; j++)
+            bq[j] = *reinterpret_cast<const v4i *>(&DBb[(c * kDB + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
     };
-    // P_red panel (rows 32 I .. + 31, union chunks kPanelS q .. + ncP - 1): item = (row r, group g of four consecutive
-    // columns); its four 8-byte words become one LDS word per digit at the group's four K positions (chunk_pos).
-    auto stage = [&](int I, int q) {
-        const int ncP = min(kPanelS, T - kPanelS * q), groups = 8 * ncP, items = kC * groups;
-        const int col0 = kC * kPanelS * q;
-        __syncthreads();   // every wave is done with the previous panel
-        for (int it0 = 0; it0 < items; it0 += 2 * 256) {
-            unsigned long long wd[2][4];
+    v4i bnext[kDB];
+    fetchB(0, bnext);
+    const int eb = EB[static_cast<size_t>(b) * kBlockSize + p];
+    double gscale[3];
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const int it = it0 + k * 256 + t;
-                const int r = it / groups, g = it - r * groups;
-                const int row = kC * I + r;
-                const bool ok = it < items && row < nu;
-                const size_t rowOff = ok ? static_cast<size_t>(uidx[row]) * n : 0;
+    for (int c = 0; c < 3; c++) gscale[c] = ldexp(1.0, EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p] - kGradBits);
+    double pr = 0.0, pgx = 0.0, pgy = 0.0, pgz = 0.0;
+    for (int I = 0; I < nChunks; I++) {
+        v16i acc[kXOrders];
+        const int r = kC * I + sr;
+        const size_t rowOff = r < nb ? static_cast<size_t>(idx[r]) * n : 0;
+        for (int c0 = 0; c0 < nChunks; c0 += kPanel) {
+            const int nc = nChunks - c0 < kPanel ? nChunks - c0 : kPanel;
+            unsigned long long wd[kPanel][4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int col = col0 + 4 * g + u;
-                    wd[k][u] = (ok && col < nu) ? Pq[rowOff + uidx[col]] : 0ull;
+            for (int m = 0; m < kPanel; m++)
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const int k = kC * (c0 + m) + 4 * s8 + jj;
+                    wd[m][jj] = (m < nc && r < nb && k < nb) ? Pq[rowOff + idx[k]] : 0ull;
                 }
-            }
+            __syncthreads();   // every wave is done with the previous panel
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const int it = it0 + k * 256 + t;
-                if (it < items) {
-                    const int r = it / groups, g = it - r * groups, s8 = g & 7;
-                    const int dw = 8 * (g >> 3) + 4 * (s8 & 1) + (s8 >> 1);
+            for (int m = 0; m < kPanel; m++) {
+                if (m < nc) {
 #pragma unroll
                     for (int j = 0; j < kDP; j++) {
                         unsigned x = 0;
 #pragma unroll
-                        for (int u = 0; u < 4; u++) x |= (static_cast<unsigned>(wd[k][u] >> (8 * j)) & 255u) << (8 * u);
-                        sP[j][r][dw] = static_cast<int>(x);
+                        for (int jj = 0; jj < 4; jj++) x |= (static_cast<unsigned>(wd[m][jj] >> (8 * j)) & 255u) << (8 * jj);
+                        sP[j][sr][8 * m + sdw] = static_cast<int>(x);
                     }
                 }
             }
-        }
-        if (q == 0 && t < kC) sPe[t] = kC * I + t < nu ? eP[uidx[kC * I + t]] : 0;
-        __syncthreads();
-    };
-    v4i bnext[kDB];
-    if (T > 0) fetchB(0, 0, bnext);
-    for (int I = 0; I < T; I++) {
-        for (int bi = 0; bi < nbk; bi++) {
-            const int b = b0 + bi;
-            v16i acc[kXOrders];
+            if (c0 == 0) {
+                if (t < kC) sPe[I & 1][t] = kC * I + t < nb ? eP[idx[kC * I + t]] : 0;
 #pragma unroll
-            for (int s = 0; s < kXOrders; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int q = 0; q < nPanels; q++) {
-                if (nPanels > 1 || bi == 0) stage(I, q);
-                const int cBase = kPanelS * q, ncP = min(kPanelS, T - cBase);
-                for (int cl = 0; cl < ncP; cl++) {
-                    const int c = cBase + cl;
-                    v4i a[kDP], bq[kDB];
+                for (int s = 0; s < kXOrders; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            }
+            __syncthreads();
+            for (int cl = 0; cl < nc; cl++) {
+                const int c = c0 + cl;
+                v4i a[kDP], bq[kDB];
 #pragma unroll
-                    for (int j = 0; j < kDB; j++) bq[j] = bnext[j];
-                    if (c == I) {   // chunk I's digits, kept for the epilogue (each thread reads back its own slot)
+                for (int j = 0; j < kDB; j++) bq[j] = bnext[j];
+                if (c == I) {   // chunk I's digits, kept for the epilogue (each thread reads back its own slot)
 #pragma unroll
-                        for (int j = 0; j < kDB; j++) sDI[j][t] = bq[j];
-                    }
-                    // The next step's operand: the next chunk of this block, else the next block's (or row chunk's) first.
-                    if (c + 1 < T) fetchB(bi, c + 1, bnext);
-                    else if (bi + 1 < nbk) fetchB(bi + 1, 0, bnext);
-                    else if (I + 1 < T) fetchB(0, 0, bnext);
-#pragma unroll
-                    for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sP[j][lr][8 * cl + 4 * lh]);
-#pragma unroll
-                    for (int i = 0; i < kDP; i++)
-#pragma unroll
-                        for (int j = 0; j < kDB; j++)
-                            if (i + j < kXOrders) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
+                    for (int j = 0; j < kDB; j++) sDI[j][t] = bq[j];
                 }
-            }
-            // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi from
-            // its fixed point (the sums over v are scaled once).  X in FP64 first, so that the accumulators are free
-            // before the epilogue's operands arrive.
-            const int eb = EB[static_cast<size_t>(b) * kBlockSize + p];
-            double x[16];
+                if (c + 1 < nChunks) fetchB(c + 1, bnext);
+                else if (I + 1 < nChunks) fetchB(0, bnext);
 #pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int s[kXOrders] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
-                x[v] = ldexp(static_cast<double>(combine<kXOrders>(s)), sPe[chunk_fun(lh, v)] + eb - 7 * (kXOrders + 1));
-            }
-            v4i dI[kDB];
+                for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sP[j][lr][8 * cl + 4 * lh]);
 #pragma unroll
-            for (int j = 0; j < kDB; j++) dI[j] = sDI[j][t];
-            double sumR = 0.0, sumG[3] = {0.0, 0.0, 0.0};
+                for (int i = 0; i < kDP; i++)
 #pragma unroll
-            for (int v = 0; v < 16; v++) sumR += static_cast<double>(digits5(dI, v)) * x[v];
-            // Rows 4 lh + 8 m .. + 3 of the chunk are in operand word m; words wholly past the union rows are zero and
-            // are not loaded.
-            const int rowsLeft = nu - kC * I;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const size_t o = (static_cast<size_t>(sCU[bi]) + I) * kChunkGrad + (c * kBlockSize + p) * kC + 16 * lh;
-                v4i gq[4];
-#pragma unroll
-                for (int m = 0; m < 4; m++)
-                    gq[m] = 4 * lh + 8 * m < rowsLeft ? *reinterpret_cast<const v4i *>(GF + o + 4 * m) : v4i{0, 0, 0, 0};
-#pragma unroll
-                for (int v = 0; v < 16; v++) sumG[c] += int_of(gq, v) * x[v];
-            }
-            double pr = ldexp(sumR, eb - 7 * kDB), pg[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-                pg[c] = sumG[c] * ldexp(1.0, EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p] - kGradBits);
-            pr += __shfl_xor(pr, 32, 64);
-#pragma unroll
-            for (int c = 0; c < 3; c++) pg[c] += __shfl_xor(pg[c], 32, 64);
-            if (lh == 0) {   // one lane per (block, point): no other lane adds to these slots
-                rsum[bi][0][p] += pr;
-#pragma unroll
-                for (int c = 0; c < 3; c++) rsum[bi][1 + c][p] += pg[c];
+                    for (int j = 0; j < kDB; j++)
+                        if (i + j < kXOrders) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
             }
         }
+        // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi from
+        // its fixed point (the sums over v are scaled once).
+        // X in FP64 first, so that the accumulators are free before the epilogue's operands arrive.
+        double x[16];
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int s[kXOrders] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
+            x[v] = ldexp(static_cast<double>(combine<kXOrders>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 7 * (kXOrders + 1));
+        }
+        v4i dI[kDB];
+#pragma unroll
+        for (int j = 0; j < kDB; j++) dI[j] = sDI[j][t];
+        double sumR = 0.0, sumG[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int v = 0; v < 16; v++) sumR += static_cast<double>(digits5(dI, v)) * x[v];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const size_t o = (C + I) * kChunkGrad + (c * kBlockSize + p) * kC + 16 * lh;
+            v4i gq[4];
+#pragma unroll
+            for (int w = 0; w < 4; w++) gq[w] = *reinterpret_cast<const v4i *>(GF + o + 4 * w);
+#pragma unroll
+            for (int v = 0; v < 16; v++) sumG[c] += int_of(gq, v) * x[v];
+        }
+        const double sumX = sumG[0], sumY = sumG[1], sumZ = sumG[2];
+        pr += ldexp(sumR, eb - 7 * kDB);
+        pgx += sumX * gscale[0];
+        pgy += sumY * gscale[1];
+        pgz += sumZ * gscale[2];
     }
+    pr += __shfl_xor(pr, 32, 64);
+    pgx += __shfl_xor(pgx, 32, 64);
+    pgy += __shfl_xor(pgy, 32, 64);
+    pgz += __shfl_xor(pgz, 32, 64);
+    if (lh == 0) { rsum[0][p] = pr; rsum[1][p] = pgx; rsum[2][p] = pgy; rsum[3][p] = pgz; }
     __syncthreads();
-    // Per block: functional, E and N, W coefficients.
-    for (int bi = 0; bi < nbk; bi++) {
-        const int b = b0 + bi;
-        const xq::DevBlock blk = blocks[b];
-        const int np = blk.nPoints;
-        double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    // Functional, E and N, W coefficients.
+    double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    if (t < np) { rho = rsum[0][t]; gx = 2.0 * rsum[1][t]; gy = 2.0 * rsum[2][t]; gz = 2.0 * rsum[3][t]; }
+    if (kRhoOut) {   // one spin of an unrestricted quadrature: the functional follows in its own kernel
         if (t < np) {
-            rho = rsum[bi][0][t];
-            gx = 2.0 * rsum[bi][1][t];
-            gy = 2.0 * rsum[bi][2][t];
-            gz = 2.0 * rsum[bi][3][t];
+            double *o = J.rhoOut + 4 * (static_cast<size_t>(blk.pStart) + t);
+            o[0] = rho; o[1] = gx; o[2] = gy; o[3] = gz;
         }
-        if (kRhoOut) {   // one spin of an unrestricted quadrature: the functional follows in its own kernel
-            if (t < np) {
-                double *o = J.rhoOut + 4 * (static_cast<size_t>(blk.pStart) + t);
-                o[0] = rho; o[1] = gx; o[2] = gy; o[3] = gz;
-            }
-            continue;
-        }
-        double mx = t < np ? fabs(rho) : 0.0;
+        return;
+    }
+    double mx = t < np ? fabs(rho) : 0.0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-        if (lane == 0) wsum[wv][0] = mx;
-        __syncthreads();
-        const bool skipped = !(fmax(fmax(wsum[0][0], wsum[1][0]), fmax(wsum[2][0], wsum[3][0])) > rhoTol);
-        __syncthreads();
-        double e = 0.0, ne = 0.0;
-        if (t < np) {
-            double vr = 0.0, vs = 0.0;
-            if (!skipped) {
-                double eps;
-                evaluate_functional(fs, rho, gx * gx + gy * gy + gz * gz, eps, vr, vs);
-                const double ww = w[blk.pStart + t];
-                vr *= ww;
-                vs *= ww;
-                e = eps * rho * ww;
-                ne = rho * ww;
-            }
-            double *cw = wcoef + 4 * (static_cast<size_t>(blk.pStart) + t);
-            cw[0] = 0.5 * vr;
-            cw[1] = 2.0 * vs * gx;
-            cw[2] = 2.0 * vs * gy;
-            cw[3] = 2.0 * vs * gz;
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) wsum[wv][0] = mx;
+    __syncthreads();
+    const bool skipped = !(fmax(fmax(wsum[0][0], wsum[1][0]), fmax(wsum[2][0], wsum[3][0])) > rhoTol);
+    __syncthreads();
+    double e = 0.0, ne = 0.0;
+    if (t < np) {
+        double vr = 0.0, vs = 0.0;
+        if (!skipped) {
+            double eps;
+            evaluate_functional(fs, rho, gx * gx + gy * gy + gz * gz, eps, vr, vs);
+            const double ww = w[blk.pStart + t];
+            vr *= ww;
+            vs *= ww;
+            e = eps * rho * ww;
+            ne = rho * ww;
         }
+        double *cw = wcoef + 4 * (static_cast<size_t>(blk.pStart) + t);
+        cw[0] = 0.5 * vr;
+        cw[1] = 2.0 * vs * gx;
+        cw[2] = 2.0 * vs * gy;
+        cw[3] = 2.0 * vs * gz;
+    }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) { e += __shfl_xor(e, o, 64); ne += __shfl_xor(ne, o, 64); }
-        if (lane == 0) { wsum[wv][0] = e; wsum[wv][1] = ne; }
-        __syncthreads();
-        if (t == 0) {
-            blockEN[2 * b] = ((wsum[0][0] + wsum[1][0]) + wsum[2][0]) + wsum[3][0];
-            blockEN[2 * b + 1] = ((wsum[0][1] + wsum[1][1]) + wsum[2][1]) + wsum[3][1];
-            skip[b] = skipped ? 1 : 0;
-        }
-        __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) { e += __shfl_xor(e, o, 64); ne += __shfl_xor(ne, o, 64); }
+    if (lane == 0) { wsum[wv][0] = e; wsum[wv][1] = ne; }
+    __syncthreads();
+    if (t == 0) {
+        blockEN[2 * b] = ((wsum[0][0] + wsum[1][0]) + wsum[2][0]) + wsum[3][0];
+        blockEN[2 * b + 1] = ((wsum[0][1] + wsum[1][1]) + wsum[2][1]) + wsum[3][1];
+        skip[b] = skipped ? 1 : 0;
     }
 }
 
@@ -886,9 +857,9 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     const size_t nc = static_cast<size_t>(chunks > 0 ? chunks : 1), ncU = static_cast<size_t>(chunksU > 0 ? chunksU : 1);
     XQ_TRY(dev_alloc(g, &g->d_DA, ncU * kChunkBytes));
     XQ_TRY(dev_alloc(g, &g->d_EA, ncU * kC));
-    XQ_TRY(dev_alloc(g, &g->d_DB, ncU * kChunkBytesB));
+    XQ_TRY(dev_alloc(g, &g->d_DB, nc * kChunkBytesB));
     XQ_TRY(dev_alloc(g, &g->d_EB, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * kBlockSize));
-    XQ_TRY(dev_alloc(g, &g->d_GF, ncU * kChunkGrad));
+    XQ_TRY(dev_alloc(g, &g->d_GF, nc * kChunkGrad));
     XQ_TRY(dev_alloc(g, &g->d_EG, static_cast<size_t>(g->nBlocks > 0 ? g->nBlocks : 1) * 3 * kBlockSize));
     XQ_TRY(dev_alloc(g, &g->d_DW, ncU * kChunkBytes));
     XQ_TRY(dev_alloc(g, &g->d_GHU, ncU * kChunkGrad));
@@ -901,9 +872,10 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     if (g->nBlocks > 0)
         hipLaunchKernelGGL(point_exponent_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, f, fx, fy, fz,
                            g->d_EB, g->d_EG);
-    if (chunksU > 0)
-        hipLaunchKernelGGL(density_operand_prepare_kernel, dim3(static_cast<unsigned>(chunksU)), dim3(256), 0, s,
-                           g->d_blocks, g->d_uChunkBlock, g->d_uRow, f, fx, fy, fz, g->d_EB, g->d_EG, g->d_DB, g->d_GF);
+    if (chunks > 0)
+        hipLaunchKernelGGL(density_operand_prepare_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s,
+                           g->d_blocks, g->d_chunkOffset, g->d_chunkBlock, f, fx, fy, fz, g->d_EB, g->d_EG, g->d_DB,
+                           g->d_GF);
     if (chunksU > 0)
         hipLaunchKernelGGL(fock_operand_prepare_kernel, dim3(static_cast<unsigned>(chunksU)), dim3(256), 0, s,
                            g->d_blocks, g->d_uChunkBlock, g->d_uRow, f, fx, fy, fz, g->d_EG, g->d_DA, g->d_EA,
@@ -998,7 +970,6 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
         J.blockBase = static_cast<int>(blocks);
         J.chunkBase = static_cast<int>(chunks);
         J.groupBase = static_cast<int>(groups);
-        J.superBase = 0;   // set by the mixed-precision launch
         rows += J.n;
         blocks += J.nBlocks;
         chunks += J.nChunks;
@@ -1014,14 +985,9 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
 // least nJobs entries; the caller synchronises the stream before h_table is written again.
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
                  MixedJob *h_table, MixedJob *d_table, int stages) {
-    long long rows, blocks, chunks, groups, supers = 0;
+    long long rows, blocks, chunks, groups;
     fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups);
-    for (int j = 0; j < nJobs; j++) {
-        h_table[j].superBase = static_cast<int>(supers);
-        supers += grids[j]->nSupers;
-    }
-    if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16 ||
-        supers > INT32_MAX / 16) {
+    if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
         set_error("batch too large for one launch");
         return kInvalidArgument;
     }
@@ -1029,14 +995,14 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
     if ((stages & kStageDensity) && rows > 0)
         hipLaunchKernelGGL(p_slice_kernel, dim3(static_cast<unsigned>(rows)), dim3(256), 0, s, d_table, nJobs);
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
-    if ((stages & kStageDensity) && supers > 0) {
-        const int perXcd = static_cast<int>((supers + 7) / 8);
+    if ((stages & kStageDensity) && blocks > 0) {
+        const int perXcd = static_cast<int>((blocks + 7) / 8);
         if (h_table[0].rhoOut != nullptr) {   // the density stage of one spin (single-job calls only)
             hipLaunchKernelGGL(density_kernel_mixed<true>, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table,
-                               nJobs, perXcd, static_cast<int>(supers));
+                               nJobs, perXcd, static_cast<int>(blocks));
         } else {
             hipLaunchKernelGGL(density_kernel_mixed<false>, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s,
-                               d_table, nJobs, perXcd, static_cast<int>(supers));
+                               d_table, nJobs, perXcd, static_cast<int>(blocks));
         }
     }
     XQ_CHECK(hipEventRecord(ev.evRho1, s));

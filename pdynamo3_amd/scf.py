@@ -39,21 +39,37 @@ def _check(status, what):
         raise QCModelError("Error in SCF integrals ({:s}: status {:d}, {:s}).".format(what, status, _lib.last_error()))
 
 
+_TRIL = {}
+
+
+def _tril(n):
+    """Row and column indices of the packed lower triangle of an n x n matrix and the positions of its diagonal in the
+    packed vector (computed once per n: SCF steps and serving jobs reuse them)."""
+    t = _TRIL.get(n)
+    if t is None:
+        i, j = np.tril_indices(n)
+        t = (i, j, np.arange(n) * (np.arange(n) + 3) // 2)
+        _TRIL[n] = t
+    return t
+
+
 def unpack(v, n):
     """Dense symmetric matrix from a packed lower triangle."""
-    M = np.zeros((n, n))
-    M[np.tril_indices(n)] = v
-    return M + np.tril(M, -1).T
+    i, j, _ = _tril(n)
+    M = np.empty((n, n))
+    M[i, j] = v
+    M[j, i] = v
+    return M
 
 
 def pack(M):
-    return np.ascontiguousarray(M[np.tril_indices(M.shape[0])])
+    i, j, _ = _tril(M.shape[0])
+    return np.ascontiguousarray(M[i, j])
 
 
 def trace_product(a, b, n):
     """Tr(A B) of two symmetric matrices given packed (off-diagonal terms twice)."""
-    d = np.zeros(a.size, dtype=bool)
-    d[np.arange(n) * (np.arange(n) + 3) // 2] = True
+    d = _tril(n)[2]
     return float(2.0 * np.dot(a, b) - np.dot(a[d], b[d]))
 
 
@@ -232,7 +248,7 @@ class RKSDFT:
         """The density of the n_occupied lowest orbitals of F (in the orthonormal basis X)."""
         t = time.perf_counter()
         Fo = self.X.T @ unpack(F, self.n) @ self.X
-        e, C = np.linalg.eigh(Fo)
+        e, C = scipy.linalg.eigh(Fo, driver="evd", overwrite_a=True, check_finite=False)
         Cocc = self.X @ C[:, :self.n_occupied]
         self.orbital_energies = e
         P = pack(2.0 * Cocc @ Cocc.T)

@@ -49,6 +49,7 @@ class Request:
     spec: np.ndarray
     n_functions: int
     seed: int
+    peak_bytes: dict = field(default_factory=dict)   # admission estimates by (precision, scf), computed once
 
 
 class SyntheticSCF:
@@ -124,30 +125,39 @@ def request_stream(n_requests=256, seed=3, min_waters=4, max_waters=64, rate=50.
     return out
 
 
-_FIT_FUNCTIONS = {}
+_FIT_SPEC = {}
 
 
-def fit_functions(atomic_numbers):
-    """Functions of the J fitting basis on the given atoms (its parameter files are spherical)."""
-    if not _FIT_FUNCTIONS:
-        b = _basis.load_basis(FIT_BASIS)
-        for z in b["elements"]:
-            _FIT_FUNCTIONS[int(z)] = _basis.number_of_functions(b, [int(z)], True)
-    return int(sum(_FIT_FUNCTIONS[int(z)] for z in atomic_numbers))
+def fit_integral_bytes(Z, xyz_bohr, spec, spherical):
+    """Device bytes of a job's electron-fit integrals and their work arrays (qs_fit_estimate_bytes: the integral set-up's
+    own pair screen, evaluated on the host)."""
+    import ctypes
+    from . import _lib
+    key = tuple(sorted(set(int(z) for z in Z)))
+    if key not in _FIT_SPEC:
+        _FIT_SPEC[key] = _lib.f64(_basis.element_spec(_basis.load_basis(FIT_BASIS), Z))
+    fspec = _FIT_SPEC[key]
+    z, x, o = _lib.i32(Z), _lib.f64(xyz_bohr), _lib.f64(spec)
+    out = ctypes.c_longlong()
+    status = _lib.lib.qs_fit_estimate_bytes(z.size, _lib.iptr(z), _lib.dptr(x), o.size, _lib.dptr(o), int(bool(spherical)),
+                                            fspec.size, _lib.dptr(fspec), 1, ctypes.byref(out))
+    if status != 0:
+        raise integrator.QCModelError("Error in the fit estimate (status {:d}, {:s}).".format(status, _lib.last_error()))
+    return out.value
 
 
-def estimated_peak_bytes(request, precision, scf="rks"):
+def estimated_peak_bytes(request, precision, scf="rks", spherical=False):
     """Peak device bytes of a request, set-up included: the quadrature's (the grid size before the weight cull and at
-    most 256 kept functions per block, an upper bound for water clusters in 6-31G*) and, for an SCF, the electron-fit
-    integrals of every orbital function pair (8 bytes per pair and fit function, before screening) and their work
-    arrays."""
-    points = integrator.estimated_points(2, request.Z)
-    q = integrator.estimate_bytes(points, request.n_functions, float(min(request.n_functions, 256)), precision)
-    if scf != "rks":
-        return q
-    n, nf = request.n_functions, fit_functions(request.Z)
-    pairs = n * (n + 1) // 2 + n * 10
-    return q + 8 * pairs * (nf + 4) + 8 * 64 * nf
+    most 256 kept functions per block, an upper bound for water clusters in 6-31G*) and, for an SCF, its electron-fit
+    integrals and their work arrays.  Computed once per request."""
+    key = (precision, scf, bool(spherical))
+    if key not in request.peak_bytes:
+        points = integrator.estimated_points(2, request.Z)
+        b = integrator.estimate_bytes(points, request.n_functions, float(min(request.n_functions, 256)), precision)
+        if scf == "rks":
+            b += fit_integral_bytes(request.Z, request.xyz_bohr, request.spec, spherical)
+        request.peak_bytes[key] = b
+    return request.peak_bytes[key]
 
 
 class Server:
@@ -243,7 +253,7 @@ class Server:
                 queue.append(pending.pop(0))
             # Admission: free slot and room under the HBM budget, in arrival order.
             while queue and len(setting_up) + len(ready) + len(stepping) < self.max_jobs:
-                need = estimated_peak_bytes(queue[0], self.precision, self.scf_kind)
+                need = estimated_peak_bytes(queue[0], self.precision, self.scf_kind, self.spherical)
                 if reserved + need > self.hbm_budget:
                     if setting_up or ready or stepping:
                         deferred_for_memory += 1
