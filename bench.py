@@ -283,10 +283,22 @@ def scf_leg(name, compare, log_fn):
     return out
 
 
+def progress(tag, log_fn, every=32):
+    """A log function for the request loop that reports every `every`-th finished request (and the 8th, for short runs)."""
+    count = [0]
+
+    def log(message):
+        if "latency" in message:
+            count[0] += 1
+            if count[0] % every == 0 or count[0] == 8:
+                log_fn("serving {:s}: {:d} requests done".format(tag, count[0]))
+    return log
+
+
 def serving_leg(args, log_fn):
     """The serving stream in both precisions: SCF requests on (H2O)N, N log-uniform in [4, 64] (seed 3), BLYP/PBE by coin
     flip, Poisson arrivals, each a restricted Kohn-Sham SCF to convergence.  The mixed loop's capacity is measured
-    first, on a calibration stream of 32 other requests (seed 7) that all arrive at once, after a warm-up pass of 16
+    first, on a calibration stream of 24 other requests (seed 7) that all arrive at once, after a warm-up pass of 8
     more (seed 8) that fills the device memory cache and loads the kernels; the arrival rate is serve-load x that
     capacity.  The FP64 loop then serves the same stream (same
     arrival times) after its own warm-up and calibration, so the two legs differ only in the precision."""
@@ -296,13 +308,14 @@ def serving_leg(args, log_fn):
     out, rate = {}, None
     for precision in ("mixed", "fp64"):
         srv = serving.Server(precision=precision, max_jobs=args.serve_slots, scf="rks")
-        srv.run(serving.request_stream(n_requests=16, seed=8, rate=1.0e6))
-        done, cl = srv.run(serving.request_stream(n_requests=32, seed=7, rate=1.0e6))
+        srv.run(serving.request_stream(n_requests=8, seed=8, rate=1.0e6), log=progress(precision + " warm-up", log_fn))
+        done, cl = srv.run(serving.request_stream(n_requests=24, seed=7, rate=1.0e6),
+                           log=progress(precision + " calibration", log_fn))
         capacity = len(done) / cl["wall_s"]
         if rate is None:
             rate = args.serve_load * capacity
         reqs = [dataclasses.replace(r, arrival=r.arrival / rate) for r in base]
-        finished, loop = srv.run(reqs)
+        finished, loop = srv.run(reqs, log=progress(precision + " stream", log_fn))
         srv.close()
         leg = serving.summarize(finished, loop)
         leg.update({"arrival_rate_per_s": rate, "capacity_requests_per_s": capacity, "load": rate / capacity,

@@ -248,7 +248,7 @@ class RKSDFT:
         """The density of the n_occupied lowest orbitals of F (in the orthonormal basis X)."""
         t = time.perf_counter()
         Fo = self.X.T @ unpack(F, self.n) @ self.X
-        e, C = scipy.linalg.eigh(Fo, driver="evd", overwrite_a=True, check_finite=False)
+        e, C = np.linalg.eigh(Fo)
         Cocc = self.X @ C[:, :self.n_occupied]
         self.orbital_energies = e
         P = pack(2.0 * Cocc @ Cocc.T)
