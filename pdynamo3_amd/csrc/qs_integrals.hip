@@ -210,8 +210,12 @@ void qs_release(T **p) {
 // integrals), and the runtime gives every hardware queue that runs one its own scratch area sized for the whole
 // device.  Two such kernels in flight at once on two queues (set-up threads of a request loop) made the second queue's
 // scratch allocation fail (HSA_STATUS_ERROR_OUT_OF_RESOURCES, queue abort), so at most one of them runs at a time in
-// the process: each holds this lock from its launch until its stream has drained.
+// the process: each holds this lock from its launch until its stream has drained.  The same allocation fails when the
+// device is nearly full (a request loop's caching allocator holds idle blocks), so each launch first makes sure that
+// kScratchHeadroom bytes are free: the fit-integral kernel's 13,760 bytes per lane at 3 waves per SIMD on 256 CUs take
+// about 2.7 GB.
 std::mutex g_integralKernels;
+constexpr size_t kScratchHeadroom = static_cast<size_t>(6) << 30;
 
 }  // namespace
 }  // namespace qs
@@ -293,10 +297,13 @@ int qs_one_electron(qs_basis *basis, double *S, double *T, double *V) {
                                                      hipMemcpyHostToDevice, s), "copy atoms");
     if (status == kOK && !pairs.empty()) {
         std::lock_guard<std::mutex> lock(g_integralKernels);
-        hipLaunchKernelGGL(one_electron_kernel, dim3((static_cast<unsigned>(pairs.size()) + kThreads - 1) / kThreads),
-                           dim3(kThreads), 0, s, static_cast<int>(pairs.size()), d_pairs, basis->d_shells, basis->d_exps,
-                           basis->d_coefs, basis->nAtoms, d_atoms, basis->spherical, d_S, d_T, d_V);
-        status = check(hipGetLastError(), "one_electron_kernel");
+        status = xq::ensure_free_device_memory(kScratchHeadroom, "one-electron integrals");
+        if (status == kOK) {
+            hipLaunchKernelGGL(one_electron_kernel, dim3((static_cast<unsigned>(pairs.size()) + kThreads - 1) / kThreads),
+                               dim3(kThreads), 0, s, static_cast<int>(pairs.size()), d_pairs, basis->d_shells,
+                               basis->d_exps, basis->d_coefs, basis->nAtoms, d_atoms, basis->spherical, d_S, d_T, d_V);
+            status = check(hipGetLastError(), "one_electron_kernel");
+        }
         const int drained = check(hipStreamSynchronize(s), "one-electron integrals");
         if (status == kOK) status = drained;
     }
@@ -376,6 +383,7 @@ int qs_fit_create(const qs_basis *orbital, const qs_basis *fitBasis, qs_fit **ou
                                                     hipMemcpyHostToDevice), "copy pairs");
         const long long nWork = F->nPairs * static_cast<long long>(fitBasis->shells.size());
         std::lock_guard<std::mutex> lock(g_integralKernels);
+        if (status == kOK && nWork > 0) status = xq::ensure_free_device_memory(kScratchHeadroom, "fit integrals");
         if (status == kOK && nWork > 0) {
             hipLaunchKernelGGL(fit_integrals_kernel, dim3(static_cast<unsigned>((nWork + kThreads - 1) / kThreads)),
                                dim3(kThreads), 0, F->stream, nWork, static_cast<int>(fitBasis->shells.size()), d_pairs,
@@ -406,6 +414,7 @@ int qs_fit_matrix(qs_fit *fit, double *M) {
                                                 hipMemcpyHostToDevice), "copy fit pairs");
     {
         std::lock_guard<std::mutex> lock(g_integralKernels);
+        if (status == kOK && !fit->fitPairs.empty()) status = xq::ensure_free_device_memory(kScratchHeadroom, "fit matrix");
         if (status == kOK && !fit->fitPairs.empty()) {
             hipLaunchKernelGGL(fit_matrix_kernel,
                                dim3((static_cast<unsigned>(fit->fitPairs.size()) + kThreads - 1) / kThreads),

@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <unordered_map>
+#include <string>
 #include <utility>
 
 #include "xq_internal.h"
@@ -131,6 +132,24 @@ void cache_fail_after(long long count) {
 void cache_release() {
     std::lock_guard<std::mutex> lock(g_mutex);
     release_cached_locked();
+}
+
+int ensure_free_device_memory(size_t bytes, const char *what) {
+    size_t freeB = 0, totalB = 0;
+    if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) {
+        set_error(std::string(what) + ": hipMemGetInfo failed");
+        return kAlgorithmError;
+    }
+    if (freeB >= bytes) return kOK;
+    cache_release();   // idle blocks of the cache go back to the device
+    if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) {
+        set_error(std::string(what) + ": hipMemGetInfo failed");
+        return kAlgorithmError;
+    }
+    if (freeB >= bytes) return kOK;
+    set_error(std::string(what) + ": " + std::to_string(freeB >> 20) + " MiB of device memory free, " +
+              std::to_string(bytes >> 20) + " MiB needed for the kernel's private memory");
+    return kOutOfMemory;
 }
 
 }  // namespace xq

@@ -169,6 +169,9 @@ void cache_host_free(void *ptr);
 size_t cache_idle_bytes();
 void cache_release();
 void cache_fail_after(long long count);   // test hook: after `count` more device allocations, one fails (-1 disarms)
+// Before a kernel whose private memory the runtime allocates at dispatch (the SCF integral kernels): at least `bytes` of
+// free device memory, releasing the cache's idle blocks if needed; kOutOfMemory (with `what` in the message) if not.
+int ensure_free_device_memory(size_t bytes, const char *what);
 
 // Allocation helpers that account device bytes on the handle.  Every stage synchronises the handle's stream before it
 // frees what it allocated.

@@ -165,7 +165,7 @@ class Server:
 
     precision      'mixed' or 'fp64' quadratures
     max_jobs       jobs set up or running at once (the largest batch)
-    hbm_budget     device bytes the loop may reserve (default 90% of the free memory when it starts)
+    hbm_budget     device bytes the loop may reserve (default 90% of the free memory when it starts, less 16 GiB)
     scf            'rks' (each request runs scf.RKSDFT to convergence) or 'synthetic' (SyntheticSCF)
     host_workers   threads for the jobs' SCF steps (default min(16, cores)); BLAS runs one thread per step meanwhile
     setup_workers  threads for the jobs' set-up
@@ -174,7 +174,7 @@ class Server:
     """
 
     def __init__(self, precision="mixed", max_jobs=16, hbm_budget=None, spherical=False, keep_fock=False, scf="rks",
-                 host_workers=None, setup_workers=1, energy_tolerance=1.0e-6, density_tolerance=1.0e-6):
+                 host_workers=None, setup_workers=4, energy_tolerance=1.0e-6, density_tolerance=1.0e-6):
         if scf not in ("rks", "synthetic"):
             raise ValueError("scf must be 'rks' or 'synthetic'")
         self.precision = precision
@@ -185,8 +185,10 @@ class Server:
         self.energy_tolerance = energy_tolerance
         self.density_tolerance = density_tolerance
         if hbm_budget is None:
+            # 90% of the free memory, less room for the runtime's own allocations: the private memory of the SCF
+            # integral kernels (a few GB per launch) is allocated by the runtime at dispatch.
             free, _ = integrator.device_memory()
-            hbm_budget = int(0.9 * free)
+            hbm_budget = max(int(0.9 * free) - (16 << 30), int(0.5 * free))
         self.hbm_budget = int(hbm_budget)
         self.batch = integrator.QuadratureBatch(max_jobs)
         self.setup_pool = concurrent.futures.ThreadPoolExecutor(max_workers=setup_workers)
