@@ -101,11 +101,11 @@ __global__ void __launch_bounds__(kThreads) fit_matrix_kernel(int nPairs, const 
 // Electron-fit integrals: one thread per (significant orbital shell pair, fit shell).
 // ------------------------------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) fit_integrals_kernel(
-    long long nWork, int nFitShells, const PairEntry *__restrict__ pairs, const QShell *__restrict__ oShells,
+    long long k0, long long nWork, int nFitShells, const PairEntry *__restrict__ pairs, const QShell *__restrict__ oShells,
     const double *__restrict__ oExps, const double *__restrict__ oCoefs, int oSpherical,
     const QShell *__restrict__ fShells, const double *__restrict__ fExps, const double *__restrict__ fCoefs, int fSpherical,
     int nFit, double *__restrict__ B) {
-    const long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    const long long k = k0 + blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
     if (k < nWork)
         fit_integrals_item(k, nFitShells, pairs, oShells, oExps, oCoefs, oSpherical, fShells, fExps, fCoefs, fSpherical, nFit,
                            g_boys, g_c2s, B);
@@ -216,6 +216,25 @@ void qs_release(T **p) {
 // about 2.7 GB.
 std::mutex g_integralKernels;
 constexpr size_t kScratchHeadroom = static_cast<size_t>(6) << 30;
+// The runtime's scratch areas belong to hardware queues and come from a pool of fixed size, separate from the free
+// device memory: several queues that each ran one of these kernels held areas sized for a whole-device dispatch and a
+// further request failed with HSA_STATUS_ERROR_OUT_OF_RESOURCES with 217 GB free.  So every launch of them goes to one
+// stream per device (one queue, one area), and no launch is larger than kSliceThreads threads, which bounds the area a
+// dispatch asks for (about 1.8 GB at 13,760 bytes per lane).
+constexpr int kMaxDevices = 64;
+hipStream_t g_integralStream[kMaxDevices] = {};
+constexpr long long kSliceThreads = 2048LL * 64;
+
+// The integral stream of the current device (created on first use); the caller holds g_integralKernels.
+int integral_stream(hipStream_t *out) {
+    int dev = 0;
+    XQ_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) { set_error("device index out of range"); return kInvalidArgument; }
+    if (g_integralStream[dev] == nullptr)
+        XQ_CHECK(hipStreamCreateWithFlags(&g_integralStream[dev], hipStreamNonBlocking));
+    *out = g_integralStream[dev];
+    return kOK;
+}
 
 }  // namespace
 }  // namespace qs
@@ -290,27 +309,30 @@ int qs_one_electron(qs_basis *basis, double *S, double *T, double *V) {
     if (status == kOK) status = qs_alloc(&d_S, size);
     if (status == kOK) status = qs_alloc(&d_T, size);
     if (status == kOK) status = qs_alloc(&d_V, size);
-    const hipStream_t s = basis->stream;
-    if (status == kOK) status = check(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(QPair),
-                                                     hipMemcpyHostToDevice, s), "copy pairs");
-    if (status == kOK) status = check(hipMemcpyAsync(d_atoms, atoms.data(), atoms.size() * sizeof(double),
-                                                     hipMemcpyHostToDevice, s), "copy atoms");
-    if (status == kOK && !pairs.empty()) {
+    {
         std::lock_guard<std::mutex> lock(g_integralKernels);
-        status = xq::ensure_free_device_memory(kScratchHeadroom, "one-electron integrals");
-        if (status == kOK) {
-            hipLaunchKernelGGL(one_electron_kernel, dim3((static_cast<unsigned>(pairs.size()) + kThreads - 1) / kThreads),
-                               dim3(kThreads), 0, s, static_cast<int>(pairs.size()), d_pairs, basis->d_shells,
-                               basis->d_exps, basis->d_coefs, basis->nAtoms, d_atoms, basis->spherical, d_S, d_T, d_V);
+        hipStream_t s = nullptr;
+        if (status == kOK) status = integral_stream(&s);
+        if (status == kOK) status = check(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(QPair),
+                                                         hipMemcpyHostToDevice, s), "copy pairs");
+        if (status == kOK) status = check(hipMemcpyAsync(d_atoms, atoms.data(), atoms.size() * sizeof(double),
+                                                         hipMemcpyHostToDevice, s), "copy atoms");
+        if (status == kOK && !pairs.empty()) status = xq::ensure_free_device_memory(kScratchHeadroom, "one-electron integrals");
+        for (long long k0 = 0; status == kOK && k0 < static_cast<long long>(pairs.size()); k0 += kSliceThreads) {
+            const long long m = std::min<long long>(kSliceThreads, static_cast<long long>(pairs.size()) - k0);
+            hipLaunchKernelGGL(one_electron_kernel, dim3(static_cast<unsigned>((m + kThreads - 1) / kThreads)),
+                               dim3(kThreads), 0, s, static_cast<int>(m), d_pairs + k0, basis->d_shells, basis->d_exps,
+                               basis->d_coefs, basis->nAtoms, d_atoms, basis->spherical, d_S, d_T, d_V);
             status = check(hipGetLastError(), "one_electron_kernel");
         }
-        const int drained = check(hipStreamSynchronize(s), "one-electron integrals");
-        if (status == kOK) status = drained;
+        if (status == kOK) status = check(hipMemcpyAsync(S, d_S, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy S");
+        if (status == kOK) status = check(hipMemcpyAsync(T, d_T, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy T");
+        if (status == kOK) status = check(hipMemcpyAsync(V, d_V, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy V");
+        if (s != nullptr) {
+            const int drained = check(hipStreamSynchronize(s), "one-electron integrals");
+            if (status == kOK) status = drained;
+        }
     }
-    if (status == kOK) status = check(hipMemcpyAsync(S, d_S, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy S");
-    if (status == kOK) status = check(hipMemcpyAsync(T, d_T, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy T");
-    if (status == kOK) status = check(hipMemcpyAsync(V, d_V, size * sizeof(double), hipMemcpyDeviceToHost, s), "copy V");
-    (void)hipStreamSynchronize(s);
     qs_release(&d_pairs);
     qs_release(&d_atoms);
     qs_release(&d_S);
@@ -383,17 +405,22 @@ int qs_fit_create(const qs_basis *orbital, const qs_basis *fitBasis, qs_fit **ou
                                                     hipMemcpyHostToDevice), "copy pairs");
         const long long nWork = F->nPairs * static_cast<long long>(fitBasis->shells.size());
         std::lock_guard<std::mutex> lock(g_integralKernels);
+        hipStream_t is = nullptr;
+        if (status == kOK) status = integral_stream(&is);
         if (status == kOK && nWork > 0) status = xq::ensure_free_device_memory(kScratchHeadroom, "fit integrals");
-        if (status == kOK && nWork > 0) {
-            hipLaunchKernelGGL(fit_integrals_kernel, dim3(static_cast<unsigned>((nWork + kThreads - 1) / kThreads)),
-                               dim3(kThreads), 0, F->stream, nWork, static_cast<int>(fitBasis->shells.size()), d_pairs,
+        for (long long k0 = 0; status == kOK && k0 < nWork; k0 += kSliceThreads) {
+            const long long m = std::min<long long>(kSliceThreads, nWork - k0);
+            hipLaunchKernelGGL(fit_integrals_kernel, dim3(static_cast<unsigned>((m + kThreads - 1) / kThreads)),
+                               dim3(kThreads), 0, is, k0, nWork, static_cast<int>(fitBasis->shells.size()), d_pairs,
                                orbital->d_shells, orbital->d_exps, orbital->d_coefs, orbital->spherical,
                                fitBasis->d_shells, fitBasis->d_exps, fitBasis->d_coefs, fitBasis->spherical, F->nFit,
                                F->d_B);
             status = check(hipGetLastError(), "fit_integrals_kernel");
         }
-        const int drained = check(hipStreamSynchronize(F->stream), "fit integrals");
-        if (status == kOK) status = drained;
+        if (is != nullptr) {
+            const int drained = check(hipStreamSynchronize(is), "fit integrals");
+            if (status == kOK) status = drained;
+        }
     }
     qs_release(&d_pairs);
     if (status == kOK) F->fitPairs = shell_pairs(fitBasis, false);
@@ -409,22 +436,26 @@ int qs_fit_matrix(qs_fit *fit, double *M) {
     QPair *d_pairs = nullptr;
     int status = qs_alloc(&d_M, size);
     if (status == kOK) status = qs_alloc(&d_pairs, fit->fitPairs.size());
-    if (status == kOK) status = check(hipMemsetAsync(d_M, 0, size * sizeof(double), fit->stream), "clear M");
     if (status == kOK) status = check(hipMemcpy(d_pairs, fit->fitPairs.data(), fit->fitPairs.size() * sizeof(QPair),
                                                 hipMemcpyHostToDevice), "copy fit pairs");
     {
         std::lock_guard<std::mutex> lock(g_integralKernels);
+        hipStream_t is = nullptr;
+        if (status == kOK) status = integral_stream(&is);
+        if (status == kOK) status = check(hipMemsetAsync(d_M, 0, size * sizeof(double), is), "clear M");
         if (status == kOK && !fit->fitPairs.empty()) status = xq::ensure_free_device_memory(kScratchHeadroom, "fit matrix");
-        if (status == kOK && !fit->fitPairs.empty()) {
-            hipLaunchKernelGGL(fit_matrix_kernel,
-                               dim3((static_cast<unsigned>(fit->fitPairs.size()) + kThreads - 1) / kThreads),
-                               dim3(kThreads), 0, fit->stream, static_cast<int>(fit->fitPairs.size()), d_pairs,
-                               fit->fit->d_shells, fit->fit->d_exps, fit->fit->d_coefs, fit->nFit, fit->fit->spherical,
-                               d_M);
+        const long long nPairs = static_cast<long long>(fit->fitPairs.size());
+        for (long long k0 = 0; status == kOK && k0 < nPairs; k0 += kSliceThreads) {
+            const long long m = std::min<long long>(kSliceThreads, nPairs - k0);
+            hipLaunchKernelGGL(fit_matrix_kernel, dim3(static_cast<unsigned>((m + kThreads - 1) / kThreads)),
+                               dim3(kThreads), 0, is, static_cast<int>(m), d_pairs + k0, fit->fit->d_shells,
+                               fit->fit->d_exps, fit->fit->d_coefs, fit->nFit, fit->fit->spherical, d_M);
             status = check(hipGetLastError(), "fit_matrix_kernel");
         }
-        const int drained = check(hipStreamSynchronize(fit->stream), "fit matrix");
-        if (status == kOK) status = drained;
+        if (is != nullptr) {
+            const int drained = check(hipStreamSynchronize(is), "fit matrix");
+            if (status == kOK) status = drained;
+        }
     }
     if (status == kOK) status = check(hipMemcpy(M, d_M, size * sizeof(double), hipMemcpyDeviceToHost), "copy M");
     qs_release(&d_M);
