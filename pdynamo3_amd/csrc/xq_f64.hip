@@ -356,6 +356,13 @@ int prepare_f64(xq_grid *g, const double *f, const double *fx, const double *fy,
     return kOK;
 }
 
+// Per-step W planes of a second job on the handle (the beta spin of an unrestricted quadrature).
+int alloc_step_f64(xq_grid *g, StepArrays *a) {
+    return dev_alloc(g, &a->WPA, static_cast<size_t>(g->nChunks > 0 ? g->nChunks : 1) * kChunk);
+}
+
+void free_step_f64(xq_grid *g, StepArrays *a) { dev_free(g, &a->WPA); }
+
 void free_f64(xq_grid *g) {
     dev_free(g, &g->d_FPB);
     dev_free(g, &g->d_GPB);
@@ -364,9 +371,9 @@ void free_f64(xq_grid *g) {
 }
 
 int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-               MixedJob *h_table, MixedJob *d_table, int stages) {
+               MixedJob *h_table, MixedJob *d_table, int stages, const StepArrays *over) {
     long long rows, blocks, chunks, groups;
-    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups);
+    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups, over);
     if (blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
         set_error("batch too large for one launch");
         return kInvalidArgument;

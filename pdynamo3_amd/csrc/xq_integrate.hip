@@ -70,6 +70,24 @@ void launch_zero_V(xq_grid *g, hipStream_t s) {
     hipLaunchKernelGGL(zero_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, n * n, g->d_V);
 }
 
+void launch_expand(int n, const double *packed, double *dense, hipStream_t s) {
+    const size_t nn = static_cast<size_t>(n) * n;
+    const int grid = static_cast<int>(std::min<size_t>((nn + 255) / 256, 16384));
+    hipLaunchKernelGGL(expand_packed_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, n, packed, dense);
+}
+
+void launch_zero(int n, double *dense, hipStream_t s) {
+    const size_t nn = static_cast<size_t>(n) * n;
+    const int grid = static_cast<int>(std::min<size_t>((nn + 255) / 256, 16384));
+    hipLaunchKernelGGL(zero_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, nn, dense);
+}
+
+void launch_pack(int n, const double *dense, double *packed, hipStream_t s) {
+    const size_t nPacked = static_cast<size_t>(n) * (n + 1) / 2;
+    const int grid = static_cast<int>(std::min<size_t>((nPacked + 255) / 256, 16384));
+    hipLaunchKernelGGL(pack_lower_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, n, dense, packed);
+}
+
 void launch_pack_V(xq_grid *g, hipStream_t s) {
     const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
     const int grid = static_cast<int>(std::min<size_t>((nPacked + 255) / 256, 16384));

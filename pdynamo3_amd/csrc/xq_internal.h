@@ -217,18 +217,32 @@ int functional_set(int nIDs, const int *ids, FunctionalSet *fs);
 void stage_bytes(const xq_grid *g, double *rho, double *fock);
 // stages: kStageDensity (P digits and the density kernel), kStageFock (W and the Fock product), or both.
 constexpr int kStageDensity = 1, kStageFock = 2;
+// Per-step arrays of one job in place of its handle's (a null field keeps the handle's): the second spin of an
+// unrestricted quadrature runs as a second job on the same handle with its own P, V, W and coefficients.
+struct StepArrays {
+    double *P = nullptr, *V = nullptr, *wcoef = nullptr, *rhoOut = nullptr, *WPA = nullptr;
+    unsigned long long *Pq = nullptr;
+    int *eP = nullptr;
+    signed char *DW = nullptr, *EW = nullptr;
+};
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-                 MixedJob *h_table, MixedJob *d_table, int stages = kStageDensity | kStageFock);
+                 MixedJob *h_table, MixedJob *d_table, int stages = kStageDensity | kStageFock,
+                 const StepArrays *over = nullptr);
 int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-               MixedJob *h_table, MixedJob *d_table, int stages = kStageDensity | kStageFock);
+               MixedJob *h_table, MixedJob *d_table, int stages = kStageDensity | kStageFock,
+               const StepArrays *over = nullptr);
 int integrate_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const double *Pb, double *eXC, double *nE,
                   double *Va, double *Vb, double *stats);
 void launch_expand_P(xq_grid *g, hipStream_t s);
 void launch_zero_V(xq_grid *g, hipStream_t s);
 void launch_pack_V(xq_grid *g, hipStream_t s);
+// The same on explicit arrays: packed lower triangle -> dense n x n, zero n x n, dense -> packed.
+void launch_expand(int n, const double *packed, double *dense, hipStream_t s);
+void launch_zero(int n, double *dense, hipStream_t s);
+void launch_pack(int n, const double *dense, double *packed, hipStream_t s);
 // Shared by both precisions: the job table entries of the given handles, the per-block chunk tables and Fock tile list.
 void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rows,
-               long long *blocks, long long *chunks, long long *groups);
+               long long *blocks, long long *chunks, long long *groups, const StepArrays *over = nullptr);
 int chunk_tables(xq_grid *g);
 int prepare_f64(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz);
 void free_f64(xq_grid *g);
@@ -249,6 +263,10 @@ int ensure_work(xq_grid *g);
 int ensure_job_table(xq_grid *g);
 int upload_density(xq_grid *g, const double *Ppacked, hipStream_t s);
 void free_mixed(xq_grid *g);
+int alloc_step_mixed(xq_grid *g, StepArrays *a);
+void free_step_mixed(xq_grid *g, StepArrays *a);
+int alloc_step_f64(xq_grid *g, StepArrays *a);
+void free_step_f64(xq_grid *g, StepArrays *a);
 // xq_integrate_noalloc (xq_noalloc.hip): the scratch array's size and what one call on the handle needs of it.
 int integrate_noalloc(xq_grid *g, int nIDs, const int *ids, const double *P, double *eXC, double *nE, double *V,
                       double *stats);

@@ -886,6 +886,25 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     return kOK;
 }
 
+// Per-step arrays of a second job on the handle (the beta spin of an unrestricted quadrature): P digits and exponents,
+// W digits and exponents.  Complete or released.
+int alloc_step_mixed(xq_grid *g, StepArrays *a) {
+    const size_t ncU = static_cast<size_t>(g->nChunksU > 0 ? g->nChunksU : 1), n = static_cast<size_t>(g->nFunctions);
+    int status = dev_alloc(g, &a->DW, ncU * kChunkBytes);
+    if (status == kOK) status = dev_alloc(g, &a->EW, ncU * kC);
+    if (status == kOK) status = dev_alloc(g, &a->Pq, n * n);
+    if (status == kOK) status = dev_alloc(g, &a->eP, n);
+    if (status != kOK) free_step_mixed(g, a);
+    return status;
+}
+
+void free_step_mixed(xq_grid *g, StepArrays *a) {
+    dev_free(g, &a->DW);
+    dev_free(g, &a->EW);
+    dev_free(g, &a->Pq);
+    dev_free(g, &a->eP);
+}
+
 void free_mixed(xq_grid *g) {
     dev_free(g, &g->d_chunkOffset);
     dev_free(g, &g->d_chunkBlock);
@@ -920,7 +939,7 @@ void free_mixed(xq_grid *g) {
 
 // The job table entries of the given handles (each with its functional) and each stage's total work-groups.
 void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rowsOut,
-               long long *blocksOut, long long *chunksOut, long long *groupsOut) {
+               long long *blocksOut, long long *chunksOut, long long *groupsOut, const StepArrays *over) {
     long long rows = 0, blocks = 0, chunks = 0, groups = 0;
     for (int j = 0; j < nJobs; j++) {
         const xq_grid *g = grids[j];
@@ -970,6 +989,18 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
         J.blockBase = static_cast<int>(blocks);
         J.chunkBase = static_cast<int>(chunks);
         J.groupBase = static_cast<int>(groups);
+        if (over != nullptr) {
+            const StepArrays &o = over[j];
+            if (o.P) J.P = o.P;
+            if (o.V) J.V = o.V;
+            if (o.wcoef) J.wcoef = o.wcoef;
+            if (o.rhoOut) J.rhoOut = o.rhoOut;
+            if (o.WPA) J.WPA = o.WPA;
+            if (o.Pq) J.Pq = o.Pq;
+            if (o.eP) J.eP = o.eP;
+            if (o.DW) J.DW = o.DW;
+            if (o.EW) J.EW = o.EW;
+        }
         rows += J.n;
         blocks += J.nBlocks;
         chunks += J.nChunks;
@@ -984,9 +1015,9 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
 // The four stage launches of the mixed path, each one launch over all the jobs.  h_table (pinned) and d_table hold at
 // least nJobs entries; the caller synchronises the stream before h_table is written again.
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
-                 MixedJob *h_table, MixedJob *d_table, int stages) {
+                 MixedJob *h_table, MixedJob *d_table, int stages, const StepArrays *over) {
     long long rows, blocks, chunks, groups;
-    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups);
+    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups, over);
     if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
         set_error("batch too large for one launch");
         return kInvalidArgument;

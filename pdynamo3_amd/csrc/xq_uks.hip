@@ -1,13 +1,13 @@
 // Unrestricted (UKS) quadrature: alpha and beta densities, alpha and beta V_xc (DFTIntegrator.c:281-402 with
 // isSpinUnrestricted, as DFTIntegrator.pyx:82-107 calls it with alpha and beta matrices).
 //
-// The restricted path's kernels do the work, in either precision: the density stage runs once per spin with the job's
-// rhoOut set, which stores (rho_s, grad rho_s) per point and stops before the functional; spin_functional_kernel then
-// evaluates the spin-polarised functional per block, skips a block whose |rho_a| and |rho_b| are all <= rhoTol, forms the
-// block's E_xc and N_e sums and the W coefficients of both spins,
+// The restricted path's kernels do the work, in either precision, with the two spins as two jobs of each launch: the
+// density stage runs with each job's rhoOut set, which stores (rho_s, grad rho_s) per point and stops before the
+// functional; spin_functional_kernel then evaluates the spin-polarised functional per block, skips a block whose |rho_a|
+// and |rho_b| are all <= rhoTol, forms the block's E_xc and N_e sums and the W coefficients of both spins,
 //   W_s = (w vrho_s / 2) Phi + (2 w vsigma_ss grad rho_s + w vsigma_ab grad rho_t) . grad Phi   (t the other spin),
 // which is DFTIntegrator_Fock plus DFTIntegrator_FockSigma(:517-600) in the form V_s = Phi W_s^T + W_s Phi^T; the Fock
-// stage then runs once per spin.
+// stage then runs once for both spins.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -83,73 +83,91 @@ __global__ void __launch_bounds__(kBlockSize) spin_functional_kernel(const DevBl
 
 }  // namespace
 
+// The two spins run as two jobs of one launch per stage on the handle (the beta job with its own per-step arrays), in
+// one queue of work with a single wait at the end: both densities in, both density stages in one launch, the
+// polarised functional, both Fock stages in one launch, both V_xc out.
 int integrate_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const double *Pb, double *eXC, double *nE,
                   double *Va, double *Vb, double *stats) {
     FunctionalSet fs;
     XQ_TRY(functional_set(nIDs, ids, &fs));
     XQ_TRY(ensure_work(g));
-    XQ_TRY(ensure_job_table(g));
     hipStream_t s = g->stream;
     const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1};
-    const size_t n = static_cast<size_t>(g->nFunctions), nPacked = n * (n + 1) / 2;
+    const int n = g->nFunctions;
+    const size_t nn = static_cast<size_t>(n) * n, nPacked = static_cast<size_t>(n) * (n + 1) / 2;
     const size_t nP = static_cast<size_t>(g->nPoints > 0 ? g->nPoints : 1);
-    xq_grid *grids[1] = {g};
-    // Each launch refills the handle's pinned job table and copies it to the device asynchronously: the copy reads the
-    // table when it executes, so the stream must be idle before the table is filled again.
-    auto launch = [&](int stages) {
-        XQ_CHECK(hipStreamSynchronize(s));
-        return g->precision == XQ_PRECISION_FP64 ? launch_f64(s, ev, 1, grids, &fs, g->h_job, g->d_job, stages)
-                                                 : launch_mixed(s, ev, 1, grids, &fs, g->h_job, g->d_job, stages);
-    };
-    double *rhoA = nullptr, *rhoB = nullptr, *wcB = nullptr;
-    double *wcA = g->d_wcoef;
+    const bool mixed = g->precision == XQ_PRECISION_MIXED;
+    xq_grid *grids[2] = {g, g};
+    const FunctionalSet fs2[2] = {fs, fs};
+    StepArrays beta;                            // the beta job's P digits and W planes (handle's arrays: alpha)
+    double *rhoA = nullptr, *rhoB = nullptr, *wcB = nullptr, *Pbd = nullptr, *Vbd = nullptr, *packedB = nullptr;
+    double *stageB = nullptr;                   // pinned: beta's packed P in, packed V out
+    MixedJob *h_tab = nullptr, *d_tab = nullptr;   // two 2-entry job tables: density stage, Fock stage
     int status = kOK;
     float msTotal = 0.0f;
     do {
+        if ((status = mixed ? alloc_step_mixed(g, &beta) : alloc_step_f64(g, &beta)) != kOK) break;
         if ((status = dev_alloc(g, &rhoA, 4 * nP)) != kOK) break;
         if ((status = dev_alloc(g, &rhoB, 4 * nP)) != kOK) break;
         if ((status = dev_alloc(g, &wcB, 4 * nP)) != kOK) break;
+        if ((status = dev_alloc(g, &Pbd, nn > 0 ? nn : 1)) != kOK) break;
+        if ((status = dev_alloc(g, &Vbd, nn > 0 ? nn : 1)) != kOK) break;
+        if ((status = dev_alloc(g, &packedB, nPacked > 0 ? nPacked : 1)) != kOK) break;
+        if ((status = dev_alloc(g, &d_tab, 4)) != kOK) break;
+        if ((status = check(cache_host_alloc(reinterpret_cast<void **>(&h_tab), 4 * sizeof(MixedJob)),
+                            "hipHostMalloc")) != kOK) break;
+        if ((status = check(cache_host_alloc(reinterpret_cast<void **>(&stageB), nPacked * sizeof(double) + 8),
+                            "hipHostMalloc")) != kOK) break;
+        beta.P = Pbd;
+        beta.V = Vbd;
+        // Both densities in, expanded to dense matrices.
+        std::memcpy(g->h_stage, Pa, nPacked * sizeof(double));
+        std::memcpy(stageB, Pb, nPacked * sizeof(double));
         if ((status = check(hipEventRecord(g->ev0, s), "event")) != kOK) break;
-        // Density stage, once per spin.
-        const double *P[2] = {Pa, Pb};
-        double *rho[2] = {rhoA, rhoB};
-        for (int sp = 0; sp < 2 && status == kOK; sp++) {
-            if ((status = check(hipStreamSynchronize(s), "stage sync")) != kOK) break;   // the staging buffer is free
-            std::memcpy(g->h_stage, P[sp], nPacked * sizeof(double));
-            if ((status = check(hipMemcpyAsync(g->d_packed, g->h_stage, nPacked * sizeof(double), hipMemcpyHostToDevice,
-                                               s), "copy P")) != kOK) break;
-            launch_expand_P(g, s);
-            g->d_rhoOut = rho[sp];
-            status = launch(kStageDensity);
-            g->d_rhoOut = nullptr;
-        }
+        if ((status = check(hipMemcpyAsync(g->d_packed, g->h_stage, nPacked * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy P alpha")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(packedB, stageB, nPacked * sizeof(double), hipMemcpyHostToDevice, s),
+                            "copy P beta")) != kOK) break;
+        launch_expand(n, g->d_packed, g->d_P, s);
+        launch_expand(n, packedB, Pbd, s);
+        // Density stage of both spins in one launch: each job stores (rho_s, grad rho_s) per point.
+        StepArrays dens[2];
+        dens[0].rhoOut = rhoA;
+        dens[1] = beta;
+        dens[1].rhoOut = rhoB;
+        status = mixed ? launch_mixed(s, ev, 2, grids, fs2, h_tab, d_tab, kStageDensity, dens)
+                       : launch_f64(s, ev, 2, grids, fs2, h_tab, d_tab, kStageDensity, dens);
         if (status != kOK) break;
         if (g->nBlocks > 0)
             hipLaunchKernelGGL(spin_functional_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, g->d_w, fs,
-                               g->rhoTol, rhoA, rhoB, wcA, wcB, g->d_blockEN, g->d_skip);
+                               g->rhoTol, rhoA, rhoB, g->d_wcoef, wcB, g->d_blockEN, g->d_skip);
         if ((status = check(hipGetLastError(), "spin_functional_kernel")) != kOK) break;
-        // Fock stage, once per spin (the job table takes the handle's W coefficients: beta's are swapped in).
-        double *V[2] = {Va, Vb};
-        for (int sp = 0; sp < 2 && status == kOK; sp++) {
-            launch_zero_V(g, s);
-            if (sp == 1) g->d_wcoef = wcB;
-            status = launch(kStageFock);
-            g->d_wcoef = wcA;
-            if (status != kOK) break;
-            launch_pack_V(g, s);
-            if ((status = check(hipMemcpyAsync(g->h_stage, g->d_packed, nPacked * sizeof(double),
-                                               hipMemcpyDeviceToHost, s), "copy V")) != kOK) break;
-            if (sp == 1 && g->nBlocks > 0) {
-                if ((status = check(hipMemcpyAsync(g->h_blockEN, g->d_blockEN, 2 * g->nBlocks * sizeof(double),
-                                                   hipMemcpyDeviceToHost, s), "copy block sums")) != kOK) break;
-                if ((status = check(hipMemcpyAsync(g->h_skip, g->d_skip, g->nBlocks * sizeof(int),
-                                                   hipMemcpyDeviceToHost, s), "copy skips")) != kOK) break;
-                if ((status = check(hipEventRecord(g->ev1, s), "event")) != kOK) break;
-            }
-            if ((status = check(hipStreamSynchronize(s), "V sync")) != kOK) break;
-            for (size_t k = 0; k < nPacked; k++) V[sp][k] += g->h_stage[k];
-        }
+        // Fock stage of both spins in one launch (the second table: the first may still be in flight).
+        launch_zero(n, g->d_V, s);
+        launch_zero(n, Vbd, s);
+        StepArrays fock[2];
+        fock[1] = beta;
+        fock[1].wcoef = wcB;
+        status = mixed ? launch_mixed(s, ev, 2, grids, fs2, h_tab + 2, d_tab + 2, kStageFock, fock)
+                       : launch_f64(s, ev, 2, grids, fs2, h_tab + 2, d_tab + 2, kStageFock, fock);
         if (status != kOK) break;
+        launch_pack(n, g->d_V, g->d_packed, s);
+        launch_pack(n, Vbd, packedB, s);
+        if ((status = check(hipGetLastError(), "pack")) != kOK) break;
+        if ((status = check(hipEventRecord(g->ev1, s), "event")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(g->h_stage, g->d_packed, nPacked * sizeof(double), hipMemcpyDeviceToHost, s),
+                            "copy V alpha")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(stageB, packedB, nPacked * sizeof(double), hipMemcpyDeviceToHost, s),
+                            "copy V beta")) != kOK) break;
+        if (g->nBlocks > 0) {
+            if ((status = check(hipMemcpyAsync(g->h_blockEN, g->d_blockEN, 2 * g->nBlocks * sizeof(double),
+                                               hipMemcpyDeviceToHost, s), "copy block sums")) != kOK) break;
+            if ((status = check(hipMemcpyAsync(g->h_skip, g->d_skip, g->nBlocks * sizeof(int), hipMemcpyDeviceToHost, s),
+                                "copy skips")) != kOK) break;
+        }
+        if ((status = check(hipStreamSynchronize(s), "unrestricted quadrature")) != kOK) break;
+        for (size_t k = 0; k < nPacked; k++) Va[k] += g->h_stage[k];
+        for (size_t k = 0; k < nPacked; k++) Vb[k] += stageB[k];
         // FP64 reductions in block order (DFTIntegrator.c:350-352).
         double e = 0.0, ne = 0.0;
         long long skipped = 0;
@@ -160,7 +178,7 @@ int integrate_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const 
         }
         *eXC = e;
         *nE = ne;
-        if (g->nBlocks > 0) (void)hipEventElapsedTime(&msTotal, g->ev0, g->ev1);
+        (void)hipEventElapsedTime(&msTotal, g->ev0, g->ev1);
         if (stats != nullptr) {
             for (int k = 0; k < XQ_NSTATS; k++) stats[k] = 0.0;
             stats[0] = g->bfValues;
@@ -170,12 +188,17 @@ int integrate_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const 
             stats[8] = g->bfSquaresPadded;
         }
     } while (false);
-    g->d_rhoOut = nullptr;
-    g->d_wcoef = wcA;
-    (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(s);   // nothing below is released while a kernel may still use it
+    if (mixed) free_step_mixed(g, &beta); else free_step_f64(g, &beta);
     dev_free(g, &rhoA);
     dev_free(g, &rhoB);
     dev_free(g, &wcB);
+    dev_free(g, &Pbd);
+    dev_free(g, &Vbd);
+    dev_free(g, &packedB);
+    dev_free(g, &d_tab);
+    if (h_tab != nullptr) cache_host_free(h_tab);
+    if (stageB != nullptr) cache_host_free(stageB);
     return status;
 }
 
