@@ -93,7 +93,7 @@ int         xq_integrate_uks    ( xq_grid *grid, int nIDs, const int *ids, const
 !  doGradients branch :405-487 with DFTGridWeights_Derivatives, DFTGridWeights.c:98-215), as DFTGridIntegrator.Gradients
 !  calls it (DFTGridIntegrator.py:72-88).  Restricted closed shell, FP64 on either precision's handle (the basis values and
 !  second derivatives are evaluated afresh).  gradients3: nAtoms x 3 in hartree/bohr, INCREMENTED as the reference's
-!  Coordinates3 is.  At most 655 atoms. */
+!  Coordinates3 is.  At most 1024 atoms. */
 int         xq_gradients        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *gradients3 ) ;
 
 /* Spin-unrestricted XC nuclear gradients: the same call with isSpinUnrestricted, alpha and beta densities and the

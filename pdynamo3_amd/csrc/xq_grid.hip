@@ -374,8 +374,9 @@ void becke_pairs(const xq_grid *g, std::vector<double> &invR, std::vector<double
     }
 }
 
-// Largest number of atoms per lane of the weight-derivative kernel (64 x 11 = 704 atoms; the LDS bounds it at 655).
-constexpr int kMaxLaneAtoms = 11;
+// Largest number of atoms per lane of the weight-derivative kernel: 64 x 16 = 1024 atoms (more atoms per lane spill
+// registers).
+constexpr int kMaxLaneAtoms = 16;
 
 // Screen of the weight-derivative terms.  The term of the ordered pair (k, m) is ifac = (ds_km/dmu) (A_k / s_km) fac_k, and
 // moves an atom or the point by at most |ifac| 2 / R_km.  |ds/dmu| <= 1.6875 x 2 = 3.375 (|ds/dnu| peaks at nu = 0 with
@@ -405,7 +406,7 @@ __device__ __forceinline__ double excluded_product(double prodNZ, int zeros, dou
     return (zeros == 1 && s == 0.0) ? prodNZ : 0.0;
 }
 
-// XC gradient weight terms (DFTGridWeights_Derivatives, DFTGridWeights.c:98-215).  One wave per point, four points per
+// XC gradient weight terms (DFTGridWeights_Derivatives, DFTGridWeights.c:98-215).  One wave per point, WPG points per
 // round of a work-group; lane l owns atoms l, l + 64, ... (at most LA of them) and accumulates their gradient in
 // registers.  Per point (grid atom g, weight w, energy density eXC = eps rho): A_i = prod_j s_ij; p = A_g / sum_i A_i;
 // ew = eXC w / (p sum_i A_i); dA_i/dmu_ij = ds_ij prod_{k != j} s_ik; for every ordered pair (i, j) the reference adds
@@ -417,9 +418,10 @@ __device__ __forceinline__ double excluded_product(double prodNZ, int zeros, dou
 // cell product is not negligible at the point.  Both factors of a pair come from one polynomial (s_km and s_mk = 1 - s_km share nu), and
 // one division gives both 1 / s_km and 1 / s_mk.
 // LDS per wave: the distances R, their inverses and the non-zero products (3 nAtoms doubles), and the zero counts (nAtoms
-// bytes, after the doubles of all four waves).
-template <int LA>
-__global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int pointsPerGroup,
+// bytes, after the doubles of all WPG waves).  WPG waves per work-group (4, or 2 for molecules whose per-wave LDS would
+// not fit four times in 64 KB); the waves share no LDS.
+template <int LA, int WPG>
+__global__ void __launch_bounds__(64 * WPG) weight_gradient_kernel(int nPoints, int pointsPerGroup,
                                                               const double *__restrict__ px, const double *__restrict__ py,
                                                               const double *__restrict__ pz, const double *__restrict__ pw,
                                                               const int *__restrict__ pointAtom,
@@ -431,13 +433,13 @@ __global__ void __launch_bounds__(256) weight_gradient_kernel(int nPoints, int p
     extern __shared__ double lds[];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     double *R = lds + static_cast<size_t>(wv) * 3 * nAtoms, *iR = R + nAtoms, *prodNZ = iR + nAtoms;
-    unsigned char *zeros = reinterpret_cast<unsigned char *>(lds + static_cast<size_t>(12) * nAtoms) + wv * nAtoms;
+    unsigned char *zeros = reinterpret_cast<unsigned char *>(lds + static_cast<size_t>(3 * WPG) * nAtoms) + wv * nAtoms;
     double G[LA][3];
 #pragma unroll
     for (int a = 0; a < LA; a++) G[a][0] = G[a][1] = G[a][2] = 0.0;
     const int p0 = blockIdx.x * pointsPerGroup;
     const int p1 = p0 + pointsPerGroup < nPoints ? p0 + pointsPerGroup : nPoints;
-    for (int base = p0; base < p1; base += 4) {
+    for (int base = p0; base < p1; base += WPG) {
         const int p = base + wv;
         const bool active = p < p1 && eXC[p] != 0.0;
         const double rx = active ? px[p] : 0.0, ry = active ? py[p] : 0.0, rz = active ? pz[p] : 0.0;
@@ -893,9 +895,13 @@ long long estimated_points(int accuracy, int nAtoms, const int *Z) {
 int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_t s) {
     const int nA = g->nAtoms, G = g->nPoints;
     if (G <= 0 || nA < 2) return kOK;
-    const size_t lds = 4 * (3 * sizeof(double) + 1) * static_cast<size_t>(nA);
-    if (lds > 64 * 1024) {
-        set_error("XC gradients support at most 655 atoms");
+    // Waves per work-group: four up to 640 atoms, two above (each wave needs 25 bytes of LDS per atom: 4 x 25 x 640 and
+    // 2 x 25 x 1024 bytes both fit in 64 KB).  The kernel instance below takes the same count.
+    const size_t perWave = (3 * sizeof(double) + 1) * static_cast<size_t>(nA);
+    const int wpg = nA <= 640 ? 4 : 2;
+    const size_t lds = wpg * perWave;
+    if (nA > 64 * kMaxLaneAtoms || lds > 64 * 1024) {
+        set_error("XC gradients support at most " + std::to_string(64 * kMaxLaneAtoms) + " atoms");
         return kInvalidArgument;
     }
     std::vector<double> invR, aij;
@@ -937,21 +943,22 @@ int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_
                             "copy xyz")) != kOK) break;
         if ((status = check(hipMemcpyAsync(d_pointAtom, pointAtom.data(), pointAtom.size() * sizeof(int),
                                            hipMemcpyHostToDevice, s), "copy point atoms")) != kOK) break;
-        const int groups = std::min(2048, (G + 63) / 64);
+        const int groups = std::min(2048 * 4 / wpg, (G + 63) / 64);
         int perGroup = (G + groups - 1) / groups;
-        perGroup = (perGroup + 3) / 4 * 4;
+        perGroup = (perGroup + wpg - 1) / wpg * wpg;
         const int nGroups = (G + perGroup - 1) / perGroup;
         auto launch = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, dim3(nGroups), dim3(256), lds, s, G, perGroup, g->d_x, g->d_y, g->d_z, g->d_w,
-                               d_pointAtom, d_eXC, d_xyz, nA, d_invR, d_aij, d_near, nNear, d_grad);
+            hipLaunchKernelGGL(kernel, dim3(nGroups), dim3(64 * wpg), lds, s, G, perGroup, g->d_x, g->d_y, g->d_z,
+                               g->d_w, d_pointAtom, d_eXC, d_xyz, nA, d_invR, d_aij, d_near, nNear, d_grad);
         };
         // Atoms per lane: the smallest instance that covers nAtoms (fewer registers for small systems).
-        if (nA <= 64) launch(weight_gradient_kernel<1>);
-        else if (nA <= 128) launch(weight_gradient_kernel<2>);
-        else if (nA <= 192) launch(weight_gradient_kernel<3>);
-        else if (nA <= 256) launch(weight_gradient_kernel<4>);
-        else if (nA <= 384) launch(weight_gradient_kernel<6>);
-        else launch(weight_gradient_kernel<kMaxLaneAtoms>);
+        if (nA <= 64) launch(weight_gradient_kernel<1, 4>);
+        else if (nA <= 128) launch(weight_gradient_kernel<2, 4>);
+        else if (nA <= 192) launch(weight_gradient_kernel<3, 4>);
+        else if (nA <= 256) launch(weight_gradient_kernel<4, 4>);
+        else if (nA <= 384) launch(weight_gradient_kernel<6, 4>);
+        else if (nA <= 640) launch(weight_gradient_kernel<10, 4>);
+        else launch(weight_gradient_kernel<kMaxLaneAtoms, 2>);
         if ((status = check(hipGetLastError(), "weight_gradient_kernel")) != kOK) break;
         status = check(hipStreamSynchronize(s), "gradient sync");
     } while (false);

@@ -4,7 +4,9 @@
   quadrature against the FP64 path on the same grid and density, |dE_xc| <= 1e-6 Eh and max|dV_xc| <= 1e-7 (the stated
   tolerance of the mixed path); the electron count within 1e-5 per electron of 10 per water;
 - (H2O)64: the XC nuclear gradients (restricted, and unrestricted at alpha = 0.55 P, beta = 0.45 P) sum to zero over the
-  atoms (the grid moves with its atoms), and the closed-shell limit of the unrestricted gradients is the restricted one.
+  atoms (the grid moves with its atoms), and the closed-shell limit of the unrestricted gradients is the restricted one;
+- (H2O)220 (660 atoms, 19 A cube, seed 7), above the 640 atoms of the four-wave weight kernel: the same two properties
+  for the two-wave instance that takes up to 1,024 atoms.
 The FP64 path itself is pinned against the oracle and the compiled reference on smaller systems (test_gpu_parity.py).
 """
 
@@ -57,4 +59,22 @@ def test_h2o64_gradients_are_translation_invariant():
           "closed-shell limit", np.max(np.abs(Gh - G)))
     assert np.max(np.abs(G)) > 1e-2
     assert np.max(np.abs(G.sum(0))) <= 1e-9 and np.max(np.abs(Gu.sum(0))) <= 1e-9
+    assert np.max(np.abs(Gh - G)) <= 1e-9
+
+
+def test_h2o220_gradients_are_translation_invariant():
+    """660 atoms: the weight terms run on the instance with 16 atoms per lane and two waves per work-group."""
+    from pdynamo3_amd import integrator
+    from pdynamo3_amd.definitions import functional_ids
+    Z, xyz, spec, P = _cluster(220, 19.0, 7)
+    assert len(Z) == 660
+    ids = list(functional_ids("pbe"))
+    g = integrator.QuadratureGrid(Z, xyz, 2)
+    g.set_basis(spec, False, "mixed")
+    G = g.gradients(ids, P)
+    Gh = g.gradients_uks(ids, 0.5 * P, 0.5 * P)
+    g.close()
+    print("max G", np.max(np.abs(G)), "sum", np.max(np.abs(G.sum(0))), "closed-shell limit", np.max(np.abs(Gh - G)))
+    assert np.max(np.abs(G)) > 1e-2
+    assert np.max(np.abs(G.sum(0))) <= 1e-9
     assert np.max(np.abs(Gh - G)) <= 1e-9
