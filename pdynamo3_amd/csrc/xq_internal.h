@@ -77,6 +77,9 @@ struct MixedJob {
     double rhoTol;
     int n, nBlocks, nChunks, nTiles;
     int rowBase, blockBase, chunkBase, groupBase;
+    // Density launch: blocks order[xcdStart[x] .. xcdStart[x + 1]) run on XCD x, at that XCD's slots from xcdBase[x].
+    const int *order;
+    int xcdStart[9], xcdBase[8];
 };
 
 }  // namespace xq
@@ -97,6 +100,11 @@ struct xq_grid {
     std::vector<xq::DevBlock> blocks;     // host copy
     double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_w = nullptr;
     xq::DevBlock *d_blocks = nullptr;
+    // Density launch order: the blocks cut into 8 contiguous ranges of equal estimated cost (one per XCD, so that each
+    // XCD's L2 holds one region's P rows), each range in order of decreasing cost (the longest blocks start first).
+    std::vector<int> h_order;
+    int *d_order = nullptr;
+    int xcdStart[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // Basis.
     int nFunctions = 0, nShells = 0, spherical = 1, precision = 0;
@@ -241,8 +249,13 @@ void launch_expand(int n, const double *packed, double *dense, hipStream_t s);
 void launch_zero(int n, double *dense, hipStream_t s);
 void launch_pack(int n, const double *dense, double *packed, hipStream_t s);
 // Shared by both precisions: the job table entries of the given handles, the per-block chunk tables and Fock tile list.
+// blockSlots (optional): the density launch's work-groups per XCD, the largest over the XCDs of the jobs' summed block
+// ranges.
 void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rows,
-               long long *blocks, long long *chunks, long long *groups, const StepArrays *over = nullptr);
+               long long *blocks, long long *chunks, long long *groups, const StepArrays *over = nullptr,
+               long long *blockSlots = nullptr);
+// The density launch's block order (d_order, xcdStart), from the blocks' kept-function counts; at basis set-up.
+int build_block_order(xq_grid *g, hipStream_t s);
 int chunk_tables(xq_grid *g);
 int prepare_f64(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz);
 void free_f64(xq_grid *g);

@@ -40,9 +40,11 @@
 //                                  value = (256 GHU + GLU) 2^(EG - 23), GHU int16, GLU uint8 (W enters V only, whose
 //                                  error its 4-digit rows bound); the W kernel reads them with 16-byte loads.
 //   DB     [C][digit (5)][point (128)][32 bytes]   Phi digits in the B operand order of the density product.  Position
-//                                  16 h + v of a point holds row 32 rc + 4 h + 8 (v / 4) + v % 4 (chunk_fun): the row
-//                                  whose result the MFMA lane (., h) receives in register v, so that the epilogue finds
-//                                  Phi for its own results in the operand it loads.  P_red tiles use the same K order.
+//                                  16 h + v of a point holds row 32 rc + 8 (v % 4) + 4 h + v / 4 (chunk_fun): the row
+//                                  whose result the MFMA lane (., h) receives in register v (P_red rows are staged in
+//                                  the tile rows tile_row), so that the epilogue finds Phi for its own results in the
+//                                  operand it loads.  P_red tiles use the same K order, in which each 4-byte group
+//                                  holds rows q, q + 8, q + 16, q + 24.
 //                                  Exponents EB [block][128].
 //   GF     [C][3][point (128)][32 int32]           grad Phi in DB's order as 32-bit fixed point: value = GF 2^(EG - 31),
 //                                  one power of two per point and component over the block's kept functions
@@ -165,9 +167,12 @@ __device__ __forceinline__ int grad_encode(double g, int e) {
     return static_cast<int>(Q);
 }
 
-// Row of a chunk held at density position 16 h + v, and the position of row k.
-__host__ __device__ constexpr int chunk_fun(int h, int v) { return 4 * h + 8 * (v >> 2) + (v & 3); }
-__host__ __device__ constexpr int chunk_pos(int k) { return 16 * ((k >> 2) & 1) + 4 * (k >> 3) + (k & 3); }
+// Row of a chunk held at density position 16 h + v (K position 4 q + j holds row 8 j + q), and the position of row k.
+__host__ __device__ constexpr int chunk_fun(int h, int v) { return 8 * (v & 3) + 4 * h + (v >> 2); }
+__host__ __device__ constexpr int chunk_pos(int k) { return 4 * (k & 7) + (k >> 3); }
+// The MFMA row that holds row r of a P_red tile: the output row (v % 4) + 4 h + 8 (v / 4) of lane half h, register v,
+// is row chunk_fun(h, v), the row whose Phi digits the lane holds at position 16 h + v.
+__host__ __device__ constexpr int tile_row(int r) { return 8 * (r & 3) + 4 * ((r >> 2) & 1) + (r >> 3); }
 
 // Job of work-group g in a launch over several jobs: the last job whose first work-group in this stage is <= g (jobs
 // without work in the stage share the next job's first work-group and are never returned for it).
@@ -367,18 +372,20 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
 // kRhoOut: the density stage of one spin of an unrestricted quadrature (the job's rhoOut receives rho and grad rho).
 template <bool kRhoOut>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
+density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
     __shared__ v4i sDI[kDB][256];   // per thread: its Phi digits of the current row chunk, for the epilogue
     __shared__ int sPe[2][kC];
     __shared__ int sIdx[kIdxCache];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
-    // Work-group g runs on XCD g % 8; each XCD takes a contiguous range of blocks (one region of the molecule), so that
-    // its L2 holds the P rows those blocks gather.
-    const int wg = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
-    if (wg >= nGroups) return;
-    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, wg, [](const xq::MixedJob &j) { return j.blockBase; })];
+    // Work-group g runs on XCD x = g % 8 and takes slot g / 8 of that XCD's list: the jobs' block ranges x in job order
+    // (each a contiguous region of its molecule, so that the XCD's L2 holds the P rows its blocks gather; the ranges
+    // of one job have equal estimated cost), each range's blocks in order of decreasing cost.
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, slot, [xcd](const xq::MixedJob &j) { return j.xcdBase[xcd]; })];
+    const int local = slot - J.xcdBase[xcd];
+    if (local >= J.xcdStart[xcd + 1] - J.xcdStart[xcd]) return;
     const xq::DevBlock *__restrict__ blocks = J.blocks;
     const int *__restrict__ chunkOffset = J.chunkOffset;
     const int *__restrict__ idxAll = J.idx;
@@ -395,7 +402,7 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     double *__restrict__ wcoef = J.wcoef;
     double *__restrict__ blockEN = J.blockEN;
     int *__restrict__ skip = J.skip;
-    const int b = wg - J.blockBase;
+    const int b = J.order[J.xcdStart[xcd] + local];
     const xq::DevBlock blk = blocks[b];
     const int nb = blk.nKept, np = blk.nPoints;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -418,8 +425,9 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
     const signed char *DBb = DB + C * kChunkBytesB;
     // P_red is staged one row panel at a time (rows 32 I .. + 31, up to kPanel chunks of columns), all of the panel's
     // words fetched before the barrier; the panel's MFMA steps then run without barriers.  Thread (row sr, slot s8)
-    // fetches columns 4 s8 .. + 3 of every chunk of the panel and writes them as one word per digit at chunk_pos.
-    const int sr = t >> 3, s8 = t & 7, sdw = 4 * (s8 & 1) + (s8 >> 1);
+    // fetches columns 8 j + s8 (j = 0 .. 3) of every chunk of the panel (eight lanes read eight consecutive kept
+    // functions) and writes them as one word per digit at K positions 4 s8 .. + 3, in row tile_row(sr).
+    const int sr = t >> 3, s8 = t & 7, srow = tile_row(sr);
     auto fetchB = [&](int c, v4i (&bq)[kDB]) {
 #pragma unroll
         for (int j = 0; j < kDB; j++)
@@ -443,7 +451,7 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
             for (int m = 0; m < kPanel; m++)
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++) {
-                    const int k = kC * (c0 + m) + 4 * s8 + jj;
+                    const int k = kC * (c0 + m) + 8 * jj + s8;
                     wd[m][jj] = (m < nc && r < nb && k < nb) ? Pq[rowOff + idx[k]] : 0ull;
                 }
             __syncthreads();   // every wave is done with the previous panel
@@ -455,7 +463,7 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int group
                         unsigned x = 0;
 #pragma unroll
                         for (int jj = 0; jj < 4; jj++) x |= (static_cast<unsigned>(wd[m][jj] >> (8 * j)) & 255u) << (8 * jj);
-                        sP[j][sr][8 * m + sdw] = static_cast<int>(x);
+                        sP[j][srow][8 * m + s8] = static_cast<int>(x);
                     }
                 }
             }
@@ -762,6 +770,39 @@ int chunk_tables(xq_grid *g) {
         XQ_CHECK(hipMemcpyAsync(g->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s));
     // The host vectors must outlive the copies.
     XQ_CHECK(hipStreamSynchronize(s));
+    return build_block_order(g, s);
+}
+
+// The density launch's block order.  A block of T row chunks costs about T^2 MFMA steps of the density kernel, plus
+// one epilogue per row chunk and the functional (counted as one step each, and one for a block without kept
+// functions).  The blocks are cut into 8 contiguous ranges of equal summed cost, one per XCD, each sorted by decreasing
+// cost: on (H2O)64 the largest blocks take about 28 times the mean, and started last they set the length of the launch.
+int build_block_order(xq_grid *g, hipStream_t s) {
+    const int nB = g->nBlocks;
+    std::vector<double> cost(static_cast<size_t>(nB > 0 ? nB : 1), 0.0);
+    double total = 0.0;
+    for (int b = 0; b < nB; b++) {
+        const double T = static_cast<double>((g->blocks[b].nKept + kC - 1) / kC);
+        cost[b] = T * T + T + 1.0;
+        total += cost[b];
+    }
+    g->xcdStart[0] = 0;
+    int b = 0;
+    double acc = 0.0;
+    for (int x = 1; x < 8; x++) {
+        const double target = total * x / 8.0;
+        while (b < nB && acc + 0.5 * cost[b] < target) acc += cost[b++];
+        g->xcdStart[x] = b;
+    }
+    g->xcdStart[8] = nB;
+    g->h_order.resize(static_cast<size_t>(nB > 0 ? nB : 1));
+    for (int i = 0; i < nB; i++) g->h_order[i] = i;
+    for (int x = 0; x < 8; x++)
+        std::stable_sort(g->h_order.begin() + g->xcdStart[x], g->h_order.begin() + g->xcdStart[x + 1],
+                         [&](int u, int v) { return cost[u] > cost[v]; });
+    XQ_TRY(dev_alloc(g, &g->d_order, g->h_order.size()));
+    XQ_CHECK(hipMemcpyAsync(g->d_order, g->h_order.data(), g->h_order.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    XQ_CHECK(hipStreamSynchronize(s));
     return kOK;
 }
 
@@ -906,6 +947,9 @@ void free_step_mixed(xq_grid *g, StepArrays *a) {
 }
 
 void free_mixed(xq_grid *g) {
+    dev_free(g, &g->d_order);
+    g->h_order.clear();
+    for (int x = 0; x <= 8; x++) g->xcdStart[x] = 0;
     dev_free(g, &g->d_chunkOffset);
     dev_free(g, &g->d_chunkBlock);
     dev_free(g, &g->d_tiles);
@@ -939,12 +983,19 @@ void free_mixed(xq_grid *g) {
 
 // The job table entries of the given handles (each with its functional) and each stage's total work-groups.
 void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rowsOut,
-               long long *blocksOut, long long *chunksOut, long long *groupsOut, const StepArrays *over) {
-    long long rows = 0, blocks = 0, chunks = 0, groups = 0;
+               long long *blocksOut, long long *chunksOut, long long *groupsOut, const StepArrays *over,
+               long long *blockSlotsOut) {
+    long long rows = 0, blocks = 0, chunks = 0, groups = 0, slots[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = 0; j < nJobs; j++) {
         const xq_grid *g = grids[j];
         MixedJob &J = h_table[j];
         J.blocks = g->d_blocks;
+        J.order = g->d_order;
+        for (int x = 0; x <= 8; x++) J.xcdStart[x] = g->xcdStart[x];
+        for (int x = 0; x < 8; x++) {
+            J.xcdBase[x] = static_cast<int>(slots[x]);
+            slots[x] += g->xcdStart[x + 1] - g->xcdStart[x];
+        }
         J.chunkOffset = g->d_chunkOffset;
         J.chunkBlock = g->d_chunkBlock;
         J.idx = g->d_idx;
@@ -1008,6 +1059,7 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
     }
     *rowsOut = rows;
     *blocksOut = blocks;
+    if (blockSlotsOut != nullptr) *blockSlotsOut = *std::max_element(slots, slots + 8);
     *chunksOut = chunks;
     *groupsOut = groups;
 }
@@ -1016,8 +1068,8 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
 // least nJobs entries; the caller synchronises the stream before h_table is written again.
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
                  MixedJob *h_table, MixedJob *d_table, int stages, const StepArrays *over) {
-    long long rows, blocks, chunks, groups;
-    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups, over);
+    long long rows, blocks, chunks, groups, blockSlots;
+    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups, over, &blockSlots);
     if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
         set_error("batch too large for one launch");
         return kInvalidArgument;
@@ -1027,13 +1079,11 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
         hipLaunchKernelGGL(p_slice_kernel, dim3(static_cast<unsigned>(rows)), dim3(256), 0, s, d_table, nJobs);
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
     if ((stages & kStageDensity) && blocks > 0) {
-        const int perXcd = static_cast<int>((blocks + 7) / 8);
-        if (h_table[0].rhoOut != nullptr) {   // the density stage of one spin (single-job calls only)
-            hipLaunchKernelGGL(density_kernel_mixed<true>, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table,
-                               nJobs, perXcd, static_cast<int>(blocks));
+        const unsigned grid = static_cast<unsigned>(8 * blockSlots);
+        if (h_table[0].rhoOut != nullptr) {   // the density stage of one spin (every job of the call)
+            hipLaunchKernelGGL(density_kernel_mixed<true>, dim3(grid), dim3(256), 0, s, d_table, nJobs);
         } else {
-            hipLaunchKernelGGL(density_kernel_mixed<false>, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s,
-                               d_table, nJobs, perXcd, static_cast<int>(blocks));
+            hipLaunchKernelGGL(density_kernel_mixed<false>, dim3(grid), dim3(256), 0, s, d_table, nJobs);
         }
     }
     XQ_CHECK(hipEventRecord(ev.evRho1, s));
