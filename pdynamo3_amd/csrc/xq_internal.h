@@ -77,8 +77,9 @@ struct MixedJob {
     double rhoTol;
     int n, nBlocks, nChunks, nTiles;
     int rowBase, blockBase, chunkBase, groupBase;
-    // Density launch: blocks order[xcdStart[x] .. xcdStart[x + 1]) run on XCD x, at that XCD's slots from xcdBase[x].
-    const int *order;
+    // Density launch: blocks order[xcdStart[x] .. xcdStart[x + 1]) run on XCD x, at that XCD's slots from xcdBase[x];
+    // each entry {block, first chunk, first kept index, kept functions x 256 + points}.
+    const int4 *order;
     int xcdStart[9], xcdBase[8];
 };
 
@@ -102,8 +103,8 @@ struct xq_grid {
     xq::DevBlock *d_blocks = nullptr;
     // Density launch order: the blocks cut into 8 contiguous ranges of equal estimated cost (one per XCD, so that each
     // XCD's L2 holds one region's P rows), each range in order of decreasing cost (the longest blocks start first).
-    std::vector<int> h_order;
-    int *d_order = nullptr;
+    std::vector<int4> h_order;
+    int4 *d_order = nullptr;
     int xcdStart[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // Basis.

@@ -387,7 +387,6 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     const int local = slot - J.xcdBase[xcd];
     if (local >= J.xcdStart[xcd + 1] - J.xcdStart[xcd]) return;
     const xq::DevBlock *__restrict__ blocks = J.blocks;
-    const int *__restrict__ chunkOffset = J.chunkOffset;
     const int *__restrict__ idxAll = J.idx;
     const signed char *__restrict__ DB = J.DB;
     const signed char *__restrict__ EB = J.EB;
@@ -402,24 +401,21 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     double *__restrict__ wcoef = J.wcoef;
     double *__restrict__ blockEN = J.blockEN;
     int *__restrict__ skip = J.skip;
-    const int b = J.order[J.xcdStart[xcd] + local];
-    const xq::DevBlock blk = blocks[b];
-    const int nb = blk.nKept, np = blk.nPoints;
+    // The block's record in launch order ({block, first chunk, first kept index, kept functions x 256 + points}): one
+    // load gives what the first fetches need.
+    const int4 ent = J.order[J.xcdStart[xcd] + local];
+    const int b = ent.x;
+    const int nb = ent.w >> 8, np = ent.w & 255;
+    const int pStart = blocks[b].pStart;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int lr = lane & 31, lh = lane >> 5;
     if (nb <= 0) {
         if (t == 0) { blockEN[2 * b] = 0.0; blockEN[2 * b + 1] = 0.0; skip[b] = 1; }
         if (kRhoOut && t < np)
-            for (int c = 0; c < 4; c++) J.rhoOut[4 * (static_cast<size_t>(blk.pStart) + t) + c] = 0.0;
+            for (int c = 0; c < 4; c++) J.rhoOut[4 * (static_cast<size_t>(pStart) + t) + c] = 0.0;
         return;
     }
-    // The block's kept-function indices, cached in LDS for the P_red gathers (blocks with more kept functions than the
-    // cache read them from memory).
-    const int *idxG = idxAll + blk.iOffset;
-    for (int k = t; k < nb && k < kIdxCache; k += 256) sIdx[k] = idxG[k];
-    __syncthreads();
-    const int *idx = nb <= kIdxCache ? sIdx : idxG;
-    const size_t C = static_cast<size_t>(chunkOffset[b]);
+    const size_t C = static_cast<size_t>(ent.y);
     const int nChunks = (nb + kC - 1) / kC;
     const int p = kC * wv + lr;
     const signed char *DBb = DB + C * kChunkBytesB;
@@ -439,6 +435,12 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     double gscale[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) gscale[c] = ldexp(1.0, EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p] - kGradBits);
+    // The block's kept-function indices (fetched after the first Phi operands, whose latency they share), cached in LDS
+    // for the P_red gathers (blocks with more kept functions than the cache read them from memory).
+    const int *idxG = idxAll + ent.z;
+    for (int k = t; k < nb && k < kIdxCache; k += 256) sIdx[k] = idxG[k];
+    __syncthreads();
+    const int *idx = nb <= kIdxCache ? sIdx : idxG;
     double pr = 0.0, pgx = 0.0, pgy = 0.0, pgz = 0.0;
     for (int I = 0; I < nChunks; I++) {
         v16i acc[kXOrders];
@@ -534,7 +536,7 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     if (t < np) { rho = rsum[0][t]; gx = 2.0 * rsum[1][t]; gy = 2.0 * rsum[2][t]; gz = 2.0 * rsum[3][t]; }
     if (kRhoOut) {   // one spin of an unrestricted quadrature: the functional follows in its own kernel
         if (t < np) {
-            double *o = J.rhoOut + 4 * (static_cast<size_t>(blk.pStart) + t);
+            double *o = J.rhoOut + 4 * (static_cast<size_t>(pStart) + t);
             o[0] = rho; o[1] = gx; o[2] = gy; o[3] = gz;
         }
         return;
@@ -552,13 +554,13 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
         if (!skipped) {
             double eps;
             evaluate_functional(fs, rho, gx * gx + gy * gy + gz * gz, eps, vr, vs);
-            const double ww = w[blk.pStart + t];
+            const double ww = w[pStart + t];
             vr *= ww;
             vs *= ww;
             e = eps * rho * ww;
             ne = rho * ww;
         }
-        double *cw = wcoef + 4 * (static_cast<size_t>(blk.pStart) + t);
+        double *cw = wcoef + 4 * (static_cast<size_t>(pStart) + t);
         cw[0] = 0.5 * vr;
         cw[1] = 2.0 * vs * gx;
         cw[2] = 2.0 * vs * gy;
@@ -795,13 +797,22 @@ int build_block_order(xq_grid *g, hipStream_t s) {
         g->xcdStart[x] = b;
     }
     g->xcdStart[8] = nB;
-    g->h_order.resize(static_cast<size_t>(nB > 0 ? nB : 1));
-    for (int i = 0; i < nB; i++) g->h_order[i] = i;
+    std::vector<int> order(static_cast<size_t>(nB > 0 ? nB : 1), 0), firstChunk(static_cast<size_t>(nB > 0 ? nB : 1), 0);
+    for (int i = 0, c = 0; i < nB; i++) {
+        order[i] = i;
+        firstChunk[i] = c;
+        c += (g->blocks[i].nKept + kC - 1) / kC;
+    }
     for (int x = 0; x < 8; x++)
-        std::stable_sort(g->h_order.begin() + g->xcdStart[x], g->h_order.begin() + g->xcdStart[x + 1],
+        std::stable_sort(order.begin() + g->xcdStart[x], order.begin() + g->xcdStart[x + 1],
                          [&](int u, int v) { return cost[u] > cost[v]; });
+    g->h_order.assign(order.size(), make_int4(0, 0, 0, 0));
+    for (int i = 0; i < nB; i++) {
+        const xq::DevBlock &blk = g->blocks[order[i]];
+        g->h_order[i] = make_int4(order[i], firstChunk[order[i]], blk.iOffset, blk.nKept * 256 + blk.nPoints);
+    }
     XQ_TRY(dev_alloc(g, &g->d_order, g->h_order.size()));
-    XQ_CHECK(hipMemcpyAsync(g->d_order, g->h_order.data(), g->h_order.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    XQ_CHECK(hipMemcpyAsync(g->d_order, g->h_order.data(), g->h_order.size() * sizeof(int4), hipMemcpyHostToDevice, s));
     XQ_CHECK(hipStreamSynchronize(s));
     return kOK;
 }
