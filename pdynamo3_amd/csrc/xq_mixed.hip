@@ -80,7 +80,7 @@ constexpr int kChunkBytesB = kDB * kBlockSize * kC;   // one chunk of DB: 5 digi
 constexpr int kChunkGrad = 3 * kBlockSize * kC;     // one chunk of GF, GHU or GLU (elements)
 constexpr int kGradBits = 31;           // GF: |value| < 2^EG is stored as a multiple of 2^(EG - 31)
 constexpr int kGradBitsW = 23;          // GHU, GLU: |value| < 2^EG is stored as a multiple of 2^(EG - 23)
-constexpr int kPanel = 4;               // chunks of P_red columns staged at once by the density kernel
+constexpr int kPanel = 1;               // chunks of P_red columns and of Phi staged at once by the density kernel
 constexpr int kPanelDwords = 8 * kPanel + 4;        // one panel row in LDS (padded against bank conflicts)
 constexpr int kIdxCache = 1024;         // kept-function indices of a block cached in LDS by the density kernel
 constexpr int kSuper = 4;               // blocks per superblock of the Fock product
@@ -367,14 +367,17 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
 
 // Density kernel (per block, 256 threads = 4 waves): X = P_red Phi on int8 MFMA, rho and grad rho, the FP64 functional,
 // E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
-// point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  P_red is staged in LDS by row panels shared by
-// the four waves; the Phi operands are fetched one step ahead, and those of chunk I are kept in LDS for its epilogue.
+// point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  Each step stages one column chunk of P_red's row
+// panel (gathered) and that chunk's Phi digits (16-byte loads of the contiguous DB chunk) in LDS, both fetched by all
+// 256 threads before one barrier.  36 KB of LDS and at most 168 VGPRs let three work-groups share a CU, whose
+// interleaving hides the staging latency better than a one-step register prefetch at two work-groups per CU (3.14
+// against 3.22 ms on (H2O)64).  The epilogue reads chunk I's Phi digits again, with its grad Phi.
 // kRhoOut: the density stage of one spin of an unrestricted quadrature (the job's rhoOut receives rho and grad rho).
 template <bool kRhoOut>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
-    __shared__ v4i sDI[kDB][256];   // per thread: its Phi digits of the current row chunk, for the epilogue
+    __shared__ v4i sB[kPanel * kChunkBytesB / 16];   // the panel's Phi digits, in DB's order
     __shared__ int sPe[2][kC];
     __shared__ int sIdx[kIdxCache];
     __shared__ double rsum[4][kBlockSize];
@@ -424,13 +427,6 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     // fetches columns 8 j + s8 (j = 0 .. 3) of every chunk of the panel (eight lanes read eight consecutive kept
     // functions) and writes them as one word per digit at K positions 4 s8 .. + 3, in row tile_row(sr).
     const int sr = t >> 3, s8 = t & 7, srow = tile_row(sr);
-    auto fetchB = [&](int c, v4i (&bq)[kDB]) {
-#pragma unroll
-        for (int j = 0; j < kDB; j++)
-            bq[j] = *reinterpret_cast<const v4i *>(&DBb[(c * kDB + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
-    };
-    v4i bnext[kDB];
-    fetchB(0, bnext);
     const int eb = EB[static_cast<size_t>(b) * kBlockSize + p];
     double gscale[3];
 #pragma unroll
@@ -456,7 +452,21 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
                     const int k = kC * (c0 + m) + 8 * jj + s8;
                     wd[m][jj] = (m < nc && r < nb && k < nb) ? Pq[rowOff + idx[k]] : 0ull;
                 }
+            // The panel's Phi digits: nc contiguous chunks of DB, 16 bytes per thread and step.
+            constexpr int kPhiPerThread = kPanel * kChunkBytesB / 16 / 256;
+            const v4i *phiSrc = reinterpret_cast<const v4i *>(DBb) + static_cast<size_t>(c0) * (kChunkBytesB / 16);
+            v4i ph[kPhiPerThread];
+#pragma unroll
+            for (int i = 0; i < kPhiPerThread; i++) {
+                const int e = t + 256 * i;
+                if (e < nc * (kChunkBytesB / 16)) ph[i] = phiSrc[e];
+            }
             __syncthreads();   // every wave is done with the previous panel
+#pragma unroll
+            for (int i = 0; i < kPhiPerThread; i++) {
+                const int e = t + 256 * i;
+                if (e < nc * (kChunkBytesB / 16)) sB[e] = ph[i];
+            }
 #pragma unroll
             for (int m = 0; m < kPanel; m++) {
                 if (m < nc) {
@@ -479,13 +489,7 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
                 const int c = c0 + cl;
                 v4i a[kDP], bq[kDB];
 #pragma unroll
-                for (int j = 0; j < kDB; j++) bq[j] = bnext[j];
-                if (c == I) {   // chunk I's digits, kept for the epilogue (each thread reads back its own slot)
-#pragma unroll
-                    for (int j = 0; j < kDB; j++) sDI[j][t] = bq[j];
-                }
-                if (c + 1 < nChunks) fetchB(c + 1, bnext);
-                else if (I + 1 < nChunks) fetchB(0, bnext);
+                for (int j = 0; j < kDB; j++) bq[j] = sB[cl * (kChunkBytesB / 16) + (j * kBlockSize + p) * 2 + lh];
 #pragma unroll
                 for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sP[j][lr][8 * cl + 4 * lh]);
 #pragma unroll
@@ -504,9 +508,10 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
             const int s[kXOrders] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
             x[v] = ldexp(static_cast<double>(combine<kXOrders>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 7 * (kXOrders + 1));
         }
-        v4i dI[kDB];
+        v4i dI[kDB];   // the lane's Phi digits of chunk I, read again with its grad Phi
 #pragma unroll
-        for (int j = 0; j < kDB; j++) dI[j] = sDI[j][t];
+        for (int j = 0; j < kDB; j++)
+            dI[j] = *reinterpret_cast<const v4i *>(&DBb[(I * kDB + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
         double sumR = 0.0, sumG[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int v = 0; v < 16; v++) sumR += static_cast<double>(digits5(dI, v)) * x[v];
