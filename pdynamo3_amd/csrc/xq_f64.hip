@@ -83,17 +83,19 @@ __global__ void __launch_bounds__(256) f64_prepare_kernel(
 // coefficients per point.  P_red is staged in LDS by row panels of kPanelF chunks; the Phi operands of the next chunk
 // are loaded while the current one is multiplied.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-density_kernel_f64(const MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
+density_kernel_f64(const MixedJob *__restrict__ jobs, int nJobs) {
     __shared__ double sPf[kC][kPanelPitch];
     __shared__ int sIdx[kIdxCache];
     __shared__ double rsum[4][kBlockSize];
     __shared__ double wsum[4][2];
-    // Work-group g runs on XCD g % 8; each XCD takes a contiguous range of blocks (one region of the molecule), so that
-    // its L2 holds the P rows those blocks gather.
-    const int wg = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
-    if (wg >= nGroups) return;
-    const MixedJob &J = jobs[job_of(jobs, nJobs, wg, [](const MixedJob &j) { return j.blockBase; })];
-    const int b = wg - J.blockBase;
+    // Work-group g runs on XCD x = g % 8 and takes slot g / 8 of that XCD's list: the jobs' block ranges x in job order
+    // (each a contiguous region of its molecule, so that the XCD's L2 holds the P rows its blocks gather; the ranges
+    // of one job have equal estimated cost), each range's blocks in order of decreasing cost.
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const MixedJob &J = jobs[job_of(jobs, nJobs, slot, [xcd](const MixedJob &j) { return j.xcdBase[xcd]; })];
+    const int local = slot - J.xcdBase[xcd];
+    if (local >= J.xcdStart[xcd + 1] - J.xcdStart[xcd]) return;
+    const int b = J.order[J.xcdStart[xcd] + local].x;
     const DevBlock blk = J.blocks[b];
     const int nb = blk.nKept, np = blk.nPoints, n = J.n;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, l16 = lane & 15, l4 = lane >> 4;
@@ -372,8 +374,8 @@ void free_f64(xq_grid *g) {
 
 int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
                MixedJob *h_table, MixedJob *d_table, int stages, const StepArrays *over) {
-    long long rows, blocks, chunks, groups;
-    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups, over);
+    long long rows, blocks, chunks, groups, blockSlots;
+    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups, over, &blockSlots);
     if (blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
         set_error("batch too large for one launch");
         return kInvalidArgument;
@@ -381,9 +383,8 @@ int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *
     XQ_CHECK(hipMemcpyAsync(d_table, h_table, nJobs * sizeof(MixedJob), hipMemcpyHostToDevice, s));
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
     if ((stages & kStageDensity) && blocks > 0) {
-        const int perXcd = static_cast<int>((blocks + 7) / 8);
-        hipLaunchKernelGGL(density_kernel_f64, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,
-                           perXcd, static_cast<int>(blocks));
+        hipLaunchKernelGGL(density_kernel_f64, dim3(static_cast<unsigned>(8 * blockSlots)), dim3(256), 0, s, d_table,
+                           nJobs);
     }
     XQ_CHECK(hipEventRecord(ev.evRho1, s));
     if ((stages & kStageFock) && chunks > 0) {
