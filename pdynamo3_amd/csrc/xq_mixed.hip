@@ -167,9 +167,8 @@ __device__ __forceinline__ int grad_encode(double g, int e) {
     return static_cast<int>(Q);
 }
 
-// Row of a chunk held at density position 16 h + v (K position 4 q + j holds row 8 j + q), and the position of row k.
+// Row of a chunk held at density position 16 h + v (K position 4 q + j holds row 8 j + q).
 __host__ __device__ constexpr int chunk_fun(int h, int v) { return 8 * (v & 3) + 4 * h + (v >> 2); }
-__host__ __device__ constexpr int chunk_pos(int k) { return 4 * (k & 7) + (k >> 3); }
 // The MFMA row that holds row r of a P_red tile: the output row (v % 4) + 4 h + 8 (v / 4) of lane half h, register v,
 // is row chunk_fun(h, v), the row whose Phi digits the lane holds at position 16 h + v.
 __host__ __device__ constexpr int tile_row(int r) { return 8 * (r & 3) + 4 * ((r >> 2) & 1) + (r >> 3); }
@@ -486,7 +485,6 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
             }
             __syncthreads();
             for (int cl = 0; cl < nc; cl++) {
-                const int c = c0 + cl;
                 v4i a[kDP], bq[kDB];
 #pragma unroll
                 for (int j = 0; j < kDB; j++) bq[j] = sB[cl * (kChunkBytesB / 16) + (j * kBlockSize + p) * 2 + lh];
