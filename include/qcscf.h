@@ -17,6 +17,9 @@
 !                        constraint entry (the caller appends the total charge)
 !   qs_fit_fock       <- Fock_MakeFockFromFitIntegrals (FockConstruction.c:74-96)
 !   qs_fit_estimate_bytes  (no counterpart: the device bytes of qs_fit_create, for admission control)
+!   qs_dense_aufbau   <- QCOrbitals.MakeFromFock with the orthogonaliser (QCOrbitals.py:27-41) and
+!                        SymmetricMatrix.MakeFromEigenSystem of the occupied orbitals (DIISSCFConverger.py:434-446)
+!   qs_dense_commutator <- the DIIS error vector of DIISSCFConverger.DIISCurrentFrame (DIISSCFConverger.py:291-303)
 ! Conventions as include/xcquad.h: bohr, packed lower triangles (i(i+1)/2 + j), the reference's Status codes, host pointers.
 !=================================================================================================================================*/
 # ifndef QCSCF_H
@@ -28,6 +31,7 @@ extern "C" {
 
 typedef struct qs_basis qs_basis ;
 typedef struct qs_fit   qs_fit   ;
+typedef struct qs_dense qs_dense ;
 
 int  qs_basis_create ( int nAtoms, const int *atomicNumbers, const double *xyzBohr, int nSpec, const double *spec,
                        int spherical, qs_basis **basis ) ;
@@ -55,6 +59,19 @@ int  qs_fit_estimate_bytes ( int nAtoms, const int *atomicNumbers, const double 
 /* Rows of stored integrals (function pairs of the kept shell pairs), kept shell pairs, device bytes of the integrals. */
 int  qs_fit_stats    ( const qs_fit *fit, long long *rows, long long *pairs, long long *deviceBytes ) ;
 void qs_fit_free     ( qs_fit *fit ) ;
+
+/* Dense SCF steps on the device (rocBLAS and rocSOLVER on a stream of the handle's own): one handle per SCF holds the
+!  orthogonaliser X (dense n x n, row-major, symmetric: the reference's symmetric orthogonalisation, QCModelBase.py:313-333)
+!  and the packed overlap S.  Every call is synchronous: it returns when its results are on the host.  Calls on one
+!  handle must not overlap.
+!  qs_dense_aufbau: P = 2 C_occ C_occ^T with C = X eigvec(X^T F X), the nOccupied lowest orbitals of the packed Fock
+!  matrix F (packed P OVERWRITTEN; orbitalEnergies, n values ascending, may be NULL).
+!  qs_dense_commutator: e = X^T (F P S - S P F) X for packed F and P (dense n x n, row-major, OVERWRITTEN). */
+int  qs_dense_create     ( int n, const double *orthogonalizer, const double *overlapS, qs_dense **out ) ;
+int  qs_dense_aufbau     ( qs_dense *dense, const double *fockF, int nOccupied, double *densityP,
+                           double *orbitalEnergies ) ;
+int  qs_dense_commutator ( qs_dense *dense, const double *fockF, const double *densityP, double *error ) ;
+void qs_dense_free       ( qs_dense *dense ) ;
 
 # ifdef __cplusplus
 }

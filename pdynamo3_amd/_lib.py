@@ -56,6 +56,10 @@ SCF_SIGNATURES = {
     "qs_fit_estimate_bytes": (ctypes.c_int, [ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                              ctypes.c_int, _llp]),
     "qs_fit_free": (None, [_vp]),
+    "qs_dense_create": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.POINTER(_vp)]),
+    "qs_dense_aufbau": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp]),
+    "qs_dense_commutator": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "qs_dense_free": (None, [_vp]),
 }
 
 

@@ -148,6 +148,49 @@ class GPUIntegrals:
             pass
 
 
+class DeviceDenseSteps:
+    """The SCF's dense steps on the GPU (include/qcscf.h, qs_dense_*: rocBLAS and rocSOLVER): the orbitals and density
+    of a Fock matrix and the DIIS error vector, with the orthogonaliser X and the overlap S resident in device memory."""
+
+    def __init__(self, X, S_packed):
+        self.n = int(X.shape[0])
+        self._h = ctypes.c_void_p()
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        _check(_lib.lib.qs_dense_create(self.n, _lib.dptr(X), _lib.dptr(_lib.f64(S_packed)), ctypes.byref(self._h)),
+               "dense SCF steps")
+
+    def aufbau(self, F, n_occupied):
+        """Packed P = 2 C_occ C_occ^T of the packed Fock matrix F, and the orbital energies."""
+        P, e = np.empty(self.n * (self.n + 1) // 2), np.empty(self.n)
+        _check(_lib.lib.qs_dense_aufbau(self._h, _lib.dptr(_lib.f64(F)), int(n_occupied), _lib.dptr(P), _lib.dptr(e)),
+               "aufbau")
+        return P, e
+
+    def commutator(self, F, P):
+        """X^T (F P S - S P F) X (dense) of packed F and P."""
+        err = np.empty((self.n, self.n))
+        _check(_lib.lib.qs_dense_commutator(self._h, _lib.dptr(_lib.f64(F)), _lib.dptr(_lib.f64(P)), _lib.dptr(err)),
+               "commutator")
+        return err
+
+    def close(self):
+        if self._h:
+            _lib.lib.qs_dense_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# Basis size from which the dense steps run on the GPU by default: below it one host core is as fast (measured on
+# MI355X: eigenproblems of 304 functions 4.6 ms on one core against 6.9 ms on the device; of 1216 functions 126 ms
+# against 29 ms, and a 1216^3 product 29 ms against 0.09 ms).
+DEVICE_DENSE_MIN_FUNCTIONS = 512
+
+
 class RKSDFT:
     """Restricted Kohn-Sham SCF with a fit basis for J; the XC quadrature on the GPU.
 
@@ -157,12 +200,19 @@ class RKSDFT:
     fit_basis        the J fitting basis (dgauss-a1-dftjfit, spherical functions as in its parameter files)
     precision        'fp64' or 'mixed' quadrature
     integrals, grid  injected providers (tests); by default the GPU integrals and a QuadratureGrid
+    dense            where the orbitals, the density and the DIIS error are formed: 'host' (numpy / LAPACK), 'device'
+                     (DeviceDenseSteps), or 'auto': the device from DEVICE_DENSE_MIN_FUNCTIONS functions when the
+                     integrals are on the GPU
     """
 
     def __init__(self, Z, xyz_bohr, basis="6-31g_st", spherical=False, functional="blyp",
                  fit_basis="dgauss-a1-dftjfit", precision="fp64", accuracy=2, charge=0, energy_tolerance=1.0e-6,
                  density_tolerance=1.0e-6, max_iterations=100, diis_onset=0.2, history=10, integrals=None, grid=None,
-                 guess=None):
+                 guess=None, dense="auto"):
+        if dense not in ("auto", "host", "device"):
+            raise ValueError("dense must be 'auto', 'host' or 'device'")
+        self.dense_mode = dense
+        self.dense = None
         self.Z = _lib.i32(Z)
         self.xyz = _lib.f64(xyz_bohr).reshape(-1, 3)
         self.functional = functional
@@ -203,6 +253,9 @@ class RKSDFT:
         if s[0] <= 0.0:
             raise QCModelError("The overlap matrix is not positive definite.")
         self.X = (U / np.sqrt(s)) @ U.T            # S^-1/2 (symmetric orthogonalisation)
+        if self.dense_mode == "device" or (self.dense_mode == "auto" and self.n >= DEVICE_DENSE_MIN_FUNCTIONS and
+                                           isinstance(self.integrals, GPUIntegrals)):
+            self.dense = DeviceDenseSteps(self.X, S)
         M = unpack(self.integrals.fit_matrix(), self.integrals.n_fit + 1)
         self.fit_lu = scipy.linalg.lu_factor(M)
         self.e_nuclear = nuclear_repulsion(self.Z, self.xyz)
@@ -215,6 +268,7 @@ class RKSDFT:
         self.converged = False
         self.diis = False
         self.history_F, self.history_e = [], []
+        self.diis_B = np.zeros((0, 0))             # inner products of the stored error vectors
         self.energies, self.records = [], []
         self.oda_F = self.oda_P = self.oda_E = None
         self.last_P = None
@@ -247,15 +301,20 @@ class RKSDFT:
     def _aufbau(self, F):
         """The density of the n_occupied lowest orbitals of F (in the orthonormal basis X)."""
         t = time.perf_counter()
-        Fo = self.X.T @ unpack(F, self.n) @ self.X
-        e, C = np.linalg.eigh(Fo)
-        Cocc = self.X @ C[:, :self.n_occupied]
-        self.orbital_energies = e
-        P = pack(2.0 * Cocc @ Cocc.T)
+        if self.dense is not None:
+            P, self.orbital_energies = self.dense.aufbau(F, self.n_occupied)
+        else:
+            Fo = self.X.T @ unpack(F, self.n) @ self.X
+            e, C = np.linalg.eigh(Fo)
+            Cocc = self.X @ C[:, :self.n_occupied]
+            self.orbital_energies = e
+            P = pack(2.0 * Cocc @ Cocc.T)
         self.timings["diagonalisation_s"] += time.perf_counter() - t
         return P
 
     def commutator_error(self, F, P):
+        if self.dense is not None:
+            return self.dense.commutator(F, P)
         Fd, Pd = unpack(F, self.n), unpack(P, self.n)
         FPS = Fd @ Pd @ self.S
         return self.X.T @ (FPS - FPS.T) @ self.X
@@ -285,10 +344,20 @@ class RKSDFT:
         if not self.diis and err_max < self.diis_onset:
             self.diis = True
         if self.diis:
+            t = time.perf_counter()
+            # The new error vector's inner products with the stored ones extend the DIIS matrix by one row.
+            flat = err.ravel()
+            row = np.array([float(np.dot(flat, e.ravel())) for e in self.history_e] + [float(np.dot(flat, flat))])
+            m = len(self.history_e)
+            B = np.zeros((m + 1, m + 1))
+            B[:m, :m] = self.diis_B
+            B[m, :] = B[:, m] = row
             self.history_F.append(F)
             self.history_e.append(err)
-            self.history_F = self.history_F[-self.history:]
-            self.history_e = self.history_e[-self.history:]
+            if len(self.history_e) > self.history:
+                self.history_F, self.history_e, B = self.history_F[1:], self.history_e[1:], B[1:, 1:]
+            self.diis_B = B
+            self.timings["diis_s"] += time.perf_counter() - t
             t = time.perf_counter()
             F_next = self._diis_extrapolate()
             self.timings["diis_s"] += time.perf_counter() - t
@@ -320,9 +389,7 @@ class RKSDFT:
         if m == 1:
             return self.history_F[0]
         B = np.zeros((m + 1, m + 1))
-        for i in range(m):
-            for j in range(i + 1):
-                B[i, j] = B[j, i] = float(np.sum(self.history_e[i] * self.history_e[j]))
+        B[:m, :m] = self.diis_B
         B[m, :m] = B[:m, m] = -1.0
         rhs = np.zeros(m + 1)
         rhs[m] = -1.0
@@ -359,6 +426,9 @@ class RKSDFT:
                 "timings": dict(self.timings)}
 
     def close(self):
+        if self.dense is not None:
+            self.dense.close()
+            self.dense = None
         if isinstance(self.integrals, GPUIntegrals):
             self.integrals.close()
         if isinstance(self.grid, QuadratureGrid):

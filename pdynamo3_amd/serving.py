@@ -31,6 +31,7 @@ import numpy as np
 
 from . import basis as _basis
 from . import densities, integrator, molecules
+from . import scf as scf_module
 from .definitions import functional_ids
 
 FIT_BASIS = "dgauss-a1-dftjfit"
@@ -156,6 +157,9 @@ def estimated_peak_bytes(request, precision, scf="rks", spherical=False):
         b = integrator.estimate_bytes(points, request.n_functions, float(min(request.n_functions, 256)), precision)
         if scf == "rks":
             b += fit_integral_bytes(request.Z, request.xyz_bohr, request.spec, spherical)
+            n = request.n_functions
+            if n >= scf_module.DEVICE_DENSE_MIN_FUNCTIONS:   # the SCF's dense steps on the device: 6 n^2 + n^2 / 2 doubles
+                b += 8 * (6 * n * n + n * (n + 1) // 2 + 2 * n)
         request.peak_bytes[key] = b
     return request.peak_bytes[key]
 

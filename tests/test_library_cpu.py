@@ -62,6 +62,11 @@ def test_argument_errors_return_status_codes():
     assert _lib.lib.xq_gradients(None, 2, _lib.iptr(ids), _lib.dptr(x), _lib.dptr(x)) == 3
     assert _lib.lib.xq_gradients_uks(None, 2, _lib.iptr(ids), _lib.dptr(x), _lib.dptr(x), _lib.dptr(x)) == 3
     assert "invalid" in _lib.last_error()
+    # The SCF's dense steps refuse an empty basis or a missing handle before any device work.
+    h = ctypes.c_void_p()
+    assert _lib.lib.qs_dense_create(0, _lib.dptr(x), _lib.dptr(x), ctypes.byref(h)) == 3 and not h
+    assert _lib.lib.qs_dense_aufbau(None, _lib.dptr(x), 1, _lib.dptr(x), None) == 3
+    assert _lib.lib.qs_dense_commutator(None, _lib.dptr(x), _lib.dptr(x), _lib.dptr(x)) == 3
 
 
 def test_memory_estimate_scales():
