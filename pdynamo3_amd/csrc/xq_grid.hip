@@ -421,7 +421,7 @@ __device__ __forceinline__ double excluded_product(double prodNZ, int zeros, dou
 // bytes, after the doubles of all WPG waves).  WPG waves per work-group (4, or 2 for molecules whose per-wave LDS would
 // not fit four times in 64 KB); the waves share no LDS.
 template <int LA, int WPG>
-__global__ void __launch_bounds__(64 * WPG) weight_gradient_kernel(int nPoints, int pointsPerGroup,
+__global__ void __launch_bounds__(64 * WPG) weight_gradient_kernel(int nPoints,
                                                               const double *__restrict__ px, const double *__restrict__ py,
                                                               const double *__restrict__ pz, const double *__restrict__ pw,
                                                               const int *__restrict__ pointAtom,
@@ -437,11 +437,11 @@ __global__ void __launch_bounds__(64 * WPG) weight_gradient_kernel(int nPoints, 
     double G[LA][3];
 #pragma unroll
     for (int a = 0; a < LA; a++) G[a][0] = G[a][1] = G[a][2] = 0.0;
-    const int p0 = blockIdx.x * pointsPerGroup;
-    const int p1 = p0 + pointsPerGroup < nPoints ? p0 + pointsPerGroup : nPoints;
-    for (int base = p0; base < p1; base += WPG) {
+    // Work-group g takes the rounds g, g + gridDim.x, ... of WPG consecutive points (a point's cost depends on the atoms
+    // whose cells matter there, so contiguous ranges would differ in cost).
+    for (int base = blockIdx.x * WPG; base < nPoints; base += gridDim.x * WPG) {
         const int p = base + wv;
-        const bool active = p < p1 && eXC[p] != 0.0;
+        const bool active = p < nPoints && eXC[p] != 0.0;
         const double rx = active ? px[p] : 0.0, ry = active ? py[p] : 0.0, rz = active ? pz[p] : 0.0;
         if (active)
             for (int i = lane; i < nAtoms; i += 64) {
@@ -943,12 +943,9 @@ int weight_gradients(xq_grid *g, const double *d_eXC, double *d_grad, hipStream_
                             "copy xyz")) != kOK) break;
         if ((status = check(hipMemcpyAsync(d_pointAtom, pointAtom.data(), pointAtom.size() * sizeof(int),
                                            hipMemcpyHostToDevice, s), "copy point atoms")) != kOK) break;
-        const int groups = std::min(2048 * 4 / wpg, (G + 63) / 64);
-        int perGroup = (G + groups - 1) / groups;
-        perGroup = (perGroup + wpg - 1) / wpg * wpg;
-        const int nGroups = (G + perGroup - 1) / perGroup;
+        const int nGroups = std::min(2048 * 4 / wpg, (G + 63) / 64);
         auto launch = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, dim3(nGroups), dim3(64 * wpg), lds, s, G, perGroup, g->d_x, g->d_y, g->d_z,
+            hipLaunchKernelGGL(kernel, dim3(nGroups), dim3(64 * wpg), lds, s, G, g->d_x, g->d_y, g->d_z,
                                g->d_w, d_pointAtom, d_eXC, d_xyz, nA, d_invR, d_aij, d_near, nNear, d_grad);
         };
         // Atoms per lane: the smallest instance that covers nAtoms (fewer registers for small systems).
