@@ -120,9 +120,14 @@ __device__ __forceinline__ void reduced_product(const double *__restrict__ P, in
 //   2 vsigma_ab grad rho_t (DFTIntegrator.c:405-487: _Gradients per spin and the two _GradientsSigma cross calls);
 //   U_s = u0 Phi + c . grad Phi, so that P_s U_s = 2 vrho X_s + P_s (grad Phi . c);
 //   pass 2: Z = P_s U_s and per row g_i = sum_p [Z grad phi + X_s (Hess phi . c)]_ip.
+// The blocks of one batch launch in launch order: XCD x takes order[start[x] .. start[x + 1]) (absolute block indices).
+struct XcdRanges {
+    int start[9];
+};
+
 template <int kSpins>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grad_direct_kernel(
-    int b0, int nGroups, int groupsPerXcd, long long rowBase, const DevBlock *__restrict__ blocks,
+    const int *__restrict__ order, XcdRanges xr, long long rowBase, const DevBlock *__restrict__ blocks,
     const int *__restrict__ idxAll, const int *__restrict__ atomOf, int n, const double *__restrict__ P0,
     const double *__restrict__ P1, const double *__restrict__ w, FunctionalSet fs, double rhoTol, Planes10 B,
     double *__restrict__ Xp0, double *__restrict__ Xp1, double *__restrict__ Up, double *__restrict__ eXC,
@@ -133,9 +138,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
     __shared__ double sCoef[4 * kSpins][kBlockSize];
     __shared__ double sRed[4][kC][3];
     __shared__ double smax[4];
-    const int wg = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
-    if (wg >= nGroups) return;
-    const int b = b0 + wg;
+    // Work-group g runs on XCD g % 8 and takes slot g / 8 of that XCD's range of the batch.
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    if (slot >= xr.start[xcd + 1] - xr.start[xcd]) return;
+    const int b = order[xr.start[xcd] + slot];
     const DevBlock blk = blocks[b];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, l16 = lane & 15, l4 = lane >> 4;
     const int nb = blk.nKept, np = blk.nPoints;
@@ -150,7 +156,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
     const size_t R0 = static_cast<size_t>(blk.vOffset - rowBase) * kBlockSize;
     const double *F = B.p[0] + R0, *Fx = B.p[1] + R0, *Fy = B.p[2] + R0, *Fz = B.p[3] + R0;
     double *U = Up + R0;
-    double *grad = gradCopies + static_cast<size_t>(wg % kGradCopies) * 3 * nAtoms;
+    double *grad = gradCopies + static_cast<size_t>(b % kGradCopies) * 3 * nAtoms;
     const double *Ps[2] = {P0, P1};
     double *Xs[2] = {Xp0 + R0, kSpins == 2 ? Xp1 + R0 : nullptr};
 
@@ -351,6 +357,33 @@ __global__ void __launch_bounds__(256) sum_copies_kernel(int nEntries, const dou
 
 namespace {
 
+// The launch order of the batch b0 .. b1 - 1, written to order[b0 .. b1): as the density launch's (xq_mixed.hip,
+// build_block_order), 8 contiguous ranges of equal estimated cost (T^2 + T + 1 for a block of T row chunks), one per
+// XCD, each sorted by decreasing cost, so that the batch's longest blocks do not start last.
+XcdRanges batch_order(const xq_grid *g, int b0, int b1, std::vector<int> &order) {
+    std::vector<double> cost(static_cast<size_t>(b1 - b0));
+    double total = 0.0;
+    for (int b = b0; b < b1; b++) {
+        const double T = static_cast<double>((g->blocks[b].nKept + kC - 1) / kC);
+        cost[b - b0] = T * T + T + 1.0;
+        total += cost[b - b0];
+        order[b] = b;
+    }
+    XcdRanges xr;
+    xr.start[0] = b0;
+    int b = b0;
+    double acc = 0.0;
+    for (int x = 1; x < 8; x++) {
+        while (b < b1 && acc + 0.5 * cost[b - b0] < total * x / 8.0) acc += cost[(b++) - b0];
+        xr.start[x] = b;
+    }
+    xr.start[8] = b1;
+    for (int x = 0; x < 8; x++)
+        std::stable_sort(order.begin() + xr.start[x], order.begin() + xr.start[x + 1],
+                         [&](int u, int v) { return cost[u - b0] > cost[v - b0]; });
+    return xr;
+}
+
 // The direct and weight terms for one (kSpins = 1) or two (alpha, beta) dense densities already on the device.
 int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const double *d_P0, const double *d_P1,
                         double *grad) {
@@ -368,7 +401,19 @@ int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const do
     const int nPlanes = 11 + spins;
     double *planes[13] = {};
     double *d_grad = nullptr, *d_eXC = nullptr, *d_copies = nullptr;
-    int *d_atomOf = nullptr;
+    int *d_atomOf = nullptr, *d_order = nullptr;
+    // Batches of blocks whose kept rows fit the planes, and their launch orders.
+    std::vector<std::pair<int, int>> batches;
+    for (int b0 = 0; b0 < nB;) {
+        const long long rowBase = g->blocks[b0].vOffset;
+        int b1 = b0;
+        while (b1 < nB && g->blocks[b1].vOffset + g->blocks[b1].nKept - rowBase <= batchRows) b1++;
+        batches.emplace_back(b0, b1);
+        b0 = b1;
+    }
+    std::vector<int> order(static_cast<size_t>(nB > 0 ? nB : 1), 0);
+    std::vector<XcdRanges> ranges;
+    for (const auto &bt : batches) ranges.push_back(batch_order(g, bt.first, bt.second, order));
     int status = kOK;
     do {
         bool ok = true;
@@ -388,25 +433,28 @@ int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const do
                                            hipMemcpyHostToDevice, s), "zero gradient copies")) != kOK) break;
         if ((status = check(hipMemcpyAsync(d_atomOf, atomOf.data(), atomOf.size() * sizeof(int), hipMemcpyHostToDevice, s),
                             "copy function atoms")) != kOK) break;
+        if ((status = dev_alloc(g, &d_order, order.size())) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                            "copy block order")) != kOK) break;
         Planes10 B;
         for (int k = 0; k < 10; k++) B.p[k] = planes[k];
         double *Xa = planes[10], *Xb = spins == 2 ? planes[11] : nullptr, *U = planes[nPlanes - 1];
-        for (int b0 = 0; b0 < nB && status == kOK;) {
+        for (size_t k = 0; k < batches.size() && status == kOK; k++) {
+            const int b0 = batches[k].first, b1 = batches[k].second;
             const long long rowBase = g->blocks[b0].vOffset;
-            int b1 = b0;
-            while (b1 < nB && g->blocks[b1].vOffset + g->blocks[b1].nKept - rowBase <= batchRows) b1++;
             if ((status = eval_basis2(g, b0, b1, rowBase, B, s)) != kOK) break;
-            const int nGroups = b1 - b0, perXcd = (nGroups + 7) / 8;
+            const XcdRanges &xr = ranges[k];
+            int perXcd = 0;
+            for (int x = 0; x < 8; x++) perXcd = std::max(perXcd, xr.start[x + 1] - xr.start[x]);
             if (spins == 1)
-                hipLaunchKernelGGL(grad_direct_kernel<1>, dim3(8 * perXcd), dim3(256), 0, s, b0, nGroups, perXcd, rowBase,
+                hipLaunchKernelGGL(grad_direct_kernel<1>, dim3(8 * perXcd), dim3(256), 0, s, d_order, xr, rowBase,
                                    g->d_blocks, g->d_idx, d_atomOf, g->nFunctions, d_P0, d_P0, g->d_w, fs, g->rhoTol, B,
                                    Xa, Xa, U, d_eXC, d_copies, nA);
             else
-                hipLaunchKernelGGL(grad_direct_kernel<2>, dim3(8 * perXcd), dim3(256), 0, s, b0, nGroups, perXcd, rowBase,
+                hipLaunchKernelGGL(grad_direct_kernel<2>, dim3(8 * perXcd), dim3(256), 0, s, d_order, xr, rowBase,
                                    g->d_blocks, g->d_idx, d_atomOf, g->nFunctions, d_P0, d_P1, g->d_w, fs, g->rhoTol, B,
                                    Xa, Xb, U, d_eXC, d_copies, nA);
             status = check(hipGetLastError(), "grad_direct_kernel");
-            b0 = b1;
         }
         if (status != kOK) break;
         const int nEntries = static_cast<int>(zeros.size());
@@ -425,6 +473,7 @@ int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const do
     dev_free(g, &d_copies);
     dev_free(g, &d_eXC);
     dev_free(g, &d_atomOf);
+    dev_free(g, &d_order);
     return status;
 }
 
