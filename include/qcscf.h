@@ -62,8 +62,9 @@ void qs_fit_free     ( qs_fit *fit ) ;
 
 /* Dense SCF steps on the device (rocBLAS and rocSOLVER on a stream of the handle's own): one handle per SCF holds the
 !  orthogonaliser X (dense n x n, row-major, symmetric: the reference's symmetric orthogonalisation, QCModelBase.py:313-333)
-!  and the packed overlap S.  Every call is synchronous: it returns when its results are on the host.  Calls on one
-!  handle must not overlap.
+!  and the packed overlap S, on the device current at qs_dense_create, which must be current at every call (status 3
+!  otherwise).  Every call is synchronous: it returns when its results are on the host.  Calls on one handle must not
+!  overlap.
 !  qs_dense_aufbau: P = 2 C_occ C_occ^T with C = X eigvec(X^T F X), the nOccupied lowest orbitals of the packed Fock
 !  matrix F (packed P OVERWRITTEN; orbitalEnergies, n values ascending, may be NULL).
 !  qs_dense_commutator: e = X^T (F P S - S P F) X for packed F and P (dense n x n, row-major, OVERWRITTEN). */
