@@ -230,7 +230,12 @@ QS_HD void fit_integrals_item(long long k, int nFitShells, const PairEntry *__re
     double blk[216], tmp[6];
     for (int i = 0; i < 216; i++) blk[i] = 0.0;
     double Ex[(kMaxL + 1) * kEJ * kET], Ey[(kMaxL + 1) * kEJ * kET], Ez[(kMaxL + 1) * kEJ * kET];
-    double G[3][9], R[kRD * kRD * kRD], work[2 * kRD * kRD * kRD];
+    double G[9], R[kRD * kRD * kRD], work[2 * kRD * kRD * kRD];
+    // Cartesian powers of the three shells' components; their normalisation factors multiply the finished block.
+    int pa[6][3], pb[6][3], pc[6][3];
+    for (int i = 0; i < na; i++) cart_powers(A.l, i, pa[i][0], pa[i][1], pa[i][2]);
+    for (int i = 0; i < nb; i++) cart_powers(Bs.l, i, pb[i][0], pb[i][1], pb[i][2]);
+    for (int i = 0; i < nc; i++) cart_powers(C.l, i, pc[i][0], pc[i][1], pc[i][2]);
     const double ABx = A.x - Bs.x, ABy = A.y - Bs.y, ABz = A.z - Bs.z;
     for (int qa = 0; qa < A.nPrim; qa++)
         for (int qb = 0; qb < Bs.nPrim; qb++) {
@@ -243,12 +248,12 @@ QS_HD void fit_integrals_item(long long k, int nFitShells, const PairEntry *__re
             for (int qc = 0; qc < C.nPrim; qc++) {
                 const double g = fExps[C.prim + qc];
                 const double alpha = p * g / (p + g);
-                const double pref = 2.0 * pow(M_PI, 2.5) / (p * g * sqrt(p + g)) * cab * fCoefs[C.prim + qc];
-                for (int dim = 0; dim < 3; dim++) hermite_single(C.l, g, G[dim]);
+                // 2 pi^(5/2) / (p g sqrt(p + g)); pi^(5/2) = 17.493418327624862.
+                const double pref = 2.0 * 17.493418327624862 / (p * g * sqrt(p + g)) * cab * fCoefs[C.prim + qc];
+                hermite_single(C.l, g, G);   // the same in x, y and z
                 hermite_r(A.l + Bs.l + C.l, alpha, Px - C.x, Py - C.y, Pz - C.z, boysTable, R, work);
                 for (int ic = 0; ic < nc; ic++) {
-                    int xc, yc, zc;
-                    cart_powers(C.l, ic, xc, yc, zc);
+                    const int xc = pc[ic][0], yc = pc[ic][1], zc = pc[ic][2];
                     // W_tuv = sum_t'u'v' (-1)^(t'+u'+v') E^c_t'u'v' R_t+t',u+u',v+v' for t + u + v <= la + lb.
                     double W[5 * 5 * 5];
                     const int Lab = A.l + Bs.l;
@@ -259,30 +264,32 @@ QS_HD void fit_integrals_item(long long k, int nFitShells, const PairEntry *__re
                                 for (int t2 = 0; t2 <= xc; t2++)
                                     for (int u2 = 0; u2 <= yc; u2++)
                                         for (int v2 = 0; v2 <= zc; v2++) {
-                                            const double e = G[0][xc * 3 + t2] * G[1][yc * 3 + u2] * G[2][zc * 3 + v2];
+                                            const double e = G[xc * 3 + t2] * G[yc * 3 + u2] * G[zc * 3 + v2];
                                             const double sg = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
                                             acc += sg * e * R[ridx(t + t2, u + u2, v + v2)];
                                         }
                                 W[(t * 5 + u) * 5 + v] = acc;
                             }
-                    const double fc = pref * cart_factor(C.l, ic);
                     for (int ia = 0; ia < na; ia++) {
-                        int xa, ya, za;
-                        cart_powers(A.l, ia, xa, ya, za);
+                        const int xa = pa[ia][0], ya = pa[ia][1], za = pa[ia][2];
                         for (int ib = 0; ib < nb; ib++) {
-                            int xb, yb, zb;
-                            cart_powers(Bs.l, ib, xb, yb, zb);
+                            const int xb = pb[ib][0], yb = pb[ib][1], zb = pb[ib][2];
                             double acc = 0.0;
                             for (int t = 0; t <= xa + xb; t++)
                                 for (int u = 0; u <= ya + yb; u++)
                                     for (int v = 0; v <= za + zb; v++)
                                         acc += Ex[(xa * kEJ + xb) * kET + t] * Ey[(ya * kEJ + yb) * kET + u] *
                                                Ez[(za * kEJ + zb) * kET + v] * W[(t * 5 + u) * 5 + v];
-                            blk[(ia * nb + ib) * nc + ic] += fc * cart_factor(A.l, ia) * cart_factor(Bs.l, ib) * acc;
+                            blk[(ia * nb + ib) * nc + ic] += pref * acc;
                         }
                     }
                 }
             }
+        }
+    for (int ia = 0; ia < na; ia++)
+        for (int ib = 0; ib < nb; ib++) {
+            const double fab = cart_factor(A.l, ia) * cart_factor(Bs.l, ib);
+            for (int ic = 0; ic < nc; ic++) blk[(ia * nb + ib) * nc + ic] *= fab * cart_factor(C.l, ic);
         }
     int ma = na, mb = nb, mc = nc;
     if (oSpherical) {
