@@ -66,7 +66,7 @@ QS_HD void one_electron_pair(int k, const QPair *__restrict__ pairs, const QShel
     double s[36], t[36], v[36], tmp[6];
     for (int i = 0; i < 36; i++) { s[i] = 0.0; t[i] = 0.0; v[i] = 0.0; }
     double Ex[(kMaxL + 1) * kEJ * kET], Ey[(kMaxL + 1) * kEJ * kET], Ez[(kMaxL + 1) * kEJ * kET];
-    double R[kRD * kRD * kRD], work[2 * kRD * kRD * kRD];
+    double R[kRD * kRD * kRD];
     const double ABx = A.x - B.x, ABy = A.y - B.y, ABz = A.z - B.z;
     for (int qa = 0; qa < A.nPrim; qa++)
         for (int qb = 0; qb < B.nPrim; qb++) {
@@ -100,7 +100,7 @@ QS_HD void one_electron_pair(int k, const QPair *__restrict__ pairs, const QShel
             const double zN = kNuclearExponent, aN = p * zN / (p + zN), fN = 2.0 * M_PI / p * sqrt(zN / (p + zN));
             for (int C = 0; C < nAtoms; C++) {
                 const double Zc = atoms[4 * C + 3];
-                hermite_r(L, aN, Px - atoms[4 * C], Py - atoms[4 * C + 1], Pz - atoms[4 * C + 2], boysTable, R, work);
+                hermite_r(L, aN, Px - atoms[4 * C], Py - atoms[4 * C + 1], Pz - atoms[4 * C + 2], boysTable, R);
                 const double f = -Zc * fN * cab;
                 for (int ia = 0; ia < na; ia++) {
                     int xa, ya, za;
@@ -147,7 +147,7 @@ QS_HD void fit_matrix_pair(int k, const QPair *__restrict__ pairs, const QShell 
                            const double *__restrict__ boysTable, const double *__restrict__ c2s, double *__restrict__ M) {
     const QShell C = shells[pairs[k].x], D = shells[pairs[k].y];
     const int nc = ncart(C.l), nd = ncart(D.l);
-    double blk[36], tmp[6], Gc[3][9], Gd[3][9], R[kRD * kRD * kRD], work[2 * kRD * kRD * kRD];
+    double blk[36], tmp[6], Gc[3][9], Gd[3][9], R[kRD * kRD * kRD];
     for (int i = 0; i < 36; i++) blk[i] = 0.0;
     for (int qc = 0; qc < C.nPrim; qc++)
         for (int qd = 0; qd < D.nPrim; qd++) {
@@ -155,7 +155,7 @@ QS_HD void fit_matrix_pair(int k, const QPair *__restrict__ pairs, const QShell 
             const double alpha = g * d / (g + d);
             const double pref = 2.0 * pow(M_PI, 2.5) / (g * d * sqrt(g + d)) * coefs[C.prim + qc] * coefs[D.prim + qd];
             for (int dim = 0; dim < 3; dim++) { hermite_single(C.l, g, Gc[dim]); hermite_single(D.l, d, Gd[dim]); }
-            hermite_r(C.l + D.l, alpha, C.x - D.x, C.y - D.y, C.z - D.z, boysTable, R, work);
+            hermite_r(C.l + D.l, alpha, C.x - D.x, C.y - D.y, C.z - D.z, boysTable, R);
             for (int ic = 0; ic < nc; ic++) {
                 int xc, yc, zc;
                 cart_powers(C.l, ic, xc, yc, zc);
@@ -230,7 +230,7 @@ QS_HD void fit_integrals_item(long long k, int nFitShells, const PairEntry *__re
     double blk[216], tmp[6];
     for (int i = 0; i < 216; i++) blk[i] = 0.0;
     double Ex[(kMaxL + 1) * kEJ * kET], Ey[(kMaxL + 1) * kEJ * kET], Ez[(kMaxL + 1) * kEJ * kET];
-    double G[9], R[kRD * kRD * kRD], work[2 * kRD * kRD * kRD];
+    double G[9], R[kRD * kRD * kRD];
     // Cartesian powers of the three shells' components; their normalisation factors multiply the finished block.
     int pa[6][3], pb[6][3], pc[6][3];
     for (int i = 0; i < na; i++) cart_powers(A.l, i, pa[i][0], pa[i][1], pa[i][2]);
@@ -251,7 +251,7 @@ QS_HD void fit_integrals_item(long long k, int nFitShells, const PairEntry *__re
                 // 2 pi^(5/2) / (p g sqrt(p + g)); pi^(5/2) = 17.493418327624862.
                 const double pref = 2.0 * 17.493418327624862 / (p * g * sqrt(p + g)) * cab * fCoefs[C.prim + qc];
                 hermite_single(C.l, g, G);   // the same in x, y and z
-                hermite_r(A.l + Bs.l + C.l, alpha, Px - C.x, Py - C.y, Pz - C.z, boysTable, R, work);
+                hermite_r(A.l + Bs.l + C.l, alpha, Px - C.x, Py - C.y, Pz - C.z, boysTable, R);
                 for (int ic = 0; ic < nc; ic++) {
                     const int xc = pc[ic][0], yc = pc[ic][1], zc = pc[ic][2];
                     // W_tuv = sum_t'u'v' (-1)^(t'+u'+v') E^c_t'u'v' R_t+t',u+u',v+v' for t + u + v <= la + lb.

@@ -114,38 +114,32 @@ QS_HD void hermite_single(int l, double g, double *G) {
         }
 }
 
-// Hermite integrals R_tuv = R^0_tuv for t + u + v <= L (L <= kMaxR), R[(t * 7 + u) * 7 + v]; work holds 2 x 343 values.
+// Hermite integrals R_tuv = R^0_tuv for t + u + v <= L (L <= kMaxR), R[(t * 7 + u) * 7 + v].  The levels R^n are formed
+// in place from n = L down to 0: the entries of total order s at level n take orders s - 1 and s - 2 of level n + 1,
+// which still hold level n + 1 while s runs downwards.
 constexpr int kRD = kMaxR + 1;
 QS_HD int ridx(int t, int u, int v) { return (t * kRD + u) * kRD + v; }
-QS_HD void hermite_r(int L, double alpha, double X, double Y, double Z, const double *__restrict__ table, double *R,
-                     double *work) {
+QS_HD void hermite_r(int L, double alpha, double X, double Y, double Z, const double *__restrict__ table, double *R) {
     double F[kBoysOrders];
     boys(L, alpha * (X * X + Y * Y + Z * Z), table, F);
-    double *cur = work, *nxt = work + kRD * kRD * kRD;
     // Level n = L: only R^L_000.
     double fac = 1.0;
     for (int n = 0; n < L; n++) fac *= -2.0 * alpha;
-    cur[0] = fac * F[L];
+    R[0] = fac * F[L];
     for (int n = L - 1; n >= 0; n--) {
-        // Level n from level n + 1 (which holds t + u + v <= L - n - 1).
         fac /= -2.0 * alpha;
-        const int top = L - n;
-        nxt[0] = fac * F[n];
-        for (int s = 1; s <= top; s++)
+        for (int s = L - n; s >= 1; s--)
             for (int t = 0; t <= s; t++)
                 for (int u = 0; u <= s - t; u++) {
                     const int v = s - t - u;
                     double r;
-                    if (t > 0) r = (t > 1 ? (t - 1) * cur[ridx(t - 2, u, v)] : 0.0) + X * cur[ridx(t - 1, u, v)];
-                    else if (u > 0) r = (u > 1 ? (u - 1) * cur[ridx(t, u - 2, v)] : 0.0) + Y * cur[ridx(t, u - 1, v)];
-                    else r = (v > 1 ? (v - 1) * cur[ridx(t, u, v - 2)] : 0.0) + Z * cur[ridx(t, u, v - 1)];
-                    nxt[ridx(t, u, v)] = r;
+                    if (t > 0) r = (t > 1 ? (t - 1) * R[ridx(t - 2, u, v)] : 0.0) + X * R[ridx(t - 1, u, v)];
+                    else if (u > 0) r = (u > 1 ? (u - 1) * R[ridx(t, u - 2, v)] : 0.0) + Y * R[ridx(t, u - 1, v)];
+                    else r = (v > 1 ? (v - 1) * R[ridx(t, u, v - 2)] : 0.0) + Z * R[ridx(t, u, v - 1)];
+                    R[ridx(t, u, v)] = r;
                 }
-        double *sw = cur; cur = nxt; nxt = sw;
+        R[0] = fac * F[n];
     }
-    for (int s = 0; s <= L; s++)
-        for (int t = 0; t <= s; t++)
-            for (int u = 0; u <= s - t; u++) R[ridx(t, u, s - t - u)] = cur[ridx(t, u, s - t - u)];
 }
 
 }  // namespace qs
