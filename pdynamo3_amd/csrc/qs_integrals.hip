@@ -206,14 +206,14 @@ void qs_release(T **p) {
     if (*p != nullptr) { (void)xq::cache_free(*p); *p = nullptr; }
 }
 
-// The integral kernels hold their McMurchie-Davidson tables in private memory (about 14 KB per lane for the fit
+// The integral kernels hold their McMurchie-Davidson tables in private memory (about 8 KB per lane for the fit
 // integrals), and the runtime gives every hardware queue that runs one its own scratch area sized for the whole
 // device.  Two such kernels in flight at once on two queues (set-up threads of a request loop) made the second queue's
 // scratch allocation fail (HSA_STATUS_ERROR_OUT_OF_RESOURCES, queue abort), so at most one of them runs at a time in
 // the process: each holds this lock from its launch until its stream has drained.  The same allocation fails when the
 // device is nearly full (a request loop's caching allocator holds idle blocks), so each launch first makes sure that
-// kScratchHeadroom bytes are free: the fit-integral kernel's 13,760 bytes per lane at 3 waves per SIMD on 256 CUs take
-// about 2.7 GB.
+// kScratchHeadroom bytes are free: the fit-integral kernel's 8,368 bytes per lane at 3 waves per SIMD on 256 CUs take
+// about 1.6 GB.
 std::mutex g_integralKernels;
 constexpr size_t kScratchHeadroom = static_cast<size_t>(6) << 30;
 // The runtime's scratch areas belong to hardware queues and come from a pool of fixed size, separate from the free
@@ -387,6 +387,12 @@ int qs_fit_create(const qs_basis *orbital, const qs_basis *fitBasis, qs_fit **ou
     const size_t nOrbPacked = static_cast<size_t>(F->nOrbital) * (F->nOrbital + 1) / 2;
     const long long nChunks = (rows + kRowChunk - 1) / kRowChunk;
     PairEntry *d_pairs = nullptr;
+    // The fit shells in order of angular momentum for the integral launch: the consecutive work items of a wave take
+    // one shell pair and consecutive fit shells, which then share their loop bounds (each shell writes its own
+    // functions, f0 .. f0 + nf - 1, wherever it sits in this list).
+    std::vector<QShell> fitSorted(fitBasis->shells);
+    std::stable_sort(fitSorted.begin(), fitSorted.end(), [](const QShell &u, const QShell &v) { return u.l < v.l; });
+    QShell *d_fitSorted = nullptr;
     if (status == kOK) status = qs_alloc(&F->d_B, static_cast<size_t>(rows) * F->nFit);
     if (status == kOK) status = qs_alloc(&F->d_rowPacked, rowPacked.size());
     if (status == kOK) status = qs_alloc(&F->d_rowDiag, rowDiag.size());
@@ -396,6 +402,10 @@ int qs_fit_create(const qs_basis *orbital, const qs_basis *fitBasis, qs_fit **ou
     if (status == kOK) status = qs_alloc(&F->d_c, F->nFit);
     if (status == kOK) status = qs_alloc(&F->d_partial, static_cast<size_t>(nChunks > 0 ? nChunks : 1) * F->nFit);
     if (status == kOK) status = qs_alloc(&d_pairs, pairs.size());
+    if (status == kOK) status = qs_alloc(&d_fitSorted, fitSorted.empty() ? 1 : fitSorted.size());
+    if (status == kOK && !fitSorted.empty())
+        status = check(hipMemcpy(d_fitSorted, fitSorted.data(), fitSorted.size() * sizeof(QShell), hipMemcpyHostToDevice),
+                       "copy fit shells");
     if (status == kOK && rows > 0) {
         status = check(hipMemcpy(F->d_rowPacked, rowPacked.data(), rowPacked.size() * sizeof(long long),
                                  hipMemcpyHostToDevice), "copy rows");
@@ -413,7 +423,7 @@ int qs_fit_create(const qs_basis *orbital, const qs_basis *fitBasis, qs_fit **ou
             hipLaunchKernelGGL(fit_integrals_kernel, dim3(static_cast<unsigned>((m + kThreads - 1) / kThreads)),
                                dim3(kThreads), 0, is, k0, nWork, static_cast<int>(fitBasis->shells.size()), d_pairs,
                                orbital->d_shells, orbital->d_exps, orbital->d_coefs, orbital->spherical,
-                               fitBasis->d_shells, fitBasis->d_exps, fitBasis->d_coefs, fitBasis->spherical, F->nFit,
+                               d_fitSorted, fitBasis->d_exps, fitBasis->d_coefs, fitBasis->spherical, F->nFit,
                                F->d_B);
             status = check(hipGetLastError(), "fit_integrals_kernel");
         }
@@ -423,6 +433,7 @@ int qs_fit_create(const qs_basis *orbital, const qs_basis *fitBasis, qs_fit **ou
         }
     }
     qs_release(&d_pairs);
+    qs_release(&d_fitSorted);
     if (status == kOK) F->fitPairs = shell_pairs(fitBasis, false);
     if (status != kOK) { qs_fit_free(F); return status; }
     *out = F;
