@@ -297,6 +297,13 @@ int qs_one_electron(qs_basis *basis, double *S, double *T, double *V) {
     }
     const size_t n = static_cast<size_t>(basis->nFunctions), size = n * (n + 1) / 2;
     std::vector<QPair> pairs = shell_pairs(basis, false);
+    // Pairs of one class (angular momenta and primitive counts) side by side, so that a wave's pairs share their loop
+    // bounds; each pair writes its own entries of S, T and V.
+    auto cls = [&](const QPair &q) {
+        const QShell &a = basis->shells[q.x], &b = basis->shells[q.y];
+        return ((a.l * 8 + b.l) * 64 + a.nPrim) * 64 + b.nPrim;
+    };
+    std::stable_sort(pairs.begin(), pairs.end(), [&](const QPair &u, const QPair &v) { return cls(u) < cls(v); });
     std::vector<double> atoms(4 * static_cast<size_t>(basis->nAtoms));
     for (int a = 0; a < basis->nAtoms; a++) {
         for (int c = 0; c < 3; c++) atoms[4 * a + c] = basis->xyz[3 * a + c];
