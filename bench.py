@@ -111,7 +111,7 @@ def run_leg(grid, ids, P, precision, steps, warmup, spec, spherical):
     for _ in range(warmup):
         V[:] = 0.0
         grid.integrate(ids, P, V)
-    dev, rho, fock, wall = [], [], [], []
+    dev, rho, product, fock, wall = [], [], [], [], []
     E = N = None
     for _ in range(steps):
         V[:] = 0.0
@@ -120,10 +120,11 @@ def run_leg(grid, ids, P, precision, steps, warmup, spec, spherical):
         wall.append((time.perf_counter() - t) * 1e3)
         dev.append(st["device_ms"])
         rho.append(st["rho_ms"])
+        product.append(st["product_ms"])
         fock.append(st["fock_ms"])
     return {"E": E, "N": N, "V": V.copy(), "device_ms": np.asarray(dev), "rho_ms": np.asarray(rho),
-            "fock_ms": np.asarray(fock), "wall_ms": np.asarray(wall), "stats": st, "basis_s": basis_s,
-            "device_bytes": grid.device_bytes()}
+            "product_ms": np.asarray(product), "fock_ms": np.asarray(fock), "wall_ms": np.asarray(wall), "stats": st,
+            "basis_s": basis_s, "device_bytes": grid.device_bytes()}
 
 
 def cpu_baseline(kind, n_waters, cube, seed, functional, spherical, threads=1, seconds=10.0):
@@ -363,12 +364,14 @@ def main():
     st = mixed["stats"]
     ms = float(np.mean(mixed["device_ms"]))
     value = G / (ms * 1e-3)
-    # Roofline of the dominant kernel: density_kernel_mixed (one kernel between the rho events; the Fock stage is two
-    # kernels).  Algorithmic bytes: every value of a kept function at a point read once (17 bytes: 5 Phi digits and
-    # three 32-bit fixed-point gradients), P digits, weights, W coefficients and block sums (stats[6]).  Its int8 matrix
-    # work at peak is far shorter than its bytes at peak, so the HBM roof is the one that binds.
-    rho_ms = float(np.mean(mixed["rho_ms"]))
-    rho_bytes = st["rho_bytes"]
+    # Roofline of the dominant kernel: density_rows_kernel, X = P_red Phi per row chunk with the chunk's share of rho and
+    # grad rho, timed alone by the library's events on its stream (stats[9]; the density stage adds the per-block
+    # functional kernel, the Fock stage is two kernels).  Algorithmic bytes (stats[10]): every value of a kept function
+    # at a point read once (17 bytes: 5 Phi digits and three 32-bit fixed-point gradients), P digits and row exponents,
+    # the point exponents, and each row chunk's four shares per point written once.  Its int8 matrix work at peak is
+    # far shorter than its bytes at peak, so the HBM roof is the one that binds.
+    rho_ms = float(np.mean(mixed["product_ms"]))
+    rho_bytes = st["product_bytes"]
     flops = 2.0 * st["bf_squares"]                    # X = P_red Phi: 2 nb^2 G per block, as algorithmic flops
     int8_ops = 2.0 * 15.0 * st["bf_squares_padded"]   # what the matrix cores issue: 15 digit products, rows padded to 32
     t_hbm, t_mfma = rho_bytes / (PEAK_HBM_GBS * 1e9), int8_ops / (PEAK_INT8_TOPS * 1e12)
@@ -376,15 +379,16 @@ def main():
         achieved, peak, unit, bound = rho_bytes / (rho_ms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s", "hbm"
     else:
         achieved, peak, unit, bound = int8_ops / (rho_ms * 1e-3) / 1e12, PEAK_INT8_TOPS, "TOP/s", "mfma"
-    traffic, traffic_commit = measured_traffic("density_kernel_mixed", args.workload)
-    roofline = {"kernel": "density_kernel_mixed", "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+    traffic, traffic_commit = measured_traffic("density_rows_kernel", args.workload)
+    roofline = {"kernel": "density_rows_kernel", "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
                 "frac": achieved / peak, "traffic": traffic,
                 "traffic_source": None if traffic is None else
-                "profiles/density_kernel_mixed_{:s}_pmc.txt (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, at commit "
+                "profiles/density_rows_kernel_{:s}_pmc.txt (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, at commit "
                 "{:s})".format(args.workload, traffic_commit or "unrecorded"),
                 "algorithmic_bytes": rho_bytes, "algorithmic_flops": flops,
                 "algorithmic_tflops_per_s": flops / (rho_ms * 1e-3) / 1e12, "hardware_int8_ops": int8_ops,
-                "kernel_ms": rho_ms, "fock_stage_ms": float(np.mean(mixed["fock_ms"])),
+                "kernel_ms": rho_ms, "density_stage_ms": float(np.mean(mixed["rho_ms"])),
+                "density_stage_bytes": st["rho_bytes"], "fock_stage_ms": float(np.mean(mixed["fock_ms"])),
                 "fock_stage_bytes": st["fock_bytes"], "fock_algorithmic_flops": 4.0 * st["bf_squares"]}
     out = {"metric": METRIC, "value": value, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

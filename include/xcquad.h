@@ -63,8 +63,10 @@ int         xq_basis_set        ( xq_grid *grid, int nSpec, const double *spec, 
 !  2 [5] each), [6] and [7] algorithmic HBM bytes of the density and Fock stages (each value of a kept function at a
 !  point read once in the stage's storage format, each result written once), [8] sum over blocks of (kept functions
 !  rounded up to 32)^2 x 128, the shape the matrix-core products run on (the mixed path's X = P_red Phi issues
-!  2 x 15 x [8] int8 operations: 15 digit products). */
-# define XQ_NSTATS 9
+!  2 x 15 x [8] int8 operations: 15 digit products), [9] device ms of the density product kernel alone (the mixed path's
+!  X = P_red Phi with each row chunk's share of rho and grad rho; FP64: the density kernel, functional included, as [3]),
+!  [10] its algorithmic HBM bytes (as [6]). */
+# define XQ_NSTATS 11
 int         xq_integrate        ( xq_grid *grid, int nIDs, const int *ids, const double *densityP, double *eXC,
                                   double *nElectrons, double *fockV, double *stats ) ;
 

@@ -53,6 +53,7 @@ int xq_grid_build(int accuracy, int nAtoms, const int *atomicNumbers, const doub
     if (status == kOK) status = check(hipEventCreate(&g->evRho0), "hipEventCreate");
     if (status == kOK) status = check(hipEventCreate(&g->evRho1), "hipEventCreate");
     if (status == kOK) status = check(hipEventCreate(&g->evFock1), "hipEventCreate");
+    if (status == kOK) status = check(hipEventCreate(&g->evDens1), "hipEventCreate");
     if (status == kOK) status = build_grid(g);
     if (status != kOK) { xq_free(g); return status; }
     *grid = g;
@@ -182,10 +183,11 @@ int xq_estimate_bytes(long long nPoints, int nFunctions, double meanKeptFunction
     // Fock tile list and chunk tables (rows padded to chunks of 32: about 10% more stored values than kept).
     double b = grid + kept + work + 32.0 * values + 16.0 * blocks * 16.0 + 32.0 * G;
     if (precision == XQ_PRECISION_MIXED)
-        // Phi digits in the density's order and 32-bit gradients (17 bytes per stored value), Phi and W digits and
+        // Phi digits in the density's order and 32-bit gradients (17 bytes per stored value), the density's row-chunk
+        // shares of rho and grad rho (4 KB per chunk of 32 x 128 values: 1 byte per stored value), Phi and W digits and
         // 24-bit gradients over the superblocks' union rows (17 bytes per stored value, about 15% more rows), per-point
         // exponents, P digits.
-        b += 1.1 * (17.0 + 1.15 * 17.0) * values + 512.0 * blocks + 8.0 * n * n + 4.0 * n;
+        b += 1.1 * (18.0 + 1.15 * 17.0) * values + 512.0 * blocks + 8.0 * n * n + 4.0 * n;
     else
         // Phi in two operand orders, grad Phi, W (48 bytes per stored value).
         b += 1.1 * 48.0 * values;
@@ -210,7 +212,7 @@ int xq_batch_create(int maxJobs, xq_batch **batch) {
     if (b == nullptr) { set_error("host allocation failed"); return kOutOfMemory; }
     b->maxJobs = maxJobs;
     int status = check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
-    hipEvent_t *evs[5] = {&b->ev0, &b->ev1, &b->evRho0, &b->evRho1, &b->evFock1};
+    hipEvent_t *evs[6] = {&b->ev0, &b->ev1, &b->evRho0, &b->evRho1, &b->evFock1, &b->evDens1};
     for (hipEvent_t *e : evs)
         if (status == kOK) status = check(hipEventCreate(e), "hipEventCreate");
     if (status == kOK)
@@ -244,7 +246,7 @@ int xq_integrate_batch(xq_batch *batch, int nJobs, xq_grid *const *grids, const 
         const int status = functional_set(nIDs[j], ids + 4 * j, &fs[j]);
         if (status != kOK) return status;
     }
-    const StageEvents ev{batch->ev0, batch->ev1, batch->evRho0, batch->evRho1, batch->evFock1};
+    const StageEvents ev{batch->ev0, batch->ev1, batch->evRho0, batch->evRho1, batch->evFock1, batch->evDens1};
     return integrate_jobs(batch->stream, ev, nJobs, grids, fs.data(), densityP, eXC, nElectrons, fockV, stats,
                           batch->h_jobs, batch->d_jobs);
 }
@@ -254,7 +256,7 @@ void xq_batch_free(xq_batch *batch) {
     if (batch->stream) (void)hipStreamSynchronize(batch->stream);
     if (batch->h_jobs) (void)hipHostFree(batch->h_jobs);
     if (batch->d_jobs) (void)hipFree(batch->d_jobs);
-    hipEvent_t evs[5] = {batch->ev0, batch->ev1, batch->evRho0, batch->evRho1, batch->evFock1};
+    hipEvent_t evs[6] = {batch->ev0, batch->ev1, batch->evRho0, batch->evRho1, batch->evFock1, batch->evDens1};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
     if (batch->stream) (void)hipStreamDestroy(batch->stream);
@@ -282,6 +284,7 @@ void xq_free(xq_grid *grid) {
     if (grid->evRho0) (void)hipEventDestroy(grid->evRho0);
     if (grid->evRho1) (void)hipEventDestroy(grid->evRho1);
     if (grid->evFock1) (void)hipEventDestroy(grid->evFock1);
+    if (grid->evDens1) (void)hipEventDestroy(grid->evDens1);
     if (grid->stream) (void)hipStreamDestroy(grid->stream);
     delete grid;
 }

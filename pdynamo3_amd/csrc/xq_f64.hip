@@ -386,6 +386,7 @@ int launch_f64(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *
         hipLaunchKernelGGL(density_kernel_f64, dim3(static_cast<unsigned>(8 * blockSlots)), dim3(256), 0, s, d_table,
                            nJobs);
     }
+    XQ_CHECK(hipEventRecord(ev.evDens1, s));
     XQ_CHECK(hipEventRecord(ev.evRho1, s));
     if ((stages & kStageFock) && chunks > 0) {
         hipLaunchKernelGGL(w_kernel_f64, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, d_table, nJobs);

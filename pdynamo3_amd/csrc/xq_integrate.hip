@@ -187,7 +187,7 @@ int functional_set(int nIDs, const int *ids, FunctionalSet *fs) {
 // Algorithmic HBM bytes of the two stages of one handle's quadrature: every stored value of a kept function at a grid
 // point (sum over blocks of kept functions x points, not the rows padded to chunks of 32) read once, every result
 // written once.
-void stage_bytes(const xq_grid *g, double *rho, double *fock) {
+void stage_bytes(const xq_grid *g, double *rho, double *fock, double *product) {
     const double n = static_cast<double>(g->nFunctions), nn = n * n, G = static_cast<double>(g->nPoints),
                  B = static_cast<double>(g->nBlocks), values = g->bfValues;
     if (g->precision == XQ_PRECISION_FP64) {
@@ -196,6 +196,7 @@ void stage_bytes(const xq_grid *g, double *rho, double *fock) {
         // the Fock product (Phi and W in: 16 bytes per value; V read and written).
         *rho = 32.0 * values + 8.0 * nn + 40.0 * G + 20.0 * B;
         *fock = 56.0 * values + 32.0 * G + 16.0 * nn;
+        *product = *rho;   // one density kernel, functional included
     } else {
         // Density: Phi as 5 digits and grad Phi as 3 x 32-bit fixed point (17 bytes per value), P digits (8 bytes per
         // entry), weights in, W coefficients out and the point exponents (44 bytes per point), block sums.  Fock stage,
@@ -205,6 +206,9 @@ void stage_bytes(const xq_grid *g, double *rho, double *fock) {
         const double union_values = g->bfUnion;
         *rho = 17.0 * values + 8.0 * nn + 44.0 * G + 20.0 * B;
         *fock = 25.0 * union_values + 35.0 * G + 16.0 * nn;
+        // The density product kernel alone: Phi digits and grad Phi, P digits and row exponents, the point exponents
+        // (4 bytes per point) in; each row chunk's share of rho and grad rho (4 x 8 bytes per point) out.
+        *product = 17.0 * values + 8.0 * nn + 4.0 * n + 4.0 * G + 32.0 * kBlockSize * static_cast<double>(g->nChunks);
     }
 }
 
@@ -263,7 +267,7 @@ static int integrate_jobs_queued(hipStream_t s, const StageEvents &ev, int nJobs
         }
     }
     XQ_CHECK(hipStreamSynchronize(s));
-    double bfv = 0.0, bfs = 0.0, bfp = 0.0, rhoBytes = 0.0, fockBytes = 0.0;
+    double bfv = 0.0, bfs = 0.0, bfp = 0.0, rhoBytes = 0.0, fockBytes = 0.0, productBytes = 0.0;
     long long skipped = 0;
     for (int j = 0; j < nJobs; j++) {
         const xq_grid *g = grids[j];
@@ -281,16 +285,18 @@ static int integrate_jobs_queued(hipStream_t s, const StageEvents &ev, int nJobs
         bfv += g->bfValues;
         bfs += g->bfSquares;
         bfp += g->bfSquaresPadded;
-        double rb, fb;
-        stage_bytes(g, &rb, &fb);
+        double rb, fb, pb;
+        stage_bytes(g, &rb, &fb, &pb);
         rhoBytes += rb;
         fockBytes += fb;
+        productBytes += pb;
     }
     if (stats != nullptr) {
-        float ms = 0.0f, msRho = 0.0f, msFock = 0.0f;
+        float ms = 0.0f, msRho = 0.0f, msFock = 0.0f, msProduct = 0.0f;
         (void)hipEventElapsedTime(&ms, ev.ev0, ev.ev1);
         (void)hipEventElapsedTime(&msRho, ev.evRho0, ev.evRho1);
         (void)hipEventElapsedTime(&msFock, ev.evRho1, ev.evFock1);
+        (void)hipEventElapsedTime(&msProduct, ev.evRho0, ev.evDens1);
         stats[0] = bfv;
         stats[1] = static_cast<double>(skipped);
         stats[2] = static_cast<double>(ms);
@@ -300,6 +306,8 @@ static int integrate_jobs_queued(hipStream_t s, const StageEvents &ev, int nJobs
         stats[6] = rhoBytes;
         stats[7] = fockBytes;
         stats[8] = bfp;
+        stats[9] = static_cast<double>(msProduct);
+        stats[10] = productBytes;
     }
     return kOK;
 }
@@ -319,7 +327,7 @@ int integrate(xq_grid *g, int nIDs, const int *ids, const double *Ppacked, doubl
     FunctionalSet fs;
     XQ_TRY(functional_set(nIDs, ids, &fs));
     XQ_TRY(ensure_job_table(g));
-    const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1};
+    const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1, g->evDens1};
     xq_grid *grids[1] = {g};
     const double *P[1] = {Ppacked};
     double *Vs[1] = {V};

@@ -81,6 +81,12 @@ struct MixedJob {
     // each entry {block, first chunk, first kept index, kept functions x 256 + points}.
     const int4 *order;
     int xcdStart[9], xcdBase[8];
+    // Mixed density launch: one work-group per row chunk of a block; items[itemStart[x] .. itemStart[x + 1]) run on
+    // XCD x from slot itemBase[x], each {chunk, kept functions x 1024 + row chunk, first kept index, block}; part
+    // receives each chunk's share of rho and grad rho, [chunk][4][point].
+    const int4 *items;
+    int itemStart[9], itemBase[8];
+    double *part;
 };
 
 }  // namespace xq
@@ -94,7 +100,7 @@ struct xq_grid {
     std::vector<double> xyz;
     double bfTol = 1.0e-10, rhoTol = 1.0e-15;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evRho0 = nullptr, evRho1 = nullptr, evFock1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evRho0 = nullptr, evRho1 = nullptr, evFock1 = nullptr, evDens1 = nullptr;
 
     // Grid.
     int nPoints = 0, nBlocks = 0;
@@ -106,6 +112,12 @@ struct xq_grid {
     std::vector<int4> h_order;
     int4 *d_order = nullptr;
     int xcdStart[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // The mixed density launch's items: each block of the order above cut into its row chunks, consecutive on the
+    // block's XCD; per chunk, its share of rho and grad rho (d_part).
+    std::vector<int4> h_items;
+    int4 *d_items = nullptr;
+    int itemStart[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double *d_part = nullptr;
 
     // Basis.
     int nFunctions = 0, nShells = 0, spherical = 1, precision = 0;
@@ -161,7 +173,7 @@ struct xq_grid {
 struct xq_batch {
     int maxJobs = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evRho0 = nullptr, evRho1 = nullptr, evFock1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evRho0 = nullptr, evRho1 = nullptr, evFock1 = nullptr, evDens1 = nullptr;
     xq::MixedJob *h_jobs = nullptr, *d_jobs = nullptr;
 };
 
@@ -217,19 +229,20 @@ int integrate(xq_grid *g, int nIDs, const int *ids, const double *P, double *eXC
 // Quadratures of several handles on one stream; the mixed-precision stages run as one launch each over all the jobs.
 struct StageEvents {
     hipEvent_t ev0, ev1, evRho0, evRho1, evFock1;
+    hipEvent_t evDens1;   // after the density product kernel (before the mixed path's functional kernel)
 };
 int integrate_jobs(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
                    const double *const *P, double *eXC, double *nE, double *const *V, double *stats, MixedJob *h_table,
                    MixedJob *d_table);
 int functional_set(int nIDs, const int *ids, FunctionalSet *fs);
 // Algorithmic HBM bytes of the density and Fock stages of one quadrature on the handle (stats[6] and [7]).
-void stage_bytes(const xq_grid *g, double *rho, double *fock);
+void stage_bytes(const xq_grid *g, double *rho, double *fock, double *product);
 // stages: kStageDensity (P digits and the density kernel), kStageFock (W and the Fock product), or both.
 constexpr int kStageDensity = 1, kStageFock = 2;
 // Per-step arrays of one job in place of its handle's (a null field keeps the handle's): the second spin of an
 // unrestricted quadrature runs as a second job on the same handle with its own P, V, W and coefficients.
 struct StepArrays {
-    double *P = nullptr, *V = nullptr, *wcoef = nullptr, *rhoOut = nullptr, *WPA = nullptr;
+    double *P = nullptr, *V = nullptr, *wcoef = nullptr, *rhoOut = nullptr, *WPA = nullptr, *part = nullptr;
     unsigned long long *Pq = nullptr;
     int *eP = nullptr;
     signed char *DW = nullptr, *EW = nullptr;
@@ -251,11 +264,12 @@ void launch_zero(int n, double *dense, hipStream_t s);
 void launch_pack(int n, const double *dense, double *packed, hipStream_t s);
 // Shared by both precisions: the job table entries of the given handles, the per-block chunk tables and Fock tile list.
 // blockSlots (optional): the density launch's work-groups per XCD, the largest over the XCDs of the jobs' summed block
-// ranges.
+// ranges; itemSlots (optional): the same for the mixed density launch's row-chunk items.
 void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rows,
                long long *blocks, long long *chunks, long long *groups, const StepArrays *over = nullptr,
-               long long *blockSlots = nullptr);
-// The density launch's block order (d_order, xcdStart), from the blocks' kept-function counts; at basis set-up.
+               long long *blockSlots = nullptr, long long *itemSlots = nullptr);
+// The density launch's block order (d_order, xcdStart) and the mixed launch's row-chunk items (d_items, itemStart),
+// from the blocks' kept-function counts; at basis set-up.
 int build_block_order(xq_grid *g, hipStream_t s);
 int chunk_tables(xq_grid *g);
 int prepare_f64(xq_grid *g, const double *f, const double *fx, const double *fy, const double *fz);

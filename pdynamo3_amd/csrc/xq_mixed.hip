@@ -80,8 +80,7 @@ constexpr int kChunkBytesB = kDB * kBlockSize * kC;   // one chunk of DB: 5 digi
 constexpr int kChunkGrad = 3 * kBlockSize * kC;     // one chunk of GF, GHU or GLU (elements)
 constexpr int kGradBits = 31;           // GF: |value| < 2^EG is stored as a multiple of 2^(EG - 31)
 constexpr int kGradBitsW = 23;          // GHU, GLU: |value| < 2^EG is stored as a multiple of 2^(EG - 23)
-constexpr int kPanel = 1;               // chunks of P_red columns and of Phi staged at once by the density kernel
-constexpr int kPanelDwords = 8 * kPanel + 4;        // one panel row in LDS (padded against bank conflicts)
+constexpr int kPRowDwords = 12;         // one staged P_red row in LDS: 8 words of K positions, padded against bank conflicts
 constexpr int kIdxCache = 1024;         // kept-function indices of a block cached in LDS by the density kernel
 constexpr int kSuper = 4;               // blocks per superblock of the Fock product
 
@@ -364,179 +363,164 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
     if (threadIdx.x == 0) eP[i] = e;
 }
 
-// Density kernel (per block, 256 threads = 4 waves): X = P_red Phi on int8 MFMA, rho and grad rho, the FP64 functional,
-// E_xc / N_e block sums, and the four W coefficients per point.  Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns
-// point 32 wv + lr and the rows chunk_fun(lh, v) of each row chunk I.  Each step stages one column chunk of P_red's row
-// panel (gathered) and that chunk's Phi digits (16-byte loads of the contiguous DB chunk) in LDS, both fetched by all
-// 256 threads before one barrier.  36 KB of LDS and at most 168 VGPRs let three work-groups share a CU, whose
-// interleaving hides the staging latency better than a one-step register prefetch at two work-groups per CU (3.14
-// against 3.22 ms on (H2O)64).  The epilogue reads chunk I's Phi digits again, with its grad Phi.
-// kRhoOut: the density stage of one spin of an unrestricted quadrature (the job's rhoOut receives rho and grad rho).
-template <bool kRhoOut>
+// Density rows (one row chunk I of one block per work-group, 256 threads = 4 waves): X = P_red Phi for the chunk's 32
+// rows on int8 MFMA, then the chunk's share of rho and grad rho at the block's points, stored in part[chunk][4][point].
+// Wave wv owns points 32 wv .. + 31; lane (lr, lh) owns point 32 wv + lr and the rows chunk_fun(lh, v) of chunk I.
+// Each step stages one column chunk of P_red's row panel (gathered) and that chunk's Phi digits (16-byte loads of the
+// contiguous DB chunk) in LDS, both fetched by all 256 threads into registers one step ahead.  The row chunks of a
+// block are consecutive items of one XCD, so that their work-groups run at about the same time and read the block's
+// Phi digit chunks, in the same order, through that XCD's L2: Phi comes from HBM about once per block instead of once
+// per row chunk.  32 KB of LDS and at most 168 VGPRs let three work-groups share a CU (four, at 128 VGPRs, were
+// slower: 3.56 against 2.75 ms on (H2O)64).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
-    __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPanelDwords];   // [digit][row][panel positions]
-    __shared__ v4i sB[kPanel * kChunkBytesB / 16];   // the panel's Phi digits, in DB's order
-    __shared__ int sPe[2][kC];
+density_rows_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
+    __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPRowDwords];   // [digit][row][K positions / 4]
+    __shared__ v4i sB[kChunkBytesB / 16];   // the step's Phi digits, in DB's order
+    __shared__ int sPe[kC];
     __shared__ int sIdx[kIdxCache];
-    __shared__ double rsum[4][kBlockSize];
-    __shared__ double wsum[4][2];
-    // Work-group g runs on XCD x = g % 8 and takes slot g / 8 of that XCD's list: the jobs' block ranges x in job order
-    // (each a contiguous region of its molecule, so that the XCD's L2 holds the P rows its blocks gather; the ranges
-    // of one job have equal estimated cost), each range's blocks in order of decreasing cost.
+    // Work-group g runs on XCD x = g % 8 and takes slot g / 8 of that XCD's item list (the jobs' lists in job order).
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, slot, [xcd](const xq::MixedJob &j) { return j.xcdBase[xcd]; })];
-    const int local = slot - J.xcdBase[xcd];
-    if (local >= J.xcdStart[xcd + 1] - J.xcdStart[xcd]) return;
-    const xq::DevBlock *__restrict__ blocks = J.blocks;
-    const int *__restrict__ idxAll = J.idx;
+    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, slot, [xcd](const xq::MixedJob &j) { return j.itemBase[xcd]; })];
+    const int local = slot - J.itemBase[xcd];
+    if (local >= J.itemStart[xcd + 1] - J.itemStart[xcd]) return;
     const signed char *__restrict__ DB = J.DB;
-    const signed char *__restrict__ EB = J.EB;
     const int *__restrict__ GF = J.GF;
-    const signed char *__restrict__ EG = J.EG;
     const int n = J.n;
     const unsigned long long *__restrict__ Pq = J.Pq;
     const int *__restrict__ eP = J.eP;
-    const double *__restrict__ w = J.w;
-    const FunctionalSet fs = J.fs;
-    const double rhoTol = J.rhoTol;
-    double *__restrict__ wcoef = J.wcoef;
-    double *__restrict__ blockEN = J.blockEN;
-    int *__restrict__ skip = J.skip;
-    // The block's record in launch order ({block, first chunk, first kept index, kept functions x 256 + points}): one
-    // load gives what the first fetches need.
-    const int4 ent = J.order[J.xcdStart[xcd] + local];
-    const int b = ent.x;
-    const int nb = ent.w >> 8, np = ent.w & 255;
-    const int pStart = blocks[b].pStart;
+    // The item's record ({chunk, kept functions x 1024 + row chunk, first kept index, block}): one load gives what the
+    // first fetches need.
+    const int4 ent = J.items[J.itemStart[xcd] + local];
+    const size_t CI = static_cast<size_t>(ent.x);
+    const int nb = ent.y >> 10, I = ent.y & 1023, b = ent.w;
+    const size_t C = CI - I;
+    const int nChunks = (nb + kC - 1) / kC;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int lr = lane & 31, lh = lane >> 5;
-    if (nb <= 0) {
-        if (t == 0) { blockEN[2 * b] = 0.0; blockEN[2 * b + 1] = 0.0; skip[b] = 1; }
-        if (kRhoOut && t < np)
-            for (int c = 0; c < 4; c++) J.rhoOut[4 * (static_cast<size_t>(pStart) + t) + c] = 0.0;
-        return;
-    }
-    const size_t C = static_cast<size_t>(ent.y);
-    const int nChunks = (nb + kC - 1) / kC;
     const int p = kC * wv + lr;
     const signed char *DBb = DB + C * kChunkBytesB;
-    // P_red is staged one row panel at a time (rows 32 I .. + 31, up to kPanel chunks of columns), all of the panel's
-    // words fetched before the barrier; the panel's MFMA steps then run without barriers.  Thread (row sr, slot s8)
-    // fetches columns 8 j + s8 (j = 0 .. 3) of every chunk of the panel (eight lanes read eight consecutive kept
-    // functions) and writes them as one word per digit at K positions 4 s8 .. + 3, in row tile_row(sr).
+    // P_red is staged one column chunk at a time: thread (row sr, slot s8) fetches columns 8 j + s8 (j = 0 .. 3) of
+    // the chunk (eight lanes read eight consecutive kept functions) and writes them as one word per digit at K
+    // positions 4 s8 .. + 3, in row tile_row(sr).
     const int sr = t >> 3, s8 = t & 7, srow = tile_row(sr);
-    const int eb = EB[static_cast<size_t>(b) * kBlockSize + p];
-    double gscale[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) gscale[c] = ldexp(1.0, EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p] - kGradBits);
-    // The block's kept-function indices (fetched after the first Phi operands, whose latency they share), cached in LDS
-    // for the P_red gathers (blocks with more kept functions than the cache read them from memory).
-    const int *idxG = idxAll + ent.z;
+    // The block's kept-function indices, cached in LDS for the P_red gathers (blocks with more kept functions than the
+    // cache read them from memory).
+    const int *idxG = J.idx + ent.z;
     for (int k = t; k < nb && k < kIdxCache; k += 256) sIdx[k] = idxG[k];
     __syncthreads();
     const int *idx = nb <= kIdxCache ? sIdx : idxG;
-    double pr = 0.0, pgx = 0.0, pgy = 0.0, pgz = 0.0;
-    for (int I = 0; I < nChunks; I++) {
-        v16i acc[kXOrders];
-        const int r = kC * I + sr;
-        const size_t rowOff = r < nb ? static_cast<size_t>(idx[r]) * n : 0;
-        for (int c0 = 0; c0 < nChunks; c0 += kPanel) {
-            const int nc = nChunks - c0 < kPanel ? nChunks - c0 : kPanel;
-            unsigned long long wd[kPanel][4];
+    if (t < kC) sPe[t] = kC * I + t < nb ? eP[idx[kC * I + t]] : 0;
+    const int r = kC * I + sr;
+    const size_t rowOff = r < nb ? static_cast<size_t>(idx[r]) * n : 0;
+    v16i acc[kXOrders];
 #pragma unroll
-            for (int m = 0; m < kPanel; m++)
+    for (int s = 0; s < kXOrders; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // One chunk per step; the next step's P_red words and Phi digits are loaded into registers while the current
+    // step's products run.
+    constexpr int kPhiPerThread = kChunkBytesB / 16 / 256;
+    const v4i *phiSrc = reinterpret_cast<const v4i *>(DBb);
+    unsigned long long wd[4];
+    v4i ph[kPhiPerThread];
+    auto fetch = [&](int c) {
 #pragma unroll
-                for (int jj = 0; jj < 4; jj++) {
-                    const int k = kC * (c0 + m) + 8 * jj + s8;
-                    wd[m][jj] = (m < nc && r < nb && k < nb) ? Pq[rowOff + idx[k]] : 0ull;
-                }
-            // The panel's Phi digits: nc contiguous chunks of DB, 16 bytes per thread and step.
-            constexpr int kPhiPerThread = kPanel * kChunkBytesB / 16 / 256;
-            const v4i *phiSrc = reinterpret_cast<const v4i *>(DBb) + static_cast<size_t>(c0) * (kChunkBytesB / 16);
-            v4i ph[kPhiPerThread];
-#pragma unroll
-            for (int i = 0; i < kPhiPerThread; i++) {
-                const int e = t + 256 * i;
-                if (e < nc * (kChunkBytesB / 16)) ph[i] = phiSrc[e];
-            }
-            __syncthreads();   // every wave is done with the previous panel
-#pragma unroll
-            for (int i = 0; i < kPhiPerThread; i++) {
-                const int e = t + 256 * i;
-                if (e < nc * (kChunkBytesB / 16)) sB[e] = ph[i];
-            }
-#pragma unroll
-            for (int m = 0; m < kPanel; m++) {
-                if (m < nc) {
-#pragma unroll
-                    for (int j = 0; j < kDP; j++) {
-                        unsigned x = 0;
-#pragma unroll
-                        for (int jj = 0; jj < 4; jj++) x |= (static_cast<unsigned>(wd[m][jj] >> (8 * j)) & 255u) << (8 * jj);
-                        sP[j][srow][8 * m + s8] = static_cast<int>(x);
-                    }
-                }
-            }
-            if (c0 == 0) {
-                if (t < kC) sPe[I & 1][t] = kC * I + t < nb ? eP[idx[kC * I + t]] : 0;
-#pragma unroll
-                for (int s = 0; s < kXOrders; s++) acc[s] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            }
-            __syncthreads();
-            for (int cl = 0; cl < nc; cl++) {
-                v4i a[kDP], bq[kDB];
-#pragma unroll
-                for (int j = 0; j < kDB; j++) bq[j] = sB[cl * (kChunkBytesB / 16) + (j * kBlockSize + p) * 2 + lh];
-#pragma unroll
-                for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sP[j][lr][8 * cl + 4 * lh]);
-#pragma unroll
-                for (int i = 0; i < kDP; i++)
-#pragma unroll
-                    for (int j = 0; j < kDB; j++)
-                        if (i + j < kXOrders) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
-            }
+        for (int jj = 0; jj < 4; jj++) {
+            const int k = kC * c + 8 * jj + s8;
+            wd[jj] = (r < nb && k < nb) ? Pq[rowOff + idx[k]] : 0ull;
         }
-        // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi from
-        // its fixed point (the sums over v are scaled once).
-        // X in FP64 first, so that the accumulators are free before the epilogue's operands arrive.
-        double x[16];
 #pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int s[kXOrders] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
-            x[v] = ldexp(static_cast<double>(combine<kXOrders>(s)), sPe[I & 1][chunk_fun(lh, v)] + eb - 7 * (kXOrders + 1));
+        for (int i = 0; i < kPhiPerThread; i++)
+            ph[i] = phiSrc[static_cast<size_t>(c) * (kChunkBytesB / 16) + t + 256 * i];
+    };
+    fetch(0);
+    for (int c = 0; c < nChunks; c++) {
+        __syncthreads();   // every wave is done with the previous chunk
+#pragma unroll
+        for (int i = 0; i < kPhiPerThread; i++) sB[t + 256 * i] = ph[i];
+#pragma unroll
+        for (int j = 0; j < kDP; j++) {
+            unsigned x = 0;
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) x |= (static_cast<unsigned>(wd[jj] >> (8 * j)) & 255u) << (8 * jj);
+            sP[j][srow][s8] = static_cast<int>(x);
         }
-        v4i dI[kDB];   // the lane's Phi digits of chunk I, read again with its grad Phi
+        __syncthreads();
+        if (c + 1 < nChunks) fetch(c + 1);
+        v4i a[kDP], bq[kDB];
 #pragma unroll
-        for (int j = 0; j < kDB; j++)
-            dI[j] = *reinterpret_cast<const v4i *>(&DBb[(I * kDB + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
-        double sumR = 0.0, sumG[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < kDB; j++) bq[j] = sB[(j * kBlockSize + p) * 2 + lh];
 #pragma unroll
-        for (int v = 0; v < 16; v++) sumR += static_cast<double>(digits5(dI, v)) * x[v];
+        for (int j = 0; j < kDP; j++) a[j] = *reinterpret_cast<const v4i *>(&sP[j][lr][4 * lh]);
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const size_t o = (C + I) * kChunkGrad + (c * kBlockSize + p) * kC + 16 * lh;
-            v4i gq[4];
+        for (int i = 0; i < kDP; i++)
 #pragma unroll
-            for (int w = 0; w < 4; w++) gq[w] = *reinterpret_cast<const v4i *>(GF + o + 4 * w);
-#pragma unroll
-            for (int v = 0; v < 16; v++) sumG[c] += int_of(gq, v) * x[v];
-        }
-        const double sumX = sumG[0], sumY = sumG[1], sumZ = sumG[2];
-        pr += ldexp(sumR, eb - 7 * kDB);
-        pgx += sumX * gscale[0];
-        pgy += sumY * gscale[1];
-        pgz += sumZ * gscale[2];
+            for (int j = 0; j < kDB; j++)
+                if (i + j < kXOrders) acc[i + j] = mfma_i8(a[i], bq[j], acc[i + j]);
     }
-    pr += __shfl_xor(pr, 32, 64);
-    pgx += __shfl_xor(pgx, 32, 64);
-    pgy += __shfl_xor(pgy, 32, 64);
-    pgz += __shfl_xor(pgz, 32, 64);
-    if (lh == 0) { rsum[0][p] = pr; rsum[1][p] = pgx; rsum[2][p] = pgy; rsum[3][p] = pgz; }
-    __syncthreads();
-    // Functional, E and N, W coefficients.
+    // Epilogue: X for rows chunk_fun(lh, v) of chunk I at point p; Phi from the chunk's own digits, grad Phi from its
+    // fixed point (the sums over v are scaled once).  X in FP64 first, so that the accumulators are free before the
+    // epilogue's operands arrive.
+    const int eb = J.EB[static_cast<size_t>(b) * kBlockSize + p];
+    double x[16];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int s[kXOrders] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
+        x[v] = ldexp(static_cast<double>(combine<kXOrders>(s)), sPe[chunk_fun(lh, v)] + eb - 7 * (kXOrders + 1));
+    }
+    v4i dI[kDB];   // the lane's Phi digits of chunk I
+#pragma unroll
+    for (int j = 0; j < kDB; j++)
+        dI[j] = *reinterpret_cast<const v4i *>(&DBb[(I * kDB + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
+    double sumR = 0.0, sumG[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 16; v++) sumR += static_cast<double>(digits5(dI, v)) * x[v];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const size_t o = CI * kChunkGrad + (c * kBlockSize + p) * kC + 16 * lh;
+        v4i gq[4];
+#pragma unroll
+        for (int w = 0; w < 4; w++) gq[w] = *reinterpret_cast<const v4i *>(GF + o + 4 * w);
+#pragma unroll
+        for (int v = 0; v < 16; v++) sumG[c] += int_of(gq, v) * x[v];
+    }
+    double out[4] = {ldexp(sumR, eb - 7 * kDB), 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        out[c + 1] = sumG[c] * ldexp(1.0, J.EG[(static_cast<size_t>(b) * 3 + c) * kBlockSize + p] - kGradBits);
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[c] += __shfl_xor(out[c], 32, 64);
+    if (lh == 0) {
+        double *q = J.part + CI * 4 * kBlockSize;
+#pragma unroll
+        for (int c = 0; c < 4; c++) q[c * kBlockSize + p] = out[c];
+    }
+}
+
+// Density functional (one block per work-group, 128 threads = one per point): rho and grad rho as the sums of the
+// block's row-chunk shares in chunk order, the FP64 functional, E_xc / N_e block sums and the four W coefficients per
+// point.  kRhoOut: the density stage of one spin of an unrestricted quadrature (the job's rhoOut receives rho and
+// grad rho).
+template <bool kRhoOut>
+__global__ void __launch_bounds__(kBlockSize)
+density_functional_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
+    __shared__ double wsum[2][2];
+    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, blockIdx.x, [](const xq::MixedJob &j) { return j.blockBase; })];
+    const int b = blockIdx.x - J.blockBase;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int pStart = J.blocks[b].pStart, np = J.blocks[b].nPoints;
+    const int c0 = J.chunkOffset[b], T = J.chunkOffset[b + 1] - c0;
     double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-    if (t < np) { rho = rsum[0][t]; gx = 2.0 * rsum[1][t]; gy = 2.0 * rsum[2][t]; gz = 2.0 * rsum[3][t]; }
+    if (t < np) {
+        const double *q = J.part + static_cast<size_t>(c0) * 4 * kBlockSize + t;
+        for (int I = 0; I < T; I++, q += 4 * kBlockSize) {
+            rho += q[0];
+            gx += q[kBlockSize];
+            gy += q[2 * kBlockSize];
+            gz += q[3 * kBlockSize];
+        }
+        gx *= 2.0;
+        gy *= 2.0;
+        gz *= 2.0;
+    }
     if (kRhoOut) {   // one spin of an unrestricted quadrature: the functional follows in its own kernel
         if (t < np) {
             double *o = J.rhoOut + 4 * (static_cast<size_t>(pStart) + t);
@@ -544,26 +528,26 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
         }
         return;
     }
-    double mx = t < np ? fabs(rho) : 0.0;
+    double mx = fabs(rho);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) wsum[wv][0] = mx;
     __syncthreads();
-    const bool skipped = !(fmax(fmax(wsum[0][0], wsum[1][0]), fmax(wsum[2][0], wsum[3][0])) > rhoTol);
+    const bool skipped = T == 0 || !(fmax(wsum[0][0], wsum[1][0]) > J.rhoTol);
     __syncthreads();
     double e = 0.0, ne = 0.0;
     if (t < np) {
         double vr = 0.0, vs = 0.0;
         if (!skipped) {
             double eps;
-            evaluate_functional(fs, rho, gx * gx + gy * gy + gz * gz, eps, vr, vs);
-            const double ww = w[pStart + t];
+            evaluate_functional(J.fs, rho, gx * gx + gy * gy + gz * gz, eps, vr, vs);
+            const double ww = J.w[pStart + t];
             vr *= ww;
             vs *= ww;
             e = eps * rho * ww;
             ne = rho * ww;
         }
-        double *cw = wcoef + 4 * (static_cast<size_t>(pStart) + t);
+        double *cw = J.wcoef + 4 * (static_cast<size_t>(pStart) + t);
         cw[0] = 0.5 * vr;
         cw[1] = 2.0 * vs * gx;
         cw[2] = 2.0 * vs * gy;
@@ -574,9 +558,9 @@ density_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     if (lane == 0) { wsum[wv][0] = e; wsum[wv][1] = ne; }
     __syncthreads();
     if (t == 0) {
-        blockEN[2 * b] = ((wsum[0][0] + wsum[1][0]) + wsum[2][0]) + wsum[3][0];
-        blockEN[2 * b + 1] = ((wsum[0][1] + wsum[1][1]) + wsum[2][1]) + wsum[3][1];
-        skip[b] = skipped ? 1 : 0;
+        J.blockEN[2 * b] = wsum[0][0] + wsum[1][0];
+        J.blockEN[2 * b + 1] = wsum[0][1] + wsum[1][1];
+        J.skip[b] = skipped ? 1 : 0;
     }
 }
 
@@ -782,6 +766,7 @@ int chunk_tables(xq_grid *g) {
 // one epilogue per row chunk and the functional (counted as one step each, and one for a block without kept
 // functions).  The blocks are cut into 8 contiguous ranges of equal summed cost, one per XCD, each sorted by decreasing
 // cost: on (H2O)64 the largest blocks take about 28 times the mean, and started last they set the length of the launch.
+// The mixed density launch takes the same order with each block cut into its row chunks, consecutive on its XCD.
 int build_block_order(xq_grid *g, hipStream_t s) {
     const int nB = g->nBlocks;
     std::vector<double> cost(static_cast<size_t>(nB > 0 ? nB : 1), 0.0);
@@ -810,12 +795,24 @@ int build_block_order(xq_grid *g, hipStream_t s) {
         std::stable_sort(order.begin() + g->xcdStart[x], order.begin() + g->xcdStart[x + 1],
                          [&](int u, int v) { return cost[u] > cost[v]; });
     g->h_order.assign(order.size(), make_int4(0, 0, 0, 0));
-    for (int i = 0; i < nB; i++) {
-        const xq::DevBlock &blk = g->blocks[order[i]];
-        g->h_order[i] = make_int4(order[i], firstChunk[order[i]], blk.iOffset, blk.nKept * 256 + blk.nPoints);
+    g->h_items.clear();
+    for (int x = 0; x < 8; x++) {
+        g->itemStart[x] = static_cast<int>(g->h_items.size());
+        for (int i = g->xcdStart[x]; i < g->xcdStart[x + 1]; i++) {
+            const xq::DevBlock &blk = g->blocks[order[i]];
+            g->h_order[i] = make_int4(order[i], firstChunk[order[i]], blk.iOffset, blk.nKept * 256 + blk.nPoints);
+            // The block's row chunks, one item each (nKept <= 26000, so that row chunk I < 1024 fits beside it).
+            for (int I = 0; I < (blk.nKept + kC - 1) / kC; I++)
+                g->h_items.push_back(make_int4(firstChunk[order[i]] + I, blk.nKept * 1024 + I, blk.iOffset, order[i]));
+        }
     }
+    g->itemStart[8] = static_cast<int>(g->h_items.size());
     XQ_TRY(dev_alloc(g, &g->d_order, g->h_order.size()));
+    XQ_TRY(dev_alloc(g, &g->d_items, g->h_items.empty() ? 1 : g->h_items.size()));
     XQ_CHECK(hipMemcpyAsync(g->d_order, g->h_order.data(), g->h_order.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    if (!g->h_items.empty())
+        XQ_CHECK(hipMemcpyAsync(g->d_items, g->h_items.data(), g->h_items.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                s));
     XQ_CHECK(hipStreamSynchronize(s));
     return kOK;
 }
@@ -923,6 +920,7 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
     XQ_TRY(dev_alloc(g, &g->d_Pq, static_cast<size_t>(g->nFunctions) * g->nFunctions));
     XQ_TRY(dev_alloc(g, &g->d_eP, static_cast<size_t>(g->nFunctions)));
     XQ_TRY(dev_alloc(g, &g->d_wcoef, 4 * static_cast<size_t>(g->nPoints > 0 ? g->nPoints : 1)));
+    XQ_TRY(dev_alloc(g, &g->d_part, nc * 4 * kBlockSize));
     hipStream_t s = g->stream;
     if (g->nBlocks > 0)
         hipLaunchKernelGGL(point_exponent_kernel, dim3(g->nBlocks), dim3(kBlockSize), 0, s, g->d_blocks, f, fx, fy, fz,
@@ -942,11 +940,13 @@ int prepare_mixed(xq_grid *g, const double *f, const double *fx, const double *f
 }
 
 // Per-step arrays of a second job on the handle (the beta spin of an unrestricted quadrature): P digits and exponents,
-// W digits and exponents.  Complete or released.
+// W digits and exponents, the density's row-chunk shares.  Complete or released.
 int alloc_step_mixed(xq_grid *g, StepArrays *a) {
     const size_t ncU = static_cast<size_t>(g->nChunksU > 0 ? g->nChunksU : 1), n = static_cast<size_t>(g->nFunctions);
+    const size_t nc = static_cast<size_t>(g->nChunks > 0 ? g->nChunks : 1);
     int status = dev_alloc(g, &a->DW, ncU * kChunkBytes);
     if (status == kOK) status = dev_alloc(g, &a->EW, ncU * kC);
+    if (status == kOK) status = dev_alloc(g, &a->part, nc * 4 * kBlockSize);
     if (status == kOK) status = dev_alloc(g, &a->Pq, n * n);
     if (status == kOK) status = dev_alloc(g, &a->eP, n);
     if (status != kOK) free_step_mixed(g, a);
@@ -958,12 +958,16 @@ void free_step_mixed(xq_grid *g, StepArrays *a) {
     dev_free(g, &a->EW);
     dev_free(g, &a->Pq);
     dev_free(g, &a->eP);
+    dev_free(g, &a->part);
 }
 
 void free_mixed(xq_grid *g) {
     dev_free(g, &g->d_order);
+    dev_free(g, &g->d_items);
+    dev_free(g, &g->d_part);
     g->h_order.clear();
-    for (int x = 0; x <= 8; x++) g->xcdStart[x] = 0;
+    g->h_items.clear();
+    for (int x = 0; x <= 8; x++) g->xcdStart[x] = g->itemStart[x] = 0;
     dev_free(g, &g->d_chunkOffset);
     dev_free(g, &g->d_chunkBlock);
     dev_free(g, &g->d_tiles);
@@ -998,8 +1002,9 @@ void free_mixed(xq_grid *g) {
 // The job table entries of the given handles (each with its functional) and each stage's total work-groups.
 void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJob *h_table, long long *rowsOut,
                long long *blocksOut, long long *chunksOut, long long *groupsOut, const StepArrays *over,
-               long long *blockSlotsOut) {
+               long long *blockSlotsOut, long long *itemSlotsOut) {
     long long rows = 0, blocks = 0, chunks = 0, groups = 0, slots[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long itemSlots[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = 0; j < nJobs; j++) {
         const xq_grid *g = grids[j];
         MixedJob &J = h_table[j];
@@ -1009,6 +1014,13 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
         for (int x = 0; x < 8; x++) {
             J.xcdBase[x] = static_cast<int>(slots[x]);
             slots[x] += g->xcdStart[x + 1] - g->xcdStart[x];
+        }
+        J.items = g->d_items;
+        J.part = g->d_part;
+        for (int x = 0; x <= 8; x++) J.itemStart[x] = g->itemStart[x];
+        for (int x = 0; x < 8; x++) {
+            J.itemBase[x] = static_cast<int>(itemSlots[x]);
+            itemSlots[x] += g->itemStart[x + 1] - g->itemStart[x];
         }
         J.chunkOffset = g->d_chunkOffset;
         J.chunkBlock = g->d_chunkBlock;
@@ -1065,6 +1077,7 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
             if (o.eP) J.eP = o.eP;
             if (o.DW) J.DW = o.DW;
             if (o.EW) J.EW = o.EW;
+            if (o.part) J.part = o.part;
         }
         rows += J.n;
         blocks += J.nBlocks;
@@ -1074,6 +1087,7 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
     *rowsOut = rows;
     *blocksOut = blocks;
     if (blockSlotsOut != nullptr) *blockSlotsOut = *std::max_element(slots, slots + 8);
+    if (itemSlotsOut != nullptr) *itemSlotsOut = *std::max_element(itemSlots, itemSlots + 8);
     *chunksOut = chunks;
     *groupsOut = groups;
 }
@@ -1082,9 +1096,10 @@ void fill_jobs(int nJobs, xq_grid *const *grids, const FunctionalSet *fs, MixedJ
 // least nJobs entries; the caller synchronises the stream before h_table is written again.
 int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const *grids, const FunctionalSet *fs,
                  MixedJob *h_table, MixedJob *d_table, int stages, const StepArrays *over) {
-    long long rows, blocks, chunks, groups, blockSlots;
-    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups, over, &blockSlots);
-    if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16) {
+    long long rows, blocks, chunks, groups, itemSlots;
+    fill_jobs(nJobs, grids, fs, h_table, &rows, &blocks, &chunks, &groups, over, nullptr, &itemSlots);
+    if (rows > INT32_MAX / 2 || blocks > INT32_MAX / 2 || chunks > INT32_MAX / 2 || groups > INT32_MAX / 16 ||
+        itemSlots > INT32_MAX / 16) {
         set_error("batch too large for one launch");
         return kInvalidArgument;
     }
@@ -1093,12 +1108,19 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
         hipLaunchKernelGGL(p_slice_kernel, dim3(static_cast<unsigned>(rows)), dim3(256), 0, s, d_table, nJobs);
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
     if ((stages & kStageDensity) && blocks > 0) {
-        const unsigned grid = static_cast<unsigned>(8 * blockSlots);
+        if (itemSlots > 0)
+            hipLaunchKernelGGL(density_rows_kernel, dim3(static_cast<unsigned>(8 * itemSlots)), dim3(256), 0, s, d_table,
+                               nJobs);
+        XQ_CHECK(hipEventRecord(ev.evDens1, s));
         if (h_table[0].rhoOut != nullptr) {   // the density stage of one spin (every job of the call)
-            hipLaunchKernelGGL(density_kernel_mixed<true>, dim3(grid), dim3(256), 0, s, d_table, nJobs);
+            hipLaunchKernelGGL(density_functional_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kBlockSize), 0,
+                               s, d_table, nJobs);
         } else {
-            hipLaunchKernelGGL(density_kernel_mixed<false>, dim3(grid), dim3(256), 0, s, d_table, nJobs);
+            hipLaunchKernelGGL(density_functional_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(kBlockSize),
+                               0, s, d_table, nJobs);
         }
+    } else {
+        XQ_CHECK(hipEventRecord(ev.evDens1, s));
     }
     XQ_CHECK(hipEventRecord(ev.evRho1, s));
     if ((stages & kStageFock) && chunks > 0) {

@@ -149,7 +149,7 @@ static int integrate_noalloc_queued(xq_grid *g, const FunctionalSet &fs, const R
     MixedJob table[1];   // pageable: copied to the device before launch_* returns; the stream is drained before return
     std::memset(static_cast<void *>(table), 0, sizeof(table));
     xq_grid *grids[1] = {g};
-    const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1};
+    const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1, g->evDens1};
     if (g->precision == XQ_PRECISION_FP64) {
         XQ_TRY(launch_f64(s, ev, 1, grids, &fs, table, r.job));
     } else {
@@ -207,12 +207,13 @@ int integrate_noalloc(xq_grid *g, int nIDs, const int *ids, const double *Ppacke
     *eXC = host3[0];
     *nE = host3[1];
     if (stats != nullptr) {
-        float ms = 0.0f, msRho = 0.0f, msFock = 0.0f;
+        float ms = 0.0f, msRho = 0.0f, msFock = 0.0f, msProduct = 0.0f;
         (void)hipEventElapsedTime(&ms, g->ev0, g->ev1);
         (void)hipEventElapsedTime(&msRho, g->evRho0, g->evRho1);
         (void)hipEventElapsedTime(&msFock, g->evRho1, g->evFock1);
-        double rb, fb;
-        stage_bytes(g, &rb, &fb);
+        (void)hipEventElapsedTime(&msProduct, g->evRho0, g->evDens1);
+        double rb, fb, pb;
+        stage_bytes(g, &rb, &fb, &pb);
         stats[0] = g->bfValues;
         stats[1] = host3[2];
         stats[2] = static_cast<double>(ms);
@@ -222,6 +223,8 @@ int integrate_noalloc(xq_grid *g, int nIDs, const int *ids, const double *Ppacke
         stats[6] = rb;
         stats[7] = fb;
         stats[8] = g->bfSquaresPadded;
+        stats[9] = static_cast<double>(msProduct);
+        stats[10] = pb;
     }
     return kOK;
 }

@@ -92,7 +92,7 @@ int integrate_uks(xq_grid *g, int nIDs, const int *ids, const double *Pa, const 
     XQ_TRY(functional_set(nIDs, ids, &fs));
     XQ_TRY(ensure_work(g));
     hipStream_t s = g->stream;
-    const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1};
+    const StageEvents ev{g->ev0, g->ev1, g->evRho0, g->evRho1, g->evFock1, g->evDens1};
     const int n = g->nFunctions;
     const size_t nn = static_cast<size_t>(n) * n, nPacked = static_cast<size_t>(n) * (n + 1) / 2;
     const size_t nP = static_cast<size_t>(g->nPoints > 0 ? g->nPoints : 1);

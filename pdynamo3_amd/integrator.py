@@ -16,7 +16,7 @@ from .definitions import (DFTGridAccuracy, LDA_C_VWN, LDA_C_VWN_RPA, LDA_X, exac
                           functional_ids)
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
-NSTATS = 9          # XQ_NSTATS of include/xcquad.h
+NSTATS = 11         # XQ_NSTATS of include/xcquad.h
 
 
 class QCModelError(Exception):
@@ -78,7 +78,8 @@ class QuadratureGrid:
                                      _lib.dptr(V), _lib.dptr(stats)), "integrate")
         return e.value, ne.value, {"bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2],
                                    "rho_ms": stats[3], "fock_ms": stats[4], "bf_squares": stats[5],
-                                   "rho_bytes": stats[6], "fock_bytes": stats[7], "bf_squares_padded": stats[8]}
+                                   "rho_bytes": stats[6], "fock_bytes": stats[7], "bf_squares_padded": stats[8],
+                                   "product_ms": stats[9], "product_bytes": stats[10]}
 
     def integrate_uks(self, ids, density_a, density_b, fock_a=None, fock_b=None):
         """One unrestricted quadrature (alpha and beta densities, packed): returns (E_xc, N_e, stats); fock_a and fock_b
@@ -192,7 +193,7 @@ class QuadratureBatch:
         return [(float(e[j]), float(ne[j])) for j in range(n)], {
             "bf_values": stats[0], "blocks_skipped": int(stats[1]), "device_ms": stats[2], "rho_ms": stats[3],
             "fock_ms": stats[4], "bf_squares": stats[5], "rho_bytes": stats[6], "fock_bytes": stats[7],
-            "bf_squares_padded": stats[8]}
+            "bf_squares_padded": stats[8], "product_ms": stats[9], "product_bytes": stats[10]}
 
     def close(self):
         if self._h:

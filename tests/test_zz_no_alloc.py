@@ -19,10 +19,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _call(name, grid, ids, P, V0):
-    from pdynamo3_amd import _lib
+    from pdynamo3_amd import _lib, integrator
     V = V0.copy()
     e, n = ctypes.c_double(), ctypes.c_double()
-    st = np.zeros(9)
+    st = np.zeros(integrator.NSTATS)
     ids = _lib.i32(ids)
     status = getattr(_lib.lib, name)(grid._h, ids.size, _lib.iptr(ids), _lib.dptr(P), ctypes.byref(e), ctypes.byref(n),
                                      _lib.dptr(V), _lib.dptr(st))
