@@ -15,8 +15,9 @@
 // The weight terms (DFTGridWeights_Derivatives) follow per point from eXC (weight_gradient_kernel, xq_grid.hip).
 //
 // Device layout: the basis data of a batch of consecutive blocks in ten planes of rows of 128 points (as the set-up's FP64
-// values), plus an X plane per spin and a U plane; batches keep the scratch bounded.  The products P_red Phi and P_red U run on FP64 MFMA
-// with P_red staged in LDS, as in the FP64 quadrature's density kernel.
+// values), plus X and U planes per spin; batches keep the scratch bounded.  The products P_red Phi and P_red U run on
+// FP64 MFMA with P_red staged in LDS, as in the FP64 quadrature's density kernel, one work-group per row chunk of a
+// block and spin; per batch: the X pass, the per-block functional (which forms U), the Z pass.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,7 +31,7 @@ namespace xq {
 
 namespace {
 
-// Rows of basis data per batch (12 planes of 128 doubles per row: about 1.6 GB).
+// Rows of basis data per batch (12 or 14 planes of 128 doubles per row: about 1.6 or 1.9 GB).
 constexpr long long kBatchRows = 131072;
 constexpr int kC = 32;                           // rows per chunk (two 16-row MFMA tiles)
 constexpr int kPanelF = 4;                       // chunks of P_red columns staged in LDS at once
@@ -48,10 +49,11 @@ __device__ __forceinline__ v4d mfma_f64(double a, double b, const v4d &c) {
 // and receives D[4 v + l / 16][l % 16]).  256 threads; wave wv owns points 32 wv .. + 31 (two 16-point tiles); for each
 // chunk I of 32 rows, epi(I, acc) receives acc[hm][hn][v] = (P_red Bp)[32 I + 16 hm + 4 v + l / 16][32 wv + 16 hn + l % 16].
 // P_red is staged in LDS by panels of kPanelF chunks of columns; Bp is the block's first row of a plane of rows of 128
-// points (rows >= nb are not read).
+// points (rows >= nb are not read).  Row chunks I0 .. I1 - 1.
 template <typename Epi>
 __device__ __forceinline__ void reduced_product(const double *__restrict__ P, int n, const int *idx, int nb,
-                                                const double *__restrict__ Bp, double (&sPf)[kC][kPanelPitch], Epi epi) {
+                                                const double *__restrict__ Bp, double (&sPf)[kC][kPanelPitch], int I0,
+                                                int I1, Epi epi) {
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, l16 = lane & 15, l4 = lane >> 4;
     const int nChunks = (nb + kC - 1) / kC;
     const int sr = t >> 3, s8 = t & 7;     // staging: row sr, columns 4 s8 .. + 3 of each chunk of the panel
@@ -64,7 +66,7 @@ __device__ __forceinline__ void reduced_product(const double *__restrict__ P, in
                 bq[s][h] = k < nb ? Bp[static_cast<size_t>(k) * kBlockSize + kC * wv + 16 * h + l16] : 0.0;
         }
     };
-    for (int I = 0; I < nChunks; I++) {
+    for (int I = I0; I < I1; I++) {
         v4d acc[2][2];
 #pragma unroll
         for (int hm = 0; hm < 2; hm++)
@@ -110,62 +112,45 @@ __device__ __forceinline__ void reduced_product(const double *__restrict__ P, in
     }
 }
 
-// Direct terms of the blocks of one batch (one work-group of 256 threads per block; work-group g runs on XCD g % 8 and
-// each XCD takes a contiguous range of blocks, so that its L2 holds the P rows they gather).  B: the ten basis planes of
-// the batch; Xp[s], Up: scratch planes with the same rows; eXC[point] = eps rho (zero for skipped or empty blocks); the
-// gradient terms go to copy (work-group % kGradCopies) of gradCopies.  kSpins = 1: P[0] is the total density; kSpins = 2:
-// P[0], P[1] are the alpha and beta densities.  Per spin s (t the other spin):
-//   pass 1: X_s = P_s Phi, rho_s, grad rho_s; then the functional (vrho, vsigma times w);
-//   restricted: u0 = 2 vrho, c = 4 vsigma grad rho; unrestricted: u0 = 2 vrho_s, c = 4 vsigma_ss grad rho_s +
-//   2 vsigma_ab grad rho_t (DFTIntegrator.c:405-487: _Gradients per spin and the two _GradientsSigma cross calls);
-//   U_s = u0 Phi + c . grad Phi, so that P_s U_s = 2 vrho X_s + P_s (grad Phi . c);
-//   pass 2: Z = P_s U_s and per row g_i = sum_p [Z grad phi + X_s (Hess phi . c)]_ip.
-// The blocks of one batch launch in launch order: XCD x takes order[start[x] .. start[x + 1]) (absolute block indices).
+// XCD ranges of a launch: XCD x takes entries start[x] .. start[x + 1] - 1 of its list.
 struct XcdRanges {
     int start[9];
 };
 
-template <int kSpins>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grad_direct_kernel(
-    const int *__restrict__ order, XcdRanges xr, long long rowBase, const DevBlock *__restrict__ blocks,
-    const int *__restrict__ idxAll, const int *__restrict__ atomOf, int n, const double *__restrict__ P0,
-    const double *__restrict__ P1, const double *__restrict__ w, FunctionalSet fs, double rhoTol, Planes10 B,
-    double *__restrict__ Xp0, double *__restrict__ Xp1, double *__restrict__ Up, double *__restrict__ eXC,
-    double *__restrict__ gradCopies, int nAtoms) {
+// The direct terms split by row chunk, in three launches per batch of blocks.  Items {block, row chunk I, slot of the
+// item's shares, batch-local block} run one work-group each; the row chunks of a block are consecutive items of one XCD
+// (work-group g runs on XCD g % 8), so that they read the block's basis rows at about the same time, through that XCD's
+// L2.
+//   grad_rows_x_kernel: per spin, X rows of chunk I = P_s Phi and the chunk's shares of rho_s and grad rho_s per point;
+//   grad_functional_kernel (one work-group per block): rho and grad rho as the sums of the shares in chunk order, the
+//     block skip, the functional, eXC and the coefficients (u0, c) per point, and U_s = u0 Phi + c . grad Phi for the
+//     block's rows;
+//   grad_rows_z_kernel: per spin, Z rows of chunk I = P_s U_s and the per-row terms of chunk I.
+// The two row-chunk kernels take spin blockIdx.y of nSpins.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grad_rows_x_kernel(
+    const int4 *__restrict__ items, XcdRanges xr, long long rowBase, const DevBlock *__restrict__ blocks,
+    const int *__restrict__ idxAll, int n, const double *__restrict__ P0, const double *__restrict__ P1, Planes10 B,
+    double *__restrict__ Xp0, double *__restrict__ Xp1, double *__restrict__ part, int nSpins) {
     __shared__ double sPf[kC][kPanelPitch];
     __shared__ int sIdx[kIdxCache];
-    __shared__ double rsum[kSpins][4][kBlockSize];
-    __shared__ double sCoef[4 * kSpins][kBlockSize];
-    __shared__ double sRed[4][kC][3];
-    __shared__ double smax[4];
-    // Work-group g runs on XCD g % 8 and takes slot g / 8 of that XCD's range of the batch.
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     if (slot >= xr.start[xcd + 1] - xr.start[xcd]) return;
-    const int b = order[xr.start[xcd] + slot];
+    const int4 it = items[xr.start[xcd] + slot];
+    const int b = it.x, I = it.y;
     const DevBlock blk = blocks[b];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, l16 = lane & 15, l4 = lane >> 4;
-    const int nb = blk.nKept, np = blk.nPoints;
-    if (nb <= 0) {
-        if (t < np) eXC[blk.pStart + t] = 0.0;
-        return;
-    }
+    const int nb = blk.nKept;
     const int *idxG = idxAll + blk.iOffset;
     for (int k = t; k < nb && k < kIdxCache; k += 256) sIdx[k] = idxG[k];
     __syncthreads();
     const int *idx = nb <= kIdxCache ? sIdx : idxG;
     const size_t R0 = static_cast<size_t>(blk.vOffset - rowBase) * kBlockSize;
     const double *F = B.p[0] + R0, *Fx = B.p[1] + R0, *Fy = B.p[2] + R0, *Fz = B.p[3] + R0;
-    double *U = Up + R0;
-    double *grad = gradCopies + static_cast<size_t>(b % kGradCopies) * 3 * nAtoms;
-    const double *Ps[2] = {P0, P1};
-    double *Xs[2] = {Xp0 + R0, kSpins == 2 ? Xp1 + R0 : nullptr};
-
-    // Pass 1, per spin: X, rho and grad rho (DFTIntegrator_GridPointRho, DFTIntegrator.c:814-877) into rsum[s].
-#pragma unroll
-    for (int sp = 0; sp < kSpins; sp++) {
-        double *X = Xs[sp];
+    const int sp = blockIdx.y;
+    {
+        double *X = (sp == 0 ? Xp0 : Xp1) + R0;
         double pr[2] = {0.0, 0.0}, pgx[2] = {0.0, 0.0}, pgy[2] = {0.0, 0.0}, pgz[2] = {0.0, 0.0};
-        reduced_product(Ps[sp], n, idx, nb, F, sPf, [&](int I, v4d (&acc)[2][2]) {
+        reduced_product(sp == 0 ? P0 : P1, n, idx, nb, F, sPf, I, I + 1, [&](int, v4d (&acc)[2][2]) {
 #pragma unroll
             for (int hm = 0; hm < 2; hm++)
 #pragma unroll
@@ -185,6 +170,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
                     }
                 }
         });
+        double *q = part + (static_cast<size_t>(it.z) * nSpins + sp) * 4 * kBlockSize;
 #pragma unroll
         for (int hn = 0; hn < 2; hn++) {
 #pragma unroll
@@ -196,61 +182,92 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
             }
             if (l4 == 0) {
                 const int p = kC * wv + 16 * hn + l16;
-                rsum[sp][0][p] = pr[hn];
-                rsum[sp][1][p] = pgx[hn];
-                rsum[sp][2][p] = pgy[hn];
-                rsum[sp][3][p] = pgz[hn];
+                q[p] = pr[hn];
+                q[kBlockSize + p] = pgx[hn];
+                q[2 * kBlockSize + p] = pgy[hn];
+                q[3 * kBlockSize + p] = pgz[hn];
             }
         }
     }
-    __syncthreads();
+}
+
+// One work-group of 256 threads per block of the batch (blockList: {block, slot of its first item's shares}).  coef
+// receives [batch-local block][4 kSpins][point]; skipOut 1 for a block without kept functions or below rhoTol.
+template <int kSpins>
+__global__ void __launch_bounds__(256) grad_functional_kernel(
+    const int2 *__restrict__ blockList, long long rowBase, const DevBlock *__restrict__ blocks,
+    const double *__restrict__ w, FunctionalSet fs, double rhoTol, Planes10 B, const double *__restrict__ part,
+    double *__restrict__ Up0, double *__restrict__ Up1, double *__restrict__ coef, int *__restrict__ skipOut,
+    double *__restrict__ eXC) {
+    __shared__ double sCoef[4 * kSpins][kBlockSize];
+    __shared__ double smax[4];
+    const int2 bl = blockList[blockIdx.x];
+    const int b = bl.x;
+    const DevBlock blk = blocks[b];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nb = blk.nKept, np = blk.nPoints, T = (nb + kC - 1) / kC;
     const bool valid = t < np;           // np <= 128: threads 128 .. 255 hold no point here
+    double rs[kSpins][4];
+#pragma unroll
+    for (int sp = 0; sp < kSpins; sp++) rs[sp][0] = rs[sp][1] = rs[sp][2] = rs[sp][3] = 0.0;
+    if (valid) {
+        for (int I = 0; I < T; I++)
+#pragma unroll
+            for (int sp = 0; sp < kSpins; sp++) {
+                const double *q = part + ((static_cast<size_t>(bl.y) + I) * kSpins + sp) * 4 * kBlockSize + t;
+#pragma unroll
+                for (int k = 0; k < 4; k++) rs[sp][k] += q[k * kBlockSize];
+            }
+    }
     double mx = 0.0;
     if (valid) {
 #pragma unroll
-        for (int sp = 0; sp < kSpins; sp++) mx = fmax(mx, fabs(rsum[sp][0][t]));
+        for (int sp = 0; sp < kSpins; sp++) mx = fmax(mx, fabs(rs[sp][0]));
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) smax[wv] = mx;
     __syncthreads();
-    if (!(fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3])) > rhoTol)) {   // the block is skipped (:332)
+    const int local = static_cast<int>(blockIdx.x);
+    if (nb <= 0 || !(fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3])) > rhoTol)) {   // the block is skipped (:332)
         if (valid) eXC[blk.pStart + t] = 0.0;
+        if (t == 0) skipOut[local] = 1;
         return;
     }
-    double coef[kSpins][4];
+    if (t == 0) skipOut[local] = 0;
+    double cf[kSpins][4];
 #pragma unroll
-    for (int sp = 0; sp < kSpins; sp++) coef[sp][0] = coef[sp][1] = coef[sp][2] = coef[sp][3] = 0.0;
+    for (int sp = 0; sp < kSpins; sp++) cf[sp][0] = cf[sp][1] = cf[sp][2] = cf[sp][3] = 0.0;
     if (valid) {
         const double ww = w[blk.pStart + t];
         if constexpr (kSpins == 1) {
-            const double rho = rsum[0][0][t], gx = 2.0 * rsum[0][1][t], gy = 2.0 * rsum[0][2][t],
-                         gz = 2.0 * rsum[0][3][t];
+            const double rho = rs[0][0], gx = 2.0 * rs[0][1], gy = 2.0 * rs[0][2], gz = 2.0 * rs[0][3];
             double eps, vr, vs;
             evaluate_functional(fs, rho, gx * gx + gy * gy + gz * gz, eps, vr, vs);
             vr *= ww;
             vs *= ww;
             eXC[blk.pStart + t] = eps * rho;
-            coef[0][0] = 2.0 * vr;
-            coef[0][1] = 4.0 * vs * gx;
-            coef[0][2] = 4.0 * vs * gy;
-            coef[0][3] = 4.0 * vs * gz;
+            cf[0][0] = 2.0 * vr;
+            cf[0][1] = 4.0 * vs * gx;
+            cf[0][2] = 4.0 * vs * gy;
+            cf[0][3] = 4.0 * vs * gz;
         } else {
-            double ra = rsum[0][0][t], rb = rsum[1][0][t], ga[3], gb[3];
+            const double ra = rs[0][0], rb = rs[1][0];
+            double ga[3], gb[3];
 #pragma unroll
-            for (int d = 0; d < 3; d++) { ga[d] = 2.0 * rsum[0][1 + d][t]; gb[d] = 2.0 * rsum[1][1 + d][t]; }
+            for (int d = 0; d < 3; d++) { ga[d] = 2.0 * rs[0][1 + d]; gb[d] = 2.0 * rs[1][1 + d]; }
             PolOut f;
             evaluate_functional_pol(fs, ra, rb, ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2],
                                     ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2],
                                     gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2], f);
             eXC[blk.pStart + t] = f.e * (ra + rb);
             const double vsaa = f.vsaa * ww, vsab = f.vsab * ww, vsbb = f.vsbb * ww;
-            coef[0][0] = 2.0 * f.va * ww;
-            coef[1][0] = 2.0 * f.vb * ww;
+            cf[0][0] = 2.0 * f.va * ww;
+            cf[1][0] = 2.0 * f.vb * ww;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                coef[0][1 + d] = 4.0 * vsaa * ga[d] + 2.0 * vsab * gb[d];
-                coef[1][1 + d] = 4.0 * vsbb * gb[d] + 2.0 * vsab * ga[d];
+                cf[0][1 + d] = 4.0 * vsaa * ga[d] + 2.0 * vsab * gb[d];
+                cf[1][1 + d] = 4.0 * vsbb * gb[d] + 2.0 * vsab * ga[d];
             }
         }
     }
@@ -258,29 +275,67 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
 #pragma unroll
         for (int sp = 0; sp < kSpins; sp++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) sCoef[4 * sp + k][t] = coef[sp][k];
+            for (int k = 0; k < 4; k++) {
+                sCoef[4 * sp + k][t] = cf[sp][k];
+                coef[(static_cast<size_t>(local) * 4 * kSpins + 4 * sp + k) * kBlockSize + t] = cf[sp][k];
+            }
     }
+    __syncthreads();
+    // U_s at point t % 128 for rows t / 128, + 2, ...
+    const size_t R0 = static_cast<size_t>(blk.vOffset - rowBase) * kBlockSize;
+    const double *F = B.p[0] + R0, *Fx = B.p[1] + R0, *Fy = B.p[2] + R0, *Fz = B.p[3] + R0;
+    const int p = t & (kBlockSize - 1);
+    double *Us[2] = {Up0 + R0, kSpins == 2 ? Up1 + R0 : nullptr};
+#pragma unroll
+    for (int sp = 0; sp < kSpins; sp++) {
+        const double c0 = sCoef[4 * sp][p], cx = sCoef[4 * sp + 1][p], cy = sCoef[4 * sp + 2][p],
+                     cz = sCoef[4 * sp + 3][p];
+        double *U = Us[sp];
+        for (int i = t >> 7; i < nb; i += 2) {
+            const size_t o = static_cast<size_t>(i) * kBlockSize + p;
+            U[o] = c0 * F[o] + cx * Fx[o] + cy * Fy[o] + cz * Fz[o];
+        }
+    }
+}
 
-    // Pass 2, per spin: Z = P_s U_s and the per-row terms (DFTIntegrator_Gradients :643-747, _GradientsSigma :752-808).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grad_rows_z_kernel(
+    const int4 *__restrict__ items, XcdRanges xr, long long rowBase, const DevBlock *__restrict__ blocks,
+    const int *__restrict__ idxAll, const int *__restrict__ atomOf, int n, const double *__restrict__ P0,
+    const double *__restrict__ P1, Planes10 B, const double *__restrict__ Xp0, const double *__restrict__ Xp1,
+    const double *__restrict__ Up0, const double *__restrict__ Up1, const double *__restrict__ coef,
+    const int *__restrict__ skip, double *__restrict__ gradCopies, int nAtoms, int nSpins) {
+    __shared__ double sPf[kC][kPanelPitch];
+    __shared__ int sIdx[kIdxCache];
+    __shared__ double sCoef[3][kBlockSize];
+    __shared__ double sRed[4][kC][3];
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    if (slot >= xr.start[xcd + 1] - xr.start[xcd]) return;
+    const int4 it = items[xr.start[xcd] + slot];
+    const int b = it.x, I = it.y, local = it.w;
+    if (skip[local]) return;
+    const DevBlock blk = blocks[b];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, l16 = lane & 15, l4 = lane >> 4;
+    const int nb = blk.nKept, np = blk.nPoints;
+    const int *idxG = idxAll + blk.iOffset;
+    const int sp = blockIdx.y;
+    for (int k = t; k < nb && k < kIdxCache; k += 256) sIdx[k] = idxG[k];
+    if (t < kBlockSize) {
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            sCoef[k][t] = coef[(static_cast<size_t>(local) * 4 * nSpins + 4 * sp + 1 + k) * kBlockSize + t];
+    }
+    __syncthreads();
+    const int *idx = nb <= kIdxCache ? sIdx : idxG;
+    const size_t R0 = static_cast<size_t>(blk.vOffset - rowBase) * kBlockSize;
+    const double *Fx = B.p[1] + R0, *Fy = B.p[2] + R0, *Fz = B.p[3] + R0;
     const double *Hxx = B.p[4] + R0, *Hxy = B.p[5] + R0, *Hxz = B.p[6] + R0, *Hyy = B.p[7] + R0, *Hyz = B.p[8] + R0,
                  *Hzz = B.p[9] + R0;
+    double *grad = gradCopies + static_cast<size_t>((b + sp) % kGradCopies) * 3 * nAtoms;
     double gs[3] = {0.0, 0.0, 0.0};
-    for (int sp = 0; sp < kSpins; sp++) {
-        __syncthreads();   // sCoef is written; the previous spin's product has finished reading U
-        {
-            // Thread t builds U at point t % 128 for rows t / 128, + 2, ...
-            const int p = t & (kBlockSize - 1);
-            const double c0 = sCoef[4 * sp][p], cx = sCoef[4 * sp + 1][p], cy = sCoef[4 * sp + 2][p],
-                         cz = sCoef[4 * sp + 3][p];
-            for (int i = t >> 7; i < nb; i += 2) {
-                const size_t o = static_cast<size_t>(i) * kBlockSize + p;
-                U[o] = c0 * F[o] + cx * Fx[o] + cy * Fy[o] + cz * Fz[o];
-            }
-        }
-        __syncthreads();   // U and X are complete for the work-group
-        const double *X = Xs[sp];
-        const double *sA = sCoef[4 * sp + 1], *sB = sCoef[4 * sp + 2], *sC = sCoef[4 * sp + 3];
-        reduced_product(Ps[sp], n, idx, nb, U, sPf, [&](int I, v4d (&acc)[2][2]) {
+    {
+        const double *X = (sp == 0 ? Xp0 : Xp1) + R0;
+        const double *sA = sCoef[0], *sB = sCoef[1], *sC = sCoef[2];
+        reduced_product(sp == 0 ? P0 : P1, n, idx, nb, (sp == 0 ? Up0 : Up1) + R0, sPf, I, I + 1, [&](int, v4d (&acc)[2][2]) {
             double c[2][4][3];
 #pragma unroll
             for (int hm = 0; hm < 2; hm++)
@@ -397,11 +452,13 @@ int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const do
     const long long batchRows = std::max(kBatchRows, maxRows);
     std::vector<double> zeros(3 * static_cast<size_t>(nA), 0.0);
     std::vector<double> zeroCopies(zeros.size() * kGradCopies, 0.0);
-    // Ten basis planes, X per spin, U.
-    const int nPlanes = 11 + spins;
-    double *planes[13] = {};
-    double *d_grad = nullptr, *d_eXC = nullptr, *d_copies = nullptr;
-    int *d_atomOf = nullptr, *d_order = nullptr;
+    // Ten basis planes, X and U per spin.
+    const int nPlanes = 10 + 2 * spins;
+    double *planes[14] = {};
+    double *d_grad = nullptr, *d_eXC = nullptr, *d_copies = nullptr, *d_part = nullptr, *d_coef = nullptr;
+    int *d_atomOf = nullptr, *d_skip = nullptr;
+    int4 *d_items = nullptr;
+    int2 *d_blockList = nullptr;
     // Batches of blocks whose kept rows fit the planes, and their launch orders.
     std::vector<std::pair<int, int>> batches;
     for (int b0 = 0; b0 < nB;) {
@@ -412,8 +469,37 @@ int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const do
         b0 = b1;
     }
     std::vector<int> order(static_cast<size_t>(nB > 0 ? nB : 1), 0);
+    // Per batch: its items (each block of the launch order cut into its row chunks, consecutive on the block's XCD),
+    // their XCD ranges, and per block {block, slot of its first item's shares} (slots in block order).
+    std::vector<int4> items;
+    std::vector<int2> blockList(static_cast<size_t>(nB > 0 ? nB : 1));
     std::vector<XcdRanges> ranges;
-    for (const auto &bt : batches) ranges.push_back(batch_order(g, bt.first, bt.second, order));
+    std::vector<size_t> itemBase;
+    size_t maxItems = 1, maxBlocks = 1;
+    for (const auto &bt : batches) {
+        const int b0 = bt.first, b1 = bt.second;
+        const XcdRanges xb = batch_order(g, b0, b1, order);
+        int slot = 0;
+        for (int b = b0; b < b1; b++) {
+            blockList[b] = make_int2(b, slot);
+            slot += (g->blocks[b].nKept + kC - 1) / kC;
+        }
+        maxItems = std::max(maxItems, static_cast<size_t>(slot));
+        maxBlocks = std::max(maxBlocks, static_cast<size_t>(b1 - b0));
+        itemBase.push_back(items.size());
+        XcdRanges xi;
+        for (int x = 0; x < 8; x++) {
+            xi.start[x] = static_cast<int>(items.size() - itemBase.back());
+            for (int k = xb.start[x]; k < xb.start[x + 1]; k++) {
+                const int b = order[k];
+                for (int I = 0; I < (g->blocks[b].nKept + kC - 1) / kC; I++)
+                    items.push_back(make_int4(b, I, blockList[b].y + I, b - b0));
+            }
+        }
+        xi.start[8] = static_cast<int>(items.size() - itemBase.back());
+        ranges.push_back(xi);
+    }
+    if (items.empty()) items.push_back(make_int4(0, 0, 0, 0));
     int status = kOK;
     do {
         bool ok = true;
@@ -433,12 +519,19 @@ int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const do
                                            hipMemcpyHostToDevice, s), "zero gradient copies")) != kOK) break;
         if ((status = check(hipMemcpyAsync(d_atomOf, atomOf.data(), atomOf.size() * sizeof(int), hipMemcpyHostToDevice, s),
                             "copy function atoms")) != kOK) break;
-        if ((status = dev_alloc(g, &d_order, order.size())) != kOK) break;
-        if ((status = check(hipMemcpyAsync(d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s),
-                            "copy block order")) != kOK) break;
+        if ((status = dev_alloc(g, &d_items, items.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_blockList, blockList.size())) != kOK) break;
+        if ((status = dev_alloc(g, &d_part, maxItems * spins * 4 * kBlockSize)) != kOK) break;
+        if ((status = dev_alloc(g, &d_coef, maxBlocks * spins * 4 * kBlockSize)) != kOK) break;
+        if ((status = dev_alloc(g, &d_skip, maxBlocks)) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s),
+                            "copy items")) != kOK) break;
+        if ((status = check(hipMemcpyAsync(d_blockList, blockList.data(), blockList.size() * sizeof(int2),
+                                           hipMemcpyHostToDevice, s), "copy block list")) != kOK) break;
         Planes10 B;
         for (int k = 0; k < 10; k++) B.p[k] = planes[k];
-        double *Xa = planes[10], *Xb = spins == 2 ? planes[11] : nullptr, *U = planes[nPlanes - 1];
+        double *Xa = planes[10], *Xb = spins == 2 ? planes[11] : planes[10];
+        double *Ua = planes[10 + spins], *Ub = spins == 2 ? planes[13] : Ua;
         for (size_t k = 0; k < batches.size() && status == kOK; k++) {
             const int b0 = batches[k].first, b1 = batches[k].second;
             const long long rowBase = g->blocks[b0].vOffset;
@@ -446,15 +539,23 @@ int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const do
             const XcdRanges &xr = ranges[k];
             int perXcd = 0;
             for (int x = 0; x < 8; x++) perXcd = std::max(perXcd, xr.start[x + 1] - xr.start[x]);
+            const int4 *it = d_items + itemBase[k];
+            const int2 *bl = d_blockList + b0;
+            const double *P1 = spins == 2 ? d_P1 : d_P0;
+            if (perXcd > 0)
+                hipLaunchKernelGGL(grad_rows_x_kernel, dim3(8 * perXcd, spins), dim3(256), 0, s, it, xr, rowBase,
+                                   g->d_blocks, g->d_idx, g->nFunctions, d_P0, P1, B, Xa, Xb, d_part, spins);
             if (spins == 1)
-                hipLaunchKernelGGL(grad_direct_kernel<1>, dim3(8 * perXcd), dim3(256), 0, s, d_order, xr, rowBase,
-                                   g->d_blocks, g->d_idx, d_atomOf, g->nFunctions, d_P0, d_P0, g->d_w, fs, g->rhoTol, B,
-                                   Xa, Xa, U, d_eXC, d_copies, nA);
+                hipLaunchKernelGGL(grad_functional_kernel<1>, dim3(b1 - b0), dim3(256), 0, s, bl, rowBase, g->d_blocks,
+                                   g->d_w, fs, g->rhoTol, B, d_part, Ua, Ub, d_coef, d_skip, d_eXC);
             else
-                hipLaunchKernelGGL(grad_direct_kernel<2>, dim3(8 * perXcd), dim3(256), 0, s, d_order, xr, rowBase,
-                                   g->d_blocks, g->d_idx, d_atomOf, g->nFunctions, d_P0, d_P1, g->d_w, fs, g->rhoTol, B,
-                                   Xa, Xb, U, d_eXC, d_copies, nA);
-            status = check(hipGetLastError(), "grad_direct_kernel");
+                hipLaunchKernelGGL(grad_functional_kernel<2>, dim3(b1 - b0), dim3(256), 0, s, bl, rowBase, g->d_blocks,
+                                   g->d_w, fs, g->rhoTol, B, d_part, Ua, Ub, d_coef, d_skip, d_eXC);
+            if (perXcd > 0)
+                hipLaunchKernelGGL(grad_rows_z_kernel, dim3(8 * perXcd, spins), dim3(256), 0, s, it, xr, rowBase,
+                                   g->d_blocks, g->d_idx, d_atomOf, g->nFunctions, d_P0, P1, B, Xa, Xb, Ua, Ub, d_coef,
+                                   d_skip, d_copies, nA, spins);
+            status = check(hipGetLastError(), "direct gradient kernels");
         }
         if (status != kOK) break;
         const int nEntries = static_cast<int>(zeros.size());
@@ -473,7 +574,11 @@ int gradients_on_device(xq_grid *g, const FunctionalSet &fs, int spins, const do
     dev_free(g, &d_copies);
     dev_free(g, &d_eXC);
     dev_free(g, &d_atomOf);
-    dev_free(g, &d_order);
+    dev_free(g, &d_items);
+    dev_free(g, &d_blockList);
+    dev_free(g, &d_part);
+    dev_free(g, &d_coef);
+    dev_free(g, &d_skip);
     return status;
 }
 

@@ -370,12 +370,13 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
 // contiguous DB chunk) in LDS, both fetched by all 256 threads into registers one step ahead.  The row chunks of a
 // block are consecutive items of one XCD, so that their work-groups run at about the same time and read the block's
 // Phi digit chunks, in the same order, through that XCD's L2: Phi comes from HBM about once per block instead of once
-// per row chunk.  32 KB of LDS and at most 168 VGPRs let three work-groups share a CU (four, at 128 VGPRs, were
-// slower: 3.56 against 2.75 ms on (H2O)64).
+// per row chunk.  Chunk I's digits, staged at step I, are kept in LDS for the epilogue.  52 KB of LDS and at most 168
+// VGPRs let three work-groups share a CU (four, at 128 VGPRs, were slower: 3.56 against 2.75 ms on (H2O)64).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 density_rows_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPRowDwords];   // [digit][row][K positions / 4]
     __shared__ v4i sB[kChunkBytesB / 16];   // the step's Phi digits, in DB's order
+    __shared__ v4i sBI[kChunkBytesB / 16];  // chunk I's Phi digits, kept from step I for the epilogue
     __shared__ int sPe[kC];
     __shared__ int sIdx[kIdxCache];
     // Work-group g runs on XCD x = g % 8 and takes slot g / 8 of that XCD's item list (the jobs' lists in job order).
@@ -436,6 +437,10 @@ density_rows_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
         __syncthreads();   // every wave is done with the previous chunk
 #pragma unroll
         for (int i = 0; i < kPhiPerThread; i++) sB[t + 256 * i] = ph[i];
+        if (c == I) {
+#pragma unroll
+            for (int i = 0; i < kPhiPerThread; i++) sBI[t + 256 * i] = ph[i];
+        }
 #pragma unroll
         for (int j = 0; j < kDP; j++) {
             unsigned x = 0;
@@ -466,10 +471,9 @@ density_rows_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
         const int s[kXOrders] = {acc[0][v], acc[1][v], acc[2][v], acc[3][v], acc[4][v]};
         x[v] = ldexp(static_cast<double>(combine<kXOrders>(s)), sPe[chunk_fun(lh, v)] + eb - 7 * (kXOrders + 1));
     }
-    v4i dI[kDB];   // the lane's Phi digits of chunk I
+    v4i dI[kDB];   // the lane's Phi digits of chunk I (written to sBI before the last step's barrier)
 #pragma unroll
-    for (int j = 0; j < kDB; j++)
-        dI[j] = *reinterpret_cast<const v4i *>(&DBb[(I * kDB + j) * (kBlockSize * kC) + p * kC + 16 * lh]);
+    for (int j = 0; j < kDB; j++) dI[j] = sBI[(j * kBlockSize + p) * 2 + lh];
     double sumR = 0.0, sumG[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int v = 0; v < 16; v++) sumR += static_cast<double>(digits5(dI, v)) * x[v];
