@@ -373,16 +373,21 @@ __global__ void __launch_bounds__(256) p_slice_kernel(const xq::MixedJob *__rest
 // per row chunk.  Chunk I's digits, staged at step I, are kept in LDS for the epilogue.  52 KB of LDS and at most 168
 // VGPRs let three work-groups share a CU (four, at 128 VGPRs, were slower: 3.56 against 2.75 ms on (H2O)64).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-density_rows_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs) {
+density_rows_kernel(const xq::MixedJob *__restrict__ jobs, int nJobs, int interleaved) {
     __shared__ __attribute__((aligned(16))) int sP[kDP][kC][kPRowDwords];   // [digit][row][K positions / 4]
     __shared__ v4i sB[kChunkBytesB / 16];   // the step's Phi digits, in DB's order
     __shared__ v4i sBI[kChunkBytesB / 16];  // chunk I's Phi digits, kept from step I for the epilogue
     __shared__ int sPe[kC];
     __shared__ int sIdx[kIdxCache];
-    // Work-group g runs on XCD x = g % 8 and takes slot g / 8 of that XCD's item list (the jobs' lists in job order).
+    // Work-group g runs on XCD x = g % 8 and takes slot g / 8 of that XCD's item list: the jobs' lists in job order, or,
+    // for the two spins of one handle (interleaved), item slot / 2 of job slot % 2, so that both spins' work-groups of
+    // a row chunk read its block's Phi chunks through L2 at about the same time.
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const xq::MixedJob &J = jobs[job_of(jobs, nJobs, slot, [xcd](const xq::MixedJob &j) { return j.itemBase[xcd]; })];
-    const int local = slot - J.itemBase[xcd];
+    const xq::MixedJob &J = interleaved ? jobs[slot & 1]
+                                        : jobs[job_of(jobs, nJobs, slot, [xcd](const xq::MixedJob &j) {
+                                              return j.itemBase[xcd];
+                                          })];
+    const int local = interleaved ? slot >> 1 : slot - J.itemBase[xcd];
     if (local >= J.itemStart[xcd + 1] - J.itemStart[xcd]) return;
     const signed char *__restrict__ DB = J.DB;
     const int *__restrict__ GF = J.GF;
@@ -648,12 +653,17 @@ __global__ void __launch_bounds__(256) w_digits_kernel(const xq::MixedJob *__res
 // Work-group g runs on XCD g % 8; tiles are dealt so that each XCD takes a contiguous range of the tile list (tiles of
 // one superblock share their chunks' digits in that XCD's L2).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-fock_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups) {
+fock_kernel_mixed(const xq::MixedJob *__restrict__ jobs, int nJobs, int groupsPerXcd, int nGroups, int interleaved) {
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 31, lh = lane >> 5;
     const int group = (blockIdx.x & 7) * groupsPerXcd + (blockIdx.x >> 3);
     if (group >= nGroups) return;
-    const xq::MixedJob &job = jobs[job_of(jobs, nJobs, group, [](const xq::MixedJob &j) { return j.groupBase; })];
-    const int tileIndex = 4 * (group - job.groupBase) + wv;
+    // The two spins of one handle (interleaved): group g is group g / 2 of job g % 2, so that both spins' tiles of a
+    // superblock share its Phi digits through L2.
+    const xq::MixedJob &job = interleaved ? jobs[group & 1]
+                                          : jobs[job_of(jobs, nJobs, group, [](const xq::MixedJob &j) {
+                                                return j.groupBase;
+                                            })];
+    const int tileIndex = 4 * (interleaved ? group >> 1 : group - job.groupBase) + wv;
     if (tileIndex >= job.nTiles) return;
     const int *__restrict__ uChunkOffset = job.uChunkOffset;
     const signed char *__restrict__ DA = job.DA;
@@ -1108,13 +1118,17 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
         return kInvalidArgument;
     }
     XQ_CHECK(hipMemcpyAsync(d_table, h_table, nJobs * sizeof(MixedJob), hipMemcpyHostToDevice, s));
+    // The two spins of one handle (an unrestricted quadrature) run interleaved, item by item and tile group by tile
+    // group.
+    const int interleaved = nJobs == 2 && grids[0] == grids[1] ? 1 : 0;
     if ((stages & kStageDensity) && rows > 0)
         hipLaunchKernelGGL(p_slice_kernel, dim3(static_cast<unsigned>(rows)), dim3(256), 0, s, d_table, nJobs);
     XQ_CHECK(hipEventRecord(ev.evRho0, s));
     if ((stages & kStageDensity) && blocks > 0) {
+        // The two spins of one handle (an unrestricted quadrature) run interleaved item by item.
         if (itemSlots > 0)
             hipLaunchKernelGGL(density_rows_kernel, dim3(static_cast<unsigned>(8 * itemSlots)), dim3(256), 0, s, d_table,
-                               nJobs);
+                               nJobs, interleaved);
         XQ_CHECK(hipEventRecord(ev.evDens1, s));
         if (h_table[0].rhoOut != nullptr) {   // the density stage of one spin (every job of the call)
             hipLaunchKernelGGL(density_functional_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kBlockSize), 0,
@@ -1131,7 +1145,7 @@ int launch_mixed(hipStream_t s, const StageEvents &ev, int nJobs, xq_grid *const
         hipLaunchKernelGGL(w_digits_kernel, dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s, d_table, nJobs);
         const int perXcd = static_cast<int>((groups + 7) / 8);
         hipLaunchKernelGGL(fock_kernel_mixed, dim3(static_cast<unsigned>(8 * perXcd)), dim3(256), 0, s, d_table, nJobs,
-                           perXcd, static_cast<int>(groups));
+                           perXcd, static_cast<int>(groups), interleaved);
     }
     XQ_CHECK(hipEventRecord(ev.evFock1, s));
     XQ_CHECK(hipGetLastError());
