@@ -372,12 +372,16 @@ __global__ void __launch_bounds__(kBlockSize) basis_eval2_kernel(
     const double x = valid ? px[blk.pStart + t] : 0.0, y = valid ? py[blk.pStart + t] : 0.0,
                  z = valid ? pz[blk.pStart + t] : 0.0;
     const int *kept = idx + blk.iOffset;
-    int k = 0;
-    while (k < blk.nKept) {
+    // Work-group (b, h) writes the kept rows h nKept / S .. (h + 1) nKept / S - 1 (S = gridDim.y); a shell that
+    // straddles two ranges is evaluated by both.
+    const int S = static_cast<int>(gridDim.y), h = static_cast<int>(blockIdx.y);
+    const int k1 = static_cast<int>((static_cast<long long>(h) + 1) * blk.nKept / S);
+    int k = static_cast<int>(static_cast<long long>(h) * blk.nKept / S);
+    while (k < k1) {
         const xq::DevShell sh = shells[shellOf[kept[k]]];
         with_shell2(sh, spherical != 0, coef, trans, x - sh.x, y - sh.y, z - sh.z, [&](int nB, const double (*v)[16]) {
             for (int j = 0; j < nB; j++) {
-                if (k < blk.nKept && kept[k] == sh.fStart + j) {
+                if (k < k1 && kept[k] == sh.fStart + j) {
                     const size_t o = static_cast<size_t>(blk.vOffset - rowBase + k) * kBlockSize + t;
 #pragma unroll
                     for (int d = 0; d < 10; d++) out.p[d][o] = valid ? v[d][j] : 0.0;
@@ -618,7 +622,9 @@ int eval_basis2(xq_grid *g, int b0, int b1, long long rowBase, const Planes10 &o
                                 g->stream));
         XQ_CHECK(hipStreamSynchronize(g->stream));
     }
-    hipLaunchKernelGGL(basis_eval2_kernel, dim3(b1 - b0), dim3(kBlockSize), 0, s, b0, rowBase, g->d_blocks, g->d_x, g->d_y,
+    // Four work-groups per block, each a quarter of its kept rows: the blocks' rows are evaluated one shell after another,
+    // so a launch would otherwise last as long as its largest block.
+    hipLaunchKernelGGL(basis_eval2_kernel, dim3(b1 - b0, 4), dim3(kBlockSize), 0, s, b0, rowBase, g->d_blocks, g->d_x, g->d_y,
                        g->d_z, g->d_shells, g->d_shellOf, g->d_coef, g->d_trans, g->spherical, g->d_idx, out);
     XQ_CHECK(hipGetLastError());
     return kOK;
