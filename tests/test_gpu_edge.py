@@ -165,3 +165,30 @@ def test_device_bytes_follow_the_precision():
     assert grid_only > 0
     assert bmix < b64
     assert b64again == b64
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_density_product_statistics(precision):
+    """stats[9] (the density product kernel's device time) lies inside the density stage's time (stats[3]); stats[10]
+    (its algorithmic bytes) is the stage's figure on the FP64 path (one kernel) and, on the mixed path, 17 bytes per
+    kept value, P digits and row exponents, 4 bytes per point and a whole number of 4 KB row-chunk shares, at least
+    one per 32 x 128 kept values."""
+    from pdynamo3_amd import basis, integrator, molecules
+    Z, xa, _, _ = molecules.water_cluster(4, 9.0 * (4 / 16.0) ** (1.0 / 3.0), 1)
+    xyz = molecules.to_bohr(xa)
+    spec = basis.element_spec(basis.load_basis(), Z)
+    n = basis.number_of_functions(basis.load_basis(), Z, False)
+    P = random_density(n, 20, seed=5)
+    g = integrator.QuadratureGrid(Z, xyz, 2)
+    g.set_basis(spec, False, precision)
+    _, _, st = g.integrate(ids("pbe"), P)
+    G = g.number_of_points
+    g.close()
+    assert 0.0 < st["product_ms"] <= st["rho_ms"]
+    if precision == "fp64":
+        assert st["product_bytes"] == st["rho_bytes"]
+    else:
+        shares = st["product_bytes"] - (17.0 * st["bf_values"] + 8.0 * n * n + 4.0 * n + 4.0 * G)
+        chunks = shares / 4096.0
+        assert chunks == round(chunks)
+        assert chunks >= st["bf_values"] / (32.0 * 128.0)
